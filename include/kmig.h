@@ -1,0 +1,162 @@
+/*
+ * kmig.h — C ABI of the MI355X Kirchhoff angle-gather imaging library
+ * (libkmig.so), a drop-in for the adjoint path of the reference's
+ * kirchmig2d_cig program.
+ *
+ * What each entry point replaces in the reference:
+ *   kmig_params_default / kmig_params_parse
+ *        the key=value parsing, defaults and validation of main(),
+ *        src/kirchmig2d_cig.cpp:548-875, with parse_output_axis :323-368
+ *        and the se_get_par_* casts of SEFILESYSTEM/src/se_par_sep.cpp:432-446;
+ *   kmig_create
+ *        the one-time set-up of main() (:924-962: buffers, tinterp_init,
+ *        kirmig_init; SERECKIRCH/src/tinterpsr.cpp:27-35, kirmig.cpp:26-32);
+ *   kmig_set_tables
+ *        the slice streamer load_table_slice / interp_volume_linear_stream
+ *        (:274-322) and the table checks of :763-815 — the whole table is made
+ *        resident in HBM instead of streamed two slices at a time;
+ *   kmig_set_data
+ *        the sequential trace reads se_fsio_read_float (:1069);
+ *   kmig_run / kmig_sync
+ *        the shot loop :979-1353 (tinterp_hermite/dinterp_hermite,
+ *        compute_grad_plane, smooth_grad_component, apply_direct_wave_mute,
+ *        doubint/diff2, the OpenMP image-point loop with kirmig_pick(adj=1));
+ *   kmig_get_cig / kmig_get_qc
+ *        the output writes se_fsio_write_float (:1071, :1080, :1359);
+ *   kmig_migrate
+ *        all of the above in one call;
+ *   kmig_last_error
+ *        the ERROR(...) message that the reference prints before exit(1)
+ *        (SEBASIC/src/se_log.cpp:426-486); here every error is a status code.
+ *
+ * Conventions: 0 = OK, negative = error class (KMIG_E_*).  No entry point
+ * exits the process.  One context per host thread; calls on one context are
+ * serialised by the caller; all device work of a context runs on one HIP
+ * stream owned by the context.  Array layouts follow RSF axis order with
+ * axis 1 fastest: data (t,h,s) = data[(is*nh + ih)*nt + it], tables
+ * (z,x,pos) = T[(ipos*nx + ix)*nz + iz], CIG (z,x,g) = cig[(ig*nx + ix)*nz + iz].
+ */
+#ifndef KMIG_H
+#define KMIG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KMIG_OK 0
+#define KMIG_E_PARAM (-1)   /* invalid or missing parameter            */
+#define KMIG_E_DIM (-2)     /* inconsistent array dimensions            */
+#define KMIG_E_IO (-3)      /* file input/output (RSF helpers, CLI)     */
+#define KMIG_E_DEVICE (-4)  /* HIP runtime error or no usable GPU       */
+#define KMIG_E_STATE (-5)   /* call made before its inputs were set     */
+#define KMIG_E_NOMEM (-6)   /* device or host allocation failed         */
+
+/* Table precision of the resident traveltime tuples. */
+#define KMIG_PREC_FP32 0
+#define KMIG_PREC_FP16 1
+#define KMIG_PREC_BF16 2
+
+/* Every key of kirchmig2d_cig's adj=1 path, same names, units and defaults
+   (src/kirchmig2d_cig.cpp:520-560, :568-598, :707-717, :816-862, :952). */
+typedef struct kmig_params {
+    int adj;                   /* 1 (migration); adj=0 is not provided      */
+    int cig;                   /* 1: common-image gathers, 0: stacked image */
+    int cig_is_angle;          /* mode=angle (1) or mode=offset (0)         */
+    int cmp;                   /* 1: data axis 2 is offset, 0: receiver x   */
+    int half_angle;            /* 0: full opening angle, 1: half opening    */
+    float oazmin, oazmax, oaztaper; /* depth-varying angle mute (degrees)  */
+    int diff;                  /* 1: centred derivative after doubint       */
+    int aperture_trace;        /* -1: off; else max |iht - ist| (table idx) */
+    int aperture_trace_taper;
+    int direct_mute, direct_mute_invert, direct_mute_qc_shot;
+    float direct_mute_slope, direct_mute_intercept, direct_mute_taper;
+    float tau, aperture, aperture_taper, aperture_rmin, aperture_rtaper, antialias;
+    int angle_interp;          /* 1: two-bin linear, 0: nearest bin         */
+    float angle_taper_start, angle_taper_end;
+    int angle_grad_radius, angle_smooth;
+    int interp_type;           /* 'h' (Hermite), 'l' (linear), 'p' (partial) */
+    float g0, dg;              /* output CIG axis (origin, step)            */
+    int ng;                    /* output CIG axis length (1 if cig == 0)    */
+    /* MI355X-side options (no counterpart keys in the reference) */
+    int precision;             /* KMIG_PREC_*                               */
+    int tile;                  /* migrate tile preset, 0 = automatic        */
+} kmig_params;
+
+typedef struct kmig_stats {
+    double updates;            /* nominal ns*nh*nz*nx of the last run       */
+    double ms_trace_prep;      /* kernel time, HIP events on the ctx stream */
+    double ms_tuple_prep;
+    double ms_migrate;
+    double ms_total;           /* first to last event of the last run       */
+    int migrate_launches;      /* migrate kernel launches in the last run   */
+    int receiver_slots;        /* resident receiver tuples                  */
+    int shot_batches;
+} kmig_stats;
+
+typedef struct kmig_ctx kmig_ctx;
+
+/* Fill *p with the reference's defaults (mode=offset, cig=0, cmp=1, ...). */
+void kmig_params_default(kmig_params *p);
+
+/* Apply key=value arguments with the reference's semantics: later keys
+   override earlier ones, integers parse as (int)double and floats as
+   (float)double; mode=angle turns cig on unless cig= is given; the output
+   axis is derived from the data axis 2 (nh,h0,dh) and the cig_ / angle_ /
+   offset_ keys.  File keys (seismic_data=, ...) are ignored here.
+   Returns KMIG_OK or KMIG_E_PARAM with the reference's message in err. */
+int kmig_params_parse(kmig_params *p, int argc, const char *const *argv, int nh, float h0,
+                      float dh, char *err, int errlen);
+
+/* Host-only validation of a parameter set against the data axes: the
+   checks of src/kirchmig2d_cig.cpp:719-738 (direct_mute options and QC
+   shot range), :359/:363 (axis) and :956-957 (interp type). */
+int kmig_params_check(const kmig_params *p, int ns, char *err, int errlen);
+
+/* Create a context on the current HIP device.  Validates p. */
+int kmig_create(kmig_ctx **out, const kmig_params *p);
+void kmig_destroy(kmig_ctx *ctx);
+const char *kmig_last_error(const kmig_ctx *ctx);
+
+/* Make one side's tables resident (side 0: source, 1: receiver).  T and
+   dTds are (nz,nx,npos); gx/gz are the optional spatial-gradient volumes
+   (either both NULL or both given on both sides before kmig_run).  Host
+   pointers are borrowed for the duration of the call.  The receiver table
+   must match the source table's nz and nx. */
+int kmig_set_tables(kmig_ctx *ctx, int side, const float *T, const float *dTds,
+                    const float *gx, const float *gz, int nz, int nx, int npos, float z0,
+                    float dz, float x0, float dx, float o_pos, float d_pos);
+
+/* Upload the seismic data (nt,nh,ns) to HBM with its axes. */
+int kmig_set_data(kmig_ctx *ctx, const float *data, int nt, float t0, float dt, int nh,
+                  float h0, float dh, int ns, float s0, float ds);
+
+/* Migrate shots [shot_begin, shot_end) into the context's CIG, which is
+   zeroed first.  Enqueues trace preparation, tuple preparation and the
+   migration kernels on the context stream and returns without waiting. */
+int kmig_run(kmig_ctx *ctx, int shot_begin, int shot_end);
+int kmig_sync(kmig_ctx *ctx);
+
+/* Copy results to host memory after kmig_sync: cig has ng*nx*nz floats;
+   qc_before/qc_after have nt*nh floats (the direct_mute_qc_shot gather). */
+int kmig_get_cig(kmig_ctx *ctx, float *cig);
+int kmig_get_qc(kmig_ctx *ctx, float *qc_before, float *qc_after);
+int kmig_get_stats(const kmig_ctx *ctx, kmig_stats *st);
+
+/* One call: set_data + run(all shots) + sync + get_cig (+ QC when not NULL). */
+int kmig_migrate(kmig_ctx *ctx, const float *data, int nt, float t0, float dt, int nh,
+                 float h0, float dh, int ns, float s0, float ds, float *cig,
+                 float *qc_before, float *qc_after);
+
+/* The HIP stream of the context (a hipStream_t), for callers that time or
+   order their own work around kmig_run. */
+void *kmig_stream(kmig_ctx *ctx);
+
+/* Library version string. */
+const char *kmig_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

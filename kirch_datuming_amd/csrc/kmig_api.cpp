@@ -1,0 +1,804 @@
+// C ABI of libkmig: context, HBM residency, per-run geometry, launches.
+//
+// The per-shot / per-trace geometry of the reference (source and receiver
+// positions, table node choice, Hermite coordinate, trace aperture, offset
+// bin) is evaluated here on the host with the reference's float arithmetic
+// (src/kirchmig2d_cig.cpp:979-1105, 1148-1165) and handed to the kernels as
+// SlotSpec / TraceMeta records.  Everything per image point runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "kmig.h"
+#include "kmig_internal.h"
+
+using namespace kmig;
+
+namespace {
+
+struct Side {
+    float *T = nullptr, *D = nullptr, *GX = nullptr, *GZ = nullptr;
+    int npos = 0;
+    float o = 0.0f, d = 0.0f;
+    bool set = false;
+    size_t cap = 0;  // elements allocated per volume
+};
+
+struct Batch {
+    int s0, s1;       // shot range
+    int nrcv;         // receiver slots
+    size_t spec_r0;   // offset of this batch's receiver specs
+};
+
+}  // namespace
+
+struct kmig_ctx {
+    kmig_params p;
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nz = 0, nx = 0;
+    float z0 = 0, dz = 0, x0 = 0, dx = 0;
+    size_t nzx = 0;
+    Side side[2];
+    int nt = 0, nh = 0, ns = 0;
+    float t0 = 0, dt = 0, h0 = 0, dh = 0, s0 = 0, ds = 0;
+    bool data_set = false;
+    float *raw = nullptr, *prep = nullptr;
+    size_t raw_cap = 0;
+    TraceMeta *meta = nullptr;
+    size_t meta_cap = 0;
+    SlotSpec *spec_s = nullptr, *spec_r = nullptr;
+    size_t spec_s_cap = 0, spec_r_cap = 0;
+    Tuples ts, tr;
+    size_t ts_cap = 0, tr_cap = 0;  // slots
+    float *tmp1 = nullptr, *tmp2 = nullptr;
+    size_t tmp1_cap = 0, tmp2_cap = 0;  // elements
+    float *cig = nullptr;
+    size_t cig_cap = 0;
+    float *qcb = nullptr, *qca = nullptr;
+    size_t qc_cap = 0;
+    bool qc_valid = false;
+    std::vector<Batch> batches;
+    std::vector<hipEvent_t> events;
+    std::vector<int> ev_kind;       // per event pair: 0 trace, 1 tuple, 2 migrate
+    int ev_used = 0;
+    int last_sb = -1, last_se = -1;
+    long long geo_version = 0, plan_version = -1;
+    kmig_stats st{};
+    bool ran = false, pending = false;
+};
+
+namespace {
+
+void dfree(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+int fail(kmig_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                      \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail((c), KMIG_E_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int ensure(kmig_ctx *c, T **ptr, size_t *cap, size_t n)
+{
+    if (*cap >= n && *ptr) return KMIG_OK;
+    if (*ptr) dfree(*ptr);
+    *ptr = nullptr;
+    *cap = 0;
+    if (n == 0) return KMIG_OK;
+    hipError_t e = hipMalloc((void **)ptr, n * sizeof(T));
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(c, KMIG_E_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T),
+                    hipGetErrorString(e));
+    }
+    *cap = n;
+    return KMIG_OK;
+}
+
+int ensure_tuples(kmig_ctx *c, Tuples &t, size_t *cap, size_t slots)
+{
+    if (*cap >= slots && t.T) return KMIG_OK;
+    float **f[5] = {&t.T, &t.A, &t.GX, &t.GZ, &t.W};
+    for (float **q : f) {
+        if (*q) dfree(*q);
+        *q = nullptr;
+    }
+    *cap = 0;
+    for (float **q : f) {
+        size_t dummy = 0;
+        int rc = ensure(c, q, &dummy, slots * c->nzx);
+        if (rc) return rc;
+    }
+    *cap = slots;
+    t.cap = (int)slots;
+    return KMIG_OK;
+}
+
+// Table node choice and interpolation coefficients for one position, as
+// src/kirchmig2d_cig.cpp:987-1028 (source) / :1148-1180 (receiver) and
+// SERECKIRCH/src/tinterpsr.cpp:37-188 compute them (float geometry, double
+// Hermite basis from the float coordinate).  Returns the clamped node.
+int make_spec(const kmig_ctx *c, const Side &sd, float pos, SlotSpec *sp)
+{
+    memset(sp, 0, sizeof(*sp));
+    sp->surf_x = pos;
+    int node = 0, interp = 0;
+    float coord = 0.0f;
+    if (!(sd.npos <= 1 || sd.d == 0.0f)) {
+        const float f = (pos - sd.o) / sd.d;
+        node = (int)floorf(f);
+        if (f <= 0.0f) {
+            node = 0;
+        } else if (f >= (float)(sd.npos - 1)) {
+            node = sd.npos - 1;
+        } else {
+            interp = 1;
+            coord = pos - (sd.o + (float)node * sd.d);
+        }
+    }
+    sp->node = node;
+    sp->interp = interp;
+    sp->coord = coord;
+    if (interp) {
+        const float dx = sd.d;
+        const float t = coord / dx;
+        sp->t = t;
+        // identical expressions (and float/double promotions) to tinterpsr.cpp:150-184
+        const double h00 = 2. * t * t * t - 3. * t * t + 1.;
+        const double h10 = t * t * t - 2. * t * t + t;
+        const double h01 = -2. * t * t * t + 3. * t * t;
+        const double h11 = t * t * t - t * t;
+        const double d00 = -6. * t * (1. - t);
+        const double d10 = (1. - t) * (1. - 3. * t);
+        const double d01 = 6. * t * (1. - t);
+        const double d11 = 3. * t * t - 2. * t;
+        sp->h[0] = h00;
+        sp->h[1] = h10 * dx;
+        sp->h[2] = h01;
+        sp->h[3] = h11 * dx;
+        sp->h[4] = d00;
+        sp->h[5] = d10 * dx;
+        sp->h[6] = d01;
+        sp->h[7] = d11 * dx;
+        sp->k1 = (int)(coord / c->dx + 0.5);
+        sp->k2 = (int)((dx - coord) / c->dx + 0.5);
+    }
+    // gradient volumes: interp_volume_linear_stream, :294-322
+    sp->gnode = 0;
+    sp->ginterp = 0;
+    if (!(sd.npos <= 1 || sd.d == 0.0f)) {
+        const float f = (pos - sd.o) / sd.d;
+        if (f <= 0.0f) {
+            sp->gnode = 0;
+        } else if (f >= (float)(sd.npos - 1)) {
+            sp->gnode = sd.npos - 1;
+        } else {
+            const int i0 = (int)floorf(f);
+            sp->gnode = i0;
+            sp->ginterp = 1;
+            sp->gw = f - (float)i0;
+        }
+    }
+    return node;
+}
+
+float nearest_bin_h(float value, float g0, float dg, int ng)
+{
+    if (ng <= 0 || dg <= 0.0f || !std::isfinite(value)) return -1;
+    int ig = (int)floorf((value - g0) / dg + 0.5f);
+    if (ig < 0 || ig >= ng) return -1;
+    return (float)ig;
+}
+
+size_t free_device_bytes()
+{
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return (size_t)8 << 30;
+    return fr;
+}
+
+// Build TraceMeta for shots [sb, se), the batches and their slot specs.
+int plan_run(kmig_ctx *c, int sb, int se, std::vector<TraceMeta> &meta,
+             std::vector<SlotSpec> &sspec, std::vector<SlotSpec> &rspec)
+{
+    const kmig_params &p = c->p;
+    const Side &S = c->side[0], &Rt = c->side[1];
+    const int nh = c->nh;
+    const float dh = (nh == 1) ? 0.0f : c->dh;        // :700
+    const float ds = (c->ns == 1) ? 0.0f : c->ds;     // :701
+    const bool angle = p.cig && p.cig_is_angle;
+    const size_t fields = angle ? 5 : 3;
+    const size_t per_slot = c->nzx * sizeof(float) * (fields + 2);
+    size_t budget = (size_t)(0.6 * (double)free_device_bytes());
+    // already-allocated tuple/tmp memory can be reused
+    budget += (c->ts_cap + c->tr_cap) * c->nzx * sizeof(float) * 5 +
+              (c->tmp1_cap + c->tmp2_cap) * sizeof(float);
+    size_t max_slots = std::max<size_t>(2, budget / per_slot);
+    // KMIG_MAX_SLOTS caps the tuple slots per batch (forces shot batching)
+    if (const char *env = getenv("KMIG_MAX_SLOTS")) {
+        const long v = atol(env);
+        if (v > 0) max_slots = std::max<size_t>(2, (size_t)v);
+    }
+
+    meta.assign((size_t)(se - sb) * nh, TraceMeta{});
+    sspec.clear();
+    rspec.clear();
+    c->batches.clear();
+    Batch cur{sb, sb, 0, 0};
+    std::unordered_map<uint32_t, int> slot_of;
+    std::vector<SlotSpec> cur_r;
+    auto close = [&](int s1) {
+        cur.s1 = s1;
+        cur.nrcv = (int)cur_r.size();
+        cur.spec_r0 = rspec.size();
+        rspec.insert(rspec.end(), cur_r.begin(), cur_r.end());
+        c->batches.push_back(cur);
+        cur = Batch{s1, s1, 0, 0};
+        cur_r.clear();
+        slot_of.clear();
+    };
+    for (int is = sb; is < se; is++) {
+        const float s = c->s0 + (float)is * ds;  // :980
+        SlotSpec ss;
+        const int ist = make_spec(c, S, s, &ss);
+        // trace records and the receivers this shot needs
+        std::vector<std::pair<int, float>> need;  // (ih, receiver_pos) of migrating traces
+        for (int ih = 0; ih < nh; ih++) {
+            TraceMeta &m = meta[(size_t)(is - sb) * nh + ih];
+            const float h = c->h0 + (float)ih * dh;
+            const float receiver_pos = p.cmp ? (s + h) : h;
+            float rf = 0.0f;
+            if (Rt.npos > 1 && Rt.d != 0.0f) rf = (receiver_pos - Rt.o) / Rt.d;
+            const int iht = (Rt.npos > 1 && Rt.d != 0.0f) ? (int)floorf(rf) : 0;
+            m.src_x = s;
+            m.rcv_x = p.cmp ? (s + h) : h;
+            m.slot = -1;
+            m.ig = 0;
+            float wap = 1.0f;
+            bool skip = false;
+            if (p.aperture_trace != -1) {  // :1082-1099
+                const float ad = fabsf((float)(iht - ist));
+                if (ad >= (float)p.aperture_trace) {
+                    skip = true;
+                } else if (p.aperture_trace_taper > 0) {
+                    const float taper_begin =
+                        std::max(0.0f, (float)(p.aperture_trace - p.aperture_trace_taper));
+                    if (ad > taper_begin) {
+                        const float denom = (float)p.aperture_trace - taper_begin;
+                        if (denom > 0.0f) {
+                            const float x = (ad - taper_begin) / denom;
+                            wap = 0.5f * (1.0f + cosf((float)M_PI * x));
+                        }
+                    }
+                }
+            }
+            if (!skip && p.cig && !p.cig_is_angle) {  // :1100-1102
+                const float ig = nearest_bin_h(h, p.g0, p.dg, p.ng);
+                if (ig < 0) skip = true;
+                else m.ig = (int)ig;
+            }
+            m.wap = skip ? -1.0f : wap;
+            if (!skip) need.push_back({ih, receiver_pos});
+        }
+        // new receiver positions of this shot
+        int fresh = 0;
+        for (auto &q : need) {
+            uint32_t key;
+            memcpy(&key, &q.second, 4);
+            if (!slot_of.count(key)) fresh++;
+        }
+        const size_t nsrc_now = (size_t)(is - cur.s0);
+        if (is > cur.s0 && (nsrc_now + 1 + cur_r.size() + fresh) > max_slots) close(is);
+        for (auto &q : need) {
+            uint32_t key;
+            memcpy(&key, &q.second, 4);
+            auto it = slot_of.find(key);
+            int slot;
+            if (it == slot_of.end()) {
+                slot = (int)cur_r.size();
+                SlotSpec rs;
+                make_spec(c, Rt, q.second, &rs);
+                cur_r.push_back(rs);
+                slot_of[key] = slot;
+            } else {
+                slot = it->second;
+            }
+            meta[(size_t)(is - sb) * nh + q.first].slot = slot;
+        }
+        sspec.push_back(ss);
+    }
+    close(se);
+    return KMIG_OK;
+}
+
+int check_tables_type_p(kmig_ctx *c, const std::vector<SlotSpec> &v)
+{
+    if (c->p.interp_type != 'p') return KMIG_OK;
+    if (c->nx < 2)
+        return fail(c, KMIG_E_PARAM, "type=p needs at least two image columns (n2 >= 2).");
+    for (const SlotSpec &s : v)
+        if (s.interp && (s.k1 > c->nx || s.k2 > c->nx || s.k1 < 0 || s.k2 < 0))
+            return fail(c, KMIG_E_PARAM,
+                        "type=p: the table position spacing spans more than the image width.");
+    return KMIG_OK;
+}
+
+hipEvent_t next_event(kmig_ctx *c)
+{
+    if (c->ev_used >= (int)c->events.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->events.push_back(e);
+    }
+    return c->events[c->ev_used++];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *kmig_version(void) { return "kmig 0.1 (gfx950)"; }
+
+int kmig_create(kmig_ctx **out, const kmig_params *p)
+{
+    if (!out || !p) return KMIG_E_PARAM;
+    *out = nullptr;
+    char err[512] = {0};
+    int rc = kmig_params_check(p, -1, err, sizeof(err));
+    kmig_ctx *c = new kmig_ctx();
+    c->p = *p;
+    if (rc) {
+        c->err = err;
+        *out = c;
+        return rc;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        *out = c;
+        return fail(c, KMIG_E_DEVICE, "no HIP device is available");
+    }
+    (void)hipGetDevice(&c->device);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        *out = c;
+        return fail(c, KMIG_E_DEVICE, "hipStreamCreate failed");
+    }
+    *out = c;
+    return KMIG_OK;
+}
+
+void kmig_destroy(kmig_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (Side &s : c->side) {
+        dfree(s.T);
+        dfree(s.D);
+        dfree(s.GX);
+        dfree(s.GZ);
+    }
+    dfree(c->raw);
+    dfree(c->prep);
+    dfree(c->meta);
+    dfree(c->spec_s);
+    dfree(c->spec_r);
+    for (Tuples *t : {&c->ts, &c->tr}) {
+        dfree(t->T);
+        dfree(t->A);
+        dfree(t->GX);
+        dfree(t->GZ);
+        dfree(t->W);
+    }
+    dfree(c->tmp1);
+    dfree(c->tmp2);
+    dfree(c->cig);
+    dfree(c->qcb);
+    dfree(c->qca);
+    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *kmig_last_error(const kmig_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+void *kmig_stream(kmig_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int kmig_set_tables(kmig_ctx *c, int side, const float *T, const float *dTds, const float *gx,
+                    const float *gz, int nz, int nx, int npos, float z0, float dz, float x0,
+                    float dx, float o_pos, float d_pos)
+{
+    if (!c) return KMIG_E_PARAM;
+    if (!c->stream) return fail(c, KMIG_E_STATE, "context has no device");
+    if (side != 0 && side != 1) return fail(c, KMIG_E_PARAM, "side must be 0 or 1");
+    if (!T || !dTds) return fail(c, KMIG_E_PARAM, "T and dTds are required");
+    if ((gx == nullptr) != (gz == nullptr))
+        return fail(c, KMIG_E_PARAM,
+                    "For mode=angle, please provide all of sgradx=, sgradz=, rgradx=, and "
+                    "rgradz=, or provide none of them.");
+    if (nz <= 0 || nx <= 0 || npos <= 0)
+        return fail(c, KMIG_E_DIM, "Need 3D data volume for %s= (z,x,%s)",
+                    side ? "rtable" : "stable", side ? "r" : "s");
+    const Side &other = c->side[1 - side];
+    if (other.set && (nz != c->nz || nx != c->nx)) {
+        if (side == 1)
+            return fail(c, KMIG_E_DIM, "Dimension mismatch in rtable=. Expected n1=%d n2=%d",
+                        c->nz, c->nx);
+        return fail(c, KMIG_E_DIM, "Dimension mismatch in stable=. Expected n1=%d n2=%d", c->nz,
+                    c->nx);
+    }
+    if (side == 0 || !other.set) {
+        c->nz = nz;
+        c->nx = nx;
+        c->z0 = z0;
+        c->dz = dz;
+        c->x0 = x0;
+        c->dx = dx;
+        c->nzx = (size_t)nz * nx;
+    }
+    Side &s = c->side[side];
+    const size_t n = (size_t)npos * c->nzx;
+    int rc;
+    if ((rc = ensure(c, &s.T, &s.cap, n))) return rc;
+    size_t cap2 = s.cap;
+    if ((rc = ensure(c, &s.D, &cap2, n))) return rc;
+    HIPCHK(c, hipMemcpy(s.T, T, n * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(s.D, dTds, n * sizeof(float), hipMemcpyHostToDevice));
+    if (gx) {
+        size_t cg = 0;
+        dfree(s.GX);
+        dfree(s.GZ);
+        s.GX = s.GZ = nullptr;
+        if ((rc = ensure(c, &s.GX, &cg, n))) return rc;
+        cg = 0;
+        if ((rc = ensure(c, &s.GZ, &cg, n))) return rc;
+        HIPCHK(c, hipMemcpy(s.GX, gx, n * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(s.GZ, gz, n * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        dfree(s.GX);
+        dfree(s.GZ);
+        s.GX = s.GZ = nullptr;
+    }
+    s.npos = npos;
+    s.o = o_pos;
+    s.d = d_pos;
+    s.set = true;
+    c->geo_version++;
+    return KMIG_OK;
+}
+
+int kmig_set_data(kmig_ctx *c, const float *data, int nt, float t0, float dt, int nh, float h0,
+                  float dh, int ns, float s0, float ds)
+{
+    if (!c) return KMIG_E_PARAM;
+    if (!c->stream) return fail(c, KMIG_E_STATE, "context has no device");
+    if (!data || nt < 2 || nh <= 0 || ns <= 0)
+        return fail(c, KMIG_E_DIM, "Need 3D data volume for seismic_data= (t,h/r,s)");
+    char err[512] = {0};
+    int rc = kmig_params_check(&c->p, ns, err, sizeof(err));
+    if (rc) return fail(c, rc, "%s", err);
+    const size_t n = (size_t)nt * nh * ns;
+    size_t cap = c->raw_cap;
+    if ((rc = ensure(c, &c->raw, &c->raw_cap, n))) return rc;
+    if ((rc = ensure(c, &c->prep, &cap, n))) return rc;
+    HIPCHK(c, hipMemcpy(c->raw, data, n * sizeof(float), hipMemcpyHostToDevice));
+    c->nt = nt;
+    c->t0 = t0;
+    c->dt = dt;
+    c->nh = nh;
+    c->h0 = h0;
+    c->dh = dh;
+    c->ns = ns;
+    c->s0 = s0;
+    c->ds = ds;
+    c->data_set = true;
+    c->geo_version++;
+    return KMIG_OK;
+}
+
+int kmig_run(kmig_ctx *c, int sb, int se)
+{
+    if (!c) return KMIG_E_PARAM;
+    if (!c->stream) return fail(c, KMIG_E_STATE, "context has no device");
+    if (!c->side[0].set || !c->side[1].set)
+        return fail(c, KMIG_E_STATE, "source and receiver tables must be set before kmig_run");
+    if (!c->data_set) return fail(c, KMIG_E_STATE, "data must be set before kmig_run");
+    const kmig_params &p = c->p;
+    const bool angle = p.cig && p.cig_is_angle;
+    const bool gfiles = c->side[0].GX != nullptr;
+    if (angle && gfiles != (c->side[1].GX != nullptr))
+        return fail(c, KMIG_E_PARAM,
+                    "For mode=angle, please provide all of sgradx=, sgradz=, rgradx=, and "
+                    "rgradz=, or provide none of them.");
+    if (sb < 0) sb = 0;
+    if (se > c->ns || se < 0) se = c->ns;
+    if (se < sb) se = sb;
+    int rc;
+    const int ng = p.cig ? p.ng : 1;
+    c->ev_used = 0;
+    c->ev_kind.clear();
+
+    // host geometry (cached while the inputs and the shot range are unchanged)
+    if (c->plan_version != c->geo_version || sb != c->last_sb || se != c->last_se) {
+        std::vector<TraceMeta> meta;
+        std::vector<SlotSpec> sspec, rspec;
+        if ((rc = plan_run(c, sb, se, meta, sspec, rspec))) return rc;
+        if ((rc = check_tables_type_p(c, sspec))) return rc;
+        if ((rc = check_tables_type_p(c, rspec))) return rc;
+        // meta is indexed by global trace number
+        if ((rc = ensure(c, &c->meta, &c->meta_cap, (size_t)c->ns * c->nh))) return rc;
+        if (!meta.empty())
+            HIPCHK(c, hipMemcpy(c->meta + (size_t)sb * c->nh, meta.data(),
+                                meta.size() * sizeof(TraceMeta), hipMemcpyHostToDevice));
+        if ((rc = ensure(c, &c->spec_s, &c->spec_s_cap, std::max<size_t>(1, sspec.size()))))
+            return rc;
+        if ((rc = ensure(c, &c->spec_r, &c->spec_r_cap, std::max<size_t>(1, rspec.size()))))
+            return rc;
+        if (!sspec.empty())
+            HIPCHK(c, hipMemcpy(c->spec_s, sspec.data(), sspec.size() * sizeof(SlotSpec),
+                                hipMemcpyHostToDevice));
+        if (!rspec.empty())
+            HIPCHK(c, hipMemcpy(c->spec_r, rspec.data(), rspec.size() * sizeof(SlotSpec),
+                                hipMemcpyHostToDevice));
+        size_t maxs = 0, maxr = 0;
+        for (const Batch &b : c->batches) {
+            maxs = std::max(maxs, (size_t)(b.s1 - b.s0));
+            maxr = std::max(maxr, (size_t)b.nrcv);
+        }
+        if ((rc = ensure_tuples(c, c->ts, &c->ts_cap, std::max<size_t>(1, maxs)))) return rc;
+        if ((rc = ensure_tuples(c, c->tr, &c->tr_cap, std::max<size_t>(1, maxr)))) return rc;
+        const size_t tslots = std::max<size_t>(1, std::max(maxs, maxr));
+        if (angle && p.angle_smooth > 0) {
+            if ((rc = ensure(c, &c->tmp1, &c->tmp1_cap, tslots * c->nzx))) return rc;
+            if ((rc = ensure(c, &c->tmp2, &c->tmp2_cap, tslots * c->nzx))) return rc;
+        }
+        c->plan_version = c->geo_version;
+        c->last_sb = sb;
+        c->last_se = se;
+    }
+    if ((rc = ensure(c, &c->cig, &c->cig_cap, (size_t)ng * c->nzx))) return rc;
+    const bool want_qc = p.direct_mute && p.direct_mute_qc_shot >= sb && p.direct_mute_qc_shot < se;
+    if (want_qc) {
+        size_t cap2 = c->qc_cap;
+        if ((rc = ensure(c, &c->qcb, &c->qc_cap, (size_t)c->nt * c->nh))) return rc;
+        if ((rc = ensure(c, &c->qca, &cap2, (size_t)c->nt * c->nh))) return rc;
+    }
+    c->qc_valid = want_qc;
+
+    hipEvent_t e0 = next_event(c);
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    HIPCHK(c, (hipError_t)launch_fill(c->cig, (size_t)ng * c->nzx, 0.0f, c->stream));
+
+    TracePrepArgs ta{};
+    ta.raw = c->raw;
+    ta.prep = c->prep;
+    ta.meta = c->meta;
+    ta.qcb = c->qcb;
+    ta.qca = c->qca;
+    ta.nt = c->nt;
+    ta.nh = c->nh;
+    ta.shot_begin = sb;
+    ta.shot_end = se;
+    ta.t0 = c->t0;
+    ta.dt = c->dt;
+    ta.mute = p.direct_mute;
+    ta.invert = p.direct_mute_invert;
+    ta.qc_shot = want_qc ? p.direct_mute_qc_shot : -1;
+    ta.diff = p.diff;
+    ta.slope = p.direct_mute_slope;
+    ta.intercept = p.direct_mute_intercept;
+    ta.taper = p.direct_mute_taper;
+    HIPCHK(c, (hipError_t)launch_trace_prep(ta, c->stream));
+    hipEvent_t e1 = next_event(c);
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    c->ev_kind.push_back(0);
+
+    const float dh = (c->nh == 1) ? 0.0f : c->dh;
+    const float ds = (c->ns == 1) ? 0.0f : c->ds;
+    size_t spec_s_off = 0;
+    int launches = 0;
+    for (const Batch &b : c->batches) {
+        const int nsb = b.s1 - b.s0;
+        TuplePrepArgs A{};
+        A.nz = c->nz;
+        A.nx = c->nx;
+        A.nzx = c->nzx;
+        A.x0 = c->x0;
+        A.dx = c->dx;
+        A.dz = c->dz;
+        A.dd = c->dx;
+        A.type = p.interp_type;
+        A.use_grad_files = gfiles ? 1 : 0;
+        A.radius = p.angle_grad_radius;
+        A.smooth = p.angle_smooth;
+        A.angle_mode = angle ? 1 : 0;
+        A.aper = p.aperture;
+        A.aper_taper = p.aperture_taper;
+        A.rmin = p.aperture_rmin;
+        A.rtaper = p.aperture_rtaper;
+        // source side
+        A.tab = c->side[0].T;
+        A.der = c->side[0].D;
+        A.gxv = c->side[0].GX;
+        A.gzv = c->side[0].GZ;
+        A.spec = c->spec_s + spec_s_off;
+        A.nslots = nsb;
+        A.dpos = c->side[0].d;
+        A.dside = ds;
+        hipEvent_t a0 = next_event(c);
+        HIPCHK(c, hipEventRecord(a0, c->stream));
+        HIPCHK(c, (hipError_t)launch_tuple_prep(A, c->ts, c->tmp1, c->tmp2, c->stream));
+        // receiver side
+        A.tab = c->side[1].T;
+        A.der = c->side[1].D;
+        A.gxv = c->side[1].GX;
+        A.gzv = c->side[1].GZ;
+        A.spec = c->spec_r + b.spec_r0;
+        A.nslots = b.nrcv;
+        A.dpos = c->side[1].d;
+        A.dside = dh;
+        HIPCHK(c, (hipError_t)launch_tuple_prep(A, c->tr, c->tmp1, c->tmp2, c->stream));
+        hipEvent_t a1 = next_event(c);
+        HIPCHK(c, hipEventRecord(a1, c->stream));
+        c->ev_kind.push_back(1);
+
+        MigArgs M{};
+        M.nz = c->nz;
+        M.nx = c->nx;
+        M.nzx = c->nzx;
+        M.nt = c->nt;
+        M.t0 = c->t0;
+        M.dt = c->dt;
+        M.inv_dt = 1.0f / c->dt;
+        M.nh = c->nh;
+        M.shot0 = b.s0;
+        M.nshots = nsb;
+        M.sT = c->ts.T;
+        M.sA = c->ts.A;
+        M.sGX = c->ts.GX;
+        M.sGZ = c->ts.GZ;
+        M.sW = c->ts.W;
+        M.rT = c->tr.T;
+        M.rA = c->tr.A;
+        M.rGX = c->tr.GX;
+        M.rGZ = c->tr.GZ;
+        M.rW = c->tr.W;
+        M.meta = c->meta;
+        M.traces = c->prep;
+        M.cig = c->cig;
+        M.tau = p.tau;
+        M.aal = p.antialias;
+        M.mode = angle ? (p.angle_interp ? 0 : 1) : 2;
+        M.half_angle = p.half_angle;
+        M.g0 = p.cig ? p.g0 : 0.0f;
+        M.dg = p.cig ? p.dg : 1.0f;
+        M.ng = ng;
+        M.oazmin = p.oazmin;
+        M.oazmax = p.oazmax;
+        M.oaztaper = p.oaztaper;
+        M.taper_start = p.angle_taper_start;
+        M.taper_end = p.angle_taper_end;
+        int tile = p.tile;
+        if (tile <= 0) {
+            tile = 2;  // 16 x 8 points
+            if (migrate_lds_bytes(ng, tile) > 64 * 1024) tile = 1;
+        }
+        if (migrate_lds_bytes(ng, tile) > 160 * 1024)
+            return fail(c, KMIG_E_PARAM, "output axis of %d samples does not fit in LDS", ng);
+        if (nsb > 0) {
+            HIPCHK(c, (hipError_t)launch_migrate(M, tile, c->stream));
+            launches++;
+        }
+        hipEvent_t m1 = next_event(c);
+        HIPCHK(c, hipEventRecord(m1, c->stream));
+        c->ev_kind.push_back(2);
+        spec_s_off += (size_t)nsb;
+    }
+    c->st = kmig_stats{};
+    c->st.updates = (double)(se - sb) * (double)c->nh * (double)c->nzx;
+    c->st.migrate_launches = launches;
+    c->st.shot_batches = (int)c->batches.size();
+    int maxr = 0;
+    for (const Batch &b : c->batches) maxr = std::max(maxr, b.nrcv);
+    c->st.receiver_slots = maxr;
+    c->pending = true;
+    c->ran = true;
+    return KMIG_OK;
+}
+
+int kmig_sync(kmig_ctx *c)
+{
+    if (!c) return KMIG_E_PARAM;
+    if (!c->stream) return fail(c, KMIG_E_STATE, "context has no device");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->pending && c->ev_used >= 2) {
+        // events: e0, e1 (trace prep), then per batch a0, a1 (tuples), m1 (migrate)
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, c->events[0], c->events[1]);
+        c->st.ms_trace_prep = ms;
+        // then per batch: a0 .. a1 (tuple prep), a1 .. m1 (migrate)
+        int k = 2;
+        double tup = 0.0, mig = 0.0;
+        for (size_t b = 0; b < c->batches.size() && k + 2 < c->ev_used; b++, k += 3) {
+            float a = 0.0f, m = 0.0f;
+            (void)hipEventElapsedTime(&a, c->events[k], c->events[k + 1]);
+            (void)hipEventElapsedTime(&m, c->events[k + 1], c->events[k + 2]);
+            tup += a;
+            mig += m;
+        }
+        c->st.ms_tuple_prep = tup;
+        c->st.ms_migrate = mig;
+        (void)hipEventElapsedTime(&ms, c->events[0], c->events[c->ev_used - 1]);
+        c->st.ms_total = ms;
+        c->pending = false;
+    }
+    return KMIG_OK;
+}
+
+int kmig_get_cig(kmig_ctx *c, float *cig)
+{
+    if (!c || !cig) return KMIG_E_PARAM;
+    if (!c->ran) return fail(c, KMIG_E_STATE, "kmig_run has not been called");
+    int rc = kmig_sync(c);
+    if (rc) return rc;
+    const int ng = c->p.cig ? c->p.ng : 1;
+    HIPCHK(c, hipMemcpy(cig, c->cig, (size_t)ng * c->nzx * sizeof(float), hipMemcpyDeviceToHost));
+    return KMIG_OK;
+}
+
+int kmig_get_qc(kmig_ctx *c, float *before, float *after)
+{
+    if (!c) return KMIG_E_PARAM;
+    if (!c->qc_valid) return fail(c, KMIG_E_STATE, "no direct-mute QC gather in the last run");
+    int rc = kmig_sync(c);
+    if (rc) return rc;
+    const size_t n = (size_t)c->nt * c->nh * sizeof(float);
+    if (before) HIPCHK(c, hipMemcpy(before, c->qcb, n, hipMemcpyDeviceToHost));
+    if (after) HIPCHK(c, hipMemcpy(after, c->qca, n, hipMemcpyDeviceToHost));
+    return KMIG_OK;
+}
+
+int kmig_get_stats(const kmig_ctx *c, kmig_stats *st)
+{
+    if (!c || !st) return KMIG_E_PARAM;
+    *st = c->st;
+    return KMIG_OK;
+}
+
+int kmig_migrate(kmig_ctx *c, const float *data, int nt, float t0, float dt, int nh, float h0,
+                 float dh, int ns, float s0, float ds, float *cig, float *qc_before,
+                 float *qc_after)
+{
+    int rc = kmig_set_data(c, data, nt, t0, dt, nh, h0, dh, ns, s0, ds);
+    if (rc) return rc;
+    if ((rc = kmig_run(c, 0, ns))) return rc;
+    if ((rc = kmig_sync(c))) return rc;
+    if (cig && (rc = kmig_get_cig(c, cig))) return rc;
+    if ((qc_before || qc_after) && c->qc_valid) rc = kmig_get_qc(c, qc_before, qc_after);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,114 @@
+// Internal declarations shared by the host orchestration (kmig_api.cpp) and
+// the HIP kernels (kmig_kernels.hip).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <string>
+
+#include "kmig.h"
+
+namespace kmig {
+
+void set_err(char *err, int errlen, const char *fmt, ...);
+int collect_kv(std::map<std::string, std::string> &kv, int argc, const char *const *argv,
+               char *err, int errlen);
+int parse_kv(kmig_params *p, const std::map<std::string, std::string> &kv, int nh, float h0,
+             float dh, char *err, int errlen);
+
+// Per-trace record written by the host from the acquisition geometry
+// (src/kirchmig2d_cig.cpp:1055-1105): trace-aperture weight (negative = the
+// trace is skipped), receiver tuple slot, offset-CIG bin, mute geometry.
+struct TraceMeta {
+    float wap;        // trace_ap_weight, or -1 when the trace does not migrate
+    int slot;         // receiver tuple slot of this trace within its batch
+    int ig;           // output bin of an offset CIG / stack (0)
+    float src_x;      // source x (mute)
+    float rcv_x;      // receiver x (mute)
+    int pad[3];
+};
+
+// How one tuple slot is read from its side's table volume
+// (src/kirchmig2d_cig.cpp:987-1028 / :1148-1180, tinterpsr.cpp:37-188).
+struct SlotSpec {
+    int node;         // lower table node (or the clamped node)
+    int interp;       // 0: copy node, 1: interpolate node..node+1
+    int k1, k2;       // 'p' shifts
+    float coord;      // in-interval coordinate (float, as the reference)
+    float t;          // coord / d
+    float surf_x;     // surface position (aperture cone vertex)
+    int gnode;        // gradient-volume lower node (linear, clamped)
+    int ginterp;      // 0: copy, 1: interpolate
+    float gw;         // gradient-volume weight
+    int pad[2];
+    double h[8];      // h00, h10*d, h01, h11*d, d00, d10*d, d01, d11*d
+};
+
+// Structure-of-arrays tuple set: one nz*nx plane per slot and field.
+struct Tuples {
+    float *T = nullptr;   // traveltime at the slot position
+    float *A = nullptr;   // |dT/dpos * spacing| (anti-alias term)
+    float *GX = nullptr;  // smoothed dT/dx
+    float *GZ = nullptr;  // smoothed dT/dz
+    float *W = nullptr;   // aperture leg weight (0 also where |grad| <= 1e-12)
+    int cap = 0;          // allocated slots
+};
+
+struct TuplePrepArgs {
+    const float *tab, *der, *gxv, *gzv;  // side volumes (npos*nzx)
+    const SlotSpec *spec;
+    int nslots;
+    int nz, nx;
+    size_t nzx;
+    float x0, dx, dz, dd;                // image grid, dd = x spacing for 'p'
+    float dpos;                          // table position spacing
+    float dside;                         // ds (source) or dh (receiver)
+    int type;                            // 'h', 'l', 'p'
+    int use_grad_files;
+    int radius, smooth;
+    int angle_mode;
+    float aper, aper_taper, rmin, rtaper;
+};
+
+struct TracePrepArgs {
+    const float *raw;
+    float *prep;
+    const TraceMeta *meta;
+    float *qcb, *qca;
+    int nt, nh;
+    int shot_begin, shot_end;
+    float t0, dt;
+    int mute, invert, qc_shot, diff;
+    float slope, intercept, taper;
+};
+
+struct MigArgs {
+    int nz, nx;
+    size_t nzx;
+    int nt;
+    float t0, dt, inv_dt;
+    int nh, shot0, nshots;
+    const float *sT, *sA, *sGX, *sGZ, *sW;
+    const float *rT, *rA, *rGX, *rGZ, *rW;
+    const TraceMeta *meta;
+    const float *traces;
+    float *cig;
+    float tau, aal;
+    int mode;            // 0 angle/linear bins, 1 angle/nearest bin, 2 offset CIG or stack
+    int half_angle;
+    float g0, dg;
+    int ng;
+    float oazmin, oazmax, oaztaper;
+    float taper_start, taper_end;
+};
+
+// Launchers (kmig_kernels.hip).  Return 0 or a hipError_t value.
+int launch_trace_prep(const TracePrepArgs &a, void *stream);
+int launch_tuple_prep(const TuplePrepArgs &a, Tuples &out, float *tmp1, float *tmp2,
+                      void *stream);
+int launch_migrate(const MigArgs &a, int tile, void *stream);
+int migrate_lds_bytes(int ng, int tile);
+int launch_fill(float *p, size_t n, float v, void *stream);
+
+}  // namespace kmig

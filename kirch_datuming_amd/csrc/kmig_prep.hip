@@ -1,0 +1,329 @@
+// HIP preparation kernels of the kirchmig2d_cig adj=1 path for gfx950.
+//
+//   k_trace_prep   per trace: direct-wave mute (+QC copies), doubint, diff2
+//                  (src/kirchmig2d_cig.cpp:110-129, :381-394, :1066-1109;
+//                   SERECKIRCH/src/kirmig.cpp:8-24)
+//   k_interp       per (slot, point): table interpolation along the position
+//                  axis, |dT/ds * spacing|, gradient-volume interpolation
+//                  (SERECKIRCH/src/tinterpsr.cpp:37-188; :294-322, :1266)
+//   k_grad         per (slot, point): radius-r finite differences (:130-152)
+//   k_box_z/k_box_x per (slot, point): one box pass of the triangular
+//                  smoothing (:438-465)
+//   k_weight       per (slot, point): aperture leg weight (:226-269,
+//                  :1221-1260), zero where |grad| <= 1e-12 (:176)
+//
+// This file is compiled with -ffp-contract=off and keeps the reference's
+// operation order, so its outputs equal the reference's float arithmetic
+// bit for bit except where the device libm (atanf, cosf) rounds differently.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "kmig_internal.h"
+
+namespace kmig {
+
+#define KPI 3.14159265358979323846
+
+// ---------------------------------------------------------------- traces --
+
+__global__ __launch_bounds__(256) void k_trace_prep(const TracePrepArgs a)
+{
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ntr = (long)(a.shot_end - a.shot_begin) * a.nh;
+    if (tid >= ntr) return;
+    const long g = (long)a.shot_begin * a.nh + tid;
+    const int is = (int)(g / a.nh), ih = (int)(g % a.nh);
+    const TraceMeta m = a.meta[g];
+    const float *src = a.raw + g * a.nt;
+    float *dst = a.prep + g * a.nt;
+    const bool qc = a.mute && is == a.qc_shot;
+    const bool keep = m.wap >= 0.0f;
+    const float offset_time = a.slope * fabsf(m.rcv_x - m.src_x);
+    const float boundary = a.invert ? a.intercept - offset_time : a.intercept + offset_time;
+    // mute, QC copies and the causal running sum of doubint
+    float tt = 0.0f;
+    for (int it = 0; it < a.nt; it++) {
+        float v = src[it];
+        if (qc) a.qcb[(size_t)ih * a.nt + it] = v;
+        if (a.mute) {
+            const float time = a.t0 + (float)it * a.dt;
+            float w = 0.0f;
+            if (time >= boundary + a.taper) {
+                w = 1.0f;
+            } else if (time > boundary) {
+                const float u = (time - boundary) / a.taper;
+                w = 0.5f * (1.0f - cosf((float)KPI * u));
+            }
+            v *= w;
+        }
+        if (qc) a.qca[(size_t)ih * a.nt + it] = v;
+        if (keep) {
+            tt = (it == 0) ? v : tt + v;
+            dst[it] = tt;
+        }
+    }
+    if (!keep) return;
+    // anticausal running sum
+    tt = dst[a.nt - 1];
+    for (int it = a.nt - 2; it >= 0; it--) {
+        tt += dst[it];
+        dst[it] = tt;
+    }
+    if (a.diff == 1) {
+        const float d = a.dt;
+        float p = dst[0], c = dst[1];
+        dst[0] = (c - p) / d;
+        for (int it = 1; it < a.nt - 1; it++) {
+            const float nx = dst[it + 1];
+            dst[it] = (nx - p) / (2.0f * d);
+            p = c;
+            c = nx;
+        }
+        dst[a.nt - 1] = (c - p) / d;
+    }
+}
+
+// ---------------------------------------------------------------- tuples --
+
+__global__ __launch_bounds__(256) void k_interp(const TuplePrepArgs a, Tuples o)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int slot = blockIdx.y;
+    if (i >= a.nzx) return;
+    const SlotSpec &s = a.spec[slot];
+    const size_t nzx = a.nzx;
+    const float *p0 = a.tab + (size_t)s.node * nzx;
+    const float *m0 = a.der + (size_t)s.node * nzx;
+    float T, Tx;
+    if (!s.interp) {
+        T = p0[i];
+        Tx = m0[i];
+    } else {
+        const float *p1 = p0 + nzx, *m1 = m0 + nzx;
+        const float dpos = a.dpos;
+        if (a.type == 'l') {
+            const float t = s.t;
+            T = (float)((1.0 - (double)t) * (double)p0[i] + (double)(t * p1[i]));
+            Tx = (-p0[i] + p1[i]) / dpos;
+        } else if (a.type == 'p') {
+            const long n1 = a.nz, nn = (long)nzx, ii = (long)i;
+            const int k1 = s.k1, k2 = s.k2;
+            const float t = s.t;
+            const double dd = (double)a.dd;
+            if (ii - k1 * n1 >= 0 && ii + k2 * n1 < nn) {
+                T = (float)((1.0 - (double)t) * (double)p0[ii - k1 * n1] +
+                            (double)(t * p1[ii + k2 * n1]));
+                float v = (-p0[ii - k1 * n1] + p1[ii + k2 * n1]) / dpos;
+                float dq;
+                double div;
+                if (ii - (k1 + 1) * n1 >= 0 && ii - (k1 - 1) * n1 < nn) {
+                    dq = -p0[ii - (k1 + 1) * n1] + p0[ii - (k1 - 1) * n1];
+                    div = 2.0 * dd;
+                } else if (ii - (k1 + 1) * n1 < 0) {
+                    dq = -p0[ii - k1 * n1] + p0[ii - (k1 - 1) * n1];
+                    div = dd;
+                } else {
+                    dq = -p0[ii - (k1 + 1) * n1] + p0[ii - k1 * n1];
+                    div = dd;
+                }
+                v = (float)((double)v - (1.0 - (double)t) * (double)dq / div);
+                // p1 offsets below the slice start read the previous slice of the
+                // contiguous volume (the reference reads before its buffer here)
+                if (ii - (k2 + 1) * n1 >= 0 && ii - (k2 - 1) * n1 < nn) {
+                    dq = -p1[ii - (k2 + 1) * n1] + p1[ii - (k2 - 1) * n1];
+                    div = 2.0 * dd;
+                } else if (ii - (k2 + 1) * n1 < 0) {
+                    dq = -p1[ii - k2 * n1] + p1[ii - (k2 - 1) * n1];
+                    div = dd;
+                } else {
+                    dq = -p1[ii - (k2 + 1) * n1] + p1[ii - k2 * n1];
+                    div = dd;
+                }
+                v = (float)((double)v - (double)(t * dq) / div);
+                Tx = v;
+            } else {
+                T = (float)((1.0 - (double)t) * (double)p0[i] + (double)(t * p1[i]));
+                Tx = (-p0[i] + p1[i]) / dpos;
+            }
+        } else {
+            const double *h = s.h;
+            T = (float)(h[0] * (double)p0[i] + h[1] * (double)m0[i] + h[2] * (double)p1[i] +
+                        h[3] * (double)m1[i]);
+            Tx = (float)((h[4] * (double)p0[i] + h[5] * (double)m0[i] + h[6] * (double)p1[i] +
+                          h[7] * (double)m1[i]) /
+                         (double)dpos);
+        }
+    }
+    const size_t oi = (size_t)slot * nzx + i;
+    o.T[oi] = T;
+    o.A[oi] = fabsf(Tx * a.dside);
+    if (a.angle_mode && a.use_grad_files) {
+        const float *g0 = a.gxv + (size_t)s.gnode * nzx, *z0 = a.gzv + (size_t)s.gnode * nzx;
+        if (!s.ginterp) {
+            o.GX[oi] = g0[i];
+            o.GZ[oi] = z0[i];
+        } else {
+            const float w = s.gw;
+            o.GX[oi] = (1.0f - w) * g0[i] + w * g0[i + nzx];
+            o.GZ[oi] = (1.0f - w) * z0[i] + w * z0[i + nzx];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_grad(const TuplePrepArgs a, Tuples o)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nzx) return;
+    const int slot = blockIdx.y;
+    const int nz = a.nz, nx = a.nx, r = a.radius < 1 ? 1 : a.radius;
+    const int iz = (int)(i % nz), ix = (int)(i / nz);
+    const float *tab = o.T + (size_t)slot * a.nzx;
+    float vx = 0.0f, vz = 0.0f;
+    if (!(nx <= 1 || a.dx == 0.0f)) {
+        const int il = max(0, ix - r), ir = min(nx - 1, ix + r);
+        if (ir != il)
+            vx = (tab[(size_t)ir * nz + iz] - tab[(size_t)il * nz + iz]) / ((float)(ir - il) * a.dx);
+    }
+    if (!(nz <= 1 || a.dz == 0.0f)) {
+        const int it = max(0, iz - r), ib = min(nz - 1, iz + r);
+        if (ib != it)
+            vz = (tab[(size_t)ix * nz + ib] - tab[(size_t)ix * nz + it]) / ((float)(ib - it) * a.dz);
+    }
+    o.GX[(size_t)slot * a.nzx + i] = vx;
+    o.GZ[(size_t)slot * a.nzx + i] = vz;
+}
+
+__global__ __launch_bounds__(256) void k_box_z(const float *in, float *out, int nz, size_t nzx,
+                                               int r)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nzx) return;
+    const size_t base = (size_t)blockIdx.y * nzx;
+    const int iz = (int)(i % nz);
+    const float *c = in + base + (i - iz);
+    const int lo = (iz - r < 0) ? 0 : iz - r;
+    const int hi = (iz + r > nz - 1) ? nz - 1 : iz + r;
+    float s = 0.0f;
+    for (int k = lo; k <= hi; k++) s += c[k];
+    out[base + i] = s / (float)(hi - lo + 1);
+}
+
+__global__ __launch_bounds__(256) void k_box_x(const float *in, float *out, int nz, int nx,
+                                               size_t nzx, int r)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nzx) return;
+    const size_t base = (size_t)blockIdx.y * nzx;
+    const int iz = (int)(i % nz), ix = (int)(i / nz);
+    const int lo = (ix - r < 0) ? 0 : ix - r;
+    const int hi = (ix + r > nx - 1) ? nx - 1 : ix + r;
+    float s = 0.0f;
+    for (int k = lo; k <= hi; k++) s += in[base + (size_t)k * nz + iz];
+    out[base + i] = s / (float)(hi - lo + 1);
+}
+
+__device__ inline float leg_weight(float leg, float aper, float band)
+{
+    if (!isfinite(leg)) return 0.0f;
+    if (band <= 0.0f) return (leg <= aper) ? 1.0f : 0.0f;
+    const float inner = aper - band;
+    if (leg <= inner) return 1.0f;
+    if (leg >= aper) return 0.0f;
+    const float u = (leg - inner) / band;
+    return 0.5f * (1.0f + cosf((float)KPI * u));
+}
+
+__device__ inline float floor_weight(float d, float rmin, float rt)
+{
+    if (rmin <= 0.0f) return 0.0f;
+    if (rt <= 0.0f) return (d <= rmin) ? 1.0f : 0.0f;
+    const float inner = rmin - rt;
+    if (inner <= 0.0f) {
+        if (d >= rmin) return 0.0f;
+        const float u = d / rmin;
+        return 0.5f * (1.0f + cosf((float)KPI * u));
+    }
+    if (d <= inner) return 1.0f;
+    if (d >= rmin) return 0.0f;
+    const float u = (d - inner) / rt;
+    return 0.5f * (1.0f + cosf((float)KPI * u));
+}
+
+__global__ __launch_bounds__(256) void k_weight(const TuplePrepArgs a, Tuples o)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nzx) return;
+    const int slot = blockIdx.y;
+    const int iz = (int)(i % a.nz), ix = (int)(i / a.nz);
+    const float dzabs = fabsf(a.dz);
+    const float zd0 = fabsf((float)iz * a.dz), zd1 = 1.0e-6f * dzabs;
+    const float zd2 = 1.0e-12f > zd1 ? 1.0e-12f : zd1;
+    const float zdist = zd0 > zd2 ? zd0 : zd2;
+    const float ximg = a.x0 + (float)ix * a.dx;
+    const float d = fabsf(a.spec[slot].surf_x - ximg);
+    const float deg = atanf(d / zdist) * 180.0f / (float)KPI;
+    const float wc = leg_weight(deg, a.aper, a.aper_taper);
+    const float wf = floor_weight(d, a.rmin, a.rtaper);
+    float w = wc > wf ? wc : wf;
+    const size_t oi = (size_t)slot * a.nzx + i;
+    if (a.angle_mode) {
+        const float gx = o.GX[oi], gz = o.GZ[oi];
+        if (sqrtf(gx * gx + gz * gz) <= 1.0e-12f) w = 0.0f;
+    }
+    o.W[oi] = w;
+}
+
+__global__ void k_fill(float *p, size_t n, float v)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// -------------------------------------------------------------- launchers --
+
+static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+int launch_trace_prep(const TracePrepArgs &a, void *stream)
+{
+    const long ntr = (long)(a.shot_end - a.shot_begin) * a.nh;
+    if (ntr <= 0) return 0;
+    hipLaunchKernelGGL(k_trace_prep, dim3(nblk((size_t)ntr, 256)), dim3(256), 0,
+                       (hipStream_t)stream, a);
+    return (int)hipGetLastError();
+}
+
+int launch_tuple_prep(const TuplePrepArgs &a, Tuples &o, float *tmp1, float *tmp2, void *stream)
+{
+    if (a.nslots <= 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 grid(nblk(a.nzx, 256), (unsigned)a.nslots), blk(256);
+    hipLaunchKernelGGL(k_interp, grid, blk, 0, st, a, o);
+    if (a.angle_mode) {
+        if (!a.use_grad_files) hipLaunchKernelGGL(k_grad, grid, blk, 0, st, a, o);
+        if (a.smooth > 0) {
+            float *comp[2] = {o.GX, o.GZ};
+            for (float *f : comp) {
+                for (int pass = 0; pass < 2; pass++) {
+                    // pass 1: f -> tmp1 via tmp2; pass 2: tmp1 -> f via tmp2
+                    const float *src = pass == 0 ? f : tmp1;
+                    float *dst = pass == 0 ? tmp1 : f;
+                    hipLaunchKernelGGL(k_box_z, grid, blk, 0, st, src, tmp2, a.nz, a.nzx, a.smooth);
+                    hipLaunchKernelGGL(k_box_x, grid, blk, 0, st, (const float *)tmp2, dst, a.nz,
+                                       a.nx, a.nzx, a.smooth);
+                }
+            }
+        }
+    }
+    hipLaunchKernelGGL(k_weight, grid, blk, 0, st, a, o);
+    return (int)hipGetLastError();
+}
+
+int launch_fill(float *p, size_t n, float v, void *stream)
+{
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, p, n, v);
+    return (int)hipGetLastError();
+}
+
+}  // namespace kmig
