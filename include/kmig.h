@@ -53,6 +53,15 @@ extern "C" {
 #define KMIG_E_STATE (-5)   /* call made before its inputs were set     */
 #define KMIG_E_NOMEM (-6)   /* device or host allocation failed         */
 
+/* Migrate kernel variants.  STREAM keeps the reference's per-point order
+   (shot-major, receiver-minor) and per-update arithmetic; RESIDENT keeps a
+   chunk of receiver tuples in registers and sweeps the shots under it
+   (receivers that do not move with the shot, cmp=0), with the opening angle
+   from per-side directions and split-sample traveltimes. */
+#define KMIG_VARIANT_AUTO 0
+#define KMIG_VARIANT_STREAM 1
+#define KMIG_VARIANT_RESIDENT 2
+
 /* Table precision of the resident traveltime tuples. */
 #define KMIG_PREC_FP32 0
 #define KMIG_PREC_FP16 1
@@ -82,6 +91,7 @@ typedef struct kmig_params {
     /* MI355X-side options (no counterpart keys in the reference) */
     int precision;             /* KMIG_PREC_*                               */
     int tile;                  /* migrate tile preset, 0 = automatic        */
+    int variant;               /* KMIG_VARIANT_*                            */
 } kmig_params;
 
 typedef struct kmig_stats {
@@ -93,6 +103,7 @@ typedef struct kmig_stats {
     int migrate_launches;      /* migrate kernel launches in the last run   */
     int receiver_slots;        /* resident receiver tuples                  */
     int shot_batches;
+    int variant;               /* KMIG_VARIANT_* actually used              */
 } kmig_stats;
 
 typedef struct kmig_ctx kmig_ctx;

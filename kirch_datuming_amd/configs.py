@@ -1,0 +1,146 @@
+"""The synthetic workloads of the benchmark and the parity tests.
+
+small   v(z) = 1.5 + 0.5 z km/s, nz=201, nx=301, dz=dx=0.01 km; 61 shots every
+        0.05 km at z=0; 201 receivers at 0.015 km (cmp=0, absolute receiver
+        axis); tables by fast marching at the 61 shot and 201 receiver
+        positions; nt=1001, dt=2 ms; 15 Hz Ricker at the two-way times of
+        flat reflectors z = 0.6 / 1.2 / 1.7 km (r = +0.1 / -0.1 / +0.1) plus
+        Gaussian noise at 20 dB SNR; angle axis 0..60 deg, 61 bins.
+salt    five-layer model 1.5 / 2.0 / 2.6 / 4.5 (salt, Gaussian-bump top) /
+        3.2 km/s smoothed with a 5-point Gaussian, nz=400, nx=1000,
+        dz=dx=0.01 km; 768 shots every 0.0125 km; 401 receivers every
+        0.025 km (cmp=0); nt=2001, dt=2 ms; 20 Hz Ricker point-scatterer
+        (Born-style) events plus a direct arrival; direct_mute=1 with slope
+        50 s/km, intercept 0, taper 0.05 s; angle axis 0..60 deg, 61 bins.
+
+Inputs are generated by libkmsynth (own C++: seeded RNG, fast-marching
+tables), so they are identical on every machine.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_SYN = None
+
+
+def synth():
+    global _SYN
+    if _SYN is None:
+        path = os.path.join(HERE, "lib", "libkmsynth.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        L = C.CDLL(path)
+        fp = C.POINTER(C.c_float)
+        f, i = C.c_float, C.c_int
+        L.kms_velocity_vz.argtypes = [fp, i, i, f, f, f]
+        L.kms_velocity_salt.argtypes = [fp, i, i, f, f]
+        L.kms_tables.argtypes = [fp, i, i, f, f, f, i, f, f, i, i, fp, fp]
+        L.kms_time_vz.restype = C.c_double
+        L.kms_time_vz.argtypes = [C.c_double] * 5
+        L.kms_data_reflectors.argtypes = [fp, i, f, f, i, f, f, i, f, f, i, f, f, i, fp, fp, f,
+                                          f, C.c_uint64]
+        L.kms_data_scatterers.argtypes = [fp, i, f, f, i, f, f, i, f, f, i, fp, i, f, f, fp, i,
+                                          f, f, i, i, i, f, f, C.c_uint64]
+        _SYN = L
+    return _SYN
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+@dataclass
+class Workload:
+    name: str
+    kv: dict
+    data: np.ndarray            # (ns, nh, nt)
+    data_axes: list             # [[nt,t0,dt],[nh,h0,dh],[ns,s0,ds]]
+    stab: np.ndarray            # (npos, nx, nz)
+    sder: np.ndarray
+    s_axes: list                # [[nz,z0,dz],[nx,x0,dx],[npos,o,d]]
+    rtab: np.ndarray
+    rder: np.ndarray
+    r_axes: list
+    vel: np.ndarray = field(default=None)
+
+    @property
+    def updates(self) -> float:
+        (nt, _, _), (nh, _, _), (ns, _, _) = self.data_axes
+        nz, nx = self.s_axes[0][0], self.s_axes[1][0]
+        return float(ns) * nh * nz * nx
+
+    def args(self):
+        return (self.data, self.data_axes, self.stab, self.sder, self.s_axes, self.rtab,
+                self.rder, self.r_axes)
+
+    def shot_subset(self, shots):
+        """Same workload restricted to the given shot indices (evenly spaced)."""
+        idx = np.asarray(shots)
+        (nt, t0, dt), hax, (ns, s0, ds) = self.data_axes
+        if len(idx) > 1:
+            step = int(idx[1] - idx[0])
+            assert np.all(np.diff(idx) == step)
+        else:
+            step = 1
+        sub_axes = [[nt, t0, dt], hax, [len(idx), s0 + idx[0] * ds, ds * step]]
+        return Workload(self.name + f"[{len(idx)} shots]", self.kv,
+                        np.ascontiguousarray(self.data[idx]), sub_axes, self.stab, self.sder,
+                        self.s_axes, self.rtab, self.rder, self.r_axes, self.vel)
+
+
+def tables(vel, dz, dx, npos, o, d, box=2, threads=0):
+    nx, nz = vel.shape
+    T = np.empty((npos, nx, nz), np.float32)
+    D = np.empty_like(T)
+    synth().kms_tables(_p(vel), nz, nx, dz, dx, 0.0, npos, o, d, box, threads, _p(T), _p(D))
+    return T, D
+
+
+def small(threads=0, seed=20260101) -> Workload:
+    L = synth()
+    nz, nx, d = 201, 301, 0.01
+    vel = np.empty((nx, nz), np.float32)
+    L.kms_velocity_vz(_p(vel), nz, nx, d, 1.5, 0.5)
+    ns, s0, ds = 61, 0.0, 0.05
+    nh, h0, dh = 201, 0.0, 0.015
+    nt, dt = 1001, 0.002
+    ST, SD = tables(vel, d, d, ns, s0, ds, 2, threads)
+    RT, RD = tables(vel, d, d, nh, h0, dh, 2, threads)
+    data = np.empty((ns, nh, nt), np.float32)
+    zr = np.array([0.6, 1.2, 1.7], np.float32)
+    amp = np.array([0.1, -0.1, 0.1], np.float32)
+    L.kms_data_reflectors(_p(data), nt, 0.0, dt, nh, h0, dh, ns, s0, ds, 0, 1.5, 0.5, 3, _p(zr),
+                          _p(amp), 15.0, 20.0, seed)
+    kv = dict(cmp=0, mode="angle", angle_min=0, angle_max=60, angle_step=1, angle_n=61)
+    tax = [[nz, 0.0, d], [nx, 0.0, d]]
+    return Workload("small v(z) 201x301, 61 shots x 201 receivers, nt=1001", kv, data,
+                    [[nt, 0.0, dt], [nh, h0, dh], [ns, s0, ds]], ST, SD, tax + [[ns, s0, ds]],
+                    RT, RD, tax + [[nh, h0, dh]], vel)
+
+
+def salt(threads=0, seed=20260202, nshots=768) -> Workload:
+    L = synth()
+    nz, nx, d = 400, 1000, 0.01
+    vel = np.empty((nx, nz), np.float32)
+    L.kms_velocity_salt(_p(vel), nz, nx, d, d)
+    ns, s0, ds = nshots, 0.0, 0.0125
+    nh, h0, dh = 401, 0.0, 0.025
+    nt, dt = 2001, 0.002
+    ST, SD = tables(vel, d, d, ns, s0, ds, 2, threads)
+    RT, RD = tables(vel, d, d, nh, h0, dh, 2, threads)
+    data = np.empty((ns, nh, nt), np.float32)
+    L.kms_data_scatterers(_p(data), nt, 0.0, dt, nh, h0, dh, ns, s0, ds, 0, _p(ST), ns, s0, ds,
+                          _p(RT), nh, h0, dh, nz, nx, 60, 20.0, 1.5, seed)
+    kv = dict(cmp=0, mode="angle", angle_min=0, angle_max=60, angle_step=1, angle_n=61,
+              direct_mute=1, direct_mute_slope=50, direct_mute_intercept=0,
+              direct_mute_taper=0.05)
+    tax = [[nz, 0.0, d], [nx, 0.0, d]]
+    return Workload("salt 400x1000, 768 shots x 401 receivers, nt=2001", kv, data,
+                    [[nt, 0.0, dt], [nh, h0, dh], [ns, s0, ds]], ST, SD, tax + [[ns, s0, ds]],
+                    RT, RD, tax + [[nh, h0, dh]], vel)

@@ -68,6 +68,17 @@ struct kmig_ctx {
     size_t qc_cap = 0;
     bool qc_valid = false;
     std::vector<Batch> batches;
+    // resident-variant state
+    PackedTuple *pk_s = nullptr, *pk_r = nullptr;
+    size_t pk_s_cap = 0, pk_r_cap = 0;
+    float *pairs = nullptr;
+    size_t pairs_cap = 0;
+    float *wap = nullptr;
+    size_t wap_cap = 0;
+    int *rslot = nullptr;
+    size_t rslot_cap = 0;
+    bool rcv_invariant = false;     // every shot uses receiver slot rslot[ih]
+    int variant_used = 0;
     std::vector<hipEvent_t> events;
     std::vector<int> ev_kind;       // per event pair: 0 trace, 1 tuple, 2 migrate
     int ev_used = 0;
@@ -224,7 +235,8 @@ size_t free_device_bytes()
 
 // Build TraceMeta for shots [sb, se), the batches and their slot specs.
 int plan_run(kmig_ctx *c, int sb, int se, std::vector<TraceMeta> &meta,
-             std::vector<SlotSpec> &sspec, std::vector<SlotSpec> &rspec)
+             std::vector<SlotSpec> &sspec, std::vector<SlotSpec> &rspec,
+             std::vector<float> &wapv, std::vector<int> &rslot)
 {
     const kmig_params &p = c->p;
     const Side &S = c->side[0], &Rt = c->side[1];
@@ -246,6 +258,9 @@ int plan_run(kmig_ctx *c, int sb, int se, std::vector<TraceMeta> &meta,
     }
 
     meta.assign((size_t)(se - sb) * nh, TraceMeta{});
+    wapv.assign((size_t)(se - sb) * nh, -1.0f);
+    rslot.assign((size_t)nh, -1);
+    bool invariant = true;
     sspec.clear();
     rspec.clear();
     c->batches.clear();
@@ -303,6 +318,7 @@ int plan_run(kmig_ctx *c, int sb, int se, std::vector<TraceMeta> &meta,
                 else m.ig = (int)ig;
             }
             m.wap = skip ? -1.0f : wap;
+            wapv[(size_t)(is - sb) * nh + ih] = m.wap;
             if (!skip) need.push_back({ih, receiver_pos});
         }
         // new receiver positions of this shot
@@ -329,10 +345,14 @@ int plan_run(kmig_ctx *c, int sb, int se, std::vector<TraceMeta> &meta,
                 slot = it->second;
             }
             meta[(size_t)(is - sb) * nh + q.first].slot = slot;
+            int &rs = rslot[(size_t)q.first];
+            if (rs < 0) rs = slot;
+            else if (rs != slot) invariant = false;
         }
         sspec.push_back(ss);
     }
     close(se);
+    c->rcv_invariant = invariant && c->batches.size() == 1;
     return KMIG_OK;
 }
 
@@ -418,6 +438,11 @@ void kmig_destroy(kmig_ctx *c)
     dfree(c->cig);
     dfree(c->qcb);
     dfree(c->qca);
+    dfree(c->pk_s);
+    dfree(c->pk_r);
+    dfree(c->pairs);
+    dfree(c->wap);
+    dfree(c->rslot);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -545,7 +570,9 @@ int kmig_run(kmig_ctx *c, int sb, int se)
     if (c->plan_version != c->geo_version || sb != c->last_sb || se != c->last_se) {
         std::vector<TraceMeta> meta;
         std::vector<SlotSpec> sspec, rspec;
-        if ((rc = plan_run(c, sb, se, meta, sspec, rspec))) return rc;
+        std::vector<float> wapv;
+        std::vector<int> rslot;
+        if ((rc = plan_run(c, sb, se, meta, sspec, rspec, wapv, rslot))) return rc;
         if ((rc = check_tables_type_p(c, sspec))) return rc;
         if ((rc = check_tables_type_p(c, rspec))) return rc;
         // meta is indexed by global trace number
@@ -575,10 +602,42 @@ int kmig_run(kmig_ctx *c, int sb, int se)
             if ((rc = ensure(c, &c->tmp1, &c->tmp1_cap, tslots * c->nzx))) return rc;
             if ((rc = ensure(c, &c->tmp2, &c->tmp2_cap, tslots * c->nzx))) return rc;
         }
+        // resident variant: angle CIG, two-bin interpolation, receivers that do
+        // not move with the shot, one batch, times within the int16 sample range
+        const double cst = ((double)p.tau - (double)c->t0) / (double)c->dt;
+        const bool res_ok = angle && p.angle_interp == 1 && c->rcv_invariant &&
+                            std::fabs(cst) + (double)c->nt < 30000.0;
+        int variant = KMIG_VARIANT_STREAM;
+        if (p.variant == KMIG_VARIANT_RESIDENT) {
+            if (!res_ok)
+                return fail(c, KMIG_E_PARAM,
+                            "variant=resident needs mode=angle, angle_interp=1, receivers that "
+                            "do not move with the shot (cmp=0) and tuples that fit one batch");
+            variant = KMIG_VARIANT_RESIDENT;
+        } else if (p.variant == KMIG_VARIANT_AUTO && res_ok) {
+            variant = KMIG_VARIANT_RESIDENT;
+        }
+        c->variant_used = variant;
+        if (variant == KMIG_VARIANT_RESIDENT) {
+            if ((rc = ensure(c, &c->pk_s, &c->pk_s_cap, std::max<size_t>(1, maxs) * c->nzx)))
+                return rc;
+            if ((rc = ensure(c, &c->pk_r, &c->pk_r_cap, std::max<size_t>(1, maxr) * c->nzx)))
+                return rc;
+            if ((rc = ensure(c, &c->pairs, &c->pairs_cap, (size_t)c->ns * c->nh * c->nt * 2)))
+                return rc;
+            if ((rc = ensure(c, &c->wap, &c->wap_cap, (size_t)c->ns * c->nh))) return rc;
+            if ((rc = ensure(c, &c->rslot, &c->rslot_cap, (size_t)c->nh))) return rc;
+            if (!wapv.empty())
+                HIPCHK(c, hipMemcpy(c->wap + (size_t)sb * c->nh, wapv.data(),
+                                    wapv.size() * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(c->rslot, rslot.data(), rslot.size() * sizeof(int),
+                                hipMemcpyHostToDevice));
+        }
         c->plan_version = c->geo_version;
         c->last_sb = sb;
         c->last_se = se;
     }
+    const bool resident = c->variant_used == KMIG_VARIANT_RESIDENT;
     if ((rc = ensure(c, &c->cig, &c->cig_cap, (size_t)ng * c->nzx))) return rc;
     const bool want_qc = p.direct_mute && p.direct_mute_qc_shot >= sb && p.direct_mute_qc_shot < se;
     if (want_qc) {
@@ -612,6 +671,10 @@ int kmig_run(kmig_ctx *c, int sb, int se)
     ta.intercept = p.direct_mute_intercept;
     ta.taper = p.direct_mute_taper;
     HIPCHK(c, (hipError_t)launch_trace_prep(ta, c->stream));
+    if (resident)
+        HIPCHK(c, (hipError_t)launch_pairs(c->prep + (size_t)sb * c->nh * c->nt,
+                                           c->pairs + (size_t)sb * c->nh * c->nt * 2,
+                                           (long)(se - sb) * c->nh, c->nt, c->stream));
     hipEvent_t e1 = next_event(c);
     HIPCHK(c, hipEventRecord(e1, c->stream));
     c->ev_kind.push_back(0);
@@ -661,6 +724,12 @@ int kmig_run(kmig_ctx *c, int sb, int se)
         A.dpos = c->side[1].d;
         A.dside = dh;
         HIPCHK(c, (hipError_t)launch_tuple_prep(A, c->tr, c->tmp1, c->tmp2, c->stream));
+        if (resident) {
+            HIPCHK(c, (hipError_t)launch_pack(c->ts, c->pk_s, nsb, c->nzx, c->dt, angle,
+                                              c->stream));
+            HIPCHK(c, (hipError_t)launch_pack(c->tr, c->pk_r, b.nrcv, c->nzx, c->dt, angle,
+                                              c->stream));
+        }
         hipEvent_t a1 = next_event(c);
         HIPCHK(c, hipEventRecord(a1, c->stream));
         c->ev_kind.push_back(1);
@@ -708,7 +777,38 @@ int kmig_run(kmig_ctx *c, int sb, int se)
         }
         if (migrate_lds_bytes(ng, tile) > 160 * 1024)
             return fail(c, KMIG_E_PARAM, "output axis of %d samples does not fit in LDS", ng);
-        if (nsb > 0) {
+        if (nsb > 0 && resident) {
+            MigResArgs R{};
+            R.nz = c->nz;
+            R.nx = c->nx;
+            R.nzx = c->nzx;
+            R.nt = c->nt;
+            R.nh = c->nh;
+            R.shot0 = b.s0;
+            R.nshots = nsb;
+            R.stup = c->pk_s;
+            R.rtup = c->pk_r;
+            R.rslot = c->rslot;
+            R.wap = c->wap;
+            R.pairs = c->pairs;
+            R.cig = c->cig;
+            const double cst = ((double)p.tau - (double)c->t0) / (double)c->dt;
+            const double ci = std::floor(cst);
+            R.c_int = (int)ci;
+            R.c_frac = (float)(cst - ci);
+            R.aal = p.antialias;
+            R.half_angle = p.half_angle;
+            R.g0 = p.g0;
+            R.inv_dg = 1.0f / p.dg;
+            R.ng = ng;
+            R.oazmin = p.oazmin;
+            R.oazmax = p.oazmax;
+            R.oaztaper = p.oaztaper;
+            R.taper_start = p.angle_taper_start;
+            R.taper_end = p.angle_taper_end;
+            HIPCHK(c, (hipError_t)launch_migrate_resident(R, tile, c->stream));
+            launches++;
+        } else if (nsb > 0) {
             HIPCHK(c, (hipError_t)launch_migrate(M, tile, c->stream));
             launches++;
         }
@@ -721,6 +821,7 @@ int kmig_run(kmig_ctx *c, int sb, int se)
     c->st.updates = (double)(se - sb) * (double)c->nh * (double)c->nzx;
     c->st.migrate_launches = launches;
     c->st.shot_batches = (int)c->batches.size();
+    c->st.variant = c->variant_used;
     int maxr = 0;
     for (const Batch &b : c->batches) maxr = std::max(maxr, b.nrcv);
     c->st.receiver_slots = maxr;

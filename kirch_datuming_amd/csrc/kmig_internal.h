@@ -103,12 +103,47 @@ struct MigArgs {
     float taper_start, taper_end;
 };
 
-// Launchers (kmig_kernels.hip).  Return 0 or a hipError_t value.
+// Packed 16-byte tuple of the resident migrate kernel, one per (slot, point):
+//   frac  fractional part of T/dt (samples, in [0,1])
+//   a     |dT/dpos * spacing| / dt (anti-alias term in samples)
+//   th    direction of the smoothed traveltime gradient, atan2(gx, gz)
+//   iw    bits 31..16: integer part of T/dt (int16), bits 15..0: the
+//         aperture weight W * 32768 (W in [0,1], 0 where |grad| <= 1e-12)
+struct PackedTuple {
+    float frac, a, th;
+    uint32_t iw;
+};
+
+struct MigResArgs {
+    int nz, nx;
+    size_t nzx;
+    int nt, nh, shot0, nshots;
+    const PackedTuple *stup;     // [nshots][nzx] (batch-local shots)
+    const PackedTuple *rtup;     // [slots][nzx]
+    const int *rslot;            // [nh] receiver slot of each receiver index (-1: unused)
+    const float *wap;            // [ns][nh] trace-aperture weight, < 0: trace skipped
+    const float *pairs;          // [ns][nh][nt][2] (tr[k], tr[k+1])
+    float *cig;
+    int c_int;                   // integer part of (tau - t0)/dt
+    float c_frac;                // fractional part of (tau - t0)/dt
+    float aal;
+    int half_angle;
+    float g0, inv_dg;
+    int ng;
+    float oazmin, oazmax, oaztaper;
+    float taper_start, taper_end;
+};
+
+// Launchers (kmig_prep.hip, kmig_migrate.hip).  Return 0 or a hipError_t value.
 int launch_trace_prep(const TracePrepArgs &a, void *stream);
 int launch_tuple_prep(const TuplePrepArgs &a, Tuples &out, float *tmp1, float *tmp2,
                       void *stream);
 int launch_migrate(const MigArgs &a, int tile, void *stream);
 int migrate_lds_bytes(int ng, int tile);
 int launch_fill(float *p, size_t n, float v, void *stream);
+int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, float dt,
+                int angle_mode, void *stream);
+int launch_pairs(const float *prep, float *pairs, long ntraces, int nt, void *stream);
+int launch_migrate_resident(const MigResArgs &a, int tile, void *stream);
 
 }  // namespace kmig

@@ -158,6 +158,155 @@ __global__ __launch_bounds__(TZ *TX) void k_migrate(const MigArgs a)
         for (int g = 0; g < a.ng; g++) a.cig[(size_t)g * a.nzx + i] = my[g * P];
 }
 
+// ------------------------------------------------------ resident variant --
+//
+// For receivers that do not move with the shot (cmp=0), each thread keeps R
+// receiver tuples of its image point in registers and sweeps every shot under
+// them: per shot it loads one source tuple and runs R independent updates.
+// Per point the accumulation order is (receiver chunk, shot, receiver).
+// The opening angle is |th_s - th_r| folded into [0, pi] (the angle between
+// the two gradient directions, the quantity atan2(|s x r|, s . r) measures),
+// and traveltimes arrive split into integer and fractional samples.
+
+struct Bins {
+    int ig0, ig1;
+    float w0, w1;
+};
+
+template <class Acc>
+__device__ __forceinline__ void res_update_s(const Acc &my, const PackedTuple &s,
+                                           const PackedTuple &r, float wap,
+                                           const float2 *__restrict__ tr, const MigResArgs &a,
+                                           float cutoff, bool taper_on)
+{
+    const float inv32k = 1.0f / 32768.0f;
+    const float ws = (float)(s.iw & 0xFFFFu) * inv32k;
+    const float wr = (float)(r.iw & 0xFFFFu) * inv32k;
+    const float apw = ws * wr;
+    if (!(apw > 0.0f)) return;
+    const float PI_F = 3.14159265358979f;
+    float d = fabsf(s.th - r.th);
+    d = d > PI_F ? 2.0f * PI_F - d : d;
+    float angle = d * (180.0f / PI_F);
+    if (a.half_angle) angle *= 0.5f;
+    float wt = 1.0f;
+    if (taper_on) {
+        if (angle >= a.taper_end) return;
+        if (angle > a.taper_start)
+            wt = 0.5f * (1.0f + cosf(PI_F * ((angle - a.taper_start) /
+                                             (a.taper_end - a.taper_start))));
+    }
+    float wd;
+    if (a.oaztaper <= 0.0f) {
+        wd = (angle <= cutoff) ? 1.0f : 0.0f;
+    } else if (angle <= cutoff) {
+        wd = 1.0f;
+    } else if (angle >= cutoff + a.oaztaper) {
+        wd = 0.0f;
+    } else {
+        wd = 0.5f * (1.0f + cosf(PI_F * ((angle - cutoff) / a.oaztaper)));
+    }
+    const float aw = wt * wd * wap * apw;
+    if (!(aw > 0.0f)) return;
+    const float u = (angle - a.g0) * a.inv_dg;
+    if (u < -1.0e-5f || u > (float)(a.ng - 1) + 1.0e-5f) return;
+    int ig0, ig1;
+    float w0, w1;
+    if (u <= 0.0f) {
+        ig0 = ig1 = 0;
+        w0 = 1.0f;
+        w1 = 0.0f;
+    } else if (u >= (float)(a.ng - 1)) {
+        ig0 = ig1 = a.ng - 1;
+        w0 = 1.0f;
+        w1 = 0.0f;
+    } else {
+        const float fu = floorf(u);
+        ig0 = (int)fu;
+        ig1 = ig0 + 1;
+        w1 = u - fu;
+        w0 = 1.0f - w1;
+    }
+    // kirmig_pick(adj=1) in split-sample time
+    const int ii = ((int)s.iw >> 16) + ((int)r.iw >> 16) + a.c_int;
+    const float ff = s.frac + r.frac + a.c_frac;
+    const float fl = floorf(ff);
+    const int it = ii + (int)fl;
+    const float ft = ff - fl;
+    const unsigned lim = (unsigned)(a.nt - 1);
+    if ((unsigned)it >= lim) return;
+    const float dA = fmaxf(s.a, r.a) * a.aal;
+    const float fm = ft - dA - 1.0f, fp = ft + dA + 1.0f;
+    const float flm = floorf(fm), flp = floorf(fp);
+    const int itm = it + (int)flm, itp = it + (int)flp;
+    if ((unsigned)itm >= lim || (unsigned)itp >= lim) return;
+    const float ftm = fm - flm, ftp = fp - flp;
+    const float rr = __builtin_amdgcn_rcpf(3.0f + 2.0f * dA);
+    const float2 q0 = tr[it], qm = tr[itm], qp = tr[itp];
+    const float sm = 2.0f * ((1.0f - ft) * q0.x + ft * q0.y) - ((1.0f - ftm) * qm.x + ftm * qm.y) -
+                     ((1.0f - ftp) * qp.x + ftp * qp.y);
+    const float picked = aw * (rr * rr) * sm;
+    my[ig0] += w0 * picked;
+    if (ig1 != ig0 && w1 > 0.0f) my[ig1] += w1 * picked;
+}
+
+template <int TZ, int TX, int R>
+__global__ __launch_bounds__(TZ *TX) void k_migrate_res(const MigResArgs a)
+{
+    constexpr int P = TZ * TX;
+    extern __shared__ float acc[];
+    const int lid = threadIdx.x;
+    const int ntz = (a.nz + TZ - 1) / TZ;
+    const int bz = blockIdx.x % ntz, bx = blockIdx.x / ntz;
+    const int iz = bz * TZ + lid % TZ, ix = bx * TX + lid / TZ;
+    const bool valid = iz < a.nz && ix < a.nx;
+    const size_t i = valid ? (size_t)ix * a.nz + iz : 0;
+    for (int g = 0; g < a.ng; g++) acc[g * P + lid] = valid ? a.cig[(size_t)g * a.nzx + i] : 0.0f;
+    // bins of this thread live at acc[g * P + lid]; index them through a
+    // strided view so res_update can address bin g as my[g]
+    struct Strided {
+        float *base;
+        __device__ float &operator[](int g) const { return base[g * P]; }
+    };
+    const float zf = (a.nz > 1) ? (float)iz / (float)(a.nz - 1) : 0.0f;
+    const float cutoff = a.oazmin + zf * (a.oazmax - a.oazmin);
+    const bool taper_on = !(a.taper_start < 0.0f || a.taper_end <= a.taper_start);
+    const float2 *__restrict__ pairs = (const float2 *)a.pairs;
+
+    for (int r0 = 0; r0 < a.nh; r0 += R) {
+        PackedTuple rt[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int ih = r0 + r;
+            const int slot = ih < a.nh ? a.rslot[ih] : -1;
+            if (slot >= 0) {
+                rt[r] = a.rtup[(size_t)slot * a.nzx + i];
+            } else {
+                rt[r].frac = rt[r].a = rt[r].th = 0.0f;
+                rt[r].iw = 0u;
+            }
+        }
+        for (int js = 0; js < a.nshots; js++) {
+            const PackedTuple st = a.stup[(size_t)js * a.nzx + i];
+            const bool live = valid && (st.iw & 0xFFFFu) != 0u;
+            if (__ballot(live) == 0) continue;
+            const int is = a.shot0 + js;
+            const float *__restrict__ wrow = a.wap + (size_t)is * a.nh + r0;
+            const float2 *__restrict__ trow = pairs + ((size_t)is * a.nh + r0) * a.nt;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if (r0 + r >= a.nh) break;
+                const float wap = wrow[r];
+                if (wap < 0.0f) continue;
+                Strided my{acc + lid};
+                res_update_s(my, st, rt[r], wap, trow + (size_t)r * a.nt, a, cutoff, taper_on);
+            }
+        }
+    }
+    if (valid)
+        for (int g = 0; g < a.ng; g++) a.cig[(size_t)g * a.nzx + i] = acc[g * P + lid];
+}
+
 static void tile_dims(int tile, int *tz, int *tx)
 {
     switch (tile) {
@@ -190,6 +339,33 @@ static int launch_mig_t(const MigArgs &a, hipStream_t st)
     hipLaunchKernelGGL((k_migrate<TZ, TX>), dim3((unsigned)(ntz * ntx)), dim3(TZ * TX), lds, st,
                        a);
     return (int)hipGetLastError();
+}
+
+template <int TZ, int TX, int R>
+static int launch_res_t(const MigResArgs &a, hipStream_t st)
+{
+    const int ntz = (a.nz + TZ - 1) / TZ, ntx = (a.nx + TX - 1) / TX;
+    const size_t lds = (size_t)a.ng * TZ * TX * sizeof(float);
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_res<TZ, TX, R>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)lds);
+        if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL((k_migrate_res<TZ, TX, R>), dim3((unsigned)(ntz * ntx)), dim3(TZ * TX), lds,
+                       st, a);
+    return (int)hipGetLastError();
+}
+
+int launch_migrate_resident(const MigResArgs &a, int tile, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    switch (tile) {
+    case 1: return launch_res_t<8, 8, 8>(a, st);
+    case 3: return launch_res_t<16, 16, 8>(a, st);
+    case 4: return launch_res_t<32, 4, 8>(a, st);
+    default: return launch_res_t<16, 8, 8>(a, st);
+    }
 }
 
 int launch_migrate(const MigArgs &a, int tile, void *stream)

@@ -200,6 +200,17 @@ int parse_kv(kmig_params *p, const KV &kv, int nh, float h0, float dh, char *err
         }
     }
     if (has(kv, "tile")) p->tile = geti(kv, "tile");
+    if (has(kv, "variant")) {
+        const std::string &t = kv.at("variant");
+        if (ieq(t, "auto")) p->variant = KMIG_VARIANT_AUTO;
+        else if (ieq(t, "stream")) p->variant = KMIG_VARIANT_STREAM;
+        else if (ieq(t, "resident")) p->variant = KMIG_VARIANT_RESIDENT;
+        else {
+            set_err(err, errlen, "Unknown variant: %s. Supported are auto, stream, resident.",
+                    t.c_str());
+            return KMIG_E_PARAM;
+        }
+    }
     if (nh == 1) dh = 0.0f; /* :700 */
     if (p->cig) {
         int rc = output_axis(kv, p->cig_is_angle, nh, h0, dh, &p->g0, &p->dg, &p->ng,
@@ -252,6 +263,7 @@ extern "C" void kmig_params_default(kmig_params *p)
     p->ng = 1;
     p->precision = KMIG_PREC_FP32;
     p->tile = 0;
+    p->variant = KMIG_VARIANT_AUTO;
 }
 
 extern "C" int kmig_params_parse(kmig_params *p, int argc, const char *const *argv, int nh,
@@ -317,6 +329,10 @@ extern "C" int kmig_params_check(const kmig_params *p, int ns, char *err, int er
             set_err(err, errlen, "CIG axis sample number must be positive.");
             return KMIG_E_PARAM;
         }
+    }
+    if (p->variant < KMIG_VARIANT_AUTO || p->variant > KMIG_VARIANT_RESIDENT) {
+        set_err(err, errlen, "Unknown kernel variant %d.", p->variant);
+        return KMIG_E_PARAM;
     }
     if (p->precision < KMIG_PREC_FP32 || p->precision > KMIG_PREC_BF16) {
         set_err(err, errlen, "Unknown precision mode %d.", p->precision);

@@ -274,6 +274,39 @@ __global__ __launch_bounds__(256) void k_weight(const TuplePrepArgs a, Tuples o)
     o.W[oi] = w;
 }
 
+// Packed tuple of the resident migrate kernel (layout in kmig_internal.h).
+// T/dt is split into an integer and a fractional part in double, so the
+// per-update sum of source and receiver times keeps ~1e-7 sample precision.
+__global__ __launch_bounds__(256) void k_pack(const float *T, const float *A, const float *GX,
+                                              const float *GZ, const float *W, PackedTuple *out,
+                                              size_t n, double inv_dt, int angle_mode)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double tt = (double)T[k] * inv_dt;
+    double ip = floor(tt);
+    if (ip > 32767.0) ip = 32767.0;
+    if (ip < -32768.0) ip = -32768.0;
+    PackedTuple q;
+    q.frac = (float)(tt - ip);
+    q.a = (float)((double)A[k] * inv_dt);
+    q.th = angle_mode ? atan2f(GX[k], GZ[k]) : 0.0f;
+    float w = W[k];
+    w = w < 0.0f ? 0.0f : (w > 1.0f ? 1.0f : w);
+    const uint32_t w16 = (uint32_t)__float2uint_rn(w * 32768.0f);
+    q.iw = ((uint32_t)(uint16_t)(int16_t)(int)ip << 16) | w16;
+    out[k] = q;
+}
+
+// (tr[k], tr[k+1]) pairs, so a kirmig_pick tap pair is one 8-byte load.
+__global__ __launch_bounds__(256) void k_pairs(const float *prep, float2 *pairs, size_t n, int nt)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int it = (int)(k % nt);
+    pairs[k] = make_float2(prep[k], it + 1 < nt ? prep[k + 1] : 0.0f);
+}
+
 __global__ void k_fill(float *p, size_t n, float v)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -316,6 +349,25 @@ int launch_tuple_prep(const TuplePrepArgs &a, Tuples &o, float *tmp1, float *tmp
         }
     }
     hipLaunchKernelGGL(k_weight, grid, blk, 0, st, a, o);
+    return (int)hipGetLastError();
+}
+
+int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, float dt,
+                int angle_mode, void *stream)
+{
+    const size_t n = (size_t)nslots * nzx;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_pack, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, in.T, in.A,
+                       in.GX, in.GZ, in.W, out, n, 1.0 / (double)dt, angle_mode);
+    return (int)hipGetLastError();
+}
+
+int launch_pairs(const float *prep, float *pairs, long ntraces, int nt, void *stream)
+{
+    const size_t n = (size_t)ntraces * nt;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_pairs, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, prep,
+                       (float2 *)pairs, n, nt);
     return (int)hipGetLastError();
 }
 

@@ -1,0 +1,388 @@
+// libkmsynth: synthetic inputs for the adj=1 imaging path (velocity models,
+// traveltime tables, seismic data), all seeded and deterministic.
+//
+// Traveltime tables: first-arrival times from a fast-marching eikonal solve
+// (second-order upwind, binary heap, double precision) per table position, with
+// a straight-ray box of radius `box` cells around the source like eikods2d's
+// b1=b2 box (src/eikods2d.cpp:251-281).  dT/ds at each position node is the
+// centred difference of neighbouring position tables (one-sided at the ends),
+// the derivative the Hermite interpolation along the position axis consumes.
+//
+// Data: 15-20 Hz Ricker wavelets at analytic two-way times (flat reflectors
+// in v(z) = v0 + g z, or point scatterers with table times), plus seeded
+// Gaussian noise.  The RNG is splitmix64 (Steele, Lea, Flood 2014) with a
+// Box-Muller transform, so every machine builds the same inputs.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed) {}
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
+    double normal()
+    {
+        double u1 = uniform(), u2 = uniform();
+        if (u1 < 1e-300) u1 = 1e-300;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+    }
+};
+
+double ricker(double t, double f)
+{
+    const double a = (M_PI * f * t) * (M_PI * f * t);
+    return (1.0 - 2.0 * a) * std::exp(-a);
+}
+
+// Indexed binary min-heap over grid nodes.
+struct Heap {
+    std::vector<int> h;      // node ids
+    std::vector<int> where;  // position in h, -1 if absent
+    const double *key;
+    explicit Heap(size_t n, const double *k) : where(n, -1), key(k) { h.reserve(4096); }
+    void swap_(int a, int b)
+    {
+        std::swap(h[a], h[b]);
+        where[h[a]] = a;
+        where[h[b]] = b;
+    }
+    void up(int i)
+    {
+        while (i > 0) {
+            int p = (i - 1) / 2;
+            if (key[h[p]] <= key[h[i]]) break;
+            swap_(p, i);
+            i = p;
+        }
+    }
+    void down(int i)
+    {
+        const int n = (int)h.size();
+        for (;;) {
+            int l = 2 * i + 1, r = l + 1, m = i;
+            if (l < n && key[h[l]] < key[h[m]]) m = l;
+            if (r < n && key[h[r]] < key[h[m]]) m = r;
+            if (m == i) break;
+            swap_(m, i);
+            i = m;
+        }
+    }
+    void push_or_decrease(int node)
+    {
+        if (where[node] < 0) {
+            h.push_back(node);
+            where[node] = (int)h.size() - 1;
+            up((int)h.size() - 1);
+        } else {
+            up(where[node]);
+        }
+    }
+    int pop()
+    {
+        int top = h[0];
+        where[top] = -1;
+        h[0] = h.back();
+        h.pop_back();
+        if (!h.empty()) {
+            where[h[0]] = 0;
+            down(0);
+        }
+        return top;
+    }
+    bool empty() const { return h.empty(); }
+};
+
+// First-arrival times on an (nx, nz) grid (z fastest) for a point source.
+void fmm(const double *slow, int nz, int nx, double dz, double dx, double zs, double xs, int box,
+         double *T, std::vector<uint8_t> &state, bool order2)
+{
+    const size_t n = (size_t)nz * nx;
+    const double INF = std::numeric_limits<double>::infinity();
+    std::fill(T, T + n, INF);
+    state.assign(n, 0);  // 0 far, 1 trial, 2 known
+    Heap heap(n, T);
+    const int izs = (int)std::floor(zs / dz + 0.5), ixs = (int)std::floor(xs / dx + 0.5);
+    const int izc = std::min(std::max(izs, 0), nz - 1), ixc = std::min(std::max(ixs, 0), nx - 1);
+    const double s0 = slow[(size_t)ixc * nz + izc];
+    for (int ix = std::max(0, ixc - box); ix <= std::min(nx - 1, ixc + box); ix++)
+        for (int iz = std::max(0, izc - box); iz <= std::min(nz - 1, izc + box); iz++) {
+            const size_t k = (size_t)ix * nz + iz;
+            T[k] = s0 * std::hypot(iz * dz - zs, ix * dx - xs);
+            state[k] = 1;
+            heap.push_or_decrease((int)k);
+        }
+    const int dzi[4] = {-1, 1, 0, 0}, dxi[4] = {0, 0, -1, 1};
+    while (!heap.empty()) {
+        const int k = heap.pop();
+        state[k] = 2;
+        const int iz = k % nz, ix = k / nz;
+        for (int q = 0; q < 4; q++) {
+            const int jz = iz + dzi[q], jx = ix + dxi[q];
+            if (jz < 0 || jz >= nz || jx < 0 || jx >= nx) continue;
+            const size_t j = (size_t)jx * nz + jz;
+            if (state[j] == 2) continue;
+            // upwind neighbours along each axis: second-order one-sided
+            // stencil (3T - 4T1 + T2) / 2h where the next node is known and
+            // earlier, first order otherwise (Sethian 1999, sec. 8.3)
+            double qa[2], qb[2];
+            int nq = 0;
+            for (int ax = 0; ax < 2; ax++) {
+                const int step = ax == 0 ? 1 : nz;
+                const int pos = ax == 0 ? jz : jx, lim = ax == 0 ? nz : nx;
+                const double hh = ax == 0 ? dz : dx;
+                double best = INF, best2 = INF;
+                for (int sg = -1; sg <= 1; sg += 2) {
+                    const int p1 = pos + sg, p2 = pos + 2 * sg;
+                    if (p1 < 0 || p1 >= lim) continue;
+                    const size_t j1 = (size_t)((long)j + (long)sg * step);
+                    if (state[j1] != 2 || T[j1] >= best) continue;
+                    best = T[j1];
+                    best2 = INF;
+                    if (p2 >= 0 && p2 < lim) {
+                        const size_t j2 = (size_t)((long)j + 2L * sg * step);
+                        if (state[j2] == 2 && T[j2] <= T[j1]) best2 = T[j2];
+                    }
+                }
+                if (best == INF) continue;
+                if (order2 && best2 < INF) {
+                    qa[nq] = 9.0 / (4.0 * hh * hh);
+                    qb[nq] = (4.0 * best - best2) / 3.0;
+                } else {
+                    qa[nq] = 1.0 / (hh * hh);
+                    qb[nq] = best;
+                }
+                nq++;
+            }
+            const double s = slow[j];
+            double t = INF;
+            if (nq == 2) {
+                // qa0 (t - qb0)^2 + qa1 (t - qb1)^2 = s^2
+                const double a = qa[0] + qa[1];
+                const double b = -2.0 * (qa[0] * qb[0] + qa[1] * qb[1]);
+                const double c = qa[0] * qb[0] * qb[0] + qa[1] * qb[1] * qb[1] - s * s;
+                const double disc = b * b - 4.0 * a * c;
+                if (disc >= 0.0) t = (-b + std::sqrt(disc)) / (2.0 * a);
+                if (!(t >= std::max(qb[0], qb[1]))) t = INF;  // not upwind: 1-D update
+            }
+            if (t == INF)
+                for (int q2 = 0; q2 < nq; q2++) t = std::min(t, qb[q2] + s / std::sqrt(qa[q2]));
+            if (t < T[j]) {
+                T[j] = t;
+                state[j] = 1;
+                heap.push_or_decrease((int)j);
+            }
+        }
+    }
+}
+
+void smooth5(std::vector<double> &v, int nz, int nx)
+{
+    // separable 5-point Gaussian [1 4 6 4 1]/16 with edge clamping, z then x
+    static const double w[5] = {1.0 / 16, 4.0 / 16, 6.0 / 16, 4.0 / 16, 1.0 / 16};
+    std::vector<double> t(v.size());
+    for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
+            double s = 0;
+            for (int k = -2; k <= 2; k++) {
+                const int jz = std::min(std::max(iz + k, 0), nz - 1);
+                s += w[k + 2] * v[(size_t)ix * nz + jz];
+            }
+            t[(size_t)ix * nz + iz] = s;
+        }
+    for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
+            double s = 0;
+            for (int k = -2; k <= 2; k++) {
+                const int jx = std::min(std::max(ix + k, 0), nx - 1);
+                s += w[k + 2] * t[(size_t)jx * nz + iz];
+            }
+            v[(size_t)ix * nz + iz] = s;
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+// v(z) = v0 + g z on (nx, nz), z fastest.
+void kms_velocity_vz(float *v, int nz, int nx, float dz, float v0, float g)
+{
+    for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) v[(size_t)ix * nz + iz] = v0 + g * (float)(iz * (double)dz);
+}
+
+// Five-layer salt-like model: 1.5 / 2.0 / 2.6 / 4.5 (salt body with a
+// Gaussian-bump top) / 3.2 km/s, smoothed with a 5-point Gaussian.
+void kms_velocity_salt(float *v, int nz, int nx, float dz, float dx)
+{
+    std::vector<double> m((size_t)nz * nx);
+    const double zmax = (nz - 1) * (double)dz, xmax = (nx - 1) * (double)dx;
+    for (int ix = 0; ix < nx; ix++) {
+        const double x = ix * (double)dx, xr = x / std::max(xmax, 1e-9);
+        const double z1 = 0.2 * zmax, z2 = 0.38 * zmax, zbase = 0.8 * zmax;
+        const double ztop = 0.62 * zmax - 0.22 * zmax * std::exp(-std::pow((xr - 0.5) / 0.14, 2));
+        const bool salt_x = xr > 0.18 && xr < 0.82;
+        for (int iz = 0; iz < nz; iz++) {
+            const double z = iz * (double)dz;
+            double vel;
+            if (z < z1) vel = 1.5;
+            else if (z < z2) vel = 2.0;
+            else if (salt_x && z >= ztop && z < zbase) vel = 4.5;
+            else if (z < zbase) vel = 2.6;
+            else vel = 3.2;
+            m[(size_t)ix * nz + iz] = vel;
+        }
+    }
+    smooth5(m, nz, nx);
+    for (size_t k = 0; k < m.size(); k++) v[k] = (float)m[k];
+}
+
+// Tables T and dT/ds (npos, nx, nz) for sources at depth zs and lateral
+// positions o + k d (k < npos).  Returns 0.
+int kms_tables(const float *vel, int nz, int nx, float dz, float dx, float zs, int npos, float o,
+               float d, int box, int nthreads, float *T, float *D)
+{
+    const size_t n = (size_t)nz * nx;
+    std::vector<double> slow(n);
+    for (size_t k = 0; k < n; k++) slow[k] = 1.0 / (double)vel[k];
+    std::vector<double> Td((size_t)npos * n);
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+    {
+        std::vector<uint8_t> state;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic)
+#endif
+        for (int k = 0; k < npos; k++) {
+            const double xs = (double)o + k * (double)d;
+            fmm(slow.data(), nz, nx, dz, dx, zs, xs, box, Td.data() + (size_t)k * n, state, true);
+        }
+    }
+    for (size_t k = 0; k < Td.size(); k++) T[k] = (float)Td[k];
+    for (int k = 0; k < npos; k++) {
+        const int a = std::max(0, k - 1), b = std::min(npos - 1, k + 1);
+        float *Dk = D + (size_t)k * n;
+        if (a == b) {
+            std::fill(Dk, Dk + n, 0.0f);
+            continue;
+        }
+        const double inv = 1.0 / ((b - a) * (double)d);
+        const double *ta = Td.data() + (size_t)a * n, *tb = Td.data() + (size_t)b * n;
+        for (size_t q = 0; q < n; q++) Dk[q] = (float)((tb[q] - ta[q]) * inv);
+    }
+    return 0;
+}
+
+// Analytic first-arrival time in v(z) = v0 + g z between (0, xs) and (z, x).
+double kms_time_vz(double v0, double g, double xs, double x, double z)
+{
+    const double r2 = (x - xs) * (x - xs) + z * z;
+    if (g == 0.0) return std::sqrt(r2) / v0;
+    return std::acosh(1.0 + g * g * r2 / (2.0 * v0 * (v0 + g * z))) / g;
+}
+
+// Flat-reflector data in v(z): for each (shot, receiver) a Ricker of peak
+// frequency f at the specular two-way time of each reflector (the midpoint
+// is the reflection point in a laterally invariant medium), then Gaussian
+// noise with sigma = rms / 10^(snr_db/20).  cmp selects offset (1) or
+// absolute receiver (0) coordinates for axis 2.  data is (ns, nh, nt).
+void kms_data_reflectors(float *data, int nt, float t0, float dt, int nh, float h0, float dh,
+                         int ns, float s0, float ds, int cmp, float v0, float g, int nref,
+                         const float *zref, const float *amp, float freq, float snr_db,
+                         uint64_t seed)
+{
+    const size_t ntr = (size_t)ns * nh;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (long tr = 0; tr < (long)ntr; tr++) {
+        const int is = (int)(tr / nh), ih = (int)(tr % nh);
+        const double s = s0 + is * (double)ds, h = h0 + ih * (double)dh;
+        const double r = cmp ? s + h : h;
+        float *out = data + (size_t)tr * nt;
+        for (int it = 0; it < nt; it++) out[it] = 0.0f;
+        for (int k = 0; k < nref; k++) {
+            const double half = 0.5 * std::fabs(r - s);
+            const double tt = 2.0 * kms_time_vz(v0, g, 0.0, half, zref[k]);
+            const int c = (int)std::floor((tt - t0) / dt);
+            const int w = (int)std::ceil(2.5 / (freq * dt));
+            for (int it = std::max(0, c - w); it <= std::min(nt - 1, c + w); it++)
+                out[it] += (float)(amp[k] * ricker(t0 + it * (double)dt - tt, freq));
+        }
+    }
+    if (snr_db > 0.0f) {
+        double e = 0.0;
+        for (size_t k = 0; k < ntr * nt; k++) e += (double)data[k] * data[k];
+        const double sigma = std::sqrt(e / (double)(ntr * nt)) / std::pow(10.0, snr_db / 20.0);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+        for (long tr = 0; tr < (long)ntr; tr++) {
+            Rng rng(seed ^ (0x5851F42D4C957F2Dull * (uint64_t)(tr + 1)));
+            float *out = data + (size_t)tr * nt;
+            for (int it = 0; it < nt; it++) out[it] += (float)(sigma * rng.normal());
+        }
+    }
+}
+
+// Point-scatterer (Born-style) data with table times plus a direct arrival:
+// nsc scatterers at seeded random grid nodes with amplitudes in [-1, 1];
+// source and receiver times are read from the position tables at the nearest
+// position node.  data is (ns, nh, nt).
+void kms_data_scatterers(float *data, int nt, float t0, float dt, int nh, float h0, float dh,
+                         int ns, float s0, float ds, int cmp, const float *ST, int snpos, float so,
+                         float sd, const float *RT, int rnpos, float ro, float rd, int nz, int nx,
+                         int nsc, float freq, float vdirect, uint64_t seed)
+{
+    const size_t n = (size_t)nz * nx;
+    Rng rng(seed);
+    std::vector<size_t> pt(nsc);
+    std::vector<double> a(nsc);
+    for (int k = 0; k < nsc; k++) {
+        const int iz = (int)(nz * (0.15 + 0.8 * rng.uniform()));
+        const int ix = (int)(nx * (0.05 + 0.9 * rng.uniform()));
+        pt[k] = (size_t)std::min(ix, nx - 1) * nz + std::min(iz, nz - 1);
+        a[k] = 2.0 * rng.uniform() - 1.0;
+    }
+    const int w = (int)std::ceil(2.5 / (freq * dt));
+    const long ntr = (long)ns * nh;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (long tr = 0; tr < ntr; tr++) {
+        const int is = (int)(tr / nh), ih = (int)(tr % nh);
+        const double s = s0 + is * (double)ds, h = h0 + ih * (double)dh;
+        const double r = cmp ? s + h : h;
+        const int ks = std::min(std::max((int)std::lround((s - so) / sd), 0), snpos - 1);
+        const int kr = std::min(std::max((int)std::lround((r - ro) / rd), 0), rnpos - 1);
+        float *out = data + (size_t)tr * nt;
+        for (int it = 0; it < nt; it++) out[it] = 0.0f;
+        auto add = [&](double tt, double amp) {
+            const int c = (int)std::floor((tt - t0) / dt);
+            for (int it = std::max(0, c - w); it <= std::min(nt - 1, c + w); it++)
+                out[it] += (float)(amp * ricker(t0 + it * (double)dt - tt, freq));
+        };
+        if (vdirect > 0.0f) add(std::fabs(r - s) / vdirect, 2.0);
+        for (int k = 0; k < nsc; k++)
+            add((double)ST[(size_t)ks * n + pt[k]] + (double)RT[(size_t)kr * n + pt[k]], a[k]);
+    }
+}
+
+}  // extern "C"

@@ -23,6 +23,7 @@ KMIG_OK = 0
 ERRORS = {-1: "KMIG_E_PARAM", -2: "KMIG_E_DIM", -3: "KMIG_E_IO", -4: "KMIG_E_DEVICE",
           -5: "KMIG_E_STATE", -6: "KMIG_E_NOMEM"}
 PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
+VARIANT = {"auto": 0, "stream": 1, "resident": 2}
 
 # every exported symbol of include/kmig.h
 EXPORTS = ("kmig_params_default", "kmig_params_parse", "kmig_params_check", "kmig_create",
@@ -54,7 +55,7 @@ class KmigParams(C.Structure):
         ("angle_taper_end", C.c_float),
         ("angle_grad_radius", C.c_int), ("angle_smooth", C.c_int), ("interp_type", C.c_int),
         ("g0", C.c_float), ("dg", C.c_float), ("ng", C.c_int),
-        ("precision", C.c_int), ("tile", C.c_int),
+        ("precision", C.c_int), ("tile", C.c_int), ("variant", C.c_int),
     ]
 
 
@@ -62,7 +63,8 @@ class KmigStats(C.Structure):
     _fields_ = [("updates", C.c_double), ("ms_trace_prep", C.c_double),
                 ("ms_tuple_prep", C.c_double), ("ms_migrate", C.c_double),
                 ("ms_total", C.c_double), ("migrate_launches", C.c_int),
-                ("receiver_slots", C.c_int), ("shot_batches", C.c_int)]
+                ("receiver_slots", C.c_int), ("shot_batches", C.c_int),
+                ("variant", C.c_int)]
 
 
 _LIB = None

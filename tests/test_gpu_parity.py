@@ -29,11 +29,12 @@ def _args(run):
             run["rder"], run["r_axes"])
 
 
+@pytest.mark.parametrize("variant", ["stream", "auto"])
 @pytest.mark.parametrize("name", gc.case_names())
-def test_golden_case(name):
+def test_golden_case(name, variant):
     kv, run, exp = gc.case(name)
-    print(f"case {name}", flush=True)
-    cig, qcb, qca = _kmig().migrate(kv, *_args(run), grads=run["grads"])
+    print(f"case {name} variant {variant}", flush=True)
+    cig, qcb, qca = _kmig().migrate(dict(kv, variant=variant), *_args(run), grads=run["grads"])
     ref = exp["cig"]
     assert cig.shape == ref.shape
     ref64 = O.migrate(kv, *_args(run), grads=run["grads"], precision="f64")[0]
@@ -50,7 +51,9 @@ def test_golden_case(name):
 
 
 def test_shot_batches_are_bitwise_identical(monkeypatch):
+    """The stream variant keeps each point's reference order across batches."""
     kv, run, _ = gc.case("mute_qc")
+    kv = dict(kv, variant="stream")
     full = _kmig().migrate(kv, *_args(run))[0]
     monkeypatch.setenv("KMIG_MAX_SLOTS", "20")  # one shot + 17 receivers per batch
     split = _kmig().migrate(kv, *_args(run))[0]
@@ -152,3 +155,25 @@ def test_parameter_errors_are_status_codes():
         kmig.migrate(kv, run["data"], run["data_axes"], run["stab"], run["sder"], run["s_axes"],
                      bad_r, run["rder"][:, :, :-1], bad_ax)
     assert "Dimension mismatch in rtable" in str(e.value)
+
+
+def test_resident_variant_is_used_and_matches_stream():
+    kmig = _kmig()
+    kv, run, _ = gc.case("angle_base")
+    ax = run["data_axes"]
+    out = {}
+    for v in ("stream", "resident"):
+        p = kmig.parse_params(dict(kv, variant=v), int(ax[1][0]), ax[1][1], ax[1][2])
+        with kmig.Migrator(p) as m:
+            m.set_tables(0, run["stab"], run["sder"], run["s_axes"])
+            m.set_tables(1, run["rtab"], run["rder"], run["r_axes"])
+            m.set_data(run["data"], ax)
+            m.run()
+            m.sync()
+            out[v] = m.cig()
+            assert m.stats()["variant"] == kmig.VARIANT[v]
+    assert O.rel_l2(out["resident"], out["stream"]) < TOL
+    # cmp=1 receivers move with the shot: resident is refused, auto streams
+    kv1, run1, _ = gc.case("angle_cmp1")
+    with pytest.raises(kmig.KmigError, match="variant=resident needs"):
+        kmig.migrate(dict(kv1, variant="resident"), *_args(run1))
