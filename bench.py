@@ -43,7 +43,21 @@ def main():
     ap.add_argument("--cpu-shots", type=int, default=2,
                     help="shots of the workload timed on the CPU oracle (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run a plain device copy of this many MiB (profiler calibration)")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        # a plain copy kernel of known size: the HBM counters of a profiler run
+        # on it give the scale factor for the migrate kernel's counters
+        import torch
+        n = int(args.copy_probe * 1024 * 1024 // 4)
+        x = torch.ones(n, device="cuda", dtype=torch.float32)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the shot-sharded "
               "multi-GPU path is a design (partial CIG per GPU + one all-reduce), not built "
