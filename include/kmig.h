@@ -104,6 +104,7 @@ typedef struct kmig_stats {
     int receiver_slots;        /* resident receiver tuples                  */
     int shot_batches;
     int variant;               /* KMIG_VARIANT_* actually used              */
+    int tile;                  /* migrate tile preset used (autotuned if 0) */
 } kmig_stats;
 
 typedef struct kmig_ctx kmig_ctx;

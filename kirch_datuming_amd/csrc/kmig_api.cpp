@@ -79,6 +79,9 @@ struct kmig_ctx {
     size_t rslot_cap = 0;
     bool rcv_invariant = false;     // every shot uses receiver slot rslot[ih]
     int variant_used = 0;
+    int tuned_tile = 0;             // autotuned resident preset (0: not tuned yet)
+    long long tuned_version = -1;
+    std::vector<std::pair<int, float>> tune_log;
     std::vector<hipEvent_t> events;
     std::vector<int> ev_kind;       // per event pair: 0 trace, 1 tuple, 2 migrate
     int ev_used = 0;
@@ -376,6 +379,100 @@ hipEvent_t next_event(kmig_ctx *c)
         c->events.push_back(e);
     }
     return c->events[c->ev_used++];
+}
+
+
+MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
+{
+    const kmig_params &p = c->p;
+    MigResArgs R{};
+    R.nz = c->nz;
+    R.nx = c->nx;
+    R.nzx = c->nzx;
+    R.nt = c->nt;
+    R.nh = c->nh;
+    R.shot0 = shot0;
+    R.nshots = nshots;
+    R.stup = c->pk_s;
+    R.rtup = c->pk_r;
+    R.rslot = c->rslot;
+    R.wap = c->wap;
+    R.pairs = c->pairs;
+    R.cig = c->cig;
+    const double cst = ((double)p.tau - (double)c->t0) / (double)c->dt;
+    const double ci = std::floor(cst);
+    R.c_int = (int)ci;
+    R.c_frac = (float)(cst - ci);
+    R.aal = p.antialias;
+    R.half_angle = p.half_angle;
+    R.g0 = p.g0;
+    R.inv_dg = 1.0f / p.dg;
+    R.ng = ng;
+    R.oazmin = p.oazmin;
+    R.oazmax = p.oazmax;
+    R.oaztaper = p.oaztaper;
+    R.inv_oaztaper = p.oaztaper > 0.0f ? 1.0f / p.oaztaper : 0.0f;
+    R.taper_start = p.angle_taper_start;
+    R.taper_end = p.angle_taper_end;
+    R.inv_taper = p.angle_taper_end > p.angle_taper_start
+                      ? 1.0f / (p.angle_taper_end - p.angle_taper_start)
+                      : 0.0f;
+    return R;
+}
+
+// Project autotuner for the resident kernel: times every tile / receiver-chunk
+// preset on a slice of the real shots (about 5e9 updates, twice each, HIP
+// events on the context stream) and keeps the fastest.  Runs once per context
+// and input set; it writes into the CIG, which the run clears afterwards.
+int autotune(kmig_ctx *c, const MigResArgs &full)
+{
+    if (c->tuned_tile > 0 && c->tuned_version == c->geo_version) return KMIG_OK;
+    static const int presets[] = {9, 15, 16, 20, 11, 10, 18};
+    const double per_shot = (double)c->nh * (double)c->nzx;
+    int S = (int)std::max(1.0, std::min((double)full.nshots, 5e9 / std::max(per_shot, 1.0)));
+    // a slice from the middle of the batch (source tuples are batch-local)
+    const int off = (full.nshots - S) / 2;
+    MigResArgs R = full;
+    R.nshots = S;
+    R.shot0 = full.shot0 + off;
+    R.stup = full.stup + (size_t)off * c->nzx;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    float best = 1e30f;
+    int best_tile = 2;
+    c->tune_log.clear();
+    for (int t : presets) {
+        if (migrate_lds_bytes(full.ng, t) > 64 * 1024) continue;
+        float tmin = 1e30f;
+        for (int rep = 0; rep < 2; rep++) {
+            HIPCHK(c, hipEventRecord(e0, c->stream));
+            HIPCHK(c, (hipError_t)launch_migrate_resident(R, t, c->stream));
+            HIPCHK(c, hipEventRecord(e1, c->stream));
+            HIPCHK(c, hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+            tmin = std::min(tmin, ms);
+        }
+        c->tune_log.push_back({t, tmin});
+        if (tmin < best) {
+            best = tmin;
+            best_tile = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (getenv("KMIG_TUNE_LOG")) {
+        fprintf(stderr, "kmig autotune (%d shots x %d receivers):", S, c->nh);
+        for (auto &q : c->tune_log) fprintf(stderr, " tile%d=%.2fms", q.first, q.second);
+        fprintf(stderr, " -> tile%d\n", best_tile);
+    }
+    c->tuned_tile = best_tile;
+    c->tuned_version = c->geo_version;
+    // the tuning launches accumulated into the CIG: clear it again
+    const int ng = full.ng;
+    HIPCHK(c, (hipError_t)launch_fill(c->cig, (size_t)ng * c->nzx, 0.0f, c->stream));
+    return KMIG_OK;
 }
 
 }  // namespace
@@ -772,40 +869,17 @@ int kmig_run(kmig_ctx *c, int sb, int se)
         M.taper_end = p.angle_taper_end;
         int tile = p.tile;
         if (tile <= 0) {
-            tile = 2;  // 16 x 8 points
-            if (migrate_lds_bytes(ng, tile) > 64 * 1024) tile = 1;
+            tile = resident ? 9 : 2;  // 16 x 8 points
+            if (migrate_lds_bytes(ng, tile) > 64 * 1024) tile = resident ? 15 : 1;
         }
         if (migrate_lds_bytes(ng, tile) > 160 * 1024)
             return fail(c, KMIG_E_PARAM, "output axis of %d samples does not fit in LDS", ng);
         if (nsb > 0 && resident) {
-            MigResArgs R{};
-            R.nz = c->nz;
-            R.nx = c->nx;
-            R.nzx = c->nzx;
-            R.nt = c->nt;
-            R.nh = c->nh;
-            R.shot0 = b.s0;
-            R.nshots = nsb;
-            R.stup = c->pk_s;
-            R.rtup = c->pk_r;
-            R.rslot = c->rslot;
-            R.wap = c->wap;
-            R.pairs = c->pairs;
-            R.cig = c->cig;
-            const double cst = ((double)p.tau - (double)c->t0) / (double)c->dt;
-            const double ci = std::floor(cst);
-            R.c_int = (int)ci;
-            R.c_frac = (float)(cst - ci);
-            R.aal = p.antialias;
-            R.half_angle = p.half_angle;
-            R.g0 = p.g0;
-            R.inv_dg = 1.0f / p.dg;
-            R.ng = ng;
-            R.oazmin = p.oazmin;
-            R.oazmax = p.oazmax;
-            R.oaztaper = p.oaztaper;
-            R.taper_start = p.angle_taper_start;
-            R.taper_end = p.angle_taper_end;
+            MigResArgs R = res_args(c, b.s0, nsb, ng);
+            if (p.tile <= 0) {
+                if ((rc = autotune(c, R))) return rc;
+                tile = c->tuned_tile;
+            }
             HIPCHK(c, (hipError_t)launch_migrate_resident(R, tile, c->stream));
             launches++;
         } else if (nsb > 0) {
@@ -822,6 +896,7 @@ int kmig_run(kmig_ctx *c, int sb, int se)
     c->st.migrate_launches = launches;
     c->st.shot_batches = (int)c->batches.size();
     c->st.variant = c->variant_used;
+    c->st.tile = resident ? (p.tile > 0 ? p.tile : c->tuned_tile) : p.tile;
     int maxr = 0;
     for (const Batch &b : c->batches) maxr = std::max(maxr, b.nrcv);
     c->st.receiver_slots = maxr;

@@ -130,8 +130,8 @@ struct MigResArgs {
     int half_angle;
     float g0, inv_dg;
     int ng;
-    float oazmin, oazmax, oaztaper;
-    float taper_start, taper_end;
+    float oazmin, oazmax, oaztaper, inv_oaztaper;
+    float taper_start, taper_end, inv_taper;
 };
 
 // Launchers (kmig_prep.hip, kmig_migrate.hip).  Return 0 or a hipError_t value.

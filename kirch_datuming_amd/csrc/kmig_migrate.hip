@@ -162,28 +162,61 @@ __global__ __launch_bounds__(TZ *TX) void k_migrate(const MigArgs a)
 //
 // For receivers that do not move with the shot (cmp=0), each thread keeps R
 // receiver tuples of its image point in registers and sweeps every shot under
-// them: per shot it loads one source tuple and runs R independent updates.
+// them: per shot it loads one source tuple (prefetched one shot ahead) and
+// runs R independent updates.
 // Per point the accumulation order is (receiver chunk, shot, receiver).
 // The opening angle is |th_s - th_r| folded into [0, pi] (the angle between
 // the two gradient directions, the quantity atan2(|s x r|, s . r) measures),
 // and traveltimes arrive split into integer and fractional samples.
 
-struct Bins {
-    int ig0, ig1;
-    float w0, w1;
+// Raised-cosine ramp 0.5 * (1 + cos(pi u)) for u in [0, 1], written as
+// 0.5 - 0.5 * sin(pi v) with v = u - 1/2 and the Taylor series of sin to
+// x^11 (truncation below 6e-8 on |x| <= pi/2).
+__device__ __forceinline__ float ramp_down(float u)
+{
+    const float x = 3.14159265358979f * (u - 0.5f);
+    const float x2 = x * x;
+    float p = -2.5052108385441720e-08f;        // -1/11!
+    p = p * x2 + 2.7557319223985893e-06f;      //  1/9!
+    p = p * x2 - 1.9841269841269841e-04f;      // -1/7!
+    p = p * x2 + 8.3333333333333333e-03f;      //  1/5!
+    p = p * x2 - 1.6666666666666667e-01f;      // -1/3!
+    p = p * x2 + 1.0f;
+    return 0.5f - 0.5f * (x * p);
+}
+
+// Optional XCD-aware tile order (KMIG_XCD_MAP=1): blocks are dealt
+// round-robin to the 8 XCDs, so this gives each XCD a contiguous run of tiles.
+// Off by default: measured slower on the salt workload, because contiguous
+// regions carry unequal work and the slowest XCD sets the kernel time, while
+// the plain order spreads every region over all XCDs.
+#ifndef KMIG_XCD_MAP
+#define KMIG_XCD_MAP 0
+#endif
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    if (!KMIG_XCD_MAP) return b;
+    const int xcd = b & 7, k = b >> 3, q = nb >> 3, rem = nb & 7;
+    return xcd < rem ? xcd * (q + 1) + k : rem * (q + 1) + (xcd - rem) * q + k;
+}
+
+// Phased form of the resident kernel: for a group of G receivers it first
+// evaluates every weight, bin and tap index without branches, then issues the
+// 3G tap-pair loads together, then accumulates.  Rejected updates keep a zero
+// coefficient and read sample 0, so every wave has up to 3G loads in flight
+// instead of one update's worth.  Per point the updates land in (receiver
+// chunk, shot, receiver) order.
+struct Pend {
+    float coef, w1, ft, ftm, ftp;
+    int ig0, it, itm, itp;
 };
 
-template <class Acc>
-__device__ __forceinline__ void res_update_s(const Acc &my, const PackedTuple &s,
-                                           const PackedTuple &r, float wap,
-                                           const float2 *__restrict__ tr, const MigResArgs &a,
-                                           float cutoff, bool taper_on)
+__device__ __forceinline__ Pend res_phase1(const PackedTuple &s, const PackedTuple &r, float wap,
+                                           const MigResArgs &a, float cutoff, bool taper_on)
 {
+    Pend q;
     const float inv32k = 1.0f / 32768.0f;
-    const float ws = (float)(s.iw & 0xFFFFu) * inv32k;
-    const float wr = (float)(r.iw & 0xFFFFu) * inv32k;
-    const float apw = ws * wr;
-    if (!(apw > 0.0f)) return;
+    const float apw = (float)(s.iw & 0xFFFFu) * inv32k * ((float)(r.iw & 0xFFFFu) * inv32k);
     const float PI_F = 3.14159265358979f;
     float d = fabsf(s.th - r.th);
     d = d > PI_F ? 2.0f * PI_F - d : d;
@@ -191,83 +224,69 @@ __device__ __forceinline__ void res_update_s(const Acc &my, const PackedTuple &s
     if (a.half_angle) angle *= 0.5f;
     float wt = 1.0f;
     if (taper_on) {
-        if (angle >= a.taper_end) return;
-        if (angle > a.taper_start)
-            wt = 0.5f * (1.0f + cosf(PI_F * ((angle - a.taper_start) /
-                                             (a.taper_end - a.taper_start))));
+        const float rt = ramp_down(fminf(fmaxf((angle - a.taper_start) * a.inv_taper, 0.0f), 1.0f));
+        wt = angle >= a.taper_end ? 0.0f : (angle > a.taper_start ? rt : 1.0f);
     }
     float wd;
     if (a.oaztaper <= 0.0f) {
         wd = (angle <= cutoff) ? 1.0f : 0.0f;
-    } else if (angle <= cutoff) {
-        wd = 1.0f;
-    } else if (angle >= cutoff + a.oaztaper) {
-        wd = 0.0f;
     } else {
-        wd = 0.5f * (1.0f + cosf(PI_F * ((angle - cutoff) / a.oaztaper)));
+        const float rd = ramp_down(fminf(fmaxf((angle - cutoff) * a.inv_oaztaper, 0.0f), 1.0f));
+        wd = angle <= cutoff ? 1.0f : (angle >= cutoff + a.oaztaper ? 0.0f : rd);
     }
     const float aw = wt * wd * wap * apw;
-    if (!(aw > 0.0f)) return;
     const float u = (angle - a.g0) * a.inv_dg;
-    if (u < -1.0e-5f || u > (float)(a.ng - 1) + 1.0e-5f) return;
-    int ig0, ig1;
-    float w0, w1;
+    const float ngm1 = (float)(a.ng - 1);
+    bool ok = aw > 0.0f && wap >= 0.0f && u >= -1.0e-5f && u <= ngm1 + 1.0e-5f;
+    const float fu = floorf(u);
+    int ig0 = (int)fu;
+    float w1 = u - fu;
     if (u <= 0.0f) {
-        ig0 = ig1 = 0;
-        w0 = 1.0f;
+        ig0 = 0;
         w1 = 0.0f;
-    } else if (u >= (float)(a.ng - 1)) {
-        ig0 = ig1 = a.ng - 1;
-        w0 = 1.0f;
-        w1 = 0.0f;
-    } else {
-        const float fu = floorf(u);
-        ig0 = (int)fu;
-        ig1 = ig0 + 1;
-        w1 = u - fu;
-        w0 = 1.0f - w1;
     }
-    // kirmig_pick(adj=1) in split-sample time
+    if (u >= ngm1) {
+        ig0 = a.ng - 1;
+        w1 = 0.0f;
+    }
     const int ii = ((int)s.iw >> 16) + ((int)r.iw >> 16) + a.c_int;
     const float ff = s.frac + r.frac + a.c_frac;
     const float fl = floorf(ff);
     const int it = ii + (int)fl;
     const float ft = ff - fl;
-    const unsigned lim = (unsigned)(a.nt - 1);
-    if ((unsigned)it >= lim) return;
     const float dA = fmaxf(s.a, r.a) * a.aal;
     const float fm = ft - dA - 1.0f, fp = ft + dA + 1.0f;
     const float flm = floorf(fm), flp = floorf(fp);
     const int itm = it + (int)flm, itp = it + (int)flp;
-    if ((unsigned)itm >= lim || (unsigned)itp >= lim) return;
-    const float ftm = fm - flm, ftp = fp - flp;
+    const unsigned lim = (unsigned)(a.nt - 1);
+    ok = ok && (unsigned)it < lim && (unsigned)itm < lim && (unsigned)itp < lim;
     const float rr = __builtin_amdgcn_rcpf(3.0f + 2.0f * dA);
-    const float2 q0 = tr[it], qm = tr[itm], qp = tr[itp];
-    const float sm = 2.0f * ((1.0f - ft) * q0.x + ft * q0.y) - ((1.0f - ftm) * qm.x + ftm * qm.y) -
-                     ((1.0f - ftp) * qp.x + ftp * qp.y);
-    const float picked = aw * (rr * rr) * sm;
-    my[ig0] += w0 * picked;
-    if (ig1 != ig0 && w1 > 0.0f) my[ig1] += w1 * picked;
+    q.coef = ok ? aw * (rr * rr) : 0.0f;
+    q.w1 = w1;
+    q.ft = ft;
+    q.ftm = fm - flm;
+    q.ftp = fp - flp;
+    q.ig0 = ok ? ig0 : 0;
+    q.it = ok ? it : 0;
+    q.itm = ok ? itm : 0;
+    q.itp = ok ? itp : 0;
+    return q;
 }
 
-template <int TZ, int TX, int R>
-__global__ __launch_bounds__(TZ *TX) void k_migrate_res(const MigResArgs a)
+template <int TZ, int TX, int R, int G>
+__global__ __launch_bounds__(TZ *TX) void k_migrate_ph(const MigResArgs a)
 {
     constexpr int P = TZ * TX;
     extern __shared__ float acc[];
     const int lid = threadIdx.x;
     const int ntz = (a.nz + TZ - 1) / TZ;
-    const int bz = blockIdx.x % ntz, bx = blockIdx.x / ntz;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int bz = tile % ntz, bx = tile / ntz;
     const int iz = bz * TZ + lid % TZ, ix = bx * TX + lid / TZ;
     const bool valid = iz < a.nz && ix < a.nx;
     const size_t i = valid ? (size_t)ix * a.nz + iz : 0;
-    for (int g = 0; g < a.ng; g++) acc[g * P + lid] = valid ? a.cig[(size_t)g * a.nzx + i] : 0.0f;
-    // bins of this thread live at acc[g * P + lid]; index them through a
-    // strided view so res_update can address bin g as my[g]
-    struct Strided {
-        float *base;
-        __device__ float &operator[](int g) const { return base[g * P]; }
-    };
+    float *my = acc + lid;
+    for (int g = 0; g < a.ng; g++) my[g * P] = valid ? a.cig[(size_t)g * a.nzx + i] : 0.0f;
     const float zf = (a.nz > 1) ? (float)iz / (float)(a.nz - 1) : 0.0f;
     const float cutoff = a.oazmin + zf * (a.oazmax - a.oazmin);
     const bool taper_on = !(a.taper_start < 0.0f || a.taper_end <= a.taper_start);
@@ -286,34 +305,58 @@ __global__ __launch_bounds__(TZ *TX) void k_migrate_res(const MigResArgs a)
                 rt[r].iw = 0u;
             }
         }
+        PackedTuple nxt = a.stup[i];
         for (int js = 0; js < a.nshots; js++) {
-            const PackedTuple st = a.stup[(size_t)js * a.nzx + i];
+            const PackedTuple st = nxt;
+            if (js + 1 < a.nshots) nxt = a.stup[(size_t)(js + 1) * a.nzx + i];
             const bool live = valid && (st.iw & 0xFFFFu) != 0u;
             if (__ballot(live) == 0) continue;
             const int is = a.shot0 + js;
             const float *__restrict__ wrow = a.wap + (size_t)is * a.nh + r0;
             const float2 *__restrict__ trow = pairs + ((size_t)is * a.nh + r0) * a.nt;
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                if (r0 + r >= a.nh) break;
-                const float wap = wrow[r];
-                if (wap < 0.0f) continue;
-                Strided my{acc + lid};
-                res_update_s(my, st, rt[r], wap, trow + (size_t)r * a.nt, a, cutoff, taper_on);
+            for (int g0 = 0; g0 < R; g0 += G) {
+                Pend q[G];
+                bool any = false;
+#pragma unroll
+                for (int k = 0; k < G; k++) {
+                    const float wap = (r0 + g0 + k < a.nh) ? wrow[g0 + k] : -1.0f;
+                    q[k] = res_phase1(st, rt[g0 + k], wap, a, cutoff, taper_on);
+                    any = any || q[k].coef != 0.0f;
+                }
+                if (__ballot(any) == 0) continue;
+                float2 v0[G], vm[G], vp[G];
+#pragma unroll
+                for (int k = 0; k < G; k++) {
+                    const float2 *tr = trow + (size_t)(g0 + k) * a.nt;
+                    v0[k] = tr[q[k].it];
+                    vm[k] = tr[q[k].itm];
+                    vp[k] = tr[q[k].itp];
+                }
+#pragma unroll
+                for (int k = 0; k < G; k++) {
+                    if (q[k].coef == 0.0f) continue;
+                    const float sm = 2.0f * ((1.0f - q[k].ft) * v0[k].x + q[k].ft * v0[k].y) -
+                                     ((1.0f - q[k].ftm) * vm[k].x + q[k].ftm * vm[k].y) -
+                                     ((1.0f - q[k].ftp) * vp[k].x + q[k].ftp * vp[k].y);
+                    const float picked = q[k].coef * sm;
+                    my[q[k].ig0 * P] += (1.0f - q[k].w1) * picked;
+                    if (q[k].w1 > 0.0f) my[(q[k].ig0 + 1) * P] += q[k].w1 * picked;
+                }
             }
         }
     }
     if (valid)
-        for (int g = 0; g < a.ng; g++) a.cig[(size_t)g * a.nzx + i] = acc[g * P + lid];
+        for (int g = 0; g < a.ng; g++) a.cig[(size_t)g * a.nzx + i] = my[g * P];
 }
 
 static void tile_dims(int tile, int *tz, int *tx)
 {
     switch (tile) {
-    case 1: *tz = 8; *tx = 8; break;
-    case 2: *tz = 16; *tx = 8; break;
-    case 3: *tz = 16; *tx = 16; break;
-    case 4: *tz = 32; *tx = 4; break;
+    case 1: case 13: case 15: case 21: *tz = 8; *tx = 8; break;
+    case 3: case 19: *tz = 16; *tx = 16; break;
+    case 4: case 16: *tz = 32; *tx = 4; break;
+    case 8: case 17: *tz = 64; *tx = 2; break;
     default: *tz = 16; *tx = 8; break;
     }
 }
@@ -341,19 +384,19 @@ static int launch_mig_t(const MigArgs &a, hipStream_t st)
     return (int)hipGetLastError();
 }
 
-template <int TZ, int TX, int R>
-static int launch_res_t(const MigResArgs &a, hipStream_t st)
+template <int TZ, int TX, int R, int G>
+static int launch_ph_t(const MigResArgs &a, hipStream_t st)
 {
     const int ntz = (a.nz + TZ - 1) / TZ, ntx = (a.nx + TX - 1) / TX;
     const size_t lds = (size_t)a.ng * TZ * TX * sizeof(float);
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_res<TZ, TX, R>,
+        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_ph<TZ, TX, R, G>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL((k_migrate_res<TZ, TX, R>), dim3((unsigned)(ntz * ntx)), dim3(TZ * TX), lds,
-                       st, a);
+    hipLaunchKernelGGL((k_migrate_ph<TZ, TX, R, G>), dim3((unsigned)(ntz * ntx)), dim3(TZ * TX),
+                       lds, st, a);
     return (int)hipGetLastError();
 }
 
@@ -361,10 +404,17 @@ int launch_migrate_resident(const MigResArgs &a, int tile, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
     switch (tile) {
-    case 1: return launch_res_t<8, 8, 8>(a, st);
-    case 3: return launch_res_t<16, 16, 8>(a, st);
-    case 4: return launch_res_t<32, 4, 8>(a, st);
-    default: return launch_res_t<16, 8, 8>(a, st);
+    case 10: return launch_ph_t<16, 8, 16, 4>(a, st);
+    case 11: return launch_ph_t<16, 8, 8, 8>(a, st);
+    case 13: return launch_ph_t<8, 8, 16, 4>(a, st);
+    case 15: return launch_ph_t<8, 8, 8, 4>(a, st);
+    case 16: return launch_ph_t<32, 4, 8, 4>(a, st);
+    case 17: return launch_ph_t<64, 2, 8, 4>(a, st);
+    case 18: return launch_ph_t<16, 8, 4, 4>(a, st);
+    case 19: return launch_ph_t<16, 16, 8, 4>(a, st);
+    case 20: return launch_ph_t<16, 8, 8, 2>(a, st);
+    case 21: return launch_ph_t<8, 8, 4, 4>(a, st);
+    default: return launch_ph_t<16, 8, 8, 4>(a, st);  // preset 9
     }
 }
 

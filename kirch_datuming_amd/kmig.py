@@ -17,7 +17,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libkmig.so")
+LIB_PATH = os.environ.get("KMIG_LIB") or os.path.join(HERE, "lib", "libkmig.so")
 
 KMIG_OK = 0
 ERRORS = {-1: "KMIG_E_PARAM", -2: "KMIG_E_DIM", -3: "KMIG_E_IO", -4: "KMIG_E_DEVICE",
@@ -64,7 +64,7 @@ class KmigStats(C.Structure):
                 ("ms_tuple_prep", C.c_double), ("ms_migrate", C.c_double),
                 ("ms_total", C.c_double), ("migrate_launches", C.c_int),
                 ("receiver_slots", C.c_int), ("shot_batches", C.c_int),
-                ("variant", C.c_int)]
+                ("variant", C.c_int), ("tile", C.c_int)]
 
 
 _LIB = None
