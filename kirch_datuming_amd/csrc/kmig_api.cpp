@@ -73,6 +73,8 @@ struct kmig_ctx {
     size_t pk_s_cap = 0, pk_r_cap = 0;
     float *pairs = nullptr;
     size_t pairs_cap = 0;
+    float *tmpd = nullptr;          // diff2 scratch (diff=1 only)
+    size_t tmpd_cap = 0;
     float *wap = nullptr;
     size_t wap_cap = 0;
     int *rslot = nullptr;
@@ -538,6 +540,7 @@ void kmig_destroy(kmig_ctx *c)
     dfree(c->pk_s);
     dfree(c->pk_r);
     dfree(c->pairs);
+    dfree(c->tmpd);
     dfree(c->wap);
     dfree(c->rslot);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
@@ -767,11 +770,11 @@ int kmig_run(kmig_ctx *c, int sb, int se)
     ta.slope = p.direct_mute_slope;
     ta.intercept = p.direct_mute_intercept;
     ta.taper = p.direct_mute_taper;
-    HIPCHK(c, (hipError_t)launch_trace_prep(ta, c->stream));
-    if (resident)
-        HIPCHK(c, (hipError_t)launch_pairs(c->prep + (size_t)sb * c->nh * c->nt,
-                                           c->pairs + (size_t)sb * c->nh * c->nt * 2,
-                                           (long)(se - sb) * c->nh, c->nt, c->stream));
+    if (p.diff == 1) {
+        if ((rc = ensure(c, &c->tmpd, &c->tmpd_cap, (size_t)c->ns * c->nh * c->nt))) return rc;
+    }
+    HIPCHK(c, (hipError_t)launch_trace_prep2(ta, resident ? c->pairs : nullptr, c->tmpd,
+                                             c->stream));
     hipEvent_t e1 = next_event(c);
     HIPCHK(c, hipEventRecord(e1, c->stream));
     c->ev_kind.push_back(0);

@@ -136,6 +136,9 @@ struct MigResArgs {
 
 // Launchers (kmig_prep.hip, kmig_migrate.hip).  Return 0 or a hipError_t value.
 int launch_trace_prep(const TracePrepArgs &a, void *stream);
+// coalesced trace preparation; pairs (nullable) gets (tr[k], tr[k+1]); tmp is a
+// trace-sized scratch volume used only when diff=1
+int launch_trace_prep2(const TracePrepArgs &a, float *pairs, float *tmp, void *stream);
 int launch_tuple_prep(const TuplePrepArgs &a, Tuples &out, float *tmp1, float *tmp2,
                       void *stream);
 int launch_migrate(const MigArgs &a, int tile, void *stream);

@@ -84,6 +84,136 @@ __global__ __launch_bounds__(256) void k_trace_prep(const TracePrepArgs a)
     }
 }
 
+// Coalesced form of the trace preparation: a block of 64 lanes owns 64
+// traces and walks them in chunks of 64 samples staged through LDS, so every
+// global access is a 256-byte row while each lane still forms its own trace's
+// running sums sequentially (the reference's rounding order).
+//   k_trace_fwd: mute (+QC copies) and the causal sum of doubint
+//   k_trace_bwd: the anticausal sum, written as samples and, when asked for,
+//                as (tr[k], tr[k+1]) pairs
+constexpr int TB = 64;  // traces per block = samples per chunk
+
+__global__ __launch_bounds__(TB) void k_trace_fwd(const TracePrepArgs a)
+{
+    __shared__ float tile[TB][TB + 1];
+    const int t = threadIdx.x;
+    const long ntr = (long)(a.shot_end - a.shot_begin) * a.nh;
+    const long g0 = (long)a.shot_begin * a.nh + (long)blockIdx.x * TB;
+    const int nloc = (int)min((long)TB, ntr - (long)blockIdx.x * TB);
+    const long gt = g0 + t;
+    const bool mine = t < nloc;
+    TraceMeta m{};
+    if (mine) m = a.meta[gt];
+    const float offset_time = a.slope * fabsf(m.rcv_x - m.src_x);
+    const float boundary = a.invert ? a.intercept - offset_time : a.intercept + offset_time;
+    bool any_qc = false;
+    if (a.mute && a.qc_shot >= 0) {
+        const long q0 = (long)a.qc_shot * a.nh, q1 = q0 + a.nh;
+        any_qc = g0 < q1 && g0 + nloc > q0;
+    }
+    float tt = 0.0f;
+    for (int c0 = 0; c0 < a.nt; c0 += TB) {
+        const int nc = min(TB, a.nt - c0);
+        for (int j = 0; j < nloc; j++)
+            if (t < nc) tile[j][t] = a.raw[(g0 + j) * a.nt + c0 + t];
+        __syncthreads();
+        if (any_qc)
+            for (int j = 0; j < nloc; j++) {
+                const long g = g0 + j;
+                if (g / a.nh == a.qc_shot && t < nc)
+                    a.qcb[(size_t)(g % a.nh) * a.nt + c0 + t] = tile[j][t];
+            }
+        if (mine && a.mute) {
+            for (int k = 0; k < nc; k++) {
+                const float time = a.t0 + (float)(c0 + k) * a.dt;
+                float w = 0.0f;
+                if (time >= boundary + a.taper) {
+                    w = 1.0f;
+                } else if (time > boundary) {
+                    const float u = (time - boundary) / a.taper;
+                    w = 0.5f * (1.0f - cosf((float)KPI * u));
+                }
+                tile[t][k] *= w;
+            }
+        }
+        __syncthreads();
+        if (any_qc)
+            for (int j = 0; j < nloc; j++) {
+                const long g = g0 + j;
+                if (g / a.nh == a.qc_shot && t < nc)
+                    a.qca[(size_t)(g % a.nh) * a.nt + c0 + t] = tile[j][t];
+            }
+        if (mine && m.wap >= 0.0f) {
+            for (int k = 0; k < nc; k++) {
+                const float v = tile[t][k];
+                tt = (c0 + k == 0) ? v : tt + v;
+                tile[t][k] = tt;
+            }
+        }
+        __syncthreads();
+        for (int j = 0; j < nloc; j++)
+            if (t < nc) a.prep[(g0 + j) * a.nt + c0 + t] = tile[j][t];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_trace_bwd(const TracePrepArgs a, float2 *pairs)
+{
+    __shared__ float tile[TB][TB + 1];
+    __shared__ float bnd[TB];
+    const int t = threadIdx.x;
+    const long ntr = (long)(a.shot_end - a.shot_begin) * a.nh;
+    const long g0 = (long)a.shot_begin * a.nh + (long)blockIdx.x * TB;
+    const int nloc = (int)min((long)TB, ntr - (long)blockIdx.x * TB);
+    const bool mine = t < nloc && a.meta[g0 + t].wap >= 0.0f;
+    float tt = 0.0f;
+    const int last = ((a.nt - 1) / TB) * TB;
+    for (int c0 = last; c0 >= 0; c0 -= TB) {
+        const int nc = min(TB, a.nt - c0);
+        for (int j = 0; j < nloc; j++)
+            if (t < nc) tile[j][t] = a.prep[(g0 + j) * a.nt + c0 + t];
+        __syncthreads();
+        if (mine) {
+            for (int k = nc - 1; k >= 0; k--) {
+                const int it = c0 + k;
+                if (it == a.nt - 1) {
+                    tt = tile[t][k];
+                } else {
+                    tt += tile[t][k];
+                    tile[t][k] = tt;
+                }
+            }
+        }
+        __syncthreads();
+        for (int j = 0; j < nloc; j++) {
+            if (t < nc) {
+                const float v = tile[j][t];
+                a.prep[(g0 + j) * a.nt + c0 + t] = v;
+                if (pairs) {
+                    const float nx = (t + 1 < nc) ? tile[j][t + 1]
+                                                  : (c0 + t + 1 < a.nt ? bnd[j] : 0.0f);
+                    pairs[(g0 + j) * a.nt + c0 + t] = make_float2(v, nx);
+                }
+            }
+        }
+        __syncthreads();
+        if (t < nloc) bnd[t] = tile[t][0];
+        __syncthreads();
+    }
+}
+
+// diff2 (src/kirchmig2d_cig.cpp:381-394) as an element-wise pass.
+__global__ __launch_bounds__(256) void k_diff2(const float *in, float *out, size_t n, int nt,
+                                               float d)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int it = (int)(k % nt);
+    if (it == 0) out[k] = (in[k + 1] - in[k]) / d;
+    else if (it == nt - 1) out[k] = (in[k] - in[k - 1]) / d;
+    else out[k] = (in[k + 1] - in[k - 1]) / (2.0f * d);
+}
+
 // ---------------------------------------------------------------- tuples --
 
 __global__ __launch_bounds__(256) void k_interp(const TuplePrepArgs a, Tuples o)
@@ -316,6 +446,31 @@ __global__ void k_fill(float *p, size_t n, float v)
 // -------------------------------------------------------------- launchers --
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+int launch_trace_prep2(const TracePrepArgs &a, float *pairs, float *tmp, void *stream)
+{
+    const long ntr = (long)(a.shot_end - a.shot_begin) * a.nh;
+    if (ntr <= 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned nb = (unsigned)((ntr + TB - 1) / TB);
+    hipLaunchKernelGGL(k_trace_fwd, dim3(nb), dim3(TB), 0, st, a);
+    const bool diff = a.diff == 1;
+    hipLaunchKernelGGL(k_trace_bwd, dim3(nb), dim3(TB), 0, st, a,
+                       diff ? (float2 *)nullptr : (float2 *)pairs);
+    if (diff) {
+        // diff2 into tmp, back into prep, then the pairs from the final trace
+        const size_t off = (size_t)a.shot_begin * a.nh * a.nt, n = (size_t)ntr * a.nt;
+        hipLaunchKernelGGL(k_diff2, dim3(nblk(n, 256)), dim3(256), 0, st, a.prep + off,
+                           tmp + off, n, a.nt, a.dt);
+        if (hipMemcpyAsync(a.prep + off, tmp + off, n * sizeof(float), hipMemcpyDeviceToDevice,
+                           st) != hipSuccess)
+            return (int)hipGetLastError();
+        if (pairs)
+            hipLaunchKernelGGL(k_pairs, dim3(nblk(n, 256)), dim3(256), 0, st, a.prep + off,
+                               (float2 *)pairs + off, n, a.nt);
+    }
+    return (int)hipGetLastError();
+}
 
 int launch_trace_prep(const TracePrepArgs &a, void *stream)
 {
