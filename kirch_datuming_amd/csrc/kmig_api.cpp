@@ -449,7 +449,7 @@ int autotune(kmig_ctx *c, const MigResArgs &full)
         float tmin = 1e30f;
         for (int rep = 0; rep < 2; rep++) {
             HIPCHK(c, hipEventRecord(e0, c->stream));
-            HIPCHK(c, (hipError_t)launch_migrate_resident(R, t, c->stream));
+            HIPCHK(c, (hipError_t)launch_migrate_resident_probe(R, t, c->stream));
             HIPCHK(c, hipEventRecord(e1, c->stream));
             HIPCHK(c, hipEventSynchronize(e1));
             float ms = 0.0f;

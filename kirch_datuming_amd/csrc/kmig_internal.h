@@ -148,5 +148,7 @@ int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, floa
                 int angle_mode, void *stream);
 int launch_pairs(const float *prep, float *pairs, long ntraces, int nt, void *stream);
 int launch_migrate_resident(const MigResArgs &a, int tile, void *stream);
+// the same kernels under a distinct symbol, for the autotuner's timing launches
+int launch_migrate_resident_probe(const MigResArgs &a, int tile, void *stream);
 
 }  // namespace kmig
