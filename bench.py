@@ -82,10 +82,12 @@ def main():
     p = kmig.parse_params(kv, int(ax[1][0]), ax[1][1], ax[1][2])
     kmig.check_params(p, int(ax[2][0]))
     m = kmig.Migrator(p)
+    tu = time.perf_counter()
     m.set_tables(0, w.stab, w.sder, w.s_axes)
     m.set_tables(1, w.rtab, w.rder, w.r_axes)
     m.set_data(w.data, ax)
-    log("tables and data resident in HBM")
+    upload_s = time.perf_counter() - tu
+    log(f"tables and data resident in HBM ({upload_s:.2f} s host-to-device)")
 
     for i in range(args.warmup):
         m.run()
@@ -183,6 +185,8 @@ def main():
                          "migrate": float(np.mean(mig_ms)) * launches,
                          "events_total": float(np.mean(tot_ms))},
         "parity": parity,
+        "upload_s": upload_s,
+        "pcie_inclusive_updates_per_s": w.updates / (wall / args.steps + upload_s),
         "cig_abs_max": float(np.abs(cig_gpu).max()),
     }
     m.close()
