@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "image-pt×trace updates/s, batched FP16/NN vs plain FP32 GPU at CIG relL2≤1e-3"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_UPDATE = {"fp32": 16.0, "fp16": 8.0, "bf16": 8.0}
+BYTES_PER_UPDATE = {"fp32": 16.0, "fp16": 12.0, "bf16": 12.0, "u16": 12.0}
 
 
 def log(msg):
@@ -37,7 +37,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="salt", choices=["salt", "small"])
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "u16"])
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--variant", default="auto", choices=["auto", "stream", "resident"])
     ap.add_argument("--cpu-shots", type=int, default=2,

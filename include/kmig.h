@@ -64,8 +64,9 @@ extern "C" {
 
 /* Table precision of the resident traveltime tuples. */
 #define KMIG_PREC_FP32 0
-#define KMIG_PREC_FP16 1
-#define KMIG_PREC_BF16 2
+#define KMIG_PREC_FP16 1   /* 16-bit IEEE half fields, per-slot offset/scale   */
+#define KMIG_PREC_BF16 2   /* 16-bit bfloat16 fields, per-slot offset/scale    */
+#define KMIG_PREC_U16 3    /* 16-bit unsigned-normalised fields, same scaling  */
 
 /* Every key of kirchmig2d_cig's adj=1 path, same names, units and defaults
    (src/kirchmig2d_cig.cpp:520-560, :568-598, :707-717, :816-862, :952). */

@@ -71,6 +71,10 @@ struct kmig_ctx {
     // resident-variant state
     PackedTuple *pk_s = nullptr, *pk_r = nullptr;
     size_t pk_s_cap = 0, pk_r_cap = 0;
+    PackedTuple16 *pk16_s = nullptr, *pk16_r = nullptr;
+    size_t pk16_s_cap = 0, pk16_r_cap = 0;
+    SlotScale *sc_s = nullptr, *sc_r = nullptr;
+    size_t sc_s_cap = 0, sc_r_cap = 0;
     float *pairs = nullptr;
     size_t pairs_cap = 0;
     float *tmpd = nullptr;          // diff2 scratch (diff=1 only)
@@ -397,6 +401,10 @@ MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
     R.nshots = nshots;
     R.stup = c->pk_s;
     R.rtup = c->pk_r;
+    R.stup16 = c->pk16_s;
+    R.rtup16 = c->pk16_r;
+    R.sscale = c->sc_s;
+    R.rscale = c->sc_r;
     R.rslot = c->rslot;
     R.wap = c->wap;
     R.pairs = c->pairs;
@@ -477,6 +485,216 @@ int autotune(kmig_ctx *c, const MigResArgs &full)
     return KMIG_OK;
 }
 
+// Host geometry, allocations and uploads for shots [sb, se); cached while the
+// inputs and the shot range are unchanged.
+int plan_if_needed(kmig_ctx *c, int sb, int se)
+{
+    const kmig_params &p = c->p;
+    const bool angle = p.cig && p.cig_is_angle;
+    int rc;
+    if (c->plan_version != c->geo_version || sb != c->last_sb || se != c->last_se) {
+        std::vector<TraceMeta> meta;
+        std::vector<SlotSpec> sspec, rspec;
+        std::vector<float> wapv;
+        std::vector<int> rslot;
+        if ((rc = plan_run(c, sb, se, meta, sspec, rspec, wapv, rslot))) return rc;
+        if ((rc = check_tables_type_p(c, sspec))) return rc;
+        if ((rc = check_tables_type_p(c, rspec))) return rc;
+        // meta is indexed by global trace number
+        if ((rc = ensure(c, &c->meta, &c->meta_cap, (size_t)c->ns * c->nh))) return rc;
+        if (!meta.empty())
+            HIPCHK(c, hipMemcpy(c->meta + (size_t)sb * c->nh, meta.data(),
+                                meta.size() * sizeof(TraceMeta), hipMemcpyHostToDevice));
+        if ((rc = ensure(c, &c->spec_s, &c->spec_s_cap, std::max<size_t>(1, sspec.size()))))
+            return rc;
+        if ((rc = ensure(c, &c->spec_r, &c->spec_r_cap, std::max<size_t>(1, rspec.size()))))
+            return rc;
+        if (!sspec.empty())
+            HIPCHK(c, hipMemcpy(c->spec_s, sspec.data(), sspec.size() * sizeof(SlotSpec),
+                                hipMemcpyHostToDevice));
+        if (!rspec.empty())
+            HIPCHK(c, hipMemcpy(c->spec_r, rspec.data(), rspec.size() * sizeof(SlotSpec),
+                                hipMemcpyHostToDevice));
+        size_t maxs = 0, maxr = 0;
+        for (const Batch &b : c->batches) {
+            maxs = std::max(maxs, (size_t)(b.s1 - b.s0));
+            maxr = std::max(maxr, (size_t)b.nrcv);
+        }
+        if ((rc = ensure_tuples(c, c->ts, &c->ts_cap, std::max<size_t>(1, maxs)))) return rc;
+        if ((rc = ensure_tuples(c, c->tr, &c->tr_cap, std::max<size_t>(1, maxr)))) return rc;
+        const size_t tslots = std::max<size_t>(1, std::max(maxs, maxr));
+        if (angle && p.angle_smooth > 0) {
+            if ((rc = ensure(c, &c->tmp1, &c->tmp1_cap, tslots * c->nzx))) return rc;
+            if ((rc = ensure(c, &c->tmp2, &c->tmp2_cap, tslots * c->nzx))) return rc;
+        }
+        // resident variant: angle CIG, two-bin interpolation, receivers that do
+        // not move with the shot, one batch, times within the int16 sample range
+        const double cst = ((double)p.tau - (double)c->t0) / (double)c->dt;
+        const bool res_ok = angle && p.angle_interp == 1 && c->rcv_invariant &&
+                            std::fabs(cst) + (double)c->nt < 30000.0;
+        int variant = KMIG_VARIANT_STREAM;
+        if (p.variant == KMIG_VARIANT_RESIDENT) {
+            if (!res_ok)
+                return fail(c, KMIG_E_PARAM,
+                            "variant=resident needs mode=angle, angle_interp=1, receivers that "
+                            "do not move with the shot (cmp=0) and tuples that fit one batch");
+            variant = KMIG_VARIANT_RESIDENT;
+        } else if (p.variant == KMIG_VARIANT_AUTO && res_ok) {
+            variant = KMIG_VARIANT_RESIDENT;
+        }
+        if (p.precision != KMIG_PREC_FP32 && variant != KMIG_VARIANT_RESIDENT)
+            return fail(c, KMIG_E_PARAM,
+                        "precision=fp16/bf16 applies to the resident kernel: it needs "
+                        "mode=angle, angle_interp=1 and receivers that do not move with the "
+                        "shot (cmp=0)");
+        c->variant_used = variant;
+        if (variant == KMIG_VARIANT_RESIDENT && p.precision != KMIG_PREC_FP32) {
+            if ((rc = ensure(c, &c->pk16_s, &c->pk16_s_cap, std::max<size_t>(1, maxs) * c->nzx)))
+                return rc;
+            if ((rc = ensure(c, &c->pk16_r, &c->pk16_r_cap, std::max<size_t>(1, maxr) * c->nzx)))
+                return rc;
+            if ((rc = ensure(c, &c->sc_s, &c->sc_s_cap, std::max<size_t>(1, maxs)))) return rc;
+            if ((rc = ensure(c, &c->sc_r, &c->sc_r_cap, std::max<size_t>(1, maxr)))) return rc;
+        }
+        if (variant == KMIG_VARIANT_RESIDENT && p.precision == KMIG_PREC_FP32) {
+            if ((rc = ensure(c, &c->pk_s, &c->pk_s_cap, std::max<size_t>(1, maxs) * c->nzx)))
+                return rc;
+            if ((rc = ensure(c, &c->pk_r, &c->pk_r_cap, std::max<size_t>(1, maxr) * c->nzx)))
+                return rc;
+        }
+        if (variant == KMIG_VARIANT_RESIDENT) {
+            if ((rc = ensure(c, &c->pairs, &c->pairs_cap, (size_t)c->ns * c->nh * c->nt * 2)))
+                return rc;
+            if ((rc = ensure(c, &c->wap, &c->wap_cap, (size_t)c->ns * c->nh))) return rc;
+            if ((rc = ensure(c, &c->rslot, &c->rslot_cap, (size_t)c->nh))) return rc;
+            if (!wapv.empty())
+                HIPCHK(c, hipMemcpy(c->wap + (size_t)sb * c->nh, wapv.data(),
+                                    wapv.size() * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(c->rslot, rslot.data(), rslot.size() * sizeof(int),
+                                hipMemcpyHostToDevice));
+        }
+        c->plan_version = c->geo_version;
+        c->last_sb = sb;
+        c->last_se = se;
+    }
+    return KMIG_OK;
+}
+
+// CIG and QC buffers of a run over shots [sb, se).
+int ensure_outputs(kmig_ctx *c, int sb, int se)
+{
+    const kmig_params &p = c->p;
+    const int ng = p.cig ? p.ng : 1;
+    int rc;
+    if ((rc = ensure(c, &c->cig, &c->cig_cap, (size_t)ng * c->nzx))) return rc;
+    const bool want_qc = p.direct_mute && p.direct_mute_qc_shot >= sb && p.direct_mute_qc_shot < se;
+    if (want_qc) {
+        size_t cap2 = c->qc_cap;
+        if ((rc = ensure(c, &c->qcb, &c->qc_cap, (size_t)c->nt * c->nh))) return rc;
+        if ((rc = ensure(c, &c->qca, &cap2, (size_t)c->nt * c->nh))) return rc;
+    }
+    c->qc_valid = want_qc;
+    return KMIG_OK;
+}
+
+// CIG clear and trace preparation of shots [sb, se) on stream st.
+int enqueue_trace_prep(kmig_ctx *c, hipStream_t st, int sb, int se)
+{
+    const kmig_params &p = c->p;
+    const int ng = p.cig ? p.ng : 1;
+    const bool resident = c->variant_used == KMIG_VARIANT_RESIDENT;
+    int rc;
+
+    HIPCHK(c, (hipError_t)launch_fill(c->cig, (size_t)ng * c->nzx, 0.0f, st));
+
+    TracePrepArgs ta{};
+    ta.raw = c->raw;
+    ta.prep = c->prep;
+    ta.meta = c->meta;
+    ta.qcb = c->qcb;
+    ta.qca = c->qca;
+    ta.nt = c->nt;
+    ta.nh = c->nh;
+    ta.shot_begin = sb;
+    ta.shot_end = se;
+    ta.t0 = c->t0;
+    ta.dt = c->dt;
+    ta.mute = p.direct_mute;
+    ta.invert = p.direct_mute_invert;
+    ta.qc_shot = c->qc_valid ? p.direct_mute_qc_shot : -1;
+    ta.diff = p.diff;
+    ta.slope = p.direct_mute_slope;
+    ta.intercept = p.direct_mute_intercept;
+    ta.taper = p.direct_mute_taper;
+    if (p.diff == 1) {
+        if ((rc = ensure(c, &c->tmpd, &c->tmpd_cap, (size_t)c->ns * c->nh * c->nt))) return rc;
+    }
+    HIPCHK(c, (hipError_t)launch_trace_prep2(ta, resident ? c->pairs : nullptr, c->tmpd,
+                                             st));
+    return KMIG_OK;
+}
+
+// Tuple preparation (and packing) of one shot batch on stream st.
+int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_s_off)
+{
+    const kmig_params &p = c->p;
+    const bool angle = p.cig && p.cig_is_angle;
+    const bool gfiles = c->side[0].GX != nullptr;
+    const bool resident = c->variant_used == KMIG_VARIANT_RESIDENT;
+    const float dh = (c->nh == 1) ? 0.0f : c->dh;
+    const float ds = (c->ns == 1) ? 0.0f : c->ds;
+    const int nsb = b.s1 - b.s0;
+    TuplePrepArgs A{};
+    A.nz = c->nz;
+    A.nx = c->nx;
+    A.nzx = c->nzx;
+    A.x0 = c->x0;
+    A.dx = c->dx;
+    A.dz = c->dz;
+    A.dd = c->dx;
+    A.type = p.interp_type;
+    A.use_grad_files = gfiles ? 1 : 0;
+    A.radius = p.angle_grad_radius;
+    A.smooth = p.angle_smooth;
+    A.angle_mode = angle ? 1 : 0;
+    A.aper = p.aperture;
+    A.aper_taper = p.aperture_taper;
+    A.rmin = p.aperture_rmin;
+    A.rtaper = p.aperture_rtaper;
+    // source side
+    A.tab = c->side[0].T;
+    A.der = c->side[0].D;
+    A.gxv = c->side[0].GX;
+    A.gzv = c->side[0].GZ;
+    A.spec = c->spec_s + spec_s_off;
+    A.nslots = nsb;
+    A.dpos = c->side[0].d;
+    A.dside = ds;
+    HIPCHK(c, (hipError_t)launch_tuple_prep(A, c->ts, c->tmp1, c->tmp2, st));
+    // receiver side
+    A.tab = c->side[1].T;
+    A.der = c->side[1].D;
+    A.gxv = c->side[1].GX;
+    A.gzv = c->side[1].GZ;
+    A.spec = c->spec_r + b.spec_r0;
+    A.nslots = b.nrcv;
+    A.dpos = c->side[1].d;
+    A.dside = dh;
+    HIPCHK(c, (hipError_t)launch_tuple_prep(A, c->tr, c->tmp1, c->tmp2, st));
+    if (resident && p.precision == KMIG_PREC_FP32) {
+        HIPCHK(c, (hipError_t)launch_pack(c->ts, c->pk_s, nsb, c->nzx, c->dt, angle,
+                                          st));
+        HIPCHK(c, (hipError_t)launch_pack(c->tr, c->pk_r, b.nrcv, c->nzx, c->dt, angle,
+                                          st));
+    } else if (resident) {
+        HIPCHK(c, (hipError_t)launch_pack16(c->ts, c->pk16_s, c->sc_s, nsb, c->nzx, c->dt,
+                                            angle, p.precision, st));
+        HIPCHK(c, (hipError_t)launch_pack16(c->tr, c->pk16_r, c->sc_r, b.nrcv, c->nzx, c->dt,
+                                            angle, p.precision, st));
+    }
+    return KMIG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -539,6 +757,10 @@ void kmig_destroy(kmig_ctx *c)
     dfree(c->qca);
     dfree(c->pk_s);
     dfree(c->pk_r);
+    dfree(c->pk16_s);
+    dfree(c->pk16_r);
+    dfree(c->sc_s);
+    dfree(c->sc_r);
     dfree(c->pairs);
     dfree(c->tmpd);
     dfree(c->wap);
@@ -666,170 +888,23 @@ int kmig_run(kmig_ctx *c, int sb, int se)
     c->ev_used = 0;
     c->ev_kind.clear();
 
-    // host geometry (cached while the inputs and the shot range are unchanged)
-    if (c->plan_version != c->geo_version || sb != c->last_sb || se != c->last_se) {
-        std::vector<TraceMeta> meta;
-        std::vector<SlotSpec> sspec, rspec;
-        std::vector<float> wapv;
-        std::vector<int> rslot;
-        if ((rc = plan_run(c, sb, se, meta, sspec, rspec, wapv, rslot))) return rc;
-        if ((rc = check_tables_type_p(c, sspec))) return rc;
-        if ((rc = check_tables_type_p(c, rspec))) return rc;
-        // meta is indexed by global trace number
-        if ((rc = ensure(c, &c->meta, &c->meta_cap, (size_t)c->ns * c->nh))) return rc;
-        if (!meta.empty())
-            HIPCHK(c, hipMemcpy(c->meta + (size_t)sb * c->nh, meta.data(),
-                                meta.size() * sizeof(TraceMeta), hipMemcpyHostToDevice));
-        if ((rc = ensure(c, &c->spec_s, &c->spec_s_cap, std::max<size_t>(1, sspec.size()))))
-            return rc;
-        if ((rc = ensure(c, &c->spec_r, &c->spec_r_cap, std::max<size_t>(1, rspec.size()))))
-            return rc;
-        if (!sspec.empty())
-            HIPCHK(c, hipMemcpy(c->spec_s, sspec.data(), sspec.size() * sizeof(SlotSpec),
-                                hipMemcpyHostToDevice));
-        if (!rspec.empty())
-            HIPCHK(c, hipMemcpy(c->spec_r, rspec.data(), rspec.size() * sizeof(SlotSpec),
-                                hipMemcpyHostToDevice));
-        size_t maxs = 0, maxr = 0;
-        for (const Batch &b : c->batches) {
-            maxs = std::max(maxs, (size_t)(b.s1 - b.s0));
-            maxr = std::max(maxr, (size_t)b.nrcv);
-        }
-        if ((rc = ensure_tuples(c, c->ts, &c->ts_cap, std::max<size_t>(1, maxs)))) return rc;
-        if ((rc = ensure_tuples(c, c->tr, &c->tr_cap, std::max<size_t>(1, maxr)))) return rc;
-        const size_t tslots = std::max<size_t>(1, std::max(maxs, maxr));
-        if (angle && p.angle_smooth > 0) {
-            if ((rc = ensure(c, &c->tmp1, &c->tmp1_cap, tslots * c->nzx))) return rc;
-            if ((rc = ensure(c, &c->tmp2, &c->tmp2_cap, tslots * c->nzx))) return rc;
-        }
-        // resident variant: angle CIG, two-bin interpolation, receivers that do
-        // not move with the shot, one batch, times within the int16 sample range
-        const double cst = ((double)p.tau - (double)c->t0) / (double)c->dt;
-        const bool res_ok = angle && p.angle_interp == 1 && c->rcv_invariant &&
-                            std::fabs(cst) + (double)c->nt < 30000.0;
-        int variant = KMIG_VARIANT_STREAM;
-        if (p.variant == KMIG_VARIANT_RESIDENT) {
-            if (!res_ok)
-                return fail(c, KMIG_E_PARAM,
-                            "variant=resident needs mode=angle, angle_interp=1, receivers that "
-                            "do not move with the shot (cmp=0) and tuples that fit one batch");
-            variant = KMIG_VARIANT_RESIDENT;
-        } else if (p.variant == KMIG_VARIANT_AUTO && res_ok) {
-            variant = KMIG_VARIANT_RESIDENT;
-        }
-        c->variant_used = variant;
-        if (variant == KMIG_VARIANT_RESIDENT) {
-            if ((rc = ensure(c, &c->pk_s, &c->pk_s_cap, std::max<size_t>(1, maxs) * c->nzx)))
-                return rc;
-            if ((rc = ensure(c, &c->pk_r, &c->pk_r_cap, std::max<size_t>(1, maxr) * c->nzx)))
-                return rc;
-            if ((rc = ensure(c, &c->pairs, &c->pairs_cap, (size_t)c->ns * c->nh * c->nt * 2)))
-                return rc;
-            if ((rc = ensure(c, &c->wap, &c->wap_cap, (size_t)c->ns * c->nh))) return rc;
-            if ((rc = ensure(c, &c->rslot, &c->rslot_cap, (size_t)c->nh))) return rc;
-            if (!wapv.empty())
-                HIPCHK(c, hipMemcpy(c->wap + (size_t)sb * c->nh, wapv.data(),
-                                    wapv.size() * sizeof(float), hipMemcpyHostToDevice));
-            HIPCHK(c, hipMemcpy(c->rslot, rslot.data(), rslot.size() * sizeof(int),
-                                hipMemcpyHostToDevice));
-        }
-        c->plan_version = c->geo_version;
-        c->last_sb = sb;
-        c->last_se = se;
-    }
+    if ((rc = plan_if_needed(c, sb, se))) return rc;
     const bool resident = c->variant_used == KMIG_VARIANT_RESIDENT;
-    if ((rc = ensure(c, &c->cig, &c->cig_cap, (size_t)ng * c->nzx))) return rc;
-    const bool want_qc = p.direct_mute && p.direct_mute_qc_shot >= sb && p.direct_mute_qc_shot < se;
-    if (want_qc) {
-        size_t cap2 = c->qc_cap;
-        if ((rc = ensure(c, &c->qcb, &c->qc_cap, (size_t)c->nt * c->nh))) return rc;
-        if ((rc = ensure(c, &c->qca, &cap2, (size_t)c->nt * c->nh))) return rc;
-    }
-    c->qc_valid = want_qc;
-
+    if ((rc = ensure_outputs(c, sb, se))) return rc;
     hipEvent_t e0 = next_event(c);
     HIPCHK(c, hipEventRecord(e0, c->stream));
-    HIPCHK(c, (hipError_t)launch_fill(c->cig, (size_t)ng * c->nzx, 0.0f, c->stream));
-
-    TracePrepArgs ta{};
-    ta.raw = c->raw;
-    ta.prep = c->prep;
-    ta.meta = c->meta;
-    ta.qcb = c->qcb;
-    ta.qca = c->qca;
-    ta.nt = c->nt;
-    ta.nh = c->nh;
-    ta.shot_begin = sb;
-    ta.shot_end = se;
-    ta.t0 = c->t0;
-    ta.dt = c->dt;
-    ta.mute = p.direct_mute;
-    ta.invert = p.direct_mute_invert;
-    ta.qc_shot = want_qc ? p.direct_mute_qc_shot : -1;
-    ta.diff = p.diff;
-    ta.slope = p.direct_mute_slope;
-    ta.intercept = p.direct_mute_intercept;
-    ta.taper = p.direct_mute_taper;
-    if (p.diff == 1) {
-        if ((rc = ensure(c, &c->tmpd, &c->tmpd_cap, (size_t)c->ns * c->nh * c->nt))) return rc;
-    }
-    HIPCHK(c, (hipError_t)launch_trace_prep2(ta, resident ? c->pairs : nullptr, c->tmpd,
-                                             c->stream));
+    if ((rc = enqueue_trace_prep(c, c->stream, sb, se))) return rc;
     hipEvent_t e1 = next_event(c);
     HIPCHK(c, hipEventRecord(e1, c->stream));
     c->ev_kind.push_back(0);
 
-    const float dh = (c->nh == 1) ? 0.0f : c->dh;
-    const float ds = (c->ns == 1) ? 0.0f : c->ds;
     size_t spec_s_off = 0;
     int launches = 0;
     for (const Batch &b : c->batches) {
         const int nsb = b.s1 - b.s0;
-        TuplePrepArgs A{};
-        A.nz = c->nz;
-        A.nx = c->nx;
-        A.nzx = c->nzx;
-        A.x0 = c->x0;
-        A.dx = c->dx;
-        A.dz = c->dz;
-        A.dd = c->dx;
-        A.type = p.interp_type;
-        A.use_grad_files = gfiles ? 1 : 0;
-        A.radius = p.angle_grad_radius;
-        A.smooth = p.angle_smooth;
-        A.angle_mode = angle ? 1 : 0;
-        A.aper = p.aperture;
-        A.aper_taper = p.aperture_taper;
-        A.rmin = p.aperture_rmin;
-        A.rtaper = p.aperture_rtaper;
-        // source side
-        A.tab = c->side[0].T;
-        A.der = c->side[0].D;
-        A.gxv = c->side[0].GX;
-        A.gzv = c->side[0].GZ;
-        A.spec = c->spec_s + spec_s_off;
-        A.nslots = nsb;
-        A.dpos = c->side[0].d;
-        A.dside = ds;
         hipEvent_t a0 = next_event(c);
         HIPCHK(c, hipEventRecord(a0, c->stream));
-        HIPCHK(c, (hipError_t)launch_tuple_prep(A, c->ts, c->tmp1, c->tmp2, c->stream));
-        // receiver side
-        A.tab = c->side[1].T;
-        A.der = c->side[1].D;
-        A.gxv = c->side[1].GX;
-        A.gzv = c->side[1].GZ;
-        A.spec = c->spec_r + b.spec_r0;
-        A.nslots = b.nrcv;
-        A.dpos = c->side[1].d;
-        A.dside = dh;
-        HIPCHK(c, (hipError_t)launch_tuple_prep(A, c->tr, c->tmp1, c->tmp2, c->stream));
-        if (resident) {
-            HIPCHK(c, (hipError_t)launch_pack(c->ts, c->pk_s, nsb, c->nzx, c->dt, angle,
-                                              c->stream));
-            HIPCHK(c, (hipError_t)launch_pack(c->tr, c->pk_r, b.nrcv, c->nzx, c->dt, angle,
-                                              c->stream));
-        }
+        if ((rc = enqueue_tuple_prep(c, c->stream, b, spec_s_off))) return rc;
         hipEvent_t a1 = next_event(c);
         HIPCHK(c, hipEventRecord(a1, c->stream));
         c->ev_kind.push_back(1);
@@ -877,7 +952,12 @@ int kmig_run(kmig_ctx *c, int sb, int se)
         }
         if (migrate_lds_bytes(ng, tile) > 160 * 1024)
             return fail(c, KMIG_E_PARAM, "output axis of %d samples does not fit in LDS", ng);
-        if (nsb > 0 && resident) {
+        if (nsb > 0 && resident && p.precision != KMIG_PREC_FP32) {
+            MigResArgs R = res_args(c, b.s0, nsb, ng);
+            if (tile <= 0 || migrate_lds_bytes(ng, tile) > 64 * 1024) tile = 15;
+            HIPCHK(c, (hipError_t)launch_migrate_resident16(R, tile, p.precision, c->stream));
+            launches++;
+        } else if (nsb > 0 && resident) {
             MigResArgs R = res_args(c, b.s0, nsb, ng);
             if (p.tile <= 0) {
                 if ((rc = autotune(c, R))) return rc;

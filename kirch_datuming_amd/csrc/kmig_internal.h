@@ -114,12 +114,34 @@ struct PackedTuple {
     uint32_t iw;
 };
 
+// 12-byte tuple of the 16-bit precision modes (FP16, BF16 or UNORM16 fields):
+//   iw  as PackedTuple::iw (int16 integer sample | W * 2^15)
+//   fa  bits 15..0: fractional sample part f in [0,1)
+//       bits 31..16: a / a_scale of the slot, in [0,1]
+//   th  gradient direction in FP32: the opening-angle bins need ~20 bits of it
+//       (16-bit directions, even UNORM16 over the slot's range, leave CIG errors
+//       of 2e-3 .. 8e-2 relL2 against the FP64 oracle)
+// UNORM16 stores f as round(f * 65536) (65535 max) and a/a_scale as
+// round(x * 65535).
+struct PackedTuple16 {
+    uint32_t iw, fa, th;
+};
+
+// Per-slot scale of the 16-bit anti-alias field (its maximum over the slot).
+struct SlotScale {
+    float a_scale;
+};
+
 struct MigResArgs {
     int nz, nx;
     size_t nzx;
     int nt, nh, shot0, nshots;
-    const PackedTuple *stup;     // [nshots][nzx] (batch-local shots)
+    const PackedTuple *stup;     // [nshots][nzx] (batch-local shots), FP32 mode
     const PackedTuple *rtup;     // [slots][nzx]
+    const PackedTuple16 *stup16; // the same in the 16-bit modes
+    const PackedTuple16 *rtup16;
+    const SlotScale *sscale;     // [nshots] (batch-local), 16-bit modes
+    const SlotScale *rscale;     // [slots]
     const int *rslot;            // [nh] receiver slot of each receiver index (-1: unused)
     const float *wap;            // [ns][nh] trace-aperture weight, < 0: trace skipped
     const float *pairs;          // [ns][nh][nt][2] (tr[k], tr[k+1])
@@ -134,6 +156,14 @@ struct MigResArgs {
     float taper_start, taper_end, inv_taper;
 };
 
+constexpr int KMIG_MAX_PROBLEMS = 8;
+
+struct MigMultiArgs {
+    int nprob;
+    int tile0[KMIG_MAX_PROBLEMS + 1];  // first block of each problem; tile0[nprob] = grid
+    MigResArgs prob[KMIG_MAX_PROBLEMS];
+};
+
 // Launchers (kmig_prep.hip, kmig_migrate.hip).  Return 0 or a hipError_t value.
 int launch_trace_prep(const TracePrepArgs &a, void *stream);
 // coalesced trace preparation; pairs (nullable) gets (tr[k], tr[k+1]); tmp is a
@@ -146,8 +176,13 @@ int migrate_lds_bytes(int ng, int tile);
 int launch_fill(float *p, size_t n, float v, void *stream);
 int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, float dt,
                 int angle_mode, void *stream);
+int launch_pack16(const Tuples &in, PackedTuple16 *out, SlotScale *scale, int nslots,
+                  size_t nzx, float dt, int angle_mode, int precision, void *stream);
 int launch_pairs(const float *prep, float *pairs, long ntraces, int nt, void *stream);
 int launch_migrate_resident(const MigResArgs &a, int tile, void *stream);
+int launch_migrate_resident16(const MigResArgs &a, int tile, int precision, void *stream);
+int launch_migrate_multi(const MigMultiArgs &m, int tile, int precision, void *stream);
+int migrate_tiles(int nz, int nx, int tile);
 // the same kernels under a distinct symbol, for the autotuner's timing launches
 int launch_migrate_resident_probe(const MigResArgs &a, int tile, void *stream);
 

@@ -212,6 +212,33 @@ struct Pend {
     int ig0, it, itm, itp;
 };
 
+// Tuple loaders: FP32 (Fmt 0) or 16-bit fields decoded to the FP32 layout
+// with the slot's offset and scale (Fmt 1: IEEE half, 2: bfloat16,
+// 3: UNORM16).  The update arithmetic is the same in every mode; only the
+// stored precision differs.
+__device__ __forceinline__ float from16(uint32_t h, int fmt)
+{
+    if (fmt == 2) return __uint_as_float(h << 16);
+    if (fmt == 3) return (float)h * (1.0f / 65535.0f);
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)h);
+}
+
+template <int Fmt>
+__device__ __forceinline__ PackedTuple load_tuple(const MigResArgs &a, bool src, size_t slot,
+                                                  size_t i)
+{
+    const size_t idx = slot * a.nzx + i;
+    if (Fmt == 0) return src ? a.stup[idx] : a.rtup[idx];
+    const PackedTuple16 q = src ? a.stup16[idx] : a.rtup16[idx];
+    const SlotScale sc = src ? a.sscale[slot] : a.rscale[slot];
+    PackedTuple t;
+    t.iw = q.iw;
+    t.frac = Fmt == 3 ? (float)(q.fa & 0xFFFFu) * (1.0f / 65536.0f) : from16(q.fa & 0xFFFFu, Fmt);
+    t.a = from16(q.fa >> 16, Fmt) * sc.a_scale;
+    t.th = __uint_as_float(q.th);
+    return t;
+}
+
 // Opening angle (degrees) from the two packed gradient directions.
 __device__ __forceinline__ float pair_angle(const PackedTuple &s, const PackedTuple &r,
                                            int half_angle)
@@ -278,14 +305,13 @@ __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, const PackedTup
 
 // Probe = true only renames the kernel for the autotuner's timing launches,
 // so profiler statistics of the production kernel cover production launches.
-template <int TZ, int TX, int R, int G, bool Probe>
-__global__ __launch_bounds__(TZ *TX) void k_migrate_ph(const MigResArgs a)
+// One tile of TZ x TX image points of one problem.
+template <int TZ, int TX, int R, int G, int Fmt>
+__device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, float *acc)
 {
     constexpr int P = TZ * TX;
-    extern __shared__ float acc[];
     const int lid = threadIdx.x;
     const int ntz = (a.nz + TZ - 1) / TZ;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int bz = tile % ntz, bx = tile / ntz;
     const int iz = bz * TZ + lid % TZ, ix = bx * TX + lid / TZ;
     const bool valid = iz < a.nz && ix < a.nx;
@@ -304,16 +330,16 @@ __global__ __launch_bounds__(TZ *TX) void k_migrate_ph(const MigResArgs a)
             const int ih = r0 + r;
             const int slot = ih < a.nh ? a.rslot[ih] : -1;
             if (slot >= 0) {
-                rt[r] = a.rtup[(size_t)slot * a.nzx + i];
+                rt[r] = load_tuple<Fmt>(a, false, (size_t)slot, i);
             } else {
                 rt[r].frac = rt[r].a = rt[r].th = 0.0f;
                 rt[r].iw = 0u;
             }
         }
-        PackedTuple nxt = a.stup[i];
+        PackedTuple nxt = load_tuple<Fmt>(a, true, 0, i);
         for (int js = 0; js < a.nshots; js++) {
             const PackedTuple st = nxt;
-            if (js + 1 < a.nshots) nxt = a.stup[(size_t)(js + 1) * a.nzx + i];
+            if (js + 1 < a.nshots) nxt = load_tuple<Fmt>(a, true, (size_t)(js + 1), i);
             const bool live = valid && (st.iw & 0xFFFFu) != 0u;
             if (__ballot(live) == 0) continue;
             const int is = a.shot0 + js;
@@ -370,6 +396,26 @@ __global__ __launch_bounds__(TZ *TX) void k_migrate_ph(const MigResArgs a)
         for (int g = 0; g < a.ng; g++) a.cig[(size_t)g * a.nzx + i] = my[g * P];
 }
 
+template <int TZ, int TX, int R, int G, bool Probe, int Fmt>
+__global__ __launch_bounds__(TZ *TX) void k_migrate_ph(const MigResArgs a)
+{
+    extern __shared__ float acc[];
+    migrate_tile<TZ, TX, R, G, Fmt>(a, xcd_tile(blockIdx.x, gridDim.x), acc);
+}
+
+// Several independent problems (e.g. the depth layers of a recursive
+// workflow, each with its own grid, shots, receivers, mute and tuples) in one
+// launch: blocks [tile0[p], tile0[p+1]) belong to problem p.  Problems are
+// listed heaviest first, so the longest blocks start first.
+template <int TZ, int TX, int R, int G, int Fmt>
+__global__ __launch_bounds__(TZ *TX) void k_migrate_multi(const MigMultiArgs m)
+{
+    extern __shared__ float acc[];
+    int p = 0;
+    while (p + 1 < m.nprob && (int)blockIdx.x >= m.tile0[p + 1]) p++;
+    migrate_tile<TZ, TX, R, G, Fmt>(m.prob[p], (int)blockIdx.x - m.tile0[p], acc);
+}
+
 static void tile_dims(int tile, int *tz, int *tx)
 {
     switch (tile) {
@@ -404,18 +450,18 @@ static int launch_mig_t(const MigArgs &a, hipStream_t st)
     return (int)hipGetLastError();
 }
 
-template <int TZ, int TX, int R, int G, bool Probe = false>
+template <int TZ, int TX, int R, int G, bool Probe = false, int Fmt = 0>
 static int launch_ph_t(const MigResArgs &a, hipStream_t st)
 {
     const int ntz = (a.nz + TZ - 1) / TZ, ntx = (a.nx + TX - 1) / TX;
     const size_t lds = (size_t)a.ng * TZ * TX * sizeof(float);
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_ph<TZ, TX, R, G, Probe>,
+        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_ph<TZ, TX, R, G, Probe, Fmt>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL((k_migrate_ph<TZ, TX, R, G, Probe>), dim3((unsigned)(ntz * ntx)), dim3(TZ * TX),
+    hipLaunchKernelGGL((k_migrate_ph<TZ, TX, R, G, Probe, Fmt>), dim3((unsigned)(ntz * ntx)), dim3(TZ * TX),
                        lds, st, a);
     return (int)hipGetLastError();
 }
@@ -446,6 +492,58 @@ int launch_migrate_resident(const MigResArgs &a, int tile, void *stream)
 int launch_migrate_resident_probe(const MigResArgs &a, int tile, void *stream)
 {
     return launch_res_preset<true>(a, tile, (hipStream_t)stream);
+}
+
+int migrate_tiles(int nz, int nx, int tile)
+{
+    int tz, tx;
+    tile_dims(tile, &tz, &tx);
+    return ((nz + tz - 1) / tz) * ((nx + tx - 1) / tx);
+}
+
+template <int TZ, int TX, int Fmt>
+static int launch_multi_t(const MigMultiArgs &m, hipStream_t st)
+{
+    const size_t lds = (size_t)m.prob[0].ng * TZ * TX * sizeof(float);
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_multi<TZ, TX, 8, 4, Fmt>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)lds);
+        if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL((k_migrate_multi<TZ, TX, 8, 4, Fmt>), dim3((unsigned)m.tile0[m.nprob]),
+                       dim3(TZ * TX), lds, st, m);
+    return (int)hipGetLastError();
+}
+
+int launch_migrate_multi(const MigMultiArgs &m, int tile, int precision, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    const bool small = tile == 15 || tile == 13 || tile == 21 || tile == 1;
+    switch (precision) {
+    case KMIG_PREC_FP16:
+        return small ? launch_multi_t<8, 8, 1>(m, st) : launch_multi_t<16, 8, 1>(m, st);
+    case KMIG_PREC_BF16:
+        return small ? launch_multi_t<8, 8, 2>(m, st) : launch_multi_t<16, 8, 2>(m, st);
+    case KMIG_PREC_U16:
+        return small ? launch_multi_t<8, 8, 3>(m, st) : launch_multi_t<16, 8, 3>(m, st);
+    default:
+        return small ? launch_multi_t<8, 8, 0>(m, st) : launch_multi_t<16, 8, 0>(m, st);
+    }
+}
+
+// 16-bit modes: the default preset shape only
+int launch_migrate_resident16(const MigResArgs &a, int tile, int precision, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    const bool small = tile == 15 || tile == 13 || tile == 21 || tile == 1;
+    if (precision == KMIG_PREC_BF16)
+        return small ? launch_ph_t<8, 8, 8, 4, false, 2>(a, st)
+                     : launch_ph_t<16, 8, 8, 4, false, 2>(a, st);
+    if (precision == KMIG_PREC_U16)
+        return small ? launch_ph_t<8, 8, 8, 4, false, 3>(a, st)
+                     : launch_ph_t<16, 8, 8, 4, false, 3>(a, st);
+    return small ? launch_ph_t<8, 8, 8, 4, false, 1>(a, st) : launch_ph_t<16, 8, 8, 4, false, 1>(a, st);
 }
 
 int launch_migrate(const MigArgs &a, int tile, void *stream)

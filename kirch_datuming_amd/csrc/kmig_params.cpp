@@ -193,8 +193,9 @@ int parse_kv(kmig_params *p, const KV &kv, int nh, float h0, float dh, char *err
         if (ieq(t, "fp32")) p->precision = KMIG_PREC_FP32;
         else if (ieq(t, "fp16")) p->precision = KMIG_PREC_FP16;
         else if (ieq(t, "bf16")) p->precision = KMIG_PREC_BF16;
+        else if (ieq(t, "u16")) p->precision = KMIG_PREC_U16;
         else {
-            set_err(err, errlen, "Unknown precision: %s. Supported are fp32, fp16, bf16.",
+            set_err(err, errlen, "Unknown precision: %s. Supported are fp32, fp16, bf16, u16.",
                     t.c_str());
             return KMIG_E_PARAM;
         }
@@ -334,7 +335,7 @@ extern "C" int kmig_params_check(const kmig_params *p, int ns, char *err, int er
         set_err(err, errlen, "Unknown kernel variant %d.", p->variant);
         return KMIG_E_PARAM;
     }
-    if (p->precision < KMIG_PREC_FP32 || p->precision > KMIG_PREC_BF16) {
+    if (p->precision < KMIG_PREC_FP32 || p->precision > KMIG_PREC_U16) {
         set_err(err, errlen, "Unknown precision mode %d.", p->precision);
         return KMIG_E_PARAM;
     }

@@ -428,6 +428,72 @@ __global__ __launch_bounds__(256) void k_pack(const float *T, const float *A, co
     out[k] = q;
 }
 
+// 16-bit fields of PackedTuple16: IEEE half (round to nearest), bfloat16
+// (round to nearest even) or UNORM16 of a value in [0,1] (round to nearest).
+__device__ inline uint32_t to16(float v, int precision)
+{
+    if (precision == KMIG_PREC_BF16) {
+        uint32_t b = __float_as_uint(v);
+        b += 0x7FFFu + ((b >> 16) & 1u);
+        return b >> 16;
+    }
+    if (precision == KMIG_PREC_U16) {
+        v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+        return (uint32_t)__float2uint_rn(v * 65535.0f);
+    }
+    const _Float16 h = (_Float16)v;
+    return (uint32_t)__builtin_bit_cast(unsigned short, h);
+}
+
+// Per-slot maximum of the anti-alias term over points with W > 0: the scale
+// of its 16-bit field.
+__global__ __launch_bounds__(256) void k_slot_range(const float *A, const float *W,
+                                                    SlotScale *out, size_t nzx, double inv_dt)
+{
+    __shared__ float samax[256];
+    const int slot = blockIdx.x;
+    float amax = 0.0f;
+    for (size_t i = threadIdx.x; i < nzx; i += blockDim.x) {
+        const size_t k = (size_t)slot * nzx + i;
+        if (W[k] > 0.0f) amax = fmaxf(amax, (float)((double)A[k] * inv_dt));
+    }
+    samax[threadIdx.x] = amax;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            samax[threadIdx.x] = fmaxf(samax[threadIdx.x], samax[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[slot].a_scale = samax[0] > 0.0f ? samax[0] : 1.0f;
+}
+
+__global__ __launch_bounds__(256) void k_pack16(const float *T, const float *A, const float *GX,
+                                                const float *GZ, const float *W,
+                                                const SlotScale *scale, PackedTuple16 *out,
+                                                size_t nzx, size_t n, double inv_dt,
+                                                int angle_mode, int precision)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const SlotScale sc = scale[k / nzx];
+    const double tt = (double)T[k] * inv_dt;
+    double ip = floor(tt);
+    if (ip > 32767.0) ip = 32767.0;
+    if (ip < -32768.0) ip = -32768.0;
+    float w = W[k];
+    w = w < 0.0f ? 0.0f : (w > 1.0f ? 1.0f : w);
+    PackedTuple16 q;
+    q.iw = ((uint32_t)(uint16_t)(int16_t)(int)ip << 16) | (uint32_t)__float2uint_rn(w * 32768.0f);
+    const float f = (float)(tt - ip);
+    const uint32_t fq = precision == KMIG_PREC_U16
+                            ? (uint32_t)min(65535.0, rint((tt - ip) * 65536.0))
+                            : to16(f, precision);
+    const float an = (float)((double)A[k] * inv_dt) / sc.a_scale;
+    q.fa = fq | (to16(an, precision) << 16);
+    q.th = angle_mode ? __float_as_uint(atan2f(GX[k], GZ[k])) : 0u;
+    out[k] = q;
+}
+
 // (tr[k], tr[k+1]) pairs, so a kirmig_pick tap pair is one 8-byte load.
 __global__ __launch_bounds__(256) void k_pairs(const float *prep, float2 *pairs, size_t n, int nt)
 {
@@ -514,6 +580,20 @@ int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, floa
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_pack, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, in.T, in.A,
                        in.GX, in.GZ, in.W, out, n, 1.0 / (double)dt, angle_mode);
+    return (int)hipGetLastError();
+}
+
+int launch_pack16(const Tuples &in, PackedTuple16 *out, SlotScale *scale, int nslots,
+                  size_t nzx, float dt, int angle_mode, int precision, void *stream)
+{
+    const size_t n = (size_t)nslots * nzx;
+    if (n == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_slot_range, dim3((unsigned)nslots), dim3(256), 0, st, in.A, in.W, scale,
+                       nzx, 1.0 / (double)dt);
+    hipLaunchKernelGGL(k_pack16, dim3(nblk(n, 256)), dim3(256), 0, st, in.T, in.A, in.GX, in.GZ,
+                       in.W, (const SlotScale *)scale, out, nzx, n, 1.0 / (double)dt, angle_mode,
+                       precision);
     return (int)hipGetLastError();
 }
 

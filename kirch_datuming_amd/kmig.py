@@ -22,7 +22,7 @@ LIB_PATH = os.environ.get("KMIG_LIB") or os.path.join(HERE, "lib", "libkmig.so")
 KMIG_OK = 0
 ERRORS = {-1: "KMIG_E_PARAM", -2: "KMIG_E_DIM", -3: "KMIG_E_IO", -4: "KMIG_E_DEVICE",
           -5: "KMIG_E_STATE", -6: "KMIG_E_NOMEM"}
-PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2}
+PRECISION = {"fp32": 0, "fp16": 1, "bf16": 2, "u16": 3}
 VARIANT = {"auto": 0, "stream": 1, "resident": 2}
 
 # every exported symbol of include/kmig.h
