@@ -36,7 +36,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="salt", choices=["salt", "small"])
+    ap.add_argument("--config", default="salt", choices=["salt", "small", "layers"])
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "u16"])
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--variant", default="auto", choices=["auto", "stream", "resident"])
@@ -70,6 +70,8 @@ def main():
 
     from kirch_datuming_amd import configs, kmig
 
+    if args.config == "layers":
+        return bench_layers(args, cores)
     t0 = time.time()
     log(f"building the {args.config} workload on {cores} host threads")
     w = configs.salt(threads=cores) if args.config == "salt" else configs.small(threads=cores)
@@ -190,6 +192,88 @@ def main():
         "cig_abs_max": float(np.abs(cig_gpu).max()),
     }
     m.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_layers(args, cores):
+    """Five depth layers of the salt model as five problems in one
+    graph-captured step (their preparation plus one migrate launch)."""
+    import numpy as np
+
+    from kirch_datuming_amd import configs, kmig
+    t0 = time.time()
+    log(f"building the five-layer workload on {cores} host threads")
+    ws = configs.layers(threads=cores)
+    total = sum(w.updates for w in ws)
+    log(f"workload ready in {time.time() - t0:.1f} s: {total:.4e} updates in 5 layers")
+    ms = []
+    tu = time.perf_counter()
+    for w in ws:
+        kv = dict(w.kv, precision=args.precision)
+        ax = w.data_axes
+        p = kmig.parse_params(kv, int(ax[1][0]), ax[1][1], ax[1][2])
+        m = kmig.Migrator(p)
+        m.set_tables(0, w.stab, w.sder, w.s_axes)
+        m.set_tables(1, w.rtab, w.rder, w.r_axes)
+        m.set_data(w.data, ax)
+        ms.append(m)
+    upload_s = time.perf_counter() - tu
+    b = kmig.Batch(ms)
+    for i in range(args.warmup):
+        b.run()
+        b.sync()
+        log(f"warmup {i + 1}/{args.warmup}: {b.stats()['ms']:.1f} ms")
+    step_ms = []
+    tstart = time.perf_counter()
+    for i in range(args.steps):
+        b.run()
+        b.sync()
+        step_ms.append(b.stats()["ms"])
+    wall = time.perf_counter() - tstart
+    st = b.stats()
+    value = total * args.steps / wall
+    log(f"timed {args.steps} steps in {wall:.3f} s ({st['graph_nodes']} graph nodes)")
+    cigs = [m.cig() for m in ms]
+    adcig = configs.stack_layers(cigs)
+    cpu = parity = None
+    if not args.no_cpu and args.cpu_shots > 0:
+        from oracle import oracle as O
+        subs = [w.shot_subset([w.data.shape[0] // 2]) for w in ws]
+        upd = sum(sw.updates for sw in subs)
+        log("CPU oracle (float) on the middle shot of each layer")
+        tc = time.perf_counter()
+        cc = [O.migrate(w.kv, *sw.args(), precision="f32")[0] for w, sw in zip(ws, subs)]
+        tcpu = time.perf_counter() - tc
+        c64 = [O.migrate(w.kv, *sw.args(), precision="f64")[0] for w, sw in zip(ws, subs)]
+        g = [kmig.migrate(dict(w.kv, precision=args.precision), *sw.args())[0]
+             for w, sw in zip(ws, subs)]
+        cpu = {"value": upd / tcpu, "unit": "updates/s", "cores": cores, "kind": "port",
+               "sample": f"middle shot of each of the 5 layers ({upd:.3e} updates), {tcpu:.1f} s"}
+        parity = {"rel_l2_gpu_vs_f64_per_layer": [O.rel_l2(a, r) for a, r in zip(g, c64)],
+                  "rel_l2_cpu_f32_vs_f64_per_layer": [O.rel_l2(a, r) for a, r in zip(cc, c64)]}
+    out = {
+        "metric": METRIC, "value": value, "unit": "updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else f"{args.precision} tables, f32 math",
+        "data": "synthetic",
+        "config": {"workload": "five salt layers nz 80/70/90/100/60 x 1000, shots "
+                               "768/640/512/384/256, receivers 401/361/321/281/241, nt=2001, "
+                               "one graph-captured step", "config": "layers",
+                   "updates_per_step": total, "precision": args.precision,
+                   "graph_nodes": st["graph_nodes"]},
+        "roofline": {"bound": "hbm", "achieved": total * BYTES_PER_UPDATE[args.precision] /
+                     (float(np.mean(step_ms)) * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": total * BYTES_PER_UPDATE[args.precision] /
+                     (float(np.mean(step_ms)) * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_migrate_multi (whole graph step timed: a lower bound)",
+                     "algorithmic_bytes_per_update": BYTES_PER_UPDATE[args.precision]},
+        "cpu_baseline": cpu, "parity": parity, "upload_s": upload_s,
+        "adcig_shape": list(adcig.shape), "adcig_abs_max": float(np.abs(adcig).max()),
+    }
+    b.close()
+    for m in ms:
+        m.close()
     print(json.dumps(out), flush=True)
 
 

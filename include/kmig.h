@@ -166,6 +166,27 @@ int kmig_migrate(kmig_ctx *ctx, const float *data, int nt, float t0, float dt, i
    order their own work around kmig_run. */
 void *kmig_stream(kmig_ctx *ctx);
 
+/* ---- batches: several problems in one step ----
+   A batch groups contexts whose tables and data are set (for example the
+   depth layers of a recursive workflow, each with its own grid, shots,
+   receivers and mute; no counterpart in the reference, which runs them as
+   separate processes, run_salt_adcig.sh:171-181).  kmig_batch_run plans each
+   context, then enqueues every context's trace and tuple preparation and ONE
+   migrate launch covering all of them; the whole step is captured in a
+   hipGraph on the first run and replayed afterwards.  Every context must use
+   the resident kernel (mode=angle, angle_interp=1, cmp=0, one shot batch) and
+   the same angle axis length and precision.  Call kmig_batch_sync before
+   reading results with kmig_get_cig / kmig_get_qc of each context. */
+typedef struct kmig_batch kmig_batch;
+int kmig_batch_create(kmig_batch **out, kmig_ctx *const *ctx, int n);
+int kmig_batch_run(kmig_batch *b);
+int kmig_batch_sync(kmig_batch *b);
+/* updates (sum of ns*nh*nz*nx) and the last step's time between HIP events
+   recorded around the graph launch on the batch stream */
+int kmig_batch_stats(const kmig_batch *b, double *updates, double *ms, int *graph_nodes);
+const char *kmig_batch_last_error(const kmig_batch *b);
+void kmig_batch_destroy(kmig_batch *b);
+
 /* Library version string. */
 const char *kmig_version(void);
 

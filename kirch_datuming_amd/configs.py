@@ -6,6 +6,14 @@ small   v(z) = 1.5 + 0.5 z km/s, nz=201, nx=301, dz=dx=0.01 km; 61 shots every
         positions; nt=1001, dt=2 ms; 15 Hz Ricker at the two-way times of
         flat reflectors z = 0.6 / 1.2 / 1.7 km (r = +0.1 / -0.1 / +0.1) plus
         Gaussian noise at 20 dB SNR; angle axis 0..60 deg, 61 bins.
+layers  the salt model cut into five depth layers migrated as separate
+        problems (nz 80/70/90/100/60 from the top; shots 768/640/512/384/256 and
+        receivers 401/361/321/281/241 spread over the same 0..9.59 / 0..10 km;
+        per-layer mutes, slopes 50/40/30/20/10 s/km, tapers 0.05..0.10 s);
+        tables by fast marching inside each layer from its top (the datum the
+        recursive workflow moves the acquisition to), Born data from scatterers
+        inside each layer; angle axis 0..60 deg for every layer, so the five
+        CIGs stack into one (400, 1000, 61) angle gather volume.
 salt    five-layer model 1.5 / 2.0 / 2.6 / 4.5 (salt, Gaussian-bump top) /
         3.2 km/s smoothed with a 5-point Gaussian, nz=400, nx=1000,
         dz=dx=0.01 km; 768 shots every 0.0125 km; 401 receivers every
@@ -144,3 +152,45 @@ def salt(threads=0, seed=20260202, nshots=768) -> Workload:
     return Workload("salt 400x1000, 768 shots x 401 receivers, nt=2001", kv, data,
                     [[nt, 0.0, dt], [nh, h0, dh], [ns, s0, ds]], ST, SD, tax + [[ns, s0, ds]],
                     RT, RD, tax + [[nh, h0, dh]], vel)
+
+
+LAYERS_NZ = (80, 70, 90, 100, 60)
+LAYERS_NS = (768, 640, 512, 384, 256)
+LAYERS_NH = (401, 361, 321, 281, 241)
+LAYERS_SLOPE = (50.0, 40.0, 30.0, 20.0, 10.0)
+LAYERS_TAPER = (0.05, 0.0625, 0.075, 0.0875, 0.10)
+
+
+def layers(threads=0, seed=20260303, scale=1.0):
+    """The five depth layers as separate Workloads (see module docstring).
+    scale < 1 shrinks shot and receiver counts (for tests)."""
+    L = synth()
+    nx, d = 1000, 0.01
+    full = np.empty((nx, 400), np.float32)
+    L.kms_velocity_salt(_p(full), 400, nx, d, d)
+    out, z0 = [], 0
+    for k, nz in enumerate(LAYERS_NZ):
+        vel = np.ascontiguousarray(full[:, z0:z0 + nz])
+        ns = max(2, int(round(LAYERS_NS[k] * scale)))
+        nh = max(2, int(round(LAYERS_NH[k] * scale)))
+        ds, dh = 9.5875 / (ns - 1), 10.0 / (nh - 1)
+        nt, dt = 2001, 0.002
+        ST, SD = tables(vel, d, d, ns, 0.0, ds, 2, threads)
+        RT, RD = tables(vel, d, d, nh, 0.0, dh, 2, threads)
+        data = np.empty((ns, nh, nt), np.float32)
+        L.kms_data_scatterers(_p(data), nt, 0.0, dt, nh, 0.0, dh, ns, 0.0, ds, 0, _p(ST), ns,
+                              0.0, ds, _p(RT), nh, 0.0, dh, nz, nx, 30, 20.0, 1.5, seed + k)
+        kv = dict(cmp=0, mode="angle", angle_min=0, angle_max=60, angle_step=1, angle_n=61,
+                  direct_mute=1, direct_mute_slope=LAYERS_SLOPE[k], direct_mute_intercept=0,
+                  direct_mute_taper=LAYERS_TAPER[k])
+        tax = [[nz, 0.0, d], [nx, 0.0, d]]
+        out.append(Workload(f"layer {k + 1}: {nz}x{nx}, {ns} shots x {nh} receivers", kv, data,
+                            [[nt, 0.0, dt], [nh, 0.0, dh], [ns, 0.0, ds]], ST, SD,
+                            tax + [[ns, 0.0, ds]], RT, RD, tax + [[nh, 0.0, dh]], vel))
+        z0 += nz
+    return out
+
+
+def stack_layers(cigs):
+    """Concatenate per-layer CIGs (ng, nx, nz_k) along depth into one volume."""
+    return np.concatenate(cigs, axis=2)

@@ -1061,3 +1061,203 @@ int kmig_migrate(kmig_ctx *c, const float *data, int nt, float t0, float dt, int
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ batches --
+
+struct kmig_batch {
+    std::vector<kmig_ctx *> ctx;
+    std::vector<long long> versions;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    size_t nodes = 0;
+    std::string err;
+    double updates = 0.0, ms = 0.0;
+    bool pending = false;
+};
+
+namespace {
+
+int bfail(kmig_batch *b, int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    b->err = buf;
+    return code;
+}
+
+#define BHIP(b, call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return bfail((b), KMIG_E_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+// Build (or rebuild) the captured graph of one batch step.
+int batch_capture(kmig_batch *b)
+{
+    const int n = (int)b->ctx.size();
+    MigMultiArgs m{};
+    m.nprob = n;
+    // heaviest problems first (longest blocks start first)
+    std::vector<int> order(n);
+    for (int k = 0; k < n; k++) order[k] = k;
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        const kmig_ctx *a = b->ctx[x], *c = b->ctx[y];
+        return (double)a->ns * a->nh > (double)c->ns * c->nh;
+    });
+    const kmig_params &p0 = b->ctx[0]->p;
+    const int ng = p0.ng;
+    int tile = 9;
+    if (migrate_lds_bytes(ng, tile) > 64 * 1024) tile = 15;
+    m.tile0[0] = 0;
+    b->updates = 0.0;
+    for (int q = 0; q < n; q++) {
+        kmig_ctx *c = b->ctx[order[q]];
+        m.prob[q] = res_args(c, 0, c->ns, ng);
+        m.tile0[q + 1] = m.tile0[q] + migrate_tiles(c->nz, c->nx, tile);
+        b->updates += (double)c->ns * c->nh * (double)c->nzx;
+    }
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = KMIG_OK;
+    for (int k = 0; k < n && rc == KMIG_OK; k++) {
+        kmig_ctx *c = b->ctx[k];
+        rc = enqueue_trace_prep(c, b->stream, 0, c->ns);
+        if (rc == KMIG_OK) rc = enqueue_tuple_prep(c, b->stream, c->batches[0], 0);
+        if (rc != KMIG_OK) b->err = c->err;
+    }
+    if (rc == KMIG_OK) {
+        const int e = launch_migrate_multi(m, tile, p0.precision, b->stream);
+        if (e) rc = bfail(b, KMIG_E_DEVICE, "multi-problem migrate launch failed: %s",
+                          hipGetErrorString((hipError_t)e));
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+    if (rc != KMIG_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ec != hipSuccess)
+        return bfail(b, KMIG_E_DEVICE, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
+    b->graph = g;
+    BHIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->nodes = 0;
+    (void)hipGraphGetNodes(b->graph, nullptr, &b->nodes);
+    return KMIG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kmig_batch_create(kmig_batch **out, kmig_ctx *const *ctx, int n)
+{
+    if (!out) return KMIG_E_PARAM;
+    *out = nullptr;
+    kmig_batch *b = new kmig_batch();
+    *out = b;
+    if (!ctx || n <= 0 || n > KMIG_MAX_PROBLEMS)
+        return bfail(b, KMIG_E_PARAM, "a batch holds 1..%d contexts", KMIG_MAX_PROBLEMS);
+    for (int k = 0; k < n; k++) {
+        if (!ctx[k] || !ctx[k]->stream)
+            return bfail(b, KMIG_E_STATE, "context %d has no device", k);
+        b->ctx.push_back(ctx[k]);
+    }
+    BHIP(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    BHIP(b, hipEventCreate(&b->e0));
+    BHIP(b, hipEventCreate(&b->e1));
+    return KMIG_OK;
+}
+
+int kmig_batch_run(kmig_batch *b)
+{
+    if (!b) return KMIG_E_PARAM;
+    if (!b->stream) return bfail(b, KMIG_E_STATE, "batch has no stream");
+    const int n = (int)b->ctx.size();
+    bool changed = b->exec == nullptr || (int)b->versions.size() != n;
+    b->versions.resize(n, -2);
+    for (int k = 0; k < n; k++) {
+        kmig_ctx *c = b->ctx[k];
+        if (!c->side[0].set || !c->side[1].set || !c->data_set)
+            return bfail(b, KMIG_E_STATE, "context %d: tables and data must be set", k);
+        int rc = plan_if_needed(c, 0, c->ns);
+        if (rc == KMIG_OK) rc = ensure_outputs(c, 0, c->ns);
+        if (rc) return bfail(b, rc, "context %d: %s", k, c->err.c_str());
+        if (c->variant_used != KMIG_VARIANT_RESIDENT || c->batches.size() != 1)
+            return bfail(b, KMIG_E_PARAM,
+                         "context %d: batches need the resident kernel (mode=angle, "
+                         "angle_interp=1, cmp=0) and tuples that fit one shot batch", k);
+        const kmig_params &p = c->p, &q = b->ctx[0]->p;
+        if (!p.cig || p.ng != q.ng || p.precision != q.precision)
+            return bfail(b, KMIG_E_PARAM,
+                         "context %d: every context of a batch needs the same angle axis "
+                         "length and precision", k);
+        if (b->versions[k] != c->plan_version) changed = true;
+        b->versions[k] = c->plan_version;
+    }
+    if (changed) {
+        BHIP(b, hipStreamSynchronize(b->stream));
+        const int rc = batch_capture(b);
+        if (rc) return rc;
+    }
+    BHIP(b, hipEventRecord(b->e0, b->stream));
+    BHIP(b, hipGraphLaunch(b->exec, b->stream));
+    BHIP(b, hipEventRecord(b->e1, b->stream));
+    for (kmig_ctx *c : b->ctx) {
+        c->ran = true;
+        c->pending = false;
+        c->st = kmig_stats{};
+        c->st.updates = (double)c->ns * c->nh * (double)c->nzx;
+        c->st.migrate_launches = 1;
+        c->st.shot_batches = 1;
+        c->st.variant = c->variant_used;
+    }
+    b->pending = true;
+    return KMIG_OK;
+}
+
+int kmig_batch_sync(kmig_batch *b)
+{
+    if (!b) return KMIG_E_PARAM;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    if (b->pending) {
+        float ms = 0.0f;
+        BHIP(b, hipEventElapsedTime(&ms, b->e0, b->e1));
+        b->ms = ms;
+        b->pending = false;
+    }
+    return KMIG_OK;
+}
+
+int kmig_batch_stats(const kmig_batch *b, double *updates, double *ms, int *graph_nodes)
+{
+    if (!b) return KMIG_E_PARAM;
+    if (updates) *updates = b->updates;
+    if (ms) *ms = b->ms;
+    if (graph_nodes) *graph_nodes = (int)b->nodes;
+    return KMIG_OK;
+}
+
+const char *kmig_batch_last_error(const kmig_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+void kmig_batch_destroy(kmig_batch *b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->e0) (void)hipEventDestroy(b->e0);
+    if (b->e1) (void)hipEventDestroy(b->e1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+}  // extern "C"

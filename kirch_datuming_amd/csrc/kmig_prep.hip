@@ -202,6 +202,12 @@ __global__ __launch_bounds__(TB) void k_trace_bwd(const TracePrepArgs a, float2 
     }
 }
 
+__global__ __launch_bounds__(256) void k_copy(const float *in, float *out, size_t n)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = in[k];
+}
+
 // diff2 (src/kirchmig2d_cig.cpp:381-394) as an element-wise pass.
 __global__ __launch_bounds__(256) void k_diff2(const float *in, float *out, size_t n, int nt,
                                                float d)
@@ -528,9 +534,9 @@ int launch_trace_prep2(const TracePrepArgs &a, float *pairs, float *tmp, void *s
         const size_t off = (size_t)a.shot_begin * a.nh * a.nt, n = (size_t)ntr * a.nt;
         hipLaunchKernelGGL(k_diff2, dim3(nblk(n, 256)), dim3(256), 0, st, a.prep + off,
                            tmp + off, n, a.nt, a.dt);
-        if (hipMemcpyAsync(a.prep + off, tmp + off, n * sizeof(float), hipMemcpyDeviceToDevice,
-                           st) != hipSuccess)
-            return (int)hipGetLastError();
+        // a copy kernel rather than a memcpy node: the step may be graph-captured
+        hipLaunchKernelGGL(k_copy, dim3(nblk(n, 256)), dim3(256), 0, st, tmp + off,
+                           a.prep + off, n);
         if (pairs)
             hipLaunchKernelGGL(k_pairs, dim3(nblk(n, 256)), dim3(256), 0, st, a.prep + off,
                                (float2 *)pairs + off, n, a.nt);

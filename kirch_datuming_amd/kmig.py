@@ -29,7 +29,8 @@ VARIANT = {"auto": 0, "stream": 1, "resident": 2}
 EXPORTS = ("kmig_params_default", "kmig_params_parse", "kmig_params_check", "kmig_create",
            "kmig_destroy", "kmig_last_error", "kmig_set_tables", "kmig_set_data", "kmig_run",
            "kmig_sync", "kmig_get_cig", "kmig_get_qc", "kmig_get_stats", "kmig_migrate",
-           "kmig_stream", "kmig_version")
+           "kmig_stream", "kmig_version", "kmig_batch_create", "kmig_batch_run",
+           "kmig_batch_sync", "kmig_batch_stats", "kmig_batch_last_error", "kmig_batch_destroy")
 
 
 class KmigError(RuntimeError):
@@ -102,6 +103,15 @@ def lib():
         L.kmig_stream.argtypes = [vp]
         L.kmig_stream.restype = vp
         L.kmig_version.restype = C.c_char_p
+        L.kmig_batch_create.argtypes = [C.POINTER(vp), C.POINTER(vp), C.c_int]
+        L.kmig_batch_run.argtypes = [vp]
+        L.kmig_batch_sync.argtypes = [vp]
+        L.kmig_batch_stats.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int)]
+        L.kmig_batch_last_error.argtypes = [vp]
+        L.kmig_batch_last_error.restype = C.c_char_p
+        L.kmig_batch_destroy.argtypes = [vp]
+        L.kmig_batch_destroy.restype = None
         _LIB = L
     return _LIB
 
@@ -207,6 +217,53 @@ class Migrator:
         if self.ctx:
             self.L.kmig_destroy(self.ctx)
             self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Several Migrators run as one graph-captured step (one migrate launch)."""
+
+    def __init__(self, migrators):
+        self.L = lib()
+        self.m = list(migrators)
+        arr = (C.c_void_p * len(self.m))(*[mm.ctx.value for mm in self.m])
+        self.b = C.c_void_p()
+        rc = self.L.kmig_batch_create(C.byref(self.b), arr, len(self.m))
+        if rc != KMIG_OK:
+            msg = self.L.kmig_batch_last_error(self.b).decode() if self.b else ""
+            self.close()
+            raise KmigError(rc, msg)
+
+    def _chk(self, rc):
+        if rc != KMIG_OK:
+            raise KmigError(rc, self.L.kmig_batch_last_error(self.b).decode())
+
+    def run(self):
+        self._chk(self.L.kmig_batch_run(self.b))
+
+    def sync(self):
+        self._chk(self.L.kmig_batch_sync(self.b))
+
+    def stats(self):
+        u, ms, nodes = C.c_double(), C.c_double(), C.c_int()
+        self._chk(self.L.kmig_batch_stats(self.b, C.byref(u), C.byref(ms), C.byref(nodes)))
+        return {"updates": u.value, "ms": ms.value, "graph_nodes": nodes.value}
+
+    def close(self):
+        if self.b:
+            self.L.kmig_batch_destroy(self.b)
+            self.b = C.c_void_p()
 
     def __enter__(self):
         return self

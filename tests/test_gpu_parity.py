@@ -177,3 +177,44 @@ def test_resident_variant_is_used_and_matches_stream():
     kv1, run1, _ = gc.case("angle_cmp1")
     with pytest.raises(kmig.KmigError, match="variant=resident needs"):
         kmig.migrate(dict(kv1, variant="resident"), *_args(run1))
+
+
+def test_batch_graph_matches_individual_runs_and_replays():
+    """Five small depth layers in one graph-captured step: each layer's CIG is
+    bitwise the one its own context computes with the same preset, and every
+    graph replay (with eager work and syncs in between) repeats it."""
+    from kirch_datuming_amd import configs, kmig
+    ws = configs.layers(threads=8, scale=0.05)
+    ms = []
+    singles = []
+    for w in ws:
+        kv = dict(w.kv, tile=9)
+        ax = w.data_axes
+        p = kmig.parse_params(kv, int(ax[1][0]), ax[1][1], ax[1][2])
+        m = kmig.Migrator(p)
+        m.set_tables(0, w.stab, w.sder, w.s_axes)
+        m.set_tables(1, w.rtab, w.rder, w.r_axes)
+        m.set_data(w.data, ax)
+        m.run()
+        m.sync()
+        singles.append(m.cig())
+        ms.append(m)
+    with kmig.Batch(ms) as b:
+        for rep in range(3):
+            b.run()
+            b.sync()
+            for k, m in enumerate(ms):
+                c = m.cig()
+                assert np.array_equal(c.view(np.uint32), singles[k].view(np.uint32)), (rep, k)
+            # eager work between replays
+            ms[0].run()
+            ms[0].sync()
+        st = b.stats()
+        assert st["graph_nodes"] > 5 and st["ms"] > 0
+        assert st["updates"] == sum(w.updates for w in ws)
+    # and one layer against the FP64 oracle
+    w = ws[2]
+    ref = O.migrate(w.kv, *w.args(), precision="f64")[0]
+    assert O.rel_l2(singles[2], ref) < TOL
+    for m in ms:
+        m.close()
