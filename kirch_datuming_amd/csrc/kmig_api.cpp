@@ -414,7 +414,7 @@ MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
     R.c_int = (int)ci;
     R.c_frac = (float)(cst - ci);
     R.aal = p.antialias;
-    R.half_angle = p.half_angle;
+    R.wrap = p.half_angle ? 180.0f : 360.0f;
     R.g0 = p.g0;
     R.inv_dg = 1.0f / p.dg;
     R.ng = ng;
@@ -437,7 +437,11 @@ MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
 int autotune(kmig_ctx *c, const MigResArgs &full)
 {
     if (c->tuned_tile > 0 && c->tuned_version == c->geo_version) return KMIG_OK;
-    static const int presets[] = {9, 15, 16, 20, 11, 10, 18};
+    static const int presets32[] = {9, 15, 16, 20, 11, 10, 18};
+    static const int presets16[] = {9, 15, 10, 13};
+    const bool p16 = c->p.precision != KMIG_PREC_FP32;
+    const int *presets = p16 ? presets16 : presets32;
+    const int npresets = p16 ? 4 : 7;
     const double per_shot = (double)c->nh * (double)c->nzx;
     int S = (int)std::max(1.0, std::min((double)full.nshots, 5e9 / std::max(per_shot, 1.0)));
     // a slice from the middle of the batch (source tuples are batch-local)
@@ -445,19 +449,23 @@ int autotune(kmig_ctx *c, const MigResArgs &full)
     MigResArgs R = full;
     R.nshots = S;
     R.shot0 = full.shot0 + off;
-    R.stup = full.stup + (size_t)off * c->nzx;
+    if (full.stup) R.stup = full.stup + (size_t)off * c->nzx;
+    if (full.stup16) R.stup16 = full.stup16 + (size_t)off * c->nzx;
+    if (full.sscale) R.sscale = full.sscale + off;
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
     float best = 1e30f;
     int best_tile = 2;
     c->tune_log.clear();
-    for (int t : presets) {
+    for (int ip = 0; ip < npresets; ip++) {
+        const int t = presets[ip];
         if (migrate_lds_bytes(full.ng, t) > 64 * 1024) continue;
         float tmin = 1e30f;
         for (int rep = 0; rep < 2; rep++) {
             HIPCHK(c, hipEventRecord(e0, c->stream));
-            HIPCHK(c, (hipError_t)launch_migrate_resident_probe(R, t, c->stream));
+            HIPCHK(c, (hipError_t)launch_migrate_resident_probe(R, t, c->p.precision,
+                                                                c->stream));
             HIPCHK(c, hipEventRecord(e1, c->stream));
             HIPCHK(c, hipEventSynchronize(e1));
             float ms = 0.0f;
@@ -681,16 +689,19 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
     A.dpos = c->side[1].d;
     A.dside = dh;
     HIPCHK(c, (hipError_t)launch_tuple_prep(A, c->tr, c->tmp1, c->tmp2, st));
+    // packed directions in output-angle degrees (the opening angle is then
+    // a difference of two directions folded at 360, or 180 for half angles)
+    const float th_scale = angle ? (float)((p.half_angle ? 90.0 : 180.0) / M_PI) : 0.0f;
     if (resident && p.precision == KMIG_PREC_FP32) {
-        HIPCHK(c, (hipError_t)launch_pack(c->ts, c->pk_s, nsb, c->nzx, c->dt, angle,
+        HIPCHK(c, (hipError_t)launch_pack(c->ts, c->pk_s, nsb, c->nzx, c->dt, th_scale,
                                           st));
-        HIPCHK(c, (hipError_t)launch_pack(c->tr, c->pk_r, b.nrcv, c->nzx, c->dt, angle,
+        HIPCHK(c, (hipError_t)launch_pack(c->tr, c->pk_r, b.nrcv, c->nzx, c->dt, th_scale,
                                           st));
     } else if (resident) {
         HIPCHK(c, (hipError_t)launch_pack16(c->ts, c->pk16_s, c->sc_s, nsb, c->nzx, c->dt,
-                                            angle, p.precision, st));
+                                            th_scale, p.precision, st));
         HIPCHK(c, (hipError_t)launch_pack16(c->tr, c->pk16_r, c->sc_r, b.nrcv, c->nzx, c->dt,
-                                            angle, p.precision, st));
+                                            th_scale, p.precision, st));
     }
     return KMIG_OK;
 }
@@ -952,18 +963,13 @@ int kmig_run(kmig_ctx *c, int sb, int se)
         }
         if (migrate_lds_bytes(ng, tile) > 160 * 1024)
             return fail(c, KMIG_E_PARAM, "output axis of %d samples does not fit in LDS", ng);
-        if (nsb > 0 && resident && p.precision != KMIG_PREC_FP32) {
-            MigResArgs R = res_args(c, b.s0, nsb, ng);
-            if (tile <= 0 || migrate_lds_bytes(ng, tile) > 64 * 1024) tile = 15;
-            HIPCHK(c, (hipError_t)launch_migrate_resident16(R, tile, p.precision, c->stream));
-            launches++;
-        } else if (nsb > 0 && resident) {
+        if (nsb > 0 && resident) {
             MigResArgs R = res_args(c, b.s0, nsb, ng);
             if (p.tile <= 0) {
                 if ((rc = autotune(c, R))) return rc;
                 tile = c->tuned_tile;
             }
-            HIPCHK(c, (hipError_t)launch_migrate_resident(R, tile, c->stream));
+            HIPCHK(c, (hipError_t)launch_migrate_resident(R, tile, p.precision, c->stream));
             launches++;
         } else if (nsb > 0) {
             HIPCHK(c, (hipError_t)launch_migrate(M, tile, c->stream));
@@ -1075,6 +1081,8 @@ struct kmig_batch {
     std::string err;
     double updates = 0.0, ms = 0.0;
     bool pending = false;
+    int tile = 0;                              // preset of the captured graph
+    std::vector<std::pair<int, float>> tune_log;
 };
 
 namespace {
@@ -1097,8 +1105,9 @@ int bfail(kmig_batch *b, int code, const char *fmt, ...)
             return bfail((b), KMIG_E_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
-// Build (or rebuild) the captured graph of one batch step.
-int batch_capture(kmig_batch *b)
+// Build (or rebuild) the captured graph of one batch step with migrate
+// preset `tile` (9, 10, 15 or 13).
+int batch_capture(kmig_batch *b, int tile)
 {
     const int n = (int)b->ctx.size();
     MigMultiArgs m{};
@@ -1112,8 +1121,6 @@ int batch_capture(kmig_batch *b)
     });
     const kmig_params &p0 = b->ctx[0]->p;
     const int ng = p0.ng;
-    int tile = 9;
-    if (migrate_lds_bytes(ng, tile) > 64 * 1024) tile = 15;
     m.tile0[0] = 0;
     b->updates = 0.0;
     for (int q = 0; q < n; q++) {
@@ -1151,7 +1158,48 @@ int batch_capture(kmig_batch *b)
     BHIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->nodes = 0;
     (void)hipGraphGetNodes(b->graph, nullptr, &b->nodes);
+    b->tile = tile;
     return KMIG_OK;
+}
+
+// Batch autotuner: captures the step with each multi-problem preset whose
+// accumulator fits 64 KiB of LDS, replays each twice (HIP events on the
+// batch stream) and keeps the fastest.  Every replay recomputes the CIGs
+// from zero, so the timing replays leave no trace in the results.
+int batch_tune(kmig_batch *b)
+{
+    static const int presets[] = {9, 10, 15, 13};
+    const int ng = b->ctx[0]->p.ng;
+    b->tune_log.clear();
+    int best = 0;
+    float tbest = 1e30f;
+    for (int t : presets) {
+        if (migrate_lds_bytes(ng, t) > 64 * 1024) continue;
+        int rc = batch_capture(b, t);
+        if (rc) return rc;
+        float tmin = 1e30f;
+        for (int rep = 0; rep < 2; rep++) {
+            BHIP(b, hipEventRecord(b->e0, b->stream));
+            BHIP(b, hipGraphLaunch(b->exec, b->stream));
+            BHIP(b, hipEventRecord(b->e1, b->stream));
+            BHIP(b, hipEventSynchronize(b->e1));
+            float ms = 0.0f;
+            BHIP(b, hipEventElapsedTime(&ms, b->e0, b->e1));
+            tmin = std::min(tmin, ms);
+        }
+        b->tune_log.push_back({t, tmin});
+        if (tmin < tbest) {
+            tbest = tmin;
+            best = t;
+        }
+    }
+    if (best == 0) best = 15;  // nothing fits 64 KiB: the smallest tile, more LDS
+    if (getenv("KMIG_TUNE_LOG")) {
+        fprintf(stderr, "kmig batch autotune:");
+        for (auto &q : b->tune_log) fprintf(stderr, " tile%d=%.2fms", q.first, q.second);
+        fprintf(stderr, " -> tile%d\n", best);
+    }
+    return b->tile == best ? KMIG_OK : batch_capture(b, best);
 }
 
 }  // namespace
@@ -1205,7 +1253,8 @@ int kmig_batch_run(kmig_batch *b)
     }
     if (changed) {
         BHIP(b, hipStreamSynchronize(b->stream));
-        const int rc = batch_capture(b);
+        const int t = b->ctx[0]->p.tile;
+        const int rc = t > 0 ? batch_capture(b, t) : batch_tune(b);
         if (rc) return rc;
     }
     BHIP(b, hipEventRecord(b->e0, b->stream));

@@ -106,7 +106,8 @@ struct MigArgs {
 // Packed 16-byte tuple of the resident migrate kernel, one per (slot, point):
 //   frac  fractional part of T/dt (samples, in [0,1])
 //   a     |dT/dpos * spacing| / dt (anti-alias term in samples)
-//   th    direction of the smoothed traveltime gradient, atan2(gx, gz)
+//   th    direction of the smoothed traveltime gradient, atan2(gx, gz), in
+//         output-angle degrees (x 180/pi, or x 90/pi for half opening angles)
 //   iw    bits 31..16: integer part of T/dt (int16), bits 15..0: the
 //         aperture weight W * 32768 (W in [0,1], 0 where |grad| <= 1e-12)
 struct PackedTuple {
@@ -144,12 +145,12 @@ struct MigResArgs {
     const SlotScale *rscale;     // [slots]
     const int *rslot;            // [nh] receiver slot of each receiver index (-1: unused)
     const float *wap;            // [ns][nh] trace-aperture weight, < 0: trace skipped
-    const float *pairs;          // [ns][nh][nt][2] (tr[k], tr[k+1])
+    const float *pairs;          // [ns][nh][nt][2] (tr[k], tr[k+1] - tr[k])
     float *cig;
     int c_int;                   // integer part of (tau - t0)/dt
     float c_frac;                // fractional part of (tau - t0)/dt
     float aal;
-    int half_angle;
+    float wrap;                  // 360 (full opening angle) or 180 (half)
     float g0, inv_dg;
     int ng;
     float oazmin, oazmax, oaztaper, inv_oaztaper;
@@ -166,7 +167,7 @@ struct MigMultiArgs {
 
 // Launchers (kmig_prep.hip, kmig_migrate.hip).  Return 0 or a hipError_t value.
 int launch_trace_prep(const TracePrepArgs &a, void *stream);
-// coalesced trace preparation; pairs (nullable) gets (tr[k], tr[k+1]); tmp is a
+// coalesced trace preparation; pairs (nullable) gets (tr[k], tr[k+1] - tr[k]); tmp is a
 // trace-sized scratch volume used only when diff=1
 int launch_trace_prep2(const TracePrepArgs &a, float *pairs, float *tmp, void *stream);
 int launch_tuple_prep(const TuplePrepArgs &a, Tuples &out, float *tmp1, float *tmp2,
@@ -174,16 +175,17 @@ int launch_tuple_prep(const TuplePrepArgs &a, Tuples &out, float *tmp1, float *t
 int launch_migrate(const MigArgs &a, int tile, void *stream);
 int migrate_lds_bytes(int ng, int tile);
 int launch_fill(float *p, size_t n, float v, void *stream);
+// th_scale: degrees per radian of the packed direction (0: no direction)
 int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, float dt,
-                int angle_mode, void *stream);
+                float th_scale, void *stream);
 int launch_pack16(const Tuples &in, PackedTuple16 *out, SlotScale *scale, int nslots,
-                  size_t nzx, float dt, int angle_mode, int precision, void *stream);
+                  size_t nzx, float dt, float th_scale, int precision, void *stream);
 int launch_pairs(const float *prep, float *pairs, long ntraces, int nt, void *stream);
-int launch_migrate_resident(const MigResArgs &a, int tile, void *stream);
-int launch_migrate_resident16(const MigResArgs &a, int tile, int precision, void *stream);
+// tile: preset 9..21 (kmig_migrate.hip); precision: KMIG_PREC_* tuple format
+int launch_migrate_resident(const MigResArgs &a, int tile, int precision, void *stream);
 int launch_migrate_multi(const MigMultiArgs &m, int tile, int precision, void *stream);
 int migrate_tiles(int nz, int nx, int tile);
 // the same kernels under a distinct symbol, for the autotuner's timing launches
-int launch_migrate_resident_probe(const MigResArgs &a, int tile, void *stream);
+int launch_migrate_resident_probe(const MigResArgs &a, int tile, int precision, void *stream);
 
 }  // namespace kmig

@@ -223,31 +223,66 @@ __device__ __forceinline__ float from16(uint32_t h, int fmt)
     return (float)__builtin_bit_cast(_Float16, (unsigned short)h);
 }
 
+// A tuple as loaded (Raw) and decoded: the loads of the next shot's tuple
+// are issued one shot ahead and decoded only when used, so the decode does
+// not wait on the load that is meant to be in flight.
+struct Raw16 {
+    PackedTuple16 q;
+    float scale;
+};
+
+template <int Fmt> struct RawOf {
+    using type = Raw16;
+};
+template <> struct RawOf<0> {
+    using type = PackedTuple;
+};
+
+template <int Fmt>
+__device__ __forceinline__ typename RawOf<Fmt>::type load_raw(const MigResArgs &a, bool src,
+                                                              size_t slot, size_t i)
+{
+    const size_t idx = slot * a.nzx + i;
+    if constexpr (Fmt == 0) {
+        return src ? a.stup[idx] : a.rtup[idx];
+    } else {
+        Raw16 r;
+        r.q = src ? a.stup16[idx] : a.rtup16[idx];
+        r.scale = src ? a.sscale[slot].a_scale : a.rscale[slot].a_scale;
+        return r;
+    }
+}
+
+template <int Fmt>
+__device__ __forceinline__ PackedTuple decode(const typename RawOf<Fmt>::type &r)
+{
+    if constexpr (Fmt == 0) {
+        return r;
+    } else {
+        PackedTuple t;
+        t.iw = r.q.iw;
+        t.frac = Fmt == 3 ? (float)(r.q.fa & 0xFFFFu) * (1.0f / 65536.0f)
+                          : from16(r.q.fa & 0xFFFFu, Fmt);
+        t.a = from16(r.q.fa >> 16, Fmt) * r.scale;
+        t.th = __uint_as_float(r.q.th);
+        return t;
+    }
+}
+
 template <int Fmt>
 __device__ __forceinline__ PackedTuple load_tuple(const MigResArgs &a, bool src, size_t slot,
                                                   size_t i)
 {
-    const size_t idx = slot * a.nzx + i;
-    if (Fmt == 0) return src ? a.stup[idx] : a.rtup[idx];
-    const PackedTuple16 q = src ? a.stup16[idx] : a.rtup16[idx];
-    const SlotScale sc = src ? a.sscale[slot] : a.rscale[slot];
-    PackedTuple t;
-    t.iw = q.iw;
-    t.frac = Fmt == 3 ? (float)(q.fa & 0xFFFFu) * (1.0f / 65536.0f) : from16(q.fa & 0xFFFFu, Fmt);
-    t.a = from16(q.fa >> 16, Fmt) * sc.a_scale;
-    t.th = __uint_as_float(q.th);
-    return t;
+    return decode<Fmt>(load_raw<Fmt>(a, src, slot, i));
 }
 
-// Opening angle (degrees) from the two packed gradient directions.
+// Opening angle from the two packed gradient directions (output-angle
+// degrees): their difference folded into [0, wrap / 2].
 __device__ __forceinline__ float pair_angle(const PackedTuple &s, const PackedTuple &r,
-                                           int half_angle)
+                                           float wrap)
 {
-    const float PI_F = 3.14159265358979f;
-    float d = fabsf(s.th - r.th);
-    d = d > PI_F ? 2.0f * PI_F - d : d;
-    const float angle = d * (180.0f / PI_F);
-    return half_angle ? 0.5f * angle : angle;
+    const float d = fabsf(s.th - r.th);
+    return fminf(d, wrap - d);
 }
 
 // Everything of one update except the opening angle and the depth-mute
@@ -268,38 +303,36 @@ __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, const PackedTup
     const float u = (angle - a.g0) * a.inv_dg;
     const float ngm1 = (float)(a.ng - 1);
     bool ok = aw > 0.0f && wap >= 0.0f && u >= -1.0e-5f && u <= ngm1 + 1.0e-5f;
-    const float fu = floorf(u);
-    int ig0 = (int)fu;
-    float w1 = u - fu;
-    if (u <= 0.0f) {
-        ig0 = 0;
-        w1 = 0.0f;
-    }
-    if (u >= ngm1) {
-        ig0 = a.ng - 1;
-        w1 = 0.0f;
-    }
+    // linear bins: u clamped to [0, ng-1] gives the edge bin with w1 = 0 at
+    // either end and (floor(u), u - floor(u)) inside
+    const float uc = __builtin_amdgcn_fmed3f(u, 0.0f, ngm1);
+    const float fu = floorf(uc);
+    const int ig0 = (int)fu;
+    const float w1 = uc - fu;
     const int ii = ((int)s.iw >> 16) + ((int)r.iw >> 16) + a.c_int;
     const float ff = s.frac + r.frac + a.c_frac;
     const float fl = floorf(ff);
     const int it = ii + (int)fl;
     const float ft = ff - fl;
-    const float dA = fmaxf(s.a, r.a) * a.aal;
-    const float fm = ft - dA - 1.0f, fp = ft + dA + 1.0f;
+    // taps at ft -/+ (delta + 1) samples; both a fields are finite and >= 0
+    const float dA1 = fmaf(s.a > r.a ? s.a : r.a, a.aal, 1.0f);
+    const float fm = ft - dA1, fp = ft + dA1;
     const float flm = floorf(fm), flp = floorf(fp);
     const int itm = it + (int)flm, itp = it + (int)flp;
     const unsigned lim = (unsigned)(a.nt - 1);
     ok = ok && (unsigned)it < lim && (unsigned)itm < lim && (unsigned)itp < lim;
-    const float rr = __builtin_amdgcn_rcpf(3.0f + 2.0f * dA);
+    const float rr = __builtin_amdgcn_rcpf(fmaf(2.0f, dA1, 1.0f));  // 1 / (3 + 2 delta)
+    // a rejected update reads the trace's last pair, (0, 0), with finite
+    // fractions and a zero coefficient: it adds an exact zero to bin 0
     q.coef = ok ? aw * (rr * rr) : 0.0f;
-    q.w1 = w1;
+    q.w1 = ok ? w1 : 0.0f;
     q.ft = ft;
     q.ftm = fm - flm;
     q.ftp = fp - flp;
     q.ig0 = ok ? ig0 : 0;
-    q.it = ok ? it : 0;
-    q.itm = ok ? itm : 0;
-    q.itp = ok ? itp : 0;
+    q.it = ok ? it : (int)lim;
+    q.itm = ok ? itm : (int)lim;
+    q.itp = ok ? itp : (int)lim;
     return q;
 }
 
@@ -336,10 +369,10 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 rt[r].iw = 0u;
             }
         }
-        PackedTuple nxt = load_tuple<Fmt>(a, true, 0, i);
+        typename RawOf<Fmt>::type nxt = load_raw<Fmt>(a, true, 0, i);
         for (int js = 0; js < a.nshots; js++) {
-            const PackedTuple st = nxt;
-            if (js + 1 < a.nshots) nxt = load_tuple<Fmt>(a, true, (size_t)(js + 1), i);
+            const PackedTuple st = decode<Fmt>(nxt);
+            if (js + 1 < a.nshots) nxt = load_raw<Fmt>(a, true, (size_t)(js + 1), i);
             const bool live = valid && (st.iw & 0xFFFFu) != 0u;
             if (__ballot(live) == 0) continue;
             const int is = a.shot0 + js;
@@ -349,13 +382,21 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             for (int g0 = 0; g0 < R; g0 += G) {
                 // stage 1: opening angle and depth-mute weight (a wave-level
                 // skip of rejected groups measured slower: few groups are empty)
+                // The depth-mute ramp is evaluated only when some lane of the
+                // wave has an angle inside the taper band (cutoff, cutoff +
+                // oaztaper); outside it the weight is 1 or 0.
                 float ang[G], wap[G], wd[G];
+                bool band = false;
 #pragma unroll
                 for (int k = 0; k < G; k++) {
                     wap[k] = (r0 + g0 + k < a.nh) ? wrow[g0 + k] : -1.0f;
-                    ang[k] = pair_angle(st, rt[g0 + k], a.half_angle);
+                    ang[k] = pair_angle(st, rt[g0 + k], a.wrap);
                     wd[k] = ang[k] <= cutoff ? 1.0f : 0.0f;
-                    if (a.oaztaper > 0.0f) {
+                    band = band || (ang[k] > cutoff && ang[k] < cutoff + a.oaztaper);
+                }
+                if (a.oaztaper > 0.0f && __ballot(band) != 0) {
+#pragma unroll
+                    for (int k = 0; k < G; k++) {
                         const float rd = ramp_down(
                             fminf(fmaxf((ang[k] - cutoff) * a.inv_oaztaper, 0.0f), 1.0f));
                         wd[k] = ang[k] <= cutoff ? 1.0f
@@ -371,23 +412,30 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                     any = any || q[k].coef != 0.0f;
                 }
                 if (__ballot(any) == 0) continue;
+                // tap offsets are 32-bit byte offsets from the wave-uniform
+                // trace row (indices are in [0, nt - 1])
                 float2 v0[G], vm[G], vp[G];
 #pragma unroll
                 for (int k = 0; k < G; k++) {
-                    const float2 *tr = trow + (size_t)(g0 + k) * a.nt;
-                    v0[k] = tr[q[k].it];
-                    vm[k] = tr[q[k].itm];
-                    vp[k] = tr[q[k].itp];
+                    const char *tr = (const char *)(trow + (size_t)(g0 + k) * a.nt);
+                    v0[k] = *(const float2 *)(tr + ((uint32_t)q[k].it << 3));
+                    vm[k] = *(const float2 *)(tr + ((uint32_t)q[k].itm << 3));
+                    vp[k] = *(const float2 *)(tr + ((uint32_t)q[k].itp << 3));
                 }
+                // Branch-free accumulation: a rejected update adds an exact
+                // zero to bin 0, and a one-bin update adds its zero second
+                // share to the same bin, so the 3G loads above are not sunk
+                // into per-update branches and stay in flight together.
 #pragma unroll
                 for (int k = 0; k < G; k++) {
-                    if (q[k].coef == 0.0f) continue;
-                    const float sm = 2.0f * ((1.0f - q[k].ft) * v0[k].x + q[k].ft * v0[k].y) -
-                                     ((1.0f - q[k].ftm) * vm[k].x + q[k].ftm * vm[k].y) -
-                                     ((1.0f - q[k].ftp) * vp[k].x + q[k].ftp * vp[k].y);
+                    // pairs hold (tr[k], tr[k+1] - tr[k])
+                    const float sm = 2.0f * (v0[k].x + q[k].ft * v0[k].y) -
+                                     (vm[k].x + q[k].ftm * vm[k].y) -
+                                     (vp[k].x + q[k].ftp * vp[k].y);
                     const float picked = q[k].coef * sm;
-                    my[q[k].ig0 * P] += (1.0f - q[k].w1) * picked;
-                    if (q[k].w1 > 0.0f) my[(q[k].ig0 + 1) * P] += q[k].w1 * picked;
+                    float *b0 = my + q[k].ig0 * P;
+                    b0[0] += (1.0f - q[k].w1) * picked;
+                    b0[q[k].w1 > 0.0f ? P : 0] += q[k].w1 * picked;
                 }
             }
         }
@@ -466,9 +514,20 @@ static int launch_ph_t(const MigResArgs &a, hipStream_t st)
     return (int)hipGetLastError();
 }
 
-template <bool Probe>
+// Presets: tile shape (TZ x TX points), receivers per chunk R, group G.  The
+// 16-bit tuple formats are built for the four presets the autotuner keeps
+// for them (16x8 and 8x8 points, R = 8 or 16).
+template <bool Probe, int Fmt>
 static int launch_res_preset(const MigResArgs &a, int tile, hipStream_t st)
 {
+    if constexpr (Fmt != 0) {
+        switch (tile) {
+        case 10: return launch_ph_t<16, 8, 16, 4, Probe, Fmt>(a, st);
+        case 13: return launch_ph_t<8, 8, 16, 4, Probe, Fmt>(a, st);
+        case 15: return launch_ph_t<8, 8, 8, 4, Probe, Fmt>(a, st);
+        default: return launch_ph_t<16, 8, 8, 4, Probe, Fmt>(a, st);
+        }
+    }
     switch (tile) {
     case 10: return launch_ph_t<16, 8, 16, 4, Probe>(a, st);
     case 11: return launch_ph_t<16, 8, 8, 8, Probe>(a, st);
@@ -484,14 +543,25 @@ static int launch_res_preset(const MigResArgs &a, int tile, hipStream_t st)
     }
 }
 
-int launch_migrate_resident(const MigResArgs &a, int tile, void *stream)
+template <bool Probe>
+static int launch_res_prec(const MigResArgs &a, int tile, int precision, hipStream_t st)
 {
-    return launch_res_preset<false>(a, tile, (hipStream_t)stream);
+    switch (precision) {
+    case KMIG_PREC_FP16: return launch_res_preset<Probe, 1>(a, tile, st);
+    case KMIG_PREC_BF16: return launch_res_preset<Probe, 2>(a, tile, st);
+    case KMIG_PREC_U16: return launch_res_preset<Probe, 3>(a, tile, st);
+    default: return launch_res_preset<Probe, 0>(a, tile, st);
+    }
 }
 
-int launch_migrate_resident_probe(const MigResArgs &a, int tile, void *stream)
+int launch_migrate_resident(const MigResArgs &a, int tile, int precision, void *stream)
 {
-    return launch_res_preset<true>(a, tile, (hipStream_t)stream);
+    return launch_res_prec<false>(a, tile, precision, (hipStream_t)stream);
+}
+
+int launch_migrate_resident_probe(const MigResArgs &a, int tile, int precision, void *stream)
+{
+    return launch_res_prec<true>(a, tile, precision, (hipStream_t)stream);
 }
 
 int migrate_tiles(int nz, int nx, int tile)
@@ -501,49 +571,43 @@ int migrate_tiles(int nz, int nx, int tile)
     return ((nz + tz - 1) / tz) * ((nx + tx - 1) / tx);
 }
 
-template <int TZ, int TX, int Fmt>
+template <int TZ, int TX, int R, int Fmt>
 static int launch_multi_t(const MigMultiArgs &m, hipStream_t st)
 {
     const size_t lds = (size_t)m.prob[0].ng * TZ * TX * sizeof(float);
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_multi<TZ, TX, 8, 4, Fmt>,
+        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_multi<TZ, TX, R, 4, Fmt>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL((k_migrate_multi<TZ, TX, 8, 4, Fmt>), dim3((unsigned)m.tile0[m.nprob]),
+    hipLaunchKernelGGL((k_migrate_multi<TZ, TX, R, 4, Fmt>), dim3((unsigned)m.tile0[m.nprob]),
                        dim3(TZ * TX), lds, st, m);
     return (int)hipGetLastError();
+}
+
+// presets 9 (16x8 points, R = 8), 10 (16x8, R = 16), 15 (8x8, R = 8) and
+// 13 (8x8, R = 16); the grid in m.tile0 must use the same tile shape
+template <int Fmt>
+static int launch_multi_fmt(const MigMultiArgs &m, int tile, hipStream_t st)
+{
+    switch (tile) {
+    case 10: return launch_multi_t<16, 8, 16, Fmt>(m, st);
+    case 13: return launch_multi_t<8, 8, 16, Fmt>(m, st);
+    case 15: return launch_multi_t<8, 8, 8, Fmt>(m, st);
+    default: return launch_multi_t<16, 8, 8, Fmt>(m, st);
+    }
 }
 
 int launch_migrate_multi(const MigMultiArgs &m, int tile, int precision, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
-    const bool small = tile == 15 || tile == 13 || tile == 21 || tile == 1;
     switch (precision) {
-    case KMIG_PREC_FP16:
-        return small ? launch_multi_t<8, 8, 1>(m, st) : launch_multi_t<16, 8, 1>(m, st);
-    case KMIG_PREC_BF16:
-        return small ? launch_multi_t<8, 8, 2>(m, st) : launch_multi_t<16, 8, 2>(m, st);
-    case KMIG_PREC_U16:
-        return small ? launch_multi_t<8, 8, 3>(m, st) : launch_multi_t<16, 8, 3>(m, st);
-    default:
-        return small ? launch_multi_t<8, 8, 0>(m, st) : launch_multi_t<16, 8, 0>(m, st);
+    case KMIG_PREC_FP16: return launch_multi_fmt<1>(m, tile, st);
+    case KMIG_PREC_BF16: return launch_multi_fmt<2>(m, tile, st);
+    case KMIG_PREC_U16: return launch_multi_fmt<3>(m, tile, st);
+    default: return launch_multi_fmt<0>(m, tile, st);
     }
-}
-
-// 16-bit modes: the default preset shape only
-int launch_migrate_resident16(const MigResArgs &a, int tile, int precision, void *stream)
-{
-    hipStream_t st = (hipStream_t)stream;
-    const bool small = tile == 15 || tile == 13 || tile == 21 || tile == 1;
-    if (precision == KMIG_PREC_BF16)
-        return small ? launch_ph_t<8, 8, 8, 4, false, 2>(a, st)
-                     : launch_ph_t<16, 8, 8, 4, false, 2>(a, st);
-    if (precision == KMIG_PREC_U16)
-        return small ? launch_ph_t<8, 8, 8, 4, false, 3>(a, st)
-                     : launch_ph_t<16, 8, 8, 4, false, 3>(a, st);
-    return small ? launch_ph_t<8, 8, 8, 4, false, 1>(a, st) : launch_ph_t<16, 8, 8, 4, false, 1>(a, st);
 }
 
 int launch_migrate(const MigArgs &a, int tile, void *stream)

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_trace_prep(const TracePrepArgs a)
 // running sums sequentially (the reference's rounding order).
 //   k_trace_fwd: mute (+QC copies) and the causal sum of doubint
 //   k_trace_bwd: the anticausal sum, written as samples and, when asked for,
-//                as (tr[k], tr[k+1]) pairs
+//                as (tr[k], tr[k+1] - tr[k]) pairs
 constexpr int TB = 64;  // traces per block = samples per chunk
 
 __global__ __launch_bounds__(TB) void k_trace_fwd(const TracePrepArgs a)
@@ -192,7 +192,10 @@ __global__ __launch_bounds__(TB) void k_trace_bwd(const TracePrepArgs a, float2 
                 if (pairs) {
                     const float nx = (t + 1 < nc) ? tile[j][t + 1]
                                                   : (c0 + t + 1 < a.nt ? bnd[j] : 0.0f);
-                    pairs[(g0 + j) * a.nt + c0 + t] = make_float2(v, nx);
+                    // the last pair of a trace is (0, 0): no update reads
+                    // it, and rejected updates are pointed at it
+                    pairs[(g0 + j) * a.nt + c0 + t] =
+                        c0 + t + 1 < a.nt ? make_float2(v, nx - v) : make_float2(0.0f, 0.0f);
                 }
             }
         }
@@ -410,27 +413,54 @@ __global__ __launch_bounds__(256) void k_weight(const TuplePrepArgs a, Tuples o)
     o.W[oi] = w;
 }
 
-// Packed tuple of the resident migrate kernel (layout in kmig_internal.h).
-// T/dt is split into an integer and a fractional part in double, so the
-// per-update sum of source and receiver times keeps ~1e-7 sample precision.
-__global__ __launch_bounds__(256) void k_pack(const float *T, const float *A, const float *GX,
-                                              const float *GZ, const float *W, PackedTuple *out,
-                                              size_t n, double inv_dt, int angle_mode)
+// Fields shared by both packed tuple layouts (kmig_internal.h).  T/dt is split
+// into an integer and a fractional part in double, so the per-update sum of
+// source and receiver times keeps ~1e-7 sample precision.  The direction is
+// atan2(gx, gz) in output-angle units (th_scale = 180/pi, or 90/pi for half
+// opening angles).  A tuple with a non-finite time, anti-alias term or
+// direction gets weight 0 and zero fields (the reference rejects such an
+// update: its time window test fails), so every field the migrate kernel
+// reads is finite.
+struct PackFields {
+    uint32_t iw;
+    float frac, a, th;
+};
+
+__device__ inline PackFields pack_fields(float T, float A, float gx, float gz, float W,
+                                         double inv_dt, float th_scale)
 {
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const double tt = (double)T[k] * inv_dt;
+    PackFields f;
+    const double tt = (double)T * inv_dt;
     double ip = floor(tt);
     if (ip > 32767.0) ip = 32767.0;
     if (ip < -32768.0) ip = -32768.0;
+    f.frac = (float)(tt - ip);
+    f.a = (float)((double)A * inv_dt);
+    f.th = th_scale != 0.0f ? atan2f(gx, gz) * th_scale : 0.0f;
+    float w = W < 0.0f ? 0.0f : (W > 1.0f ? 1.0f : W);
+    if (!isfinite(f.frac) || !isfinite(f.a) || !isfinite(f.th)) {
+        w = 0.0f;
+        ip = 0.0;
+        f.frac = f.a = f.th = 0.0f;
+    }
+    f.iw = ((uint32_t)(uint16_t)(int16_t)(int)ip << 16) |
+           (uint32_t)__float2uint_rn(w * 32768.0f);
+    return f;
+}
+
+__global__ __launch_bounds__(256) void k_pack(const float *T, const float *A, const float *GX,
+                                              const float *GZ, const float *W, PackedTuple *out,
+                                              size_t n, double inv_dt, float th_scale)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const PackFields f = pack_fields(T[k], A[k], th_scale != 0.0f ? GX[k] : 0.0f,
+                                     th_scale != 0.0f ? GZ[k] : 0.0f, W[k], inv_dt, th_scale);
     PackedTuple q;
-    q.frac = (float)(tt - ip);
-    q.a = (float)((double)A[k] * inv_dt);
-    q.th = angle_mode ? atan2f(GX[k], GZ[k]) : 0.0f;
-    float w = W[k];
-    w = w < 0.0f ? 0.0f : (w > 1.0f ? 1.0f : w);
-    const uint32_t w16 = (uint32_t)__float2uint_rn(w * 32768.0f);
-    q.iw = ((uint32_t)(uint16_t)(int16_t)(int)ip << 16) | w16;
+    q.frac = f.frac;
+    q.a = f.a;
+    q.th = f.th;
+    q.iw = f.iw;
     out[k] = q;
 }
 
@@ -477,36 +507,33 @@ __global__ __launch_bounds__(256) void k_pack16(const float *T, const float *A, 
                                                 const float *GZ, const float *W,
                                                 const SlotScale *scale, PackedTuple16 *out,
                                                 size_t nzx, size_t n, double inv_dt,
-                                                int angle_mode, int precision)
+                                                float th_scale, int precision)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const SlotScale sc = scale[k / nzx];
-    const double tt = (double)T[k] * inv_dt;
-    double ip = floor(tt);
-    if (ip > 32767.0) ip = 32767.0;
-    if (ip < -32768.0) ip = -32768.0;
-    float w = W[k];
-    w = w < 0.0f ? 0.0f : (w > 1.0f ? 1.0f : w);
+    const PackFields f = pack_fields(T[k], A[k], th_scale != 0.0f ? GX[k] : 0.0f,
+                                     th_scale != 0.0f ? GZ[k] : 0.0f, W[k], inv_dt, th_scale);
     PackedTuple16 q;
-    q.iw = ((uint32_t)(uint16_t)(int16_t)(int)ip << 16) | (uint32_t)__float2uint_rn(w * 32768.0f);
-    const float f = (float)(tt - ip);
+    q.iw = f.iw;
     const uint32_t fq = precision == KMIG_PREC_U16
-                            ? (uint32_t)min(65535.0, rint((tt - ip) * 65536.0))
-                            : to16(f, precision);
-    const float an = (float)((double)A[k] * inv_dt) / sc.a_scale;
-    q.fa = fq | (to16(an, precision) << 16);
-    q.th = angle_mode ? __float_as_uint(atan2f(GX[k], GZ[k])) : 0u;
+                            ? (uint32_t)fminf(65535.0f, rintf(f.frac * 65536.0f))
+                            : to16(f.frac, precision);
+    q.fa = fq | (to16(f.a / sc.a_scale, precision) << 16);
+    q.th = __float_as_uint(f.th);
     out[k] = q;
 }
 
-// (tr[k], tr[k+1]) pairs, so a kirmig_pick tap pair is one 8-byte load.
+// (tr[k], tr[k+1] - tr[k]) pairs, so a kirmig_pick tap is one 8-byte load and
+// its linear interpolation one fused multiply-add.  The last pair of each
+// trace, which no update reads, is (0, 0).
 __global__ __launch_bounds__(256) void k_pairs(const float *prep, float2 *pairs, size_t n, int nt)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int it = (int)(k % nt);
-    pairs[k] = make_float2(prep[k], it + 1 < nt ? prep[k + 1] : 0.0f);
+    const float v = prep[k];
+    pairs[k] = it + 1 < nt ? make_float2(v, prep[k + 1] - v) : make_float2(0.0f, 0.0f);
 }
 
 __global__ void k_fill(float *p, size_t n, float v)
@@ -580,17 +607,17 @@ int launch_tuple_prep(const TuplePrepArgs &a, Tuples &o, float *tmp1, float *tmp
 }
 
 int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, float dt,
-                int angle_mode, void *stream)
+                float th_scale, void *stream)
 {
     const size_t n = (size_t)nslots * nzx;
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_pack, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, in.T, in.A,
-                       in.GX, in.GZ, in.W, out, n, 1.0 / (double)dt, angle_mode);
+                       in.GX, in.GZ, in.W, out, n, 1.0 / (double)dt, th_scale);
     return (int)hipGetLastError();
 }
 
 int launch_pack16(const Tuples &in, PackedTuple16 *out, SlotScale *scale, int nslots,
-                  size_t nzx, float dt, int angle_mode, int precision, void *stream)
+                  size_t nzx, float dt, float th_scale, int precision, void *stream)
 {
     const size_t n = (size_t)nslots * nzx;
     if (n == 0) return 0;
@@ -598,7 +625,7 @@ int launch_pack16(const Tuples &in, PackedTuple16 *out, SlotScale *scale, int ns
     hipLaunchKernelGGL(k_slot_range, dim3((unsigned)nslots), dim3(256), 0, st, in.A, in.W, scale,
                        nzx, 1.0 / (double)dt);
     hipLaunchKernelGGL(k_pack16, dim3(nblk(n, 256)), dim3(256), 0, st, in.T, in.A, in.GX, in.GZ,
-                       in.W, (const SlotScale *)scale, out, nzx, n, 1.0 / (double)dt, angle_mode,
+                       in.W, (const SlotScale *)scale, out, nzx, n, 1.0 / (double)dt, th_scale,
                        precision);
     return (int)hipGetLastError();
 }
