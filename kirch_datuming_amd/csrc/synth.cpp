@@ -8,6 +8,13 @@
 // centred difference of neighbouring position tables (one-sided at the ends),
 // the derivative the Hermite interpolation along the position axis consumes.
 //
+// Surrogate training data: Gaussian-random-field velocity models (white noise
+// smoothed by a separable Gaussian filter, rescaled, added to a random linear
+// v(z) trend, clipped) and records (s, z, x, T, dT/dx, dT/dz, dT/ds) sampled
+// from fast-marching tables: dT/dx, dT/dz are centred differences of the
+// table on the grid (one-sided at edges), dT/ds the centred difference of
+// the tables at s -/+ one grid step.
+//
 // Data: 15-20 Hz Ricker wavelets at analytic two-way times (flat reflectors
 // in v(z) = v0 + g z, or point scatterers with table times), plus seeded
 // Gaussian noise.  The RNG is splitmix64 (Steele, Lea, Flood 2014) with a
@@ -215,9 +222,130 @@ void smooth5(std::vector<double> &v, int nz, int nx)
         }
 }
 
+// separable Gaussian filter of standard deviation sig (cells), radius 3 sig,
+// edge-normalised
+void gauss2(std::vector<double> &v, int nz, int nx, double sig_z, double sig_x)
+{
+    auto kern = [](double sig) {
+        const int r = std::max(1, (int)std::ceil(3.0 * sig));
+        std::vector<double> k(2 * r + 1);
+        for (int i = -r; i <= r; i++) k[i + r] = std::exp(-0.5 * i * i / (sig * sig));
+        return k;
+    };
+    const std::vector<double> kz = kern(sig_z), kx = kern(sig_x);
+    const int rz = (int)kz.size() / 2, rx = (int)kx.size() / 2;
+    std::vector<double> t(v.size());
+    for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
+            double s = 0.0, w = 0.0;
+            for (int k = -rz; k <= rz; k++) {
+                const int jz = iz + k;
+                if (jz < 0 || jz >= nz) continue;
+                s += kz[k + rz] * v[(size_t)ix * nz + jz];
+                w += kz[k + rz];
+            }
+            t[(size_t)ix * nz + iz] = s / w;
+        }
+    for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
+            double s = 0.0, w = 0.0;
+            for (int k = -rx; k <= rx; k++) {
+                const int jx = ix + k;
+                if (jx < 0 || jx >= nx) continue;
+                s += kx[k + rx] * t[(size_t)jx * nz + iz];
+                w += kx[k + rx];
+            }
+            v[(size_t)ix * nz + iz] = s / w;
+        }
+}
+
 }  // namespace
 
 extern "C" {
+
+// Gaussian-random-field model: v = v0 + g z + amp * f(z, x), f a unit-variance
+// field with Gaussian correlation exp(-r^2 / corr^2) (white noise through a
+// Gaussian filter of standard deviation corr / 2), clipped to [vmin, vmax].
+// v0 in [1.5, 2.0], g in [0.2, 0.6] 1/s and amp in [0.15, 0.45] km/s are
+// drawn from the seed.
+void kms_velocity_grf(float *v, int nz, int nx, float dz, float dx, float corr, float vmin,
+                      float vmax, uint64_t seed)
+{
+    Rng rng(seed);
+    const double v0 = 1.5 + 0.5 * rng.uniform(), g = 0.2 + 0.4 * rng.uniform();
+    const double amp = 0.15 + 0.3 * rng.uniform();
+    std::vector<double> f((size_t)nz * nx);
+    for (double &q : f) q = rng.normal();
+    gauss2(f, nz, nx, 0.5 * corr / dz, 0.5 * corr / dx);
+    double m = 0.0, e = 0.0;
+    for (double q : f) m += q;
+    m /= (double)f.size();
+    for (double q : f) e += (q - m) * (q - m);
+    const double sd = std::sqrt(e / (double)f.size());
+    for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
+            const size_t k = (size_t)ix * nz + iz;
+            double vel = v0 + g * iz * (double)dz + amp * (f[k] - m) / (sd > 0.0 ? sd : 1.0);
+            vel = std::min(std::max(vel, (double)vmin), (double)vmax);
+            v[k] = (float)vel;
+        }
+}
+
+// Training records of one model: for each of npos source positions drawn
+// uniformly in [s_lo, s_hi] (sources at depth zs), nper records at grid
+// nodes drawn uniformly in the window iz < nz_win, ix < nx_win.  out holds
+// npos * nper records of 7 floats.  Returns 0.
+int kms_nn_records(const float *vel, int nz, int nx, float dz, float dx, float zs, int npos,
+                   float s_lo, float s_hi, int nz_win, int nx_win, int nper, int box,
+                   uint64_t seed, int nthreads, float *out)
+{
+    const size_t n = (size_t)nz * nx;
+    std::vector<double> slow(n);
+    for (size_t k = 0; k < n; k++) slow[k] = 1.0 / (double)vel[k];
+    nz_win = std::min(std::max(nz_win, 1), nz);
+    nx_win = std::min(std::max(nx_win, 1), nx);
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+    {
+        std::vector<uint8_t> state;
+        std::vector<double> T0(n), Tm(n), Tp(n);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic)
+#endif
+        for (int k = 0; k < npos; k++) {
+            Rng rng(seed ^ (0x2545F4914F6CDD1Dull * (uint64_t)(k + 1)));
+            const double xs = s_lo + (s_hi - s_lo) * rng.uniform();
+            const double h = dx;
+            fmm(slow.data(), nz, nx, dz, dx, zs, xs, box, T0.data(), state, true);
+            fmm(slow.data(), nz, nx, dz, dx, zs, xs - h, box, Tm.data(), state, true);
+            fmm(slow.data(), nz, nx, dz, dx, zs, xs + h, box, Tp.data(), state, true);
+            float *o = out + (size_t)k * nper * 7;
+            for (int q = 0; q < nper; q++) {
+                const int iz = std::min(nz_win - 1, (int)(nz_win * rng.uniform()));
+                const int ix = std::min(nx_win - 1, (int)(nx_win * rng.uniform()));
+                const size_t c = (size_t)ix * nz + iz;
+                const int za = std::max(iz - 1, 0), zb = std::min(iz + 1, nz - 1);
+                const int xa = std::max(ix - 1, 0), xb = std::min(ix + 1, nx - 1);
+                const double tz = (T0[(size_t)ix * nz + zb] - T0[(size_t)ix * nz + za]) /
+                                  ((zb - za) * (double)dz);
+                const double tx = (T0[(size_t)xb * nz + iz] - T0[(size_t)xa * nz + iz]) /
+                                  ((xb - xa) * (double)dx);
+                const double ts = (Tp[c] - Tm[c]) / (2.0 * h);
+                float *r = o + (size_t)q * 7;
+                r[0] = (float)xs;
+                r[1] = (float)(iz * (double)dz);
+                r[2] = (float)(ix * (double)dx);
+                r[3] = (float)T0[c];
+                r[4] = (float)tx;
+                r[5] = (float)tz;
+                r[6] = (float)ts;
+            }
+        }
+    }
+    return 0;
+}
 
 // v(z) = v0 + g z on (nx, nz), z fastest.
 void kms_velocity_vz(float *v, int nz, int nx, float dz, float v0, float g)
