@@ -36,7 +36,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="salt", choices=["salt", "small", "layers"])
+    ap.add_argument("--config", default="salt",
+                    choices=["salt", "small", "layers", "surrogate"])
+    ap.add_argument("--grf-models", type=int, default=256,
+                    help="surrogate: velocity models of the pretraining set")
+    ap.add_argument("--fit-steps", type=int, default=8000,
+                    help="surrogate: Adam steps on the workload's own tables")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "u16"])
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--variant", default="auto", choices=["auto", "stream", "resident"])
@@ -72,6 +77,8 @@ def main():
 
     if args.config == "layers":
         return bench_layers(args, cores)
+    if args.config == "surrogate":
+        return bench_surrogate(args, cores)
     t0 = time.time()
     log(f"building the {args.config} workload on {cores} host threads")
     w = configs.salt(threads=cores) if args.config == "salt" else configs.small(threads=cores)
@@ -151,8 +158,11 @@ def main():
             cig_sub = ms.cig()
         log("FP64 oracle on the same sample")
         cig64 = O.migrate(w.kv, *sub.args(), precision="f64")[0]
+        bins = [O.rel_l2(cig_sub[g], cig64[g]) for g in range(cig64.shape[0])
+                if np.any(cig64[g])]
         parity = {"sample": f"shots {shots}",
                   "rel_l2_gpu_vs_f64": O.rel_l2(cig_sub, cig64),
+                  "rel_l2_gpu_vs_f64_worst_angle_bin": float(max(bins)) if bins else None,
                   "rel_l2_cpu_f32_vs_f64": O.rel_l2(cig_cpu, cig64),
                   "rel_l2_gpu_vs_cpu_f32": O.rel_l2(cig_sub, cig_cpu)}
         log(f"CPU {cpu['value']:.3e} updates/s; relL2 on the sample: GPU vs f64 "
@@ -180,7 +190,8 @@ def main():
                    "variant": {0: "auto", 1: "stream", 2: "resident"}[st["variant"]]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_migrate", "algorithmic_bytes_per_update": bpu,
+                     "kernel": "k_migrate_ph" if st["variant"] == 2 else "k_migrate",
+                     "algorithmic_bytes_per_update": bpu,
                      "updates_per_launch": upd_per_launch, "avg_launch_ms": mig_avg},
         "cpu_baseline": cpu,
         "breakdown_ms": {"trace_prep": float(np.mean(prep_ms)), "tuple_prep": float(np.mean(tup_ms)),
@@ -274,6 +285,166 @@ def bench_layers(args, cores):
     b.close()
     for m in ms:
         m.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_surrogate(args, cores):
+    """Learned-surrogate form of the small workload: the network is pretrained
+    on fast-marching tables of seeded random velocity models, fitted to the
+    workload's own tables (and, for comparison, fitted from scratch), then
+    evaluated inside every step's tuple preparation on both sides, with the
+    tables as the per-point fallback (|dT| > 1 ms) and the judge."""
+    import numpy as np
+
+    from kirch_datuming_amd import configs, kmig, nn
+    from oracle import oracle as O
+    FLOP_PER_EVAL = 2.0 * (128 * 48 + 3 * 128 * 128 + 4 * 128)
+    BF16_PEAK = 2.5e15
+    t0 = time.time()
+    w = configs.small(threads=cores)
+    ns, nh = w.data.shape[0], w.data.shape[1]
+    pos = np.concatenate([w.s_axes[2][1] + np.arange(ns) * w.s_axes[2][2],
+                          w.r_axes[2][1] + np.arange(nh) * w.r_axes[2][2]])
+    rec_t = configs.table_records(np.concatenate([w.stab, w.rtab]),
+                                  np.concatenate([w.sder, w.rder]), pos, 8192, 20260606)
+    log(f"small workload and {rec_t.shape[0]} table records in {time.time() - t0:.1f} s")
+    t0 = time.time()
+    rec_g = configs.grf_records(nmodels=args.grf_models, threads=cores)
+    t_grf = time.time() - t0
+    log(f"{rec_g.shape[0]} records from {args.grf_models} random models in {t_grf:.1f} s")
+    Z, X = np.meshgrid(w.s_axes[0][1] + np.arange(w.s_axes[0][0]) * w.s_axes[0][2],
+                       w.s_axes[1][1] + np.arange(w.s_axes[1][0]) * w.s_axes[1][2])
+
+    def check(net):
+        err = []
+        for k in range(ns):
+            s = pos[k]
+            o = net.eval(np.stack([np.full(Z.size, s), Z.ravel(), X.ravel()], 1).astype(np.float32))
+            err.append(np.abs(o[:, 0] - w.stab[k].ravel()))
+        e = np.concatenate(err)
+        return {"mean_abs_dt_s": float(e.mean()), "p99_abs_dt_s": float(np.quantile(e, 0.99)),
+                "max_abs_dt_s": float(e.max()), "frac_within_1ms": float((e <= 1e-3).mean())}
+
+    nets, report = {}, {}
+    pre = nn.Surrogate(**configs.SURROGATE)
+    pre.set_data(rec_g)
+    lp = pre.train(3000)
+    report["pretrain"] = {"steps": 3000, "loss_first": float(lp[0]), "loss_last": float(lp[-1]),
+                          **pre.stats(), "check_vs_workload_tables": check(pre)}
+    pre.set_data(rec_t)
+    lf = pre.train(args.fit_steps)
+    nets["pretrained_then_fitted"] = pre
+    report["pretrained_then_fitted"] = {"steps": args.fit_steps, "loss_last": float(lf[-1]),
+                                        **pre.stats(), "check": check(pre)}
+    scratch = nn.Surrogate(**configs.SURROGATE)
+    scratch.set_data(rec_t)
+    ls = scratch.train(args.fit_steps)
+    nets["fitted_from_scratch"] = scratch
+    report["fitted_from_scratch"] = {"steps": args.fit_steps, "loss_last": float(ls[-1]),
+                                     **scratch.stats(), "check": check(scratch)}
+    best = min(nets, key=lambda k: report[k]["check"]["mean_abs_dt_s"])
+    net = nets[best]
+    tr = report[best]
+    log(f"surrogate used: {best}; check {report[best]['check']}")
+
+    kv = dict(w.kv, variant=args.variant)
+    if args.tile:
+        kv["tile"] = args.tile
+    ax = w.data_axes
+    p = kmig.parse_params(kv, int(ax[1][0]), ax[1][1], ax[1][2])
+
+    def migrator(surrogate, data_w):
+        m = kmig.Migrator(p)
+        m.set_tables(0, w.stab, w.sder, w.s_axes)
+        m.set_tables(1, w.rtab, w.rder, w.r_axes)
+        m.set_data(data_w.data, data_w.data_axes)
+        if surrogate:
+            m.set_surrogate(0, net, 1e-3)
+            m.set_surrogate(1, net, 1e-3)
+        return m
+
+    def timed(m):
+        for _ in range(args.warmup):
+            m.run()
+            m.sync()
+        tup, tot = [], []
+        ts = time.perf_counter()
+        for _ in range(args.steps):
+            m.run()
+            m.sync()
+            st = m.stats()
+            tup.append(st["ms_tuple_prep"])
+            tot.append(st["ms_total"])
+        wall = time.perf_counter() - ts
+        return wall, float(np.mean(tup)), float(np.mean(tot))
+
+    mt = migrator(False, w)
+    wall_t, tup_t, tot_t = timed(mt)
+    mt.close()
+    m = migrator(True, w)
+    wall, tup_n, tot_n = timed(m)
+    sst = [m.surrogate_stats(k) for k in (0, 1)]
+    m.close()
+    points = sum(s["points"] for s in sst)
+    used = sum(s["used"] for s in sst)
+    nn_ms = max(tup_n - tup_t, 1e-6)
+    mfma = points * FLOP_PER_EVAL / (nn_ms * 1e-3)
+    value = w.updates * args.steps / wall
+    log(f"surrogate step {wall / args.steps * 1e3:.2f} ms ({value:.3e} updates/s), table step "
+        f"{wall_t / args.steps * 1e3:.2f} ms; network step ~{nn_ms:.2f} ms, "
+        f"{mfma / 1e12:.1f} TFLOP/s; {used / points:.3f} of the points from the network")
+
+    # parity on evenly spaced shots: surrogate and table CIGs against FP64
+    k = max(2, args.cpu_shots)
+    shots = [ns * (j + 1) // (k + 1) for j in range(k)]
+    step = shots[1] - shots[0]
+    shots = [shots[0] + j * step for j in range(k)]
+    subw = w.shot_subset(shots)
+    cig = {}
+    for name, sur in (("table", False), ("surrogate", True)):
+        ms = migrator(sur, subw)
+        ms.run()
+        ms.sync()
+        cig[name] = ms.cig()
+        ms.close()
+    cig64 = O.migrate(w.kv, *subw.args(), precision="f64")[0]
+    cpu = None
+    if not args.no_cpu:
+        tc = time.perf_counter()
+        O.migrate(w.kv, *subw.args(), precision="f32")
+        tcpu = time.perf_counter() - tc
+        cpu = {"value": subw.updates / tcpu, "unit": "updates/s", "cores": cores, "kind": "port",
+               "sample": f"shots {shots} ({subw.updates:.3e} updates), {tcpu:.1f} s, "
+                         f"tables (the reference has no surrogate)"}
+    table_bytes = 4 * 2 * (w.stab.size + w.rtab.size)
+    out = {
+        "metric": METRIC, "value": value, "unit": "updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 migration, bf16 network (fp32 accumulate)", "data": "synthetic",
+        "config": {"workload": w.name + ", traveltimes from the learned surrogate (table "
+                               "fallback where |dT| > 1 ms)", "config": "surrogate",
+                   "updates_per_step": w.updates, "network": "48 Fourier -> 4x128 SiLU -> 4"},
+        "roofline": {"bound": "mfma", "achieved": mfma / 1e12, "peak": BF16_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": mfma / BF16_PEAK, "traffic": None,
+                     "kernel": "k_nn_infer (network step = tuple preparation with the network "
+                               "minus without it)",
+                     "flop_per_evaluation": FLOP_PER_EVAL, "evaluations_per_step": points},
+        "cpu_baseline": cpu,
+        "table_step": {"ms_per_step": wall_t / args.steps * 1e3,
+                       "updates_per_s": w.updates * args.steps / wall_t,
+                       "tuple_prep_ms": tup_t, "events_total_ms": tot_t},
+        "surrogate_step": {"tuple_prep_ms": tup_n, "events_total_ms": tot_n,
+                           "points_from_network": used, "points": points,
+                           "max_residual_s": max(s["max_residual"] for s in sst)},
+        "parity": {"sample": f"shots {shots}",
+                   "rel_l2_surrogate_vs_f64": O.rel_l2(cig["surrogate"], cig64),
+                   "rel_l2_table_vs_f64": O.rel_l2(cig["table"], cig64)},
+        "training": report, "network_used": best, "grf_records_s": t_grf,
+        "memory_bytes": {"tables": table_bytes, "network_fp32": 4 * nn.PARAMS},
+    }
+    for q in nets.values():
+        q.close()
     print(json.dumps(out), flush=True)
 
 

@@ -187,6 +187,22 @@ int kmig_batch_stats(const kmig_batch *b, double *updates, double *ms, int *grap
 const char *kmig_batch_last_error(const kmig_batch *b);
 void kmig_batch_destroy(kmig_batch *b);
 
+/* ---- learned traveltime surrogate (include/kmig_nn.h) ----
+   No counterpart in the reference, whose traveltimes always come from the
+   tables (load_table_slice and tinterp_hermite, src/kirchmig2d_cig.cpp:274-284,
+   :987-1029, :1148-1180).  With a network set on a side, every run evaluates
+   it at each (slot position, grid point) right after the table interpolation
+   and takes its traveltime and dT/ds wherever |T_nn - T_table| <= max_dt
+   (seconds); elsewhere the table values stay.  Gradients, smoothing, weights
+   and the migration then proceed as without it.  nn = NULL removes it.  The
+   network must outlive its use by the context. */
+struct kmig_nn;
+int kmig_set_surrogate(kmig_ctx *ctx, int side, struct kmig_nn *nn, float max_dt);
+/* After a run: grid points evaluated on that side, points that took the
+   network's values, and the largest |T_nn - T_table| seen. */
+int kmig_get_surrogate_stats(kmig_ctx *ctx, int side, double *points, double *used,
+                             float *max_residual);
+
 /* Library version string. */
 const char *kmig_version(void);
 

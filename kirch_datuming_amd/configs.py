@@ -55,6 +55,8 @@ def synth():
                                           f, C.c_uint64]
         L.kms_data_scatterers.argtypes = [fp, i, f, f, i, f, f, i, f, f, i, fp, i, f, f, fp, i,
                                           f, f, i, i, i, f, f, C.c_uint64]
+        L.kms_velocity_grf.argtypes = [fp, i, i, f, f, f, f, f, C.c_uint64]
+        L.kms_nn_records.argtypes = [fp, i, i, f, f, f, i, f, f, i, i, i, i, C.c_uint64, i, fp]
         _SYN = L
     return _SYN
 
@@ -194,3 +196,47 @@ def layers(threads=0, seed=20260303, scale=1.0):
 def stack_layers(cigs):
     """Concatenate per-layer CIGs (ng, nx, nz_k) along depth into one volume."""
     return np.concatenate(cigs, axis=2)
+
+
+# ----------------------------------------------------------- surrogate --
+# The learned traveltime surrogate of the small workload: inputs scaled over
+# its 3 x 2 km section (positions 0..3 km), reference slowness 1/2 s/km.
+SURROGATE = dict(s_min=0.0, s_max=3.0, z_min=0.0, z_max=2.0, x_min=0.0, x_max=3.0, z_src=0.0,
+                 slowness=0.5, sigma_t=0.001, sigma_g=0.3, batch=16384, lr=2e-3, lr_final=2e-5)
+
+
+def table_records(T, D, pos, nper, seed, dz=0.01, dx=0.01):
+    """Training records (s, z, x, T, dT/dx, dT/dz, dT/ds) drawn from position
+    tables T, D (npos, nx, nz): nper random grid nodes per position; dT/dx and
+    dT/dz are centred differences of the table (one-sided at the edges)."""
+    rng = np.random.default_rng(seed)
+    _, nx, nz = T.shape
+    out = []
+    for k, s in enumerate(pos):
+        Tk = T[k].astype(np.float64)
+        gx = np.gradient(Tk, dx, axis=0)
+        gz = np.gradient(Tk, dz, axis=1)
+        ix = rng.integers(0, nx, nper)
+        iz = rng.integers(0, nz, nper)
+        out.append(np.stack([np.full(nper, s), iz * dz, ix * dx, Tk[ix, iz], gx[ix, iz],
+                             gz[ix, iz], D[k][ix, iz]], axis=1))
+    return np.concatenate(out).astype(np.float32)
+
+
+def grf_records(nmodels=256, npos=8, nper=2048, seed=20260505, threads=0, corr=0.3,
+                window=(201, 301), s_hi=3.0):
+    """Records from fast-marching tables of nmodels seeded Gaussian-random-field
+    velocity models (nz=400, nx=1000, dz=dx=0.01 km, v in 1.5..4.5 km/s,
+    correlation length corr km): npos random surface sources in [0, s_hi] km
+    per model (tables at s and s -/+ one cell for dT/ds), nper records per
+    source inside the top-left window (nz, nx) of the small workload."""
+    L = synth()
+    nz, nx = 400, 1000
+    v = np.empty((nx, nz), np.float32)
+    out = np.empty((nmodels * npos * nper, 7), np.float32)
+    for m in range(nmodels):
+        L.kms_velocity_grf(_p(v), nz, nx, 0.01, 0.01, corr, 1.5, 4.5, seed + 7919 * m)
+        blk = out[m * npos * nper:(m + 1) * npos * nper]
+        L.kms_nn_records(_p(v), nz, nx, 0.01, 0.01, 0.0, npos, 0.0, s_hi, window[0], window[1],
+                         nper, 2, seed + 104729 * (m + 1), threads, _p(blk))
+    return out

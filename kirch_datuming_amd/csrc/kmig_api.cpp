@@ -19,6 +19,7 @@
 
 #include "kmig.h"
 #include "kmig_internal.h"
+#include "kmig_nn_internal.h"
 
 using namespace kmig;
 
@@ -30,6 +31,13 @@ struct Side {
     float o = 0.0f, d = 0.0f;
     bool set = false;
     size_t cap = 0;  // elements allocated per volume
+    // learned surrogate of the table interpolation (kmig_set_surrogate)
+    kmig_nn *nn = nullptr;
+    float nn_max_dt = 0.0f;
+    int *nn_used = nullptr;          // [slot] points taken from the network
+    unsigned *nn_maxres = nullptr;   // [slot] max |T_nn - T_table| (float bits)
+    size_t nn_cap = 0, nn_cap_m = 0; // slots allocated
+    size_t nn_slots = 0;             // slots of the last plan
 };
 
 struct Batch {
@@ -500,6 +508,9 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
     const kmig_params &p = c->p;
     const bool angle = p.cig && p.cig_is_angle;
     int rc;
+    for (Side &sd : c->side)
+        if (sd.nn && nn_prepare(sd.nn) != KMIG_OK)
+            return fail(c, KMIG_E_DEVICE, "surrogate: %s", kmig_nn_last_error(sd.nn));
     if (c->plan_version != c->geo_version || sb != c->last_sb || se != c->last_se) {
         std::vector<TraceMeta> meta;
         std::vector<SlotSpec> sspec, rspec;
@@ -523,6 +534,15 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
         if (!rspec.empty())
             HIPCHK(c, hipMemcpy(c->spec_r, rspec.data(), rspec.size() * sizeof(SlotSpec),
                                 hipMemcpyHostToDevice));
+        for (int k = 0; k < 2; k++) {
+            Side &sd = c->side[k];
+            sd.nn_slots = k == 0 ? sspec.size() : rspec.size();
+            if (!sd.nn) continue;
+            if ((rc = ensure(c, &sd.nn_used, &sd.nn_cap, std::max<size_t>(1, sd.nn_slots))))
+                return rc;
+            if ((rc = ensure(c, &sd.nn_maxres, &sd.nn_cap_m, std::max<size_t>(1, sd.nn_slots))))
+                return rc;
+        }
         size_t maxs = 0, maxr = 0;
         for (const Batch &b : c->batches) {
             maxs = std::max(maxs, (size_t)(b.s1 - b.s0));
@@ -642,6 +662,28 @@ int enqueue_trace_prep(kmig_ctx *c, hipStream_t st, int sb, int se)
     return KMIG_OK;
 }
 
+// Table interpolation, then (for a side with a surrogate) the network step
+// on the slots [slot0, slot0 + a.nslots) of that side's counters, then
+// gradients, smoothing and weights.
+int side_tuples(kmig_ctx *c, int side, const TuplePrepArgs &a, Tuples &o, size_t slot0,
+                hipStream_t st)
+{
+    Side &sd = c->side[side];
+    HIPCHK(c, (hipError_t)launch_tuple_interp(a, o, st));
+    if (sd.nn && a.nslots > 0) {
+        HIPCHK(c, (hipError_t)launch_fill((float *)(sd.nn_used + slot0), (size_t)a.nslots, 0.0f,
+                                          st));
+        HIPCHK(c, (hipError_t)launch_fill((float *)(sd.nn_maxres + slot0), (size_t)a.nslots,
+                                          0.0f, st));
+        HIPCHK(c, (hipError_t)nn_replace_tuples(sd.nn, a.spec, a.nslots, c->nz, c->nzx, c->z0,
+                                                c->dz, c->x0, c->dx, a.dside, sd.nn_max_dt, o.T,
+                                                o.A, sd.nn_used + slot0, sd.nn_maxres + slot0,
+                                                st));
+    }
+    HIPCHK(c, (hipError_t)launch_tuple_finish(a, o, c->tmp1, c->tmp2, st));
+    return KMIG_OK;
+}
+
 // Tuple preparation (and packing) of one shot batch on stream st.
 int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_s_off)
 {
@@ -678,7 +720,8 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
     A.nslots = nsb;
     A.dpos = c->side[0].d;
     A.dside = ds;
-    HIPCHK(c, (hipError_t)launch_tuple_prep(A, c->ts, c->tmp1, c->tmp2, st));
+    int rc = side_tuples(c, 0, A, c->ts, spec_s_off, st);
+    if (rc) return rc;
     // receiver side
     A.tab = c->side[1].T;
     A.der = c->side[1].D;
@@ -688,7 +731,7 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
     A.nslots = b.nrcv;
     A.dpos = c->side[1].d;
     A.dside = dh;
-    HIPCHK(c, (hipError_t)launch_tuple_prep(A, c->tr, c->tmp1, c->tmp2, st));
+    if ((rc = side_tuples(c, 1, A, c->tr, b.spec_r0, st))) return rc;
     // packed directions in output-angle degrees (the opening angle is then
     // a difference of two directions folded at 360, or 180 for half angles)
     const float th_scale = angle ? (float)((p.half_angle ? 90.0 : 180.0) / M_PI) : 0.0f;
@@ -748,6 +791,8 @@ void kmig_destroy(kmig_ctx *c)
         dfree(s.D);
         dfree(s.GX);
         dfree(s.GZ);
+        dfree(s.nn_used);
+        dfree(s.nn_maxres);
     }
     dfree(c->raw);
     dfree(c->prep);
@@ -784,6 +829,49 @@ void kmig_destroy(kmig_ctx *c)
 const char *kmig_last_error(const kmig_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
 void *kmig_stream(kmig_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int kmig_set_surrogate(kmig_ctx *c, int side, kmig_nn *nn, float max_dt)
+{
+    if (!c) return KMIG_E_PARAM;
+    if (side != 0 && side != 1) return fail(c, KMIG_E_PARAM, "side must be 0 or 1");
+    if (nn && !(max_dt >= 0.0f))
+        return fail(c, KMIG_E_PARAM, "surrogate residual limit must be >= 0");
+    c->side[side].nn = nn;
+    c->side[side].nn_max_dt = max_dt;
+    c->geo_version++;  // the next run plans its counters
+    return KMIG_OK;
+}
+
+int kmig_get_surrogate_stats(kmig_ctx *c, int side, double *points, double *used,
+                             float *max_residual)
+{
+    if (!c) return KMIG_E_PARAM;
+    if (side != 0 && side != 1) return fail(c, KMIG_E_PARAM, "side must be 0 or 1");
+    Side &sd = c->side[side];
+    if (!sd.nn || !sd.nn_used || !c->ran)
+        return fail(c, KMIG_E_STATE, "no surrogate run on this side");
+    int rc = kmig_sync(c);
+    if (rc) return rc;
+    std::vector<int> u(sd.nn_slots);
+    std::vector<unsigned> m(sd.nn_slots);
+    if (!u.empty()) {
+        HIPCHK(c, hipMemcpy(u.data(), sd.nn_used, u.size() * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(m.data(), sd.nn_maxres, m.size() * sizeof(unsigned),
+                            hipMemcpyDeviceToHost));
+    }
+    double su = 0.0;
+    float mx = 0.0f;
+    for (size_t k = 0; k < u.size(); k++) {
+        su += u[k];
+        float f;
+        memcpy(&f, &m[k], sizeof(f));
+        mx = std::max(mx, f);
+    }
+    if (points) *points = (double)sd.nn_slots * (double)c->nzx;
+    if (used) *used = su;
+    if (max_residual) *max_residual = mx;
+    return KMIG_OK;
+}
 
 int kmig_set_tables(kmig_ctx *c, int side, const float *T, const float *dTds, const float *gx,
                     const float *gz, int nz, int nx, int npos, float z0, float dz, float x0,

@@ -172,6 +172,11 @@ int launch_trace_prep(const TracePrepArgs &a, void *stream);
 int launch_trace_prep2(const TracePrepArgs &a, float *pairs, float *tmp, void *stream);
 int launch_tuple_prep(const TuplePrepArgs &a, Tuples &out, float *tmp1, float *tmp2,
                       void *stream);
+// the same in two parts: table interpolation (T, A, gradient files), then
+// gradients, smoothing and weights
+int launch_tuple_interp(const TuplePrepArgs &a, Tuples &out, void *stream);
+int launch_tuple_finish(const TuplePrepArgs &a, Tuples &out, float *tmp1, float *tmp2,
+                        void *stream);
 int launch_migrate(const MigArgs &a, int tile, void *stream);
 int migrate_lds_bytes(int ng, int tile);
 int launch_fill(float *p, size_t n, float v, void *stream);

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "kmig.h"
+#include "kmig_internal.h"
 #include "kmig_nn.h"
 #include "kmig_nn_internal.h"
 
@@ -194,8 +195,26 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x)
     return x ^ (x >> 31);
 }
 
+// sin and cos of pi 2^f u for f = f0 .. f0+3, as (sin, cos) pairs: one
+// sincosf, then double-angle steps (the error of the highest frequency is a
+// few 1e-6, far below the BF16 rounding of the features)
+__device__ __forceinline__ void fourier4(float u, int f0, float *o)
+{
+    float sv, cv;
+    sincosf(3.14159265358979f * (float)(1 << f0) * u, &sv, &cv);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        o[2 * k] = sv;
+        o[2 * k + 1] = cv;
+        const float s2 = 2.0f * sv * cv, c2 = 1.0f - 2.0f * sv * sv;
+        sv = s2;
+        cv = c2;
+    }
+}
+
 // Fourier features of one point, BF16, into row n of X [n][64] (48 used) and
-// column n of Xt [64][n] (nullable)
+// column n of Xt [64][n] (nullable); column 16 c + 2 f + {0, 1} holds
+// {sin, cos}(pi 2^f u_c)
 __device__ void features(const NnScale &sc, float s, float z, float x, bf16 *X, bf16 *Xt,
                          long n, long N)
 {
@@ -203,19 +222,16 @@ __device__ void features(const NnScale &sc, float s, float z, float x, bf16 *X, 
     bf16 *row = X + (size_t)n * NN_KIN;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        float f = 3.14159265358979f * u[c];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            float sv, cv;
-            sincosf(f, &sv, &cv);
-            const int col = c * 16 + 2 * k;
-            row[col] = (bf16)sv;
-            row[col + 1] = (bf16)cv;
-            if (Xt) {
-                Xt[(size_t)col * N + n] = (bf16)sv;
-                Xt[(size_t)(col + 1) * N + n] = (bf16)cv;
+        for (int half = 0; half < 2; half++) {
+            float f[8];
+            fourier4(u[c], 4 * half, f);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int col = c * 16 + half * 8 + k;
+                row[col] = (bf16)f[k];
+                if (Xt) Xt[(size_t)col * N + n] = (bf16)f[k];
             }
-            f *= 2.0f;
         }
     }
 #pragma unroll
@@ -260,28 +276,6 @@ __global__ void k_records(const float *rec, long nreal, NnScale sc, bf16 *X, bf1
     dist[n] = src_dist(sc, s, z, x);
     for (int k = 0; k < 4; k++) tgt[(size_t)n * 4 + k] = real ? r[3 + k] : 0.0f;
     wt[n] = real ? 1.0f / (float)nreal : 0.0f;
-}
-
-__global__ void k_points(const float *pts, long n0, long N, NnScale sc, bf16 *X, float *dist)
-{
-    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    const float *p = pts + (size_t)(n0 + n) * 3;
-    features(sc, p[0], p[1], p[2], X, nullptr, n, N);
-    dist[n] = src_dist(sc, p[0], p[1], p[2]);
-}
-
-// network outputs -> T and derivatives
-__global__ void k_outputs(const float *Y, const float *dist, float S0, float *out, long N)
-{
-    const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    const float *y = Y + (size_t)n * 4;
-    float *o = out + (size_t)n * 4;
-    o[0] = dist[n] * S0 * (1.0f + y[0]);
-    o[1] = S0 * y[1];
-    o[2] = S0 * y[2];
-    o[3] = S0 * y[3];
 }
 
 // db[m] += sum_n G[m][n] (one block per m)
@@ -370,31 +364,33 @@ __global__ void k_cast(const float *W, int M, int K, bf16 *Wb, int ldw, bf16 *WT
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
-// ----------------------------------------------------- network plumbing --
-
 // layer l: fan-in K, outputs M, padded K of the BF16 operand, offsets of W, b
-static const int LK[5] = {NN_FEAT, NN_W, NN_W, NN_W, NN_W};
-static const int LM[5] = {NN_W, NN_W, NN_W, NN_W, NN_OUT};
-static const int LKP[5] = {NN_KIN, NN_W, NN_W, NN_W, NN_W};
+__host__ __device__ constexpr int lk(int l) { return l == 0 ? NN_FEAT : NN_W; }
+__host__ __device__ constexpr int lm(int l) { return l == 4 ? NN_OUT : NN_W; }
+__host__ __device__ constexpr int lkp(int l) { return l == 0 ? NN_KIN : NN_W; }
 
-static size_t woff(int l)
+__host__ __device__ constexpr size_t woff(int l)
 {
     size_t o = 0;
-    for (int k = 0; k < l; k++) o += (size_t)LM[k] * LK[k] + LM[k];
+    for (int k = 0; k < l; k++) o += (size_t)lm(k) * lk(k) + lm(k);
     return o;
 }
-static size_t boff(int l) { return woff(l) + (size_t)LM[l] * LK[l]; }
+__host__ __device__ constexpr size_t boff(int l) { return woff(l) + (size_t)lm(l) * lk(l); }
+static_assert(boff(4) + NN_OUT == NN_PARAMS, "parameter layout");
+
+// ----------------------------------------------------- network plumbing --
+
 
 // columns of the transposed BF16 weight (the M of the backward-data product,
 // padded to a multiple of 32 for the output layer)
-static int ldt(int l) { return l == 4 ? 32 : LM[l]; }
+static int ldt(int l) { return l == 4 ? 32 : lm(l); }
 
 void nn_cast_weights(const NnNet &net, hipStream_t st)
 {
     for (int l = 0; l < 5; l++) {
-        const int rows = std::max(LM[l], ldt(l)), cols = std::max(LK[l], LKP[l]);
+        const int rows = std::max(lm(l), ldt(l)), cols = std::max(lk(l), lkp(l));
         hipLaunchKernelGGL(k_cast, dim3(nblk((size_t)rows * cols, 256)), dim3(256), 0, st,
-                           net.w + woff(l), LM[l], LK[l], net.Wb[l], LKP[l], net.WT[l], ldt(l));
+                           net.w + woff(l), lm(l), lk(l), net.Wb[l], lkp(l), net.WT[l], ldt(l));
     }
 }
 
@@ -406,12 +402,300 @@ static void forward(const NnNet &net, hipStream_t st, const bf16 *X, long N, bf1
     const bf16 *in = X;
     for (int l = 0; l < 4; l++) {
         EpiFwd e{net.w + boff(l), Z ? Z[l] : nullptr, H[l], Ht ? Ht[l] : nullptr, NN_W, N};
-        gemm(st, net.Wb[l], LKP[l], in, LKP[l], NN_W, (int)N, LKP[l], 1, e);
+        gemm(st, net.Wb[l], lkp(l), in, lkp(l), NN_W, (int)N, lkp(l), 1, e);
         in = H[l];
     }
     EpiOut o = out;
     o.bias = net.w + boff(4);
     gemm(st, net.Wb[4], NN_W, in, NN_W, NN_OUT, (int)N, NN_W, 1, o);
+}
+
+// ------------------------------------------------------ fused inference --
+//
+// k_nn_infer evaluates the whole network for 64 points per wave with the
+// activations kept in registers.  The accumulator of a 16 x 16 MFMA tile
+// gives lane (r = l % 16, q = l / 16) the outputs m = 16 a + 4 q + e
+// (e = 0..3) of point n = 16 b + r.  The next layer's B operand for K step
+// ks needs 8 values of point n in that lane: it takes e = 0..3 of tiles
+// a = 2 ks and 2 ks + 1, i.e. K positions
+//     pi(ks, q, j) = 32 ks + (j < 4 ? 4 q + j : 16 + 4 q + j - 4),
+// and the weights are stored with their columns in that order, so the sum
+// over K is unchanged.  The first layer's B operand is the Fourier features
+// in natural order (lane group q, step ks: features 32 ks + 8 q + j).
+//
+// Weights live in LDS as BF16 (118 784 bytes, one copy per block; blocks are
+// persistent): per layer, row m holds its 16-byte K chunks c = 4 ks + q at
+// chunk position c ^ (m % 16) (c ^ (m % 8) for the 64-column first layer),
+// so the 16 rows a fragment load touches fall in distinct banks.
+
+constexpr int NN_BLOB_W1 = NN_W * NN_KIN;               // 8192
+constexpr int NN_BLOB_WH = NN_W * NN_W;                 // 16384
+constexpr int NN_BLOB_W5 = 16 * NN_W;                   // 2048 (rows 4..15 zero)
+constexpr int NN_BLOB_BF16 = NN_BLOB_W1 + 3 * NN_BLOB_WH + NN_BLOB_W5;
+constexpr int NN_BLOB_BIAS = 4 * NN_W + NN_OUT;         // FP32 biases
+
+__device__ __forceinline__ int perm_k(int ks, int q, int j)
+{
+    return 32 * ks + (j < 4 ? 4 * q + j : 16 + 4 * q + j - 4);
+}
+
+// master FP32 parameters -> the swizzled BF16 blob and the bias block
+__global__ void k_nn_blob(const float *w, bf16 *blob, float *bias)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < NN_BLOB_BF16) {
+        int l, rem;
+        if (t < NN_BLOB_W1) {
+            l = 0;
+            rem = t;
+        } else if (t < NN_BLOB_W1 + 3 * NN_BLOB_WH) {
+            l = 1 + (t - NN_BLOB_W1) / NN_BLOB_WH;
+            rem = (t - NN_BLOB_W1) % NN_BLOB_WH;
+        } else {
+            l = 4;
+            rem = t - NN_BLOB_W1 - 3 * NN_BLOB_WH;
+        }
+        const int cols = l == 0 ? NN_KIN : NN_W;
+        const int m = rem / cols, pos = rem % cols;
+        const int cpos = pos / 8, j = pos % 8;
+        const int c = l == 0 ? (cpos ^ (m & 7)) : (cpos ^ (m & 15));  // logical chunk
+        const int ks = c / 4, q = c % 4;
+        float v = 0.0f;
+        if (l == 0) {
+            const int k = 32 * ks + 8 * q + j;
+            if (k < NN_FEAT) v = w[woff(0) + (size_t)m * NN_FEAT + k];
+        } else {
+            const int k = perm_k(ks, q, j);
+            if (m < lm(l)) v = w[woff(l) + (size_t)m * NN_W + k];
+        }
+        blob[t] = (bf16)v;
+    }
+    if (t < NN_BLOB_BIAS) {
+        const int l = t / NN_W, m = t % NN_W;
+        bias[t] = l < 4 ? w[boff(l) + m] : w[boff(4) + (t - 4 * NN_W)];
+    }
+}
+
+__device__ __forceinline__ float silu_f(float z)
+{
+    return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+
+// 8 features of lane group q at K step ks for coordinates u (s, z, x)
+__device__ __forceinline__ bf16x8 feat_frag(const float *u, int ks, int q)
+{
+    bf16x8 f = {};
+    if (ks == 1 && q >= 2) return f;
+    const float uc = ks == 1 ? u[2] : ((q >> 1) == 0 ? u[0] : u[1]);
+    float v[8];
+    fourier4(uc, 4 * (q & 1), v);
+#pragma unroll
+    for (int j = 0; j < 8; j++) f[j] = (bf16)v[j];
+    return f;
+}
+
+struct NnInferArgs {
+    const bf16 *blob;
+    const float *bias;
+    NnScale sc;
+    long npts;
+    // points mode: pts [n][3] (s, z, x) -> out [n][4]
+    const float *pts;
+    float *out;
+    // tuple mode: point p = slot * nzx + i at (surf_x of slot, grid point i);
+    // T and A of the table path are replaced where |T_nn - T| <= max_dt
+    const SlotSpec *spec;
+    int nz;
+    long nzx;
+    float z0, dz, x0, dx, dside, max_dt;
+    float *T, *A;
+    int *used;          // [slot] points taken from the network
+    unsigned *maxres;   // [slot] max |T_nn - T| (float bits)
+};
+
+template <bool Tuples>
+__global__ __launch_bounds__(256) void k_nn_infer(const NnInferArgs a)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    bf16 *W = (bf16 *)lds_raw;
+    float *B = (float *)(lds_raw + NN_BLOB_BF16 * sizeof(bf16));
+    for (int k = threadIdx.x; k < NN_BLOB_BF16 / 8; k += blockDim.x)
+        ((uint4 *)W)[k] = ((const uint4 *)a.blob)[k];
+    for (int k = threadIdx.x; k < NN_BLOB_BIAS; k += blockDim.x) B[k] = a.bias[k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+    const long ntiles = (a.npts + 63) / 64;
+    const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6), nwaves = (long)gridDim.x * 4;
+    for (long tile = wave; tile < ntiles; tile += nwaves) {
+        // the lane's four points n = 16 b + r
+        float u[4][3], dist[4];
+        long pidx[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            long p = tile * 64 + 16 * b + r;
+            pidx[b] = p;
+            if (p >= a.npts) p = a.npts - 1;
+            float s, z, x;
+            if (Tuples) {
+                const int slot = (int)(p / a.nzx);
+                const long i = p % a.nzx;
+                s = a.spec[slot].surf_x;
+                z = a.z0 + (float)(i % a.nz) * a.dz;
+                x = a.x0 + (float)(i / a.nz) * a.dx;
+            } else {
+                s = a.pts[p * 3];
+                z = a.pts[p * 3 + 1];
+                x = a.pts[p * 3 + 2];
+            }
+            u[b][0] = (s - a.sc.s0) * a.sc.s_inv;
+            u[b][1] = (z - a.sc.z0) * a.sc.z_inv;
+            u[b][2] = (x - a.sc.x0) * a.sc.x_inv;
+            const float ddx = x - s, ddz = z - a.sc.zsrc;
+            dist[b] = sqrtf(ddx * ddx + ddz * ddz);
+        }
+        bf16x8 bf[4][4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            bf[b][0] = feat_frag(u[b], 0, q);
+            bf[b][1] = feat_frag(u[b], 1, q);
+        }
+        f32x4 acc[8][4];
+        const bf16 *Wl = W;
+        for (int l = 0; l < 4; l++) {
+            const int nks = l == 0 ? 2 : 4;
+            const int cols = l == 0 ? NN_KIN : NN_W;
+            const int cmask = l == 0 ? 7 : 15;
+#pragma unroll
+            for (int am = 0; am < 8; am++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[am][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++) {
+                if (ks < nks) {
+#pragma unroll
+                    for (int am = 0; am < 8; am++) {
+                        const int m = 16 * am + r;
+                        const int c = (4 * ks + q) ^ (m & cmask);
+                        const bf16x8 wa = *(const bf16x8 *)(Wl + (size_t)m * cols + 8 * c);
+#pragma unroll
+                        for (int b = 0; b < 4; b++)
+                            acc[am][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, bf[b][ks],
+                                                                                 acc[am][b], 0, 0, 0);
+                    }
+                }
+            }
+            Wl += (size_t)NN_W * cols;
+            // SiLU in registers -> next layer's B fragments (K order pi)
+            const float *bl = B + l * NN_W;
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+#pragma unroll
+                for (int ks = 0; ks < 4; ks++) {
+                    bf16x8 h;
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        h[e] = (bf16)silu_f(acc[2 * ks][b][e] + bl[32 * ks + 4 * q + e]);
+                        h[4 + e] = (bf16)silu_f(acc[2 * ks + 1][b][e] + bl[32 * ks + 16 + 4 * q + e]);
+                    }
+                    bf[b][ks] = h;
+                }
+        }
+        // output layer: one 16-row tile, rows 0..3 real (lane group q = 0)
+        f32x4 y[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) y[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            const int c = (4 * ks + q) ^ (r & 15);
+            const bf16x8 wa = *(const bf16x8 *)(Wl + (size_t)r * NN_W + 8 * c);
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                y[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, bf[b][ks], y[b], 0, 0, 0);
+        }
+        const float *b5 = B + 4 * NN_W;
+        int rep[4] = {0, 0, 0, 0};
+        float resb[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const long p = pidx[b];
+            if (q != 0 || p >= a.npts) continue;
+            const float S0 = a.sc.S0;
+            const float T = dist[b] * S0 * (1.0f + y[b][0] + b5[0]);
+            const float Ts = S0 * (y[b][3] + b5[3]);
+            if (Tuples) {
+                float res = fabsf(T - a.T[p]);
+                if (res <= a.max_dt) {
+                    a.T[p] = T;
+                    a.A[p] = fabsf(Ts * a.dside);
+                    rep[b] = 1;
+                }
+                resb[b] = res == res ? res : INFINITY;
+            } else {
+                float *o = a.out + p * 4;
+                o[0] = T;
+                o[1] = S0 * (y[b][1] + b5[1]);
+                o[2] = S0 * (y[b][2] + b5[2]);
+                o[3] = Ts;
+            }
+        }
+        if (Tuples) {
+            // per-slot counters: one atomic per wave when its 64 points share
+            // a slot, per point otherwise (tiles across a slot boundary)
+            const long pf = tile * 64, pl = min(pf + 63, a.npts - 1);
+            const int s0 = (int)(pf / a.nzx), s1 = (int)(pl / a.nzx);
+            if (s0 == s1) {
+                int cnt = rep[0] + rep[1] + rep[2] + rep[3];
+                float mx = fmaxf(fmaxf(resb[0], resb[1]), fmaxf(resb[2], resb[3]));
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    cnt += __shfl_xor(cnt, off, 64);
+                    mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+                }
+                if (lane == 0) {
+                    atomicAdd(a.used + s0, cnt);
+                    atomicMax(a.maxres + s0, __float_as_uint(mx));
+                }
+            } else if (q == 0) {
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    if (pidx[b] >= a.npts) continue;
+                    const int slot = (int)(pidx[b] / a.nzx);
+                    if (rep[b]) atomicAdd(a.used + slot, 1);
+                    atomicMax(a.maxres + slot, __float_as_uint(resb[b]));
+                }
+            }
+        }
+    }
+}
+
+static size_t infer_lds() { return NN_BLOB_BF16 * sizeof(bf16) + NN_BLOB_BIAS * sizeof(float); }
+
+static int launch_infer(const NnInferArgs &a, bool tuples, hipStream_t st)
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+    }
+    const long tiles = (a.npts + 63) / 64;
+    const unsigned grid = (unsigned)std::max<long>(1, std::min<long>((tiles + 3) / 4, cus));
+    const int lds = (int)infer_lds();
+    hipError_t e;
+    if (tuples) {
+        e = hipFuncSetAttribute((const void *)k_nn_infer<true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e == hipSuccess)
+            hipLaunchKernelGGL(k_nn_infer<true>, dim3(grid), dim3(256), lds, st, a);
+    } else {
+        e = hipFuncSetAttribute((const void *)k_nn_infer<false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e == hipSuccess)
+            hipLaunchKernelGGL(k_nn_infer<false>, dim3(grid), dim3(256), lds, st, a);
+    }
+    return e != hipSuccess ? (int)e : (int)hipGetLastError();
 }
 
 // ---------------------------------------------------------------- host --
@@ -441,6 +725,9 @@ struct kmig_nn {
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double train_ms = 0.0, train_flops = 0.0;
+    bf16 *blob = nullptr;       // inference weights (k_nn_infer layout)
+    float *bias = nullptr;
+    bool blob_valid = false;
     std::string err;
 };
 
@@ -584,8 +871,8 @@ double step_flops(long N)
     // forward 2 * MACs, backward data + weight gradients 4 * MACs (the input
     // layer has no backward-data product)
     double mac = 0.0, macx = 0.0;
-    for (int l = 0; l < 5; l++) mac += (double)LM[l] * LK[l];
-    macx = (double)LM[0] * LK[0];
+    for (int l = 0; l < 5; l++) mac += (double)lm(l) * lk(l);
+    macx = (double)lm(0) * lk(0);
     return (double)N * (2.0 * mac + 4.0 * mac - 2.0 * macx);
 }
 
@@ -617,6 +904,53 @@ int capture_step(kmig_nn *n)
 }
 
 }  // namespace
+
+namespace kmig {
+
+int nn_prepare(kmig_nn *n)
+{
+    if (!n) return KMIG_E_PARAM;
+    if (n->blob_valid) return KMIG_OK;
+    int rc = KMIG_OK;
+    if (!n->blob) rc = nalloc(n, &n->blob, NN_BLOB_BF16);
+    if (!rc && !n->bias) rc = nalloc(n, &n->bias, NN_BLOB_BIAS);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_nn_blob, dim3(nblk(NN_BLOB_BF16, 256)), dim3(256), 0, n->stream,
+                       (const float *)n->net.w, n->blob, n->bias);
+    NHIP(n, hipGetLastError());
+    NHIP(n, hipStreamSynchronize(n->stream));
+    n->blob_valid = true;
+    return KMIG_OK;
+}
+
+int nn_replace_tuples(const kmig_nn *n, const SlotSpec *spec, int nslots, int nz, size_t nzx,
+                      float z0, float dz, float x0, float dx, float dside, float max_dt, float *T,
+                      float *A, int *used, unsigned *maxres, hipStream_t st)
+{
+    if (!n || !n->blob_valid) return (int)hipErrorNotReady;
+    if (nslots <= 0 || nzx == 0) return 0;
+    NnInferArgs a{};
+    a.blob = n->blob;
+    a.bias = n->bias;
+    a.sc = make_scale(n->cfg);
+    a.npts = (long)nslots * (long)nzx;
+    a.spec = spec;
+    a.nz = nz;
+    a.nzx = (long)nzx;
+    a.z0 = z0;
+    a.dz = dz;
+    a.x0 = x0;
+    a.dx = dx;
+    a.dside = dside;
+    a.max_dt = max_dt;
+    a.T = T;
+    a.A = A;
+    a.used = used;
+    a.maxres = maxres;
+    return launch_infer(a, true, st);
+}
+
+}  // namespace kmig
 
 extern "C" {
 
@@ -670,8 +1004,8 @@ int kmig_nn_create(kmig_nn **out, const kmig_nn_config *cfg)
     rc = rc ? rc : nalloc(n, &n->ctr, 4);
     rc = rc ? rc : nalloc(n, &n->gloss, 1);
     for (int l = 0; l < 5 && !rc; l++) {
-        rc = nalloc(n, &n->net.Wb[l], (size_t)LM[l] * LKP[l]);
-        if (!rc) rc = nalloc(n, &n->net.WT[l], (size_t)LK[l] * ldt(l));
+        rc = nalloc(n, &n->net.Wb[l], (size_t)lm(l) * lkp(l));
+        if (!rc) rc = nalloc(n, &n->net.WT[l], (size_t)lk(l) * ldt(l));
     }
     if (rc) return rc;
     // initial weights: uniform with variance 1 / fan-in, the output layer
@@ -679,8 +1013,8 @@ int kmig_nn_create(kmig_nn **out, const kmig_nn_config *cfg)
     std::vector<float> w(NN_PARAMS, 0.0f);
     uint64_t st = cfg->seed;
     for (int l = 0; l < 5; l++) {
-        const float a = std::sqrt(3.0f / (float)LK[l]) * (l == 4 ? 0.1f : 1.0f);
-        for (size_t k = 0; k < (size_t)LM[l] * LK[l]; k++) {
+        const float a = std::sqrt(3.0f / (float)lk(l)) * (l == 4 ? 0.1f : 1.0f);
+        for (size_t k = 0; k < (size_t)lm(l) * lk(l); k++) {
             st = hmix(st);
             const float u = (float)((st >> 40) * (1.0 / 16777216.0));
             w[woff(l) + k] = a * (2.0f * u - 1.0f);
@@ -700,6 +1034,7 @@ int kmig_nn_set_params(kmig_nn *n, const float *params)
     hipLaunchKernelGGL(k_fillf, dim3(nblk(NN_PARAMS, 256)), dim3(256), 0, n->stream, n->v,
                        (size_t)NN_PARAMS, 0.0f);
     nn_cast_weights(n->net, n->stream);
+    n->blob_valid = false;
     const int zero[4] = {0, 0, 0, 0};
     NHIP(n, hipMemcpyAsync(n->ctr, zero, sizeof(zero), hipMemcpyHostToDevice, n->stream));
     NHIP(n, hipStreamSynchronize(n->stream));
@@ -763,6 +1098,7 @@ int kmig_nn_train(kmig_nn *n, int steps, float *loss)
     host[1] = 0;
     host[2] = steps;
     NHIP(n, hipMemcpy(n->ctr, host, sizeof(host), hipMemcpyHostToDevice));
+    n->blob_valid = false;
     NHIP(n, hipEventRecord(n->e0, n->stream));
     for (int s = 0; s < steps; s++) NHIP(n, hipGraphLaunch(n->exec, n->stream));
     NHIP(n, hipEventRecord(n->e1, n->stream));
@@ -808,9 +1144,7 @@ int kmig_nn_grad(kmig_nn *n, const float *rec, long count, float *grad, float *l
 int kmig_nn_eval(kmig_nn *n, const float *pts, long count, float *out)
 {
     if (!n || !pts || !out || count <= 0) return KMIG_E_PARAM;
-    const long chunk = 65536;
-    const long N = std::min<long>((count + 63) / 64 * 64, chunk);
-    int rc = ensure_batch(n, std::max<long>(N, n->N));
+    int rc = nn_prepare(n);
     if (rc) return rc;
     float *dp = nullptr, *dout = nullptr;
     if ((rc = nalloc(n, &dp, (size_t)count * 3))) return rc;
@@ -819,23 +1153,17 @@ int kmig_nn_eval(kmig_nn *n, const float *pts, long count, float *out)
         return rc;
     }
     hipError_t e = hipMemcpy(dp, pts, (size_t)count * 3 * sizeof(float), hipMemcpyHostToDevice);
-    hipStream_t st = n->stream;
-    for (long n0 = 0; n0 < count && e == hipSuccess; n0 += chunk) {
-        const long m = std::min(chunk, count - n0);
-        const long Np = (m + 63) / 64 * 64;
-        hipLaunchKernelGGL(k_points, dim3(nblk((size_t)m, 256)), dim3(256), 0, st, dp, n0, m,
-                           make_scale(n->cfg), n->X, n->dist);
-        if (Np > m)  // padding rows of X: zeros
-            hipLaunchKernelGGL(k_fillb, dim3(nblk((size_t)(Np - m) * NN_KIN, 256)), dim3(256), 0,
-                               st, n->X + (size_t)m * NN_KIN, (size_t)(Np - m) * NN_KIN);
-        EpiOut o{};
-        o.Y = n->Y;
-        forward(n->net, st, n->X, Np, n->H, nullptr, nullptr, o);
-        hipLaunchKernelGGL(k_outputs, dim3(nblk((size_t)m, 256)), dim3(256), 0, st, n->Y,
-                           n->dist, n->cfg.slowness, dout + (size_t)n0 * 4, m);
-        e = hipGetLastError();
+    if (e == hipSuccess) {
+        NnInferArgs a{};
+        a.blob = n->blob;
+        a.bias = n->bias;
+        a.sc = make_scale(n->cfg);
+        a.npts = count;
+        a.pts = dp;
+        a.out = dout;
+        e = (hipError_t)launch_infer(a, false, n->stream);
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
     if (e == hipSuccess)
         e = hipMemcpy(out, dout, (size_t)count * 4 * sizeof(float), hipMemcpyDeviceToHost);
     nfree(dp);
@@ -868,7 +1196,7 @@ void kmig_nn_destroy(kmig_nn *n)
     }
     void *bufs[] = {n->m,    n->v,     n->grad, n->dev_rec, n->X,  n->Xt,   n->dist,
                     n->tgt,  n->wt,    n->Y,    n->lossb,   n->dY, n->dYt,  n->ctr,
-                    n->hist, n->gloss};
+                    n->hist, n->gloss, n->blob, n->bias};
     for (void *p : bufs) nfree(p);
     for (int l = 0; l < 4; l++) {
         nfree(n->Z[l]);

@@ -34,4 +34,20 @@ struct NnNet {
 
 void nn_cast_weights(const NnNet &net, hipStream_t st);
 
+struct SlotSpec;
+
+// Brings the network's inference weights up to date (synchronous on the
+// network's stream); call before enqueueing nn_replace_tuples.
+int nn_prepare(kmig_nn *nn);
+
+// Surrogate step of the tuple preparation: for every (slot, grid point) of
+// T/A ([nslots][nzx], filled by the table interpolation), evaluates the
+// network at (surf_x of the slot, z0 + iz dz, x0 + ix dx) and, where
+// |T_nn - T| <= max_dt, replaces T by T_nn and A by |dT/ds_nn * dside|.
+// used[slot] counts the replaced points, maxres[slot] keeps the largest
+// |T_nn - T| (float bits); both must be cleared by the caller.
+int nn_replace_tuples(const kmig_nn *nn, const SlotSpec *spec, int nslots, int nz, size_t nzx,
+                      float z0, float dz, float x0, float dx, float dside, float max_dt, float *T,
+                      float *A, int *used, unsigned *maxres, hipStream_t st);
+
 }  // namespace kmig

@@ -580,12 +580,20 @@ int launch_trace_prep(const TracePrepArgs &a, void *stream)
     return (int)hipGetLastError();
 }
 
-int launch_tuple_prep(const TuplePrepArgs &a, Tuples &o, float *tmp1, float *tmp2, void *stream)
+int launch_tuple_interp(const TuplePrepArgs &a, Tuples &o, void *stream)
+{
+    if (a.nslots <= 0) return 0;
+    const dim3 grid(nblk(a.nzx, 256), (unsigned)a.nslots), blk(256);
+    hipLaunchKernelGGL(k_interp, grid, blk, 0, (hipStream_t)stream, a, o);
+    return (int)hipGetLastError();
+}
+
+int launch_tuple_finish(const TuplePrepArgs &a, Tuples &o, float *tmp1, float *tmp2,
+                        void *stream)
 {
     if (a.nslots <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
     const dim3 grid(nblk(a.nzx, 256), (unsigned)a.nslots), blk(256);
-    hipLaunchKernelGGL(k_interp, grid, blk, 0, st, a, o);
     if (a.angle_mode) {
         if (!a.use_grad_files) hipLaunchKernelGGL(k_grad, grid, blk, 0, st, a, o);
         if (a.smooth > 0) {
@@ -604,6 +612,12 @@ int launch_tuple_prep(const TuplePrepArgs &a, Tuples &o, float *tmp1, float *tmp
     }
     hipLaunchKernelGGL(k_weight, grid, blk, 0, st, a, o);
     return (int)hipGetLastError();
+}
+
+int launch_tuple_prep(const TuplePrepArgs &a, Tuples &o, float *tmp1, float *tmp2, void *stream)
+{
+    const int e = launch_tuple_interp(a, o, stream);
+    return e ? e : launch_tuple_finish(a, o, tmp1, tmp2, stream);
 }
 
 int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, float dt,

@@ -30,7 +30,8 @@ EXPORTS = ("kmig_params_default", "kmig_params_parse", "kmig_params_check", "kmi
            "kmig_destroy", "kmig_last_error", "kmig_set_tables", "kmig_set_data", "kmig_run",
            "kmig_sync", "kmig_get_cig", "kmig_get_qc", "kmig_get_stats", "kmig_migrate",
            "kmig_stream", "kmig_version", "kmig_batch_create", "kmig_batch_run",
-           "kmig_batch_sync", "kmig_batch_stats", "kmig_batch_last_error", "kmig_batch_destroy")
+           "kmig_batch_sync", "kmig_batch_stats", "kmig_batch_last_error", "kmig_batch_destroy",
+           "kmig_set_surrogate", "kmig_get_surrogate_stats")
 
 
 class KmigError(RuntimeError):
@@ -112,6 +113,9 @@ def lib():
         L.kmig_batch_last_error.restype = C.c_char_p
         L.kmig_batch_destroy.argtypes = [vp]
         L.kmig_batch_destroy.restype = None
+        L.kmig_set_surrogate.argtypes = [vp, C.c_int, vp, C.c_float]
+        L.kmig_get_surrogate_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double), C.POINTER(C.c_float)]
         _LIB = L
     return _LIB
 
@@ -212,6 +216,21 @@ class Migrator:
 
     def stream(self):
         return self.L.kmig_stream(self.ctx)
+
+    def set_surrogate(self, side, net, max_dt=1.0e-3):
+        """Take traveltimes of `side` from a trained nn.Surrogate wherever they
+        are within max_dt seconds of the table (net=None removes it)."""
+        h = net.h if net is not None else None
+        if not hasattr(self, "_nets"):
+            self._nets = {}
+        self._nets[side] = net  # keep the network alive while the context uses it
+        self._chk(self.L.kmig_set_surrogate(self.ctx, int(side), h, float(max_dt)))
+
+    def surrogate_stats(self, side) -> dict:
+        pts, used, mx = C.c_double(), C.c_double(), C.c_float()
+        self._chk(self.L.kmig_get_surrogate_stats(self.ctx, int(side), C.byref(pts),
+                                                  C.byref(used), C.byref(mx)))
+        return {"points": pts.value, "used": used.value, "max_residual": mx.value}
 
     def close(self):
         if self.ctx:

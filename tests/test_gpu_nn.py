@@ -84,3 +84,60 @@ def test_training_fits_vz_traveltimes():
     loss2 = net.train(50)
     assert loss2.mean() < 2.0 * loss[-50:].mean()
     net.close()
+
+
+def test_surrogate_in_the_migration():
+    """The trained network serves most grid points of the small v(z) workload
+    within 1 ms of the fast-marching tables (an untrained one almost none),
+    with weights 0.2 % of the table bytes it stands in for; removing it gives
+    the table-only CIG bit for bit."""
+    from kirch_datuming_amd import configs, kmig, nn
+    w = configs.small(threads=8).shot_subset(list(range(4, 61, 8)))
+    full = configs.small(threads=8)
+    pos = np.concatenate([np.arange(61) * 0.05, np.arange(201) * 0.015])
+    rec = configs.table_records(np.concatenate([full.stab, full.rtab]),
+                                np.concatenate([full.sder, full.rder]), pos, 8192, 5)
+    p = kmig.parse_params(w.kv, 201, 0.0, 0.015)
+
+    def run(net):
+        m = kmig.Migrator(p)
+        m.set_tables(0, w.stab, w.sder, w.s_axes)
+        m.set_tables(1, w.rtab, w.rder, w.r_axes)
+        m.set_data(w.data, w.data_axes)
+        if net is not None:
+            m.set_surrogate(0, net, 1e-3)
+            m.set_surrogate(1, net, 1e-3)
+        m.run()
+        m.sync()
+        c = m.cig()
+        st = [m.surrogate_stats(k) for k in (0, 1)] if net is not None else None
+        if net is not None:
+            m.set_surrogate(0, None)
+            m.set_surrogate(1, None)
+            m.run()
+            m.sync()
+            c_off = m.cig()
+        else:
+            c_off = None
+        m.close()
+        return c, st, c_off
+
+    cig_t, _, _ = run(None)
+    untrained = nn.Surrogate(**configs.SURROGATE)
+    _, st0, _ = run(untrained)
+    frac0 = sum(s["used"] for s in st0) / sum(s["points"] for s in st0)
+    net = nn.Surrogate(**configs.SURROGATE)
+    net.set_data(rec)
+    net.train(8000)
+    cig_n, st, cig_off = run(net)
+    frac = sum(s["used"] for s in st) / sum(s["points"] for s in st)
+    rel = _rel(cig_n, cig_t)
+    print(f"points from the network: untrained {frac0:.4f}, trained {frac:.4f}; "
+          f"max residual {max(s['max_residual'] for s in st):.2e} s; CIG relL2 vs tables {rel:.3f}")
+    assert frac0 < 0.1
+    assert frac > 0.85
+    assert rel < 0.5          # network traveltimes within 1 ms, CIG within 50 % (relL2)
+    assert np.array_equal(cig_off, cig_t)
+    assert 4 * nn.PARAMS < 0.01 * 4 * 2 * (full.stab.size + full.rtab.size)
+    untrained.close()
+    net.close()

@@ -16,19 +16,23 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "kirch_datuming_amd", "bin", "kirchmig2d_cig")
 
 
-def _header_functions():
-    txt = open(os.path.join(ROOT, "include", "kmig.h")).read()
+def _header_functions(name="kmig.h"):
+    txt = open(os.path.join(ROOT, "include", name)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(kmig_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(kmig_[a-z_0-9]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():
+    from kirch_datuming_amd import nn
     L = kmig.lib()
     names = _header_functions()
     assert set(names) == set(kmig.EXPORTS)
-    for n in names:
+    nn_names = _header_functions("kmig_nn.h")
+    assert set(nn_names) == set(nn.EXPORTS)
+    for n in names + nn_names:
         assert hasattr(L, n), n
     assert b"gfx950" in L.kmig_version()
+    assert set(os.listdir(os.path.join(ROOT, "include"))) == {"kmig.h", "kmig_nn.h"}
 
 
 @pytest.mark.parametrize("name", gc.case_names())
