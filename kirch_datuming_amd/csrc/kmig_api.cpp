@@ -664,9 +664,13 @@ int enqueue_trace_prep(kmig_ctx *c, hipStream_t st, int sb, int se)
 
 // Table interpolation, then (for a side with a surrogate) the network step
 // on the slots [slot0, slot0 + a.nslots) of that side's counters, then
-// gradients, smoothing and weights.
+// gradients, smoothing and weights.  With `packed` given (resident FP32
+// path), the last part and the packing run as one fused pass where its radii
+// allow (KMIG_TUPLE_FUSED=0 turns the fused pass off); returns in *fused
+// whether it did.
 int side_tuples(kmig_ctx *c, int side, const TuplePrepArgs &a, Tuples &o, size_t slot0,
-                hipStream_t st)
+                hipStream_t st, PackedTuple *packed = nullptr, float th_scale = 0.0f,
+                bool *fused = nullptr)
 {
     Side &sd = c->side[side];
     HIPCHK(c, (hipError_t)launch_tuple_interp(a, o, st));
@@ -679,6 +683,13 @@ int side_tuples(kmig_ctx *c, int side, const TuplePrepArgs &a, Tuples &o, size_t
                                                 c->dz, c->x0, c->dx, a.dside, sd.nn_max_dt, o.T,
                                                 o.A, sd.nn_used + slot0, sd.nn_maxres + slot0,
                                                 st));
+    }
+    const char *env = getenv("KMIG_TUPLE_FUSED");
+    const bool use_fused = packed && tuple_fused_ok(a) && !(env && env[0] == '0');
+    if (fused) *fused = use_fused;
+    if (use_fused) {
+        HIPCHK(c, (hipError_t)launch_tuple_fused(a, o, packed, c->dt, th_scale, st));
+        return KMIG_OK;
     }
     HIPCHK(c, (hipError_t)launch_tuple_finish(a, o, c->tmp1, c->tmp2, st));
     return KMIG_OK;
@@ -711,6 +722,11 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
     A.aper_taper = p.aperture_taper;
     A.rmin = p.aperture_rmin;
     A.rtaper = p.aperture_rtaper;
+    // packed directions in output-angle degrees (the opening angle is then
+    // a difference of two directions folded at 360, or 180 for half angles)
+    const float th_scale = angle ? (float)((p.half_angle ? 90.0 : 180.0) / M_PI) : 0.0f;
+    const bool pack32 = resident && p.precision == KMIG_PREC_FP32;
+    bool fused_s = false, fused_r = false;
     // source side
     A.tab = c->side[0].T;
     A.der = c->side[0].D;
@@ -720,7 +736,8 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
     A.nslots = nsb;
     A.dpos = c->side[0].d;
     A.dside = ds;
-    int rc = side_tuples(c, 0, A, c->ts, spec_s_off, st);
+    int rc = side_tuples(c, 0, A, c->ts, spec_s_off, st, pack32 ? c->pk_s : nullptr, th_scale,
+                         &fused_s);
     if (rc) return rc;
     // receiver side
     A.tab = c->side[1].T;
@@ -731,15 +748,16 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
     A.nslots = b.nrcv;
     A.dpos = c->side[1].d;
     A.dside = dh;
-    if ((rc = side_tuples(c, 1, A, c->tr, b.spec_r0, st))) return rc;
-    // packed directions in output-angle degrees (the opening angle is then
-    // a difference of two directions folded at 360, or 180 for half angles)
-    const float th_scale = angle ? (float)((p.half_angle ? 90.0 : 180.0) / M_PI) : 0.0f;
-    if (resident && p.precision == KMIG_PREC_FP32) {
-        HIPCHK(c, (hipError_t)launch_pack(c->ts, c->pk_s, nsb, c->nzx, c->dt, th_scale,
-                                          st));
-        HIPCHK(c, (hipError_t)launch_pack(c->tr, c->pk_r, b.nrcv, c->nzx, c->dt, th_scale,
-                                          st));
+    if ((rc = side_tuples(c, 1, A, c->tr, b.spec_r0, st, pack32 ? c->pk_r : nullptr, th_scale,
+                          &fused_r)))
+        return rc;
+    if (pack32) {
+        if (!fused_s)
+            HIPCHK(c, (hipError_t)launch_pack(c->ts, c->pk_s, nsb, c->nzx, c->dt, th_scale,
+                                              st));
+        if (!fused_r)
+            HIPCHK(c, (hipError_t)launch_pack(c->tr, c->pk_r, b.nrcv, c->nzx, c->dt, th_scale,
+                                              st));
     } else if (resident) {
         HIPCHK(c, (hipError_t)launch_pack16(c->ts, c->pk16_s, c->sc_s, nsb, c->nzx, c->dt,
                                             th_scale, p.precision, st));

@@ -177,6 +177,12 @@ int launch_tuple_prep(const TuplePrepArgs &a, Tuples &out, float *tmp1, float *t
 int launch_tuple_interp(const TuplePrepArgs &a, Tuples &out, void *stream);
 int launch_tuple_finish(const TuplePrepArgs &a, Tuples &out, float *tmp1, float *tmp2,
                         void *stream);
+// gradients, smoothing, weights and FP32 packing in one LDS-tiled pass (angle
+// mode, radius and smooth <= 4); the packed tuples equal those of
+// launch_tuple_finish + launch_pack bit for bit
+bool tuple_fused_ok(const TuplePrepArgs &a);
+int launch_tuple_fused(const TuplePrepArgs &a, Tuples &o, PackedTuple *out, float dt,
+                       float th_scale, void *stream);
 int launch_migrate(const MigArgs &a, int tile, void *stream);
 int migrate_lds_bytes(int ng, int tile);
 int launch_fill(float *p, size_t n, float v, void *stream);

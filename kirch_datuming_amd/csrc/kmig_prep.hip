@@ -524,6 +524,136 @@ __global__ __launch_bounds__(256) void k_pack16(const float *T, const float *A, 
     out[k] = q;
 }
 
+// Fused tuple finish for the resident FP32 path: FD gradients (or the
+// gradient files' values), the two box-z / box-x smoothing passes, the
+// aperture weight and the packing, for one 64 (z) x 16 (x) tile of one slot,
+// through LDS.  Every stage repeats the arithmetic of k_grad, k_box_z,
+// k_box_x, k_weight and k_pack term for term (this file has no contraction),
+// so the packed tuples are bitwise those of the separate passes.  The tile
+// carries a halo of 2 rs (+ rf for the T tile); a stage's values are exact
+// wherever the remaining halo still covers its window (positions nearer the
+// tile border read a window clipped by the tile and are never used).
+constexpr int FZ = 64, FX = 16, FUSED_MAX_R = 4;
+
+__device__ inline float box_sum(const float *buf, int stride, int l, int g, int r, int n_img,
+                                int l_lo, int l_hi)
+{
+    // window [g - r, g + r] of image positions, clipped to the image and to the
+    // tile's local range [l_lo, l_hi]; summed from its low end as k_box_*
+    int lo = g - r < 0 ? 0 : g - r, hi = g + r > n_img - 1 ? n_img - 1 : g + r;
+    const int off = l - g;  // local = global + off
+    if (lo + off < l_lo) lo = l_lo - off;
+    if (hi + off > l_hi) hi = l_hi - off;
+    float s = 0.0f;
+    for (int k = lo; k <= hi; k++) s += buf[(k + off) * stride];
+    return s / (float)(hi - lo + 1);
+}
+
+__global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, const Tuples o,
+                                                     PackedTuple *out, double inv_dt,
+                                                     float th_scale)
+{
+    extern __shared__ float sm[];
+    const int slot = blockIdx.y;
+    const int nz = a.nz, nx = a.nx;
+    const int ntz = (nz + FZ - 1) / FZ;
+    const int z0 = (blockIdx.x % ntz) * FZ, x0 = (blockIdx.x / ntz) * FX;
+    const int rf = a.radius < 1 ? 1 : a.radius, rs = a.smooth > 0 ? a.smooth : 0;
+    const int H1 = 2 * rs, H0 = H1 + rf;
+    const int w0z = FZ + 2 * H0, w0x = FX + 2 * H0;
+    const int w1z = FZ + 2 * H1, w1x = FX + 2 * H1, n1 = w1z * w1x;
+    float *sT = sm;
+    float *ax = sT + (a.use_grad_files ? 0 : w0z * w0x), *az = ax + n1;
+    float *bx = az + n1, *bz = bx + n1;
+    const size_t base = (size_t)slot * a.nzx;
+    const int tid = threadIdx.x;
+    if (!a.use_grad_files) {
+        for (int k = tid; k < w0z * w0x; k += blockDim.x) {
+            const int iz = z0 - H0 + k % w0z, ix = x0 - H0 + k / w0z;
+            sT[k] = (iz >= 0 && iz < nz && ix >= 0 && ix < nx) ? o.T[base + (size_t)ix * nz + iz]
+                                                                 : 0.0f;
+        }
+        __syncthreads();
+    }
+    // gradients on the region with halo H1
+    for (int k = tid; k < n1; k += blockDim.x) {
+        const int lz = k % w1z, lx = k / w1z;
+        const int iz = z0 - H1 + lz, ix = x0 - H1 + lx;
+        float vx = 0.0f, vz = 0.0f;
+        if (iz >= 0 && iz < nz && ix >= 0 && ix < nx) {
+            if (a.use_grad_files) {
+                vx = o.GX[base + (size_t)ix * nz + iz];
+                vz = o.GZ[base + (size_t)ix * nz + iz];
+            } else {
+                // sT local index of global (z, x): (x - x0 + H0) * w0z + (z - z0 + H0)
+                if (!(nx <= 1 || a.dx == 0.0f)) {
+                    const int il = max(0, ix - rf), ir = min(nx - 1, ix + rf);
+                    if (ir != il)
+                        vx = (sT[(ir - x0 + H0) * w0z + (iz - z0 + H0)] -
+                              sT[(il - x0 + H0) * w0z + (iz - z0 + H0)]) /
+                             ((float)(ir - il) * a.dx);
+                }
+                if (!(nz <= 1 || a.dz == 0.0f)) {
+                    const int it = max(0, iz - rf), ib = min(nz - 1, iz + rf);
+                    if (ib != it)
+                        vz = (sT[(ix - x0 + H0) * w0z + (ib - z0 + H0)] -
+                              sT[(ix - x0 + H0) * w0z + (it - z0 + H0)]) /
+                             ((float)(ib - it) * a.dz);
+                }
+            }
+        }
+        ax[k] = vx;
+        az[k] = vz;
+    }
+    __syncthreads();
+    // two passes of (box z, box x) per component: a -> b -> a -> b -> a
+    float *src_x = ax, *src_z = az, *dst_x = bx, *dst_z = bz;
+    for (int pass = 0; rs > 0 && pass < 4; pass++) {
+        const bool zdir = (pass % 2) == 0;
+        for (int k = tid; k < n1; k += blockDim.x) {
+            const int lz = k % w1z, lx = k / w1z;
+            const int iz = z0 - H1 + lz, ix = x0 - H1 + lx;
+            if (!(iz >= 0 && iz < nz && ix >= 0 && ix < nx)) continue;
+            if (zdir) {
+                dst_x[k] = box_sum(src_x + lx * w1z, 1, lz, iz, rs, nz, 0, w1z - 1);
+                dst_z[k] = box_sum(src_z + lx * w1z, 1, lz, iz, rs, nz, 0, w1z - 1);
+            } else {
+                dst_x[k] = box_sum(src_x + lz, w1z, lx, ix, rs, nx, 0, w1x - 1);
+                dst_z[k] = box_sum(src_z + lz, w1z, lx, ix, rs, nx, 0, w1x - 1);
+            }
+        }
+        __syncthreads();
+        float *t = src_x; src_x = dst_x; dst_x = t;
+        t = src_z; src_z = dst_z; dst_z = t;
+    }
+    // weights and packing of the output tile
+    const float dzabs = fabsf(a.dz);
+    for (int k = tid; k < FZ * FX; k += blockDim.x) {
+        const int iz = z0 + k % FZ, ix = x0 + k / FZ;
+        if (iz >= nz || ix >= nx) continue;
+        const int l = (ix - x0 + H1) * w1z + (iz - z0 + H1);
+        const float gx = src_x[l], gz = src_z[l];
+        const float zd0 = fabsf((float)iz * a.dz), zd1 = 1.0e-6f * dzabs;
+        const float zd2 = 1.0e-12f > zd1 ? 1.0e-12f : zd1;
+        const float zdist = zd0 > zd2 ? zd0 : zd2;
+        const float ximg = a.x0 + (float)ix * a.dx;
+        const float d = fabsf(a.spec[slot].surf_x - ximg);
+        const float deg = atanf(d / zdist) * 180.0f / (float)KPI;
+        const float wc = leg_weight(deg, a.aper, a.aper_taper);
+        const float wf = floor_weight(d, a.rmin, a.rtaper);
+        float w = wc > wf ? wc : wf;
+        if (sqrtf(gx * gx + gz * gz) <= 1.0e-12f) w = 0.0f;
+        const size_t oi = base + (size_t)ix * nz + iz;
+        const PackFields f = pack_fields(o.T[oi], o.A[oi], gx, gz, w, inv_dt, th_scale);
+        PackedTuple q;
+        q.frac = f.frac;
+        q.a = f.a;
+        q.th = f.th;
+        q.iw = f.iw;
+        out[oi] = q;
+    }
+}
+
 // (tr[k], tr[k+1] - tr[k]) pairs, so a kirmig_pick tap is one 8-byte load and
 // its linear interpolation one fused multiply-add.  The last pair of each
 // trace, which no update reads, is (0, 0).
@@ -611,6 +741,33 @@ int launch_tuple_finish(const TuplePrepArgs &a, Tuples &o, float *tmp1, float *t
         }
     }
     hipLaunchKernelGGL(k_weight, grid, blk, 0, st, a, o);
+    return (int)hipGetLastError();
+}
+
+bool tuple_fused_ok(const TuplePrepArgs &a)
+{
+    return a.angle_mode && a.radius <= FUSED_MAX_R && a.smooth <= FUSED_MAX_R && a.nslots > 0 &&
+           a.nslots <= 65535;
+}
+
+int launch_tuple_fused(const TuplePrepArgs &a, Tuples &o, PackedTuple *out, float dt,
+                       float th_scale, void *stream)
+{
+    if (a.nslots <= 0) return 0;
+    const int rf = a.radius < 1 ? 1 : a.radius, rs = a.smooth > 0 ? a.smooth : 0;
+    const int H1 = 2 * rs, H0 = H1 + rf;
+    const size_t n0 = a.use_grad_files ? 0 : (size_t)(FZ + 2 * H0) * (FX + 2 * H0);
+    const size_t n1 = (size_t)(FZ + 2 * H1) * (FX + 2 * H1);
+    const size_t lds = (n0 + 4 * n1) * sizeof(float);
+    const unsigned tiles = (unsigned)(((a.nz + FZ - 1) / FZ) * ((a.nx + FX - 1) / FX));
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void *)k_tuple_fused,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)lds);
+        if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL(k_tuple_fused, dim3(tiles, (unsigned)a.nslots), dim3(256), lds,
+                       (hipStream_t)stream, a, o, out, 1.0 / (double)dt, th_scale);
     return (int)hipGetLastError();
 }
 

@@ -218,3 +218,46 @@ def test_batch_graph_matches_individual_runs_and_replays():
     assert O.rel_l2(singles[2], ref) < TOL
     for m in ms:
         m.close()
+
+
+@pytest.mark.parametrize("radius,smooth,half", [(2, 2, 0), (1, 0, 0), (3, 4, 1), (4, 1, 0)])
+def test_fused_tuple_pass_is_bitwise_the_separate_passes(monkeypatch, radius, smooth, half):
+    """The one-pass LDS-tiled tuple finish (FD, 2x smoothing, weights, packing)
+    gives the same packed tuples, hence the same CIG bit for bit, as the
+    separate passes, on a grid of several tiles in z and x with partial edge
+    tiles (130 x 75 points against 64 x 16 tiles)."""
+    rng = np.random.default_rng(21)
+    nz, nx, d = 130, 75, 0.02
+    npos, dpos = 31, 0.05
+    pos = dpos * np.arange(npos)
+    T, D = _vz_tables(nz, nx, d, pos)
+    nt, dt, ns, nh = 400, 0.004, 5, 40
+    data = rng.normal(size=(ns, nh, nt)).astype(np.float32)
+    data_axes = [[nt, 0.0, dt], [nh, 0.02, 0.037], [ns, 0.1, 0.3]]
+    tax = [[nz, 0.0, d], [nx, 0.0, d], [npos, 0.0, dpos]]
+    kv = dict(cmp=0, mode="angle", angle_min=0, angle_max=60, angle_step=1,
+              angle_grad_radius=radius, angle_smooth=smooth, half_angle=half, aperture=60,
+              aperture_rmin=0.3, variant="resident")
+    print(f"fused tuple pass: radius {radius} smooth {smooth} half {half}", flush=True)
+    monkeypatch.setenv("KMIG_TUPLE_FUSED", "0")
+    sep = _kmig().migrate(kv, data, data_axes, T, D, tax, T, D, tax)[0]
+    monkeypatch.setenv("KMIG_TUPLE_FUSED", "1")
+    fus = _kmig().migrate(kv, data, data_axes, T, D, tax, T, D, tax)[0]
+    assert np.array_equal(sep.view(np.uint32), fus.view(np.uint32))
+    ref64 = O.migrate(kv, data, data_axes, T, D, tax, T, D, tax, precision="f64")[0]
+    ref32 = O.migrate(kv, data, data_axes, T, D, tax, T, D, tax, precision="f32")[0]
+    r, r32 = O.rel_l2(fus, ref64), O.rel_l2(ref32, ref64)
+    print(f"rel vs f64: gpu {r:.2e}, reference-order cpu f32 {r32:.2e}")
+    # unsmoothed directions make the bins more sensitive to rounding: the GPU
+    # is held to 1e-4 or to 1.5x the reference arithmetic's own error
+    assert r < max(TOL, 1.5 * r32)
+
+
+def test_fused_tuple_pass_with_gradient_files(monkeypatch):
+    kv, run, _ = gc.case("gradfiles")
+    kv = dict(kv, variant="resident")
+    monkeypatch.setenv("KMIG_TUPLE_FUSED", "0")
+    sep = _kmig().migrate(kv, *_args(run), grads=run["grads"])[0]
+    monkeypatch.setenv("KMIG_TUPLE_FUSED", "1")
+    fus = _kmig().migrate(kv, *_args(run), grads=run["grads"])[0]
+    assert np.array_equal(sep.view(np.uint32), fus.view(np.uint32))
