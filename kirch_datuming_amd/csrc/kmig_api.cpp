@@ -559,13 +559,14 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
         // not move with the shot, one batch, times within the int16 sample range
         const double cst = ((double)p.tau - (double)c->t0) / (double)c->dt;
         const bool res_ok = angle && p.angle_interp == 1 && c->rcv_invariant &&
-                            std::fabs(cst) + (double)c->nt < 30000.0;
+                            p.antialias >= 0.0f && std::fabs(cst) + (double)c->nt < 30000.0;
         int variant = KMIG_VARIANT_STREAM;
         if (p.variant == KMIG_VARIANT_RESIDENT) {
             if (!res_ok)
                 return fail(c, KMIG_E_PARAM,
-                            "variant=resident needs mode=angle, angle_interp=1, receivers that "
-                            "do not move with the shot (cmp=0) and tuples that fit one batch");
+                            "variant=resident needs mode=angle, angle_interp=1, antialias >= 0, "
+                            "receivers that do not move with the shot (cmp=0) and tuples that "
+                            "fit one batch");
             variant = KMIG_VARIANT_RESIDENT;
         } else if (p.variant == KMIG_VARIANT_AUTO && res_ok) {
             variant = KMIG_VARIANT_RESIDENT;

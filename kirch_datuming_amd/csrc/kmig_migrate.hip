@@ -287,9 +287,11 @@ __device__ __forceinline__ float pair_angle(const PackedTuple &s, const PackedTu
 
 // Everything of one update except the opening angle and the depth-mute
 // weight, which the caller evaluates first (wd = 0 rejects the update).
-__device__ __forceinline__ Pend res_phase1(const PackedTuple &s, const PackedTuple &r, float wap,
-                                           float angle, float wd, const MigResArgs &a,
-                                           bool taper_on)
+// s_int, s_frac: the source tuple's integer and fractional samples with the
+// constant (tau - t0)/dt already added (once per shot).
+__device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int, float s_frac,
+                                           const PackedTuple &r, float wap, float angle,
+                                           float wd, const MigResArgs &a, bool taper_on)
 {
     Pend q;
     const float inv32k = 1.0f / 32768.0f;
@@ -299,32 +301,38 @@ __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, const PackedTup
         const float rt = ramp_down(fminf(fmaxf((angle - a.taper_start) * a.inv_taper, 0.0f), 1.0f));
         wt = angle >= a.taper_end ? 0.0f : (angle > a.taper_start ? rt : 1.0f);
     }
+    // wt, wd, apw >= 0: aw > 0 also rejects skipped traces (wap < 0)
     const float aw = wt * wd * wap * apw;
     const float u = (angle - a.g0) * a.inv_dg;
     const float ngm1 = (float)(a.ng - 1);
-    bool ok = aw > 0.0f && wap >= 0.0f && u >= -1.0e-5f && u <= ngm1 + 1.0e-5f;
+    bool ok = aw > 0.0f && u >= -1.0e-5f && u <= ngm1 + 1.0e-5f;
     // linear bins: u clamped to [0, ng-1] gives the edge bin with w1 = 0 at
     // either end and (floor(u), u - floor(u)) inside
     const float uc = __builtin_amdgcn_fmed3f(u, 0.0f, ngm1);
     const float fu = floorf(uc);
     const int ig0 = (int)fu;
     const float w1 = uc - fu;
-    const int ii = ((int)s.iw >> 16) + ((int)r.iw >> 16) + a.c_int;
-    const float ff = s.frac + r.frac + a.c_frac;
+    const int ii = s_int + ((int)r.iw >> 16);
+    const float ff = s_frac + r.frac;
     const float fl = floorf(ff);
     const int it = ii + (int)fl;
     const float ft = ff - fl;
     // taps at ft -/+ (delta + 1) samples; both a fields are finite and >= 0
-    const float dA1 = fmaf(s.a > r.a ? s.a : r.a, a.aal, 1.0f);
+    // and antialias >= 0 (the resident variant requires it), so delta + 1 >= 1
+    // and itm < it < itp: the outer taps bound the middle one
+    // (non-negative floats order as their bit patterns: one integer max)
+    const float amax = __int_as_float(max(__float_as_int(s.a), __float_as_int(r.a)));
+    const float dA1 = fmaf(amax, a.aal, 1.0f);
     const float fm = ft - dA1, fp = ft + dA1;
     const float flm = floorf(fm), flp = floorf(fp);
     const int itm = it + (int)flm, itp = it + (int)flp;
     const unsigned lim = (unsigned)(a.nt - 1);
-    ok = ok && (unsigned)it < lim && (unsigned)itm < lim && (unsigned)itp < lim;
+    ok = ok && (unsigned)itm < lim && (unsigned)itp < lim;
     const float rr = __builtin_amdgcn_rcpf(fmaf(2.0f, dA1, 1.0f));  // 1 / (3 + 2 delta)
     // a rejected update reads the trace's last pair, (0, 0), with finite
     // fractions and a zero coefficient: it adds an exact zero to bin 0
-    q.coef = ok ? aw * (rr * rr) : 0.0f;
+    // selects rather than a branch: (ok ? 1 : 0) scales the coefficient
+    q.coef = aw * (rr * rr) * (ok ? 1.0f : 0.0f);
     q.w1 = ok ? w1 : 0.0f;
     q.ft = ft;
     q.ftm = fm - flm;
@@ -375,6 +383,8 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             if (js + 1 < a.nshots) nxt = load_raw<Fmt>(a, true, (size_t)(js + 1), i);
             const bool live = valid && (st.iw & 0xFFFFu) != 0u;
             if (__ballot(live) == 0) continue;
+            const int s_int = ((int)st.iw >> 16) + a.c_int;
+            const float s_frac = st.frac + a.c_frac;
             const int is = a.shot0 + js;
             const float *__restrict__ wrow = a.wap + (size_t)is * a.nh + r0;
             const float2 *__restrict__ trow = pairs + ((size_t)is * a.nh + r0) * a.nt;
@@ -408,7 +418,8 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 bool any = false;
 #pragma unroll
                 for (int k = 0; k < G; k++) {
-                    q[k] = res_phase1(st, rt[g0 + k], wap[k], ang[k], wd[k], a, taper_on);
+                    q[k] = res_phase1(st, s_int, s_frac, rt[g0 + k], wap[k], ang[k], wd[k], a,
+                                      taper_on);
                     any = any || q[k].coef != 0.0f;
                 }
                 if (__ballot(any) == 0) continue;
