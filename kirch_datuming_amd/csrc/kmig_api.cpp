@@ -425,6 +425,7 @@ MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
     R.wrap = p.half_angle ? 180.0f : 360.0f;
     R.g0 = p.g0;
     R.inv_dg = 1.0f / p.dg;
+    R.g0_inv_dg = -p.g0 / p.dg;
     R.ng = ng;
     R.oazmin = p.oazmin;
     R.oazmax = p.oazmax;

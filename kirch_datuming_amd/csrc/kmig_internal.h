@@ -151,7 +151,7 @@ struct MigResArgs {
     float c_frac;                // fractional part of (tau - t0)/dt
     float aal;
     float wrap;                  // 360 (full opening angle) or 180 (half)
-    float g0, inv_dg;
+    float g0, inv_dg, g0_inv_dg; // g0_inv_dg = -g0 / dg
     int ng;
     float oazmin, oazmax, oaztaper, inv_oaztaper;
     float taper_start, taper_end, inv_taper;

@@ -289,54 +289,56 @@ __device__ __forceinline__ float pair_angle(const PackedTuple &s, const PackedTu
 // weight, which the caller evaluates first (wd = 0 rejects the update).
 // s_int, s_frac: the source tuple's integer and fractional samples with the
 // constant (tau - t0)/dt already added (once per shot).
+// floor(x) as an integer in one instruction (the compiler emits v_floor and
+// v_cvt_i32 for (int)floorf(x)); x is finite here
+__device__ __forceinline__ int cvt_floor(float x)
+{
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int, float s_frac,
                                            const PackedTuple &r, float wap, float angle,
-                                           float wd, const MigResArgs &a, bool taper_on)
+                                           float wd, const MigResArgs &a)
 {
     Pend q;
     const float inv32k = 1.0f / 32768.0f;
     const float apw = (float)(s.iw & 0xFFFFu) * inv32k * ((float)(r.iw & 0xFFFFu) * inv32k);
-    float wt = 1.0f;
-    if (taper_on) {
-        const float rt = ramp_down(fminf(fmaxf((angle - a.taper_start) * a.inv_taper, 0.0f), 1.0f));
-        wt = angle >= a.taper_end ? 0.0f : (angle > a.taper_start ? rt : 1.0f);
-    }
-    // wt, wd, apw >= 0: aw > 0 also rejects skipped traces (wap < 0)
-    const float aw = wt * wd * wap * apw;
-    const float u = (angle - a.g0) * a.inv_dg;
+    // wd (the depth-mute weight times the angle taper) and apw are >= 0, so
+    // aw > 0 also rejects skipped traces (wap < 0)
+    const float aw = wd * wap * apw;
+    const float u = fmaf(angle, a.inv_dg, a.g0_inv_dg);  // (angle - g0) / dg
     const float ngm1 = (float)(a.ng - 1);
     bool ok = aw > 0.0f && u >= -1.0e-5f && u <= ngm1 + 1.0e-5f;
     // linear bins: u clamped to [0, ng-1] gives the edge bin with w1 = 0 at
     // either end and (floor(u), u - floor(u)) inside
     const float uc = __builtin_amdgcn_fmed3f(u, 0.0f, ngm1);
-    const float fu = floorf(uc);
-    const int ig0 = (int)fu;
-    const float w1 = uc - fu;
-    const int ii = s_int + ((int)r.iw >> 16);
+    const int ig0 = cvt_floor(uc);
+    const float w1 = __builtin_amdgcn_fractf(uc);
+    // split-sample time: integer parts add as integers, fractions in float
     const float ff = s_frac + r.frac;
-    const float fl = floorf(ff);
-    const int it = ii + (int)fl;
-    const float ft = ff - fl;
+    const int it = s_int + ((int)r.iw >> 16) + cvt_floor(ff);
+    const float ft = __builtin_amdgcn_fractf(ff);
     // taps at ft -/+ (delta + 1) samples; both a fields are finite and >= 0
     // and antialias >= 0 (the resident variant requires it), so delta + 1 >= 1
-    // and itm < it < itp: the outer taps bound the middle one
-    // (non-negative floats order as their bit patterns: one integer max)
+    // and itm < it < itp: the outer taps bound the middle one.  Non-negative
+    // floats order as their bit patterns: one integer max.
     const float amax = __int_as_float(max(__float_as_int(s.a), __float_as_int(r.a)));
     const float dA1 = fmaf(amax, a.aal, 1.0f);
     const float fm = ft - dA1, fp = ft + dA1;
-    const float flm = floorf(fm), flp = floorf(fp);
-    const int itm = it + (int)flm, itp = it + (int)flp;
+    const int itm = it + cvt_floor(fm), itp = it + cvt_floor(fp);
     const unsigned lim = (unsigned)(a.nt - 1);
     ok = ok && (unsigned)itm < lim && (unsigned)itp < lim;
     const float rr = __builtin_amdgcn_rcpf(fmaf(2.0f, dA1, 1.0f));  // 1 / (3 + 2 delta)
     // a rejected update reads the trace's last pair, (0, 0), with finite
-    // fractions and a zero coefficient: it adds an exact zero to bin 0
-    // selects rather than a branch: (ok ? 1 : 0) scales the coefficient
+    // fractions and a zero coefficient: it adds an exact zero to bin 0.
+    // Selects rather than a branch: (ok ? 1 : 0) scales the coefficient.
     q.coef = aw * (rr * rr) * (ok ? 1.0f : 0.0f);
     q.w1 = ok ? w1 : 0.0f;
     q.ft = ft;
-    q.ftm = fm - flm;
-    q.ftp = fp - flp;
+    q.ftm = __builtin_amdgcn_fractf(fm);
+    q.ftp = __builtin_amdgcn_fractf(fp);
     q.ig0 = ok ? ig0 : 0;
     q.it = ok ? it : (int)lim;
     q.itm = ok ? itm : (int)lim;
@@ -413,13 +415,22 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                                                  : (ang[k] >= cutoff + a.oaztaper ? 0.0f : rd);
                     }
                 }
+                // the angle taper (angle_taper_start/end) multiplies in here
+                if (taper_on) {
+#pragma unroll
+                    for (int k = 0; k < G; k++) {
+                        const float rt_ = ramp_down(fminf(
+                            fmaxf((ang[k] - a.taper_start) * a.inv_taper, 0.0f), 1.0f));
+                        wd[k] *= ang[k] >= a.taper_end ? 0.0f
+                                                       : (ang[k] > a.taper_start ? rt_ : 1.0f);
+                    }
+                }
                 // stage 2: weights, bins and tap indices
                 Pend q[G];
                 bool any = false;
 #pragma unroll
                 for (int k = 0; k < G; k++) {
-                    q[k] = res_phase1(st, s_int, s_frac, rt[g0 + k], wap[k], ang[k], wd[k], a,
-                                      taper_on);
+                    q[k] = res_phase1(st, s_int, s_frac, rt[g0 + k], wap[k], ang[k], wd[k], a);
                     any = any || q[k].coef != 0.0f;
                 }
                 if (__ballot(any) == 0) continue;
