@@ -210,6 +210,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 struct Pend {
     float coef, w1, ft, ftm, ftp;
     int ig0, it, itm, itp;
+    bool ok;
 };
 
 // Tuple loaders: FP32 (Fmt 0) or 16-bit fields decoded to the FP32 layout
@@ -331,15 +332,17 @@ __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int, floa
     const unsigned lim = (unsigned)(a.nt - 1);
     ok = ok && (unsigned)itm < lim && (unsigned)itp < lim;
     const float rr = __builtin_amdgcn_rcpf(fmaf(2.0f, dA1, 1.0f));  // 1 / (3 + 2 delta)
-    // a rejected update reads the trace's last pair, (0, 0), with finite
-    // fractions and a zero coefficient: it adds an exact zero to bin 0.
-    // Selects rather than a branch: (ok ? 1 : 0) scales the coefficient.
-    q.coef = aw * (rr * rr) * (ok ? 1.0f : 0.0f);
-    q.w1 = ok ? w1 : 0.0f;
+    // A rejected update reads the trace's last pair, (0, 0), at all three taps:
+    // with its finite fractions and coefficient its picked value is an exact
+    // zero, added to a bin inside [0, ng) (uc is clamped, and w1 > 0 only
+    // below ng - 1), so only the tap indices are selected.
+    q.ok = ok;
+    q.coef = aw * (rr * rr);
+    q.w1 = w1;
     q.ft = ft;
     q.ftm = __builtin_amdgcn_fractf(fm);
     q.ftp = __builtin_amdgcn_fractf(fp);
-    q.ig0 = ok ? ig0 : 0;
+    q.ig0 = ig0;
     q.it = ok ? it : (int)lim;
     q.itm = ok ? itm : (int)lim;
     q.itp = ok ? itp : (int)lim;
@@ -431,7 +434,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
 #pragma unroll
                 for (int k = 0; k < G; k++) {
                     q[k] = res_phase1(st, s_int, s_frac, rt[g0 + k], wap[k], ang[k], wd[k], a);
-                    any = any || q[k].coef != 0.0f;
+                    any = any || q[k].ok;
                 }
                 if (__ballot(any) == 0) continue;
                 // tap offsets are 32-bit byte offsets from the wave-uniform
@@ -445,9 +448,9 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                     vp[k] = *(const float2 *)(tr + ((uint32_t)q[k].itp << 3));
                 }
                 // Branch-free accumulation: a rejected update adds an exact
-                // zero to bin 0, and a one-bin update adds its zero second
-                // share to the same bin, so the 3G loads above are not sunk
-                // into per-update branches and stay in flight together.
+                // zero, and a one-bin update adds its zero second share to the
+                // same bin, so the 3G loads above are not sunk into
+                // per-update branches and stay in flight together.
 #pragma unroll
                 for (int k = 0; k < G; k++) {
                     // pairs hold (tr[k], tr[k+1] - tr[k])
