@@ -209,7 +209,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 // chunk, shot, receiver) order.
 struct Pend {
     float coef, w1, ft, ftm, ftp;
-    int ig0, it, itm, itp;
+    int ig0, ig1;          // bins; ig1 = min(ig0 + 1, ng - 1)
+    unsigned o0, om, op;   // byte offsets of the three taps in the trace row
     bool ok;
 };
 
@@ -288,8 +289,9 @@ __device__ __forceinline__ float pair_angle(const PackedTuple &s, const PackedTu
 
 // Everything of one update except the opening angle and the depth-mute
 // weight, which the caller evaluates first (wd = 0 rejects the update).
-// s_int, s_frac: the source tuple's integer and fractional samples with the
-// constant (tau - t0)/dt already added (once per shot).
+// s_int8, s_frac: the source tuple's integer samples (as a byte offset of
+// 8-byte tap pairs) and fractional samples, with the constant (tau - t0)/dt
+// already added (once per shot).
 // floor(x) as an integer in one instruction (the compiler emits v_floor and
 // v_cvt_i32 for (int)floorf(x)); x is finite here
 __device__ __forceinline__ int cvt_floor(float x)
@@ -299,7 +301,7 @@ __device__ __forceinline__ int cvt_floor(float x)
     return r;
 }
 
-__device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int, float s_frac,
+__device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int8, float s_frac,
                                            const PackedTuple &r, float wap, float angle,
                                            float wd, const MigResArgs &a)
 {
@@ -317,9 +319,10 @@ __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int, floa
     const float uc = __builtin_amdgcn_fmed3f(u, 0.0f, ngm1);
     const int ig0 = cvt_floor(uc);
     const float w1 = __builtin_amdgcn_fractf(uc);
-    // split-sample time: integer parts add as integers, fractions in float
+    // split-sample time: integer parts add as integers (in bytes of 8-byte
+    // tap pairs), fractions in float
     const float ff = s_frac + r.frac;
-    const int it = s_int + ((int)r.iw >> 16) + cvt_floor(ff);
+    const int it8 = s_int8 + (((int)r.iw >> 16) << 3) + (cvt_floor(ff) << 3);
     const float ft = __builtin_amdgcn_fractf(ff);
     // taps at ft -/+ (delta + 1) samples; both a fields are finite and >= 0
     // and antialias >= 0 (the resident variant requires it), so delta + 1 >= 1
@@ -328,9 +331,9 @@ __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int, floa
     const float amax = __int_as_float(max(__float_as_int(s.a), __float_as_int(r.a)));
     const float dA1 = fmaf(amax, a.aal, 1.0f);
     const float fm = ft - dA1, fp = ft + dA1;
-    const int itm = it + cvt_floor(fm), itp = it + cvt_floor(fp);
-    const unsigned lim = (unsigned)(a.nt - 1);
-    ok = ok && (unsigned)itm < lim && (unsigned)itp < lim;
+    const int itm8 = it8 + (cvt_floor(fm) << 3), itp8 = it8 + (cvt_floor(fp) << 3);
+    const unsigned lim8 = (unsigned)(a.nt - 1) << 3;
+    ok = ok && (unsigned)itm8 < lim8 && (unsigned)itp8 < lim8;
     const float rr = __builtin_amdgcn_rcpf(fmaf(2.0f, dA1, 1.0f));  // 1 / (3 + 2 delta)
     // A rejected update reads the trace's last pair, (0, 0), at all three taps:
     // with its finite fractions and coefficient its picked value is an exact
@@ -343,9 +346,10 @@ __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int, floa
     q.ftm = __builtin_amdgcn_fractf(fm);
     q.ftp = __builtin_amdgcn_fractf(fp);
     q.ig0 = ig0;
-    q.it = ok ? it : (int)lim;
-    q.itm = ok ? itm : (int)lim;
-    q.itp = ok ? itp : (int)lim;
+    q.ig1 = min(ig0 + 1, a.ng - 1);  // w1 = 0 wherever this clamps
+    q.o0 = ok ? (unsigned)it8 : lim8;
+    q.om = ok ? (unsigned)itm8 : lim8;
+    q.op = ok ? (unsigned)itp8 : lim8;
     return q;
 }
 
@@ -388,7 +392,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             if (js + 1 < a.nshots) nxt = load_raw<Fmt>(a, true, (size_t)(js + 1), i);
             const bool live = valid && (st.iw & 0xFFFFu) != 0u;
             if (__ballot(live) == 0) continue;
-            const int s_int = ((int)st.iw >> 16) + a.c_int;
+            const int s_int8 = (((int)st.iw >> 16) + a.c_int) << 3;
             const float s_frac = st.frac + a.c_frac;
             const int is = a.shot0 + js;
             const float *__restrict__ wrow = a.wap + (size_t)is * a.nh + r0;
@@ -433,7 +437,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 bool any = false;
 #pragma unroll
                 for (int k = 0; k < G; k++) {
-                    q[k] = res_phase1(st, s_int, s_frac, rt[g0 + k], wap[k], ang[k], wd[k], a);
+                    q[k] = res_phase1(st, s_int8, s_frac, rt[g0 + k], wap[k], ang[k], wd[k], a);
                     any = any || q[k].ok;
                 }
                 if (__ballot(any) == 0) continue;
@@ -443,9 +447,9 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
 #pragma unroll
                 for (int k = 0; k < G; k++) {
                     const char *tr = (const char *)(trow + (size_t)(g0 + k) * a.nt);
-                    v0[k] = *(const float2 *)(tr + ((uint32_t)q[k].it << 3));
-                    vm[k] = *(const float2 *)(tr + ((uint32_t)q[k].itm << 3));
-                    vp[k] = *(const float2 *)(tr + ((uint32_t)q[k].itp << 3));
+                    v0[k] = *(const float2 *)(tr + q[k].o0);
+                    vm[k] = *(const float2 *)(tr + q[k].om);
+                    vp[k] = *(const float2 *)(tr + q[k].op);
                 }
                 // Branch-free accumulation: a rejected update adds an exact
                 // zero, and a one-bin update adds its zero second share to the
@@ -458,9 +462,8 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                                      (vm[k].x + q[k].ftm * vm[k].y) -
                                      (vp[k].x + q[k].ftp * vp[k].y);
                     const float picked = q[k].coef * sm;
-                    float *b0 = my + q[k].ig0 * P;
-                    b0[0] += (1.0f - q[k].w1) * picked;
-                    b0[q[k].w1 > 0.0f ? P : 0] += q[k].w1 * picked;
+                    my[q[k].ig0 * P] += (1.0f - q[k].w1) * picked;
+                    my[q[k].ig1 * P] += q[k].w1 * picked;
                 }
             }
         }
