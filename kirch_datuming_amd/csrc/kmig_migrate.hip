@@ -370,6 +370,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
     for (int g = 0; g < a.ng; g++) my[g * P] = valid ? a.cig[(size_t)g * a.nzx + i] : 0.0f;
     const float zf = (a.nz > 1) ? (float)iz / (float)(a.nz - 1) : 0.0f;
     const float cutoff = a.oazmin + zf * (a.oazmax - a.oazmin);
+    const float cutoff_hi = cutoff + a.oaztaper;
     const bool taper_on = !(a.taper_start < 0.0f || a.taper_end <= a.taper_start);
     const float2 *__restrict__ pairs = (const float2 *)a.pairs;
 
@@ -411,7 +412,8 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                     wap[k] = (r0 + g0 + k < a.nh) ? wrow[g0 + k] : -1.0f;
                     ang[k] = pair_angle(st, rt[g0 + k], a.wrap);
                     wd[k] = ang[k] <= cutoff ? 1.0f : 0.0f;
-                    band = band || (ang[k] > cutoff && ang[k] < cutoff + a.oaztaper);
+                    // bitwise, not short-circuit: no per-update branches
+                    band |= (ang[k] > cutoff) & (ang[k] < cutoff_hi);
                 }
                 if (a.oaztaper > 0.0f && __ballot(band) != 0) {
 #pragma unroll
@@ -438,7 +440,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
 #pragma unroll
                 for (int k = 0; k < G; k++) {
                     q[k] = res_phase1(st, s_int8, s_frac, rt[g0 + k], wap[k], ang[k], wd[k], a);
-                    any = any || q[k].ok;
+                    any |= q[k].ok;
                 }
                 if (__ballot(any) == 0) continue;
                 // tap offsets are 32-bit byte offsets from the wave-uniform
