@@ -191,11 +191,13 @@ def main():
         "config": {"workload": w.name, "config": args.config, "nz": int(w.s_axes[0][0]),
                    "nx": int(w.s_axes[1][0]), "ns": int(ns), "nh": int(nh), "nt": int(nt),
                    "ng": int(p.ng), "updates_per_step": w.updates,
-                   "precision": args.precision, "tile": p.tile or "auto",
+                   "precision": args.precision,
+                   "tile": p.tile or f"auto (preset {st['tile']})",
                    "variant": {0: "auto", 1: "stream", 2: "resident"}[st["variant"]]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_migrate_ph" if st["variant"] == 2 else "k_migrate",
+                     "kernel": (f"k_migrate_ph (preset {st['tile']})" if st["variant"] == 2
+                                else "k_migrate"),
                      "algorithmic_bytes_per_update": bpu,
                      "updates_per_launch": upd_per_launch, "avg_launch_ms": mig_avg},
         "cpu_baseline": cpu,
