@@ -447,10 +447,10 @@ int autotune(kmig_ctx *c, const MigResArgs &full)
 {
     if (c->tuned_tile > 0 && c->tuned_version == c->geo_version) return KMIG_OK;
     static const int presets32[] = {9, 15, 16, 20, 11, 10, 18};
-    static const int presets16[] = {9, 15, 10, 13};
+    static const int presets16[] = {9, 15, 10, 13, 11};
     const bool p16 = c->p.precision != KMIG_PREC_FP32;
     const int *presets = p16 ? presets16 : presets32;
-    const int npresets = p16 ? 4 : 7;
+    const int npresets = p16 ? 5 : 7;
     const double per_shot = (double)c->nh * (double)c->nzx;
     int S = (int)std::max(1.0, std::min((double)full.nshots, 5e9 / std::max(per_shot, 1.0)));
     // a slice from the middle of the batch (source tuples are batch-local)
@@ -666,13 +666,21 @@ int enqueue_trace_prep(kmig_ctx *c, hipStream_t st, int sb, int se)
 
 // Table interpolation, then (for a side with a surrogate) the network step
 // on the slots [slot0, slot0 + a.nslots) of that side's counters, then
-// gradients, smoothing and weights.  With `packed` given (resident FP32
-// path), the last part and the packing run as one fused pass where its radii
-// allow (KMIG_TUPLE_FUSED=0 turns the fused pass off); returns in *fused
-// whether it did.
+// gradients, smoothing and weights.  With an output given (resident path:
+// `packed` for FP32, `packed16` + `scale` for the 16-bit formats), the last
+// part and the packing run as one fused pass where its radii allow
+// (KMIG_TUPLE_FUSED=0 turns the fused pass off); returns in *fused whether
+// it did.
+struct PackOut {
+    PackedTuple *packed = nullptr;
+    PackedTuple16 *packed16 = nullptr;
+    SlotScale *scale = nullptr;
+    int precision = KMIG_PREC_FP32;
+    float th_scale = 0.0f;
+};
+
 int side_tuples(kmig_ctx *c, int side, const TuplePrepArgs &a, Tuples &o, size_t slot0,
-                hipStream_t st, PackedTuple *packed = nullptr, float th_scale = 0.0f,
-                bool *fused = nullptr)
+                hipStream_t st, const PackOut &po, bool *fused)
 {
     Side &sd = c->side[side];
     HIPCHK(c, (hipError_t)launch_tuple_interp(a, o, st));
@@ -687,10 +695,16 @@ int side_tuples(kmig_ctx *c, int side, const TuplePrepArgs &a, Tuples &o, size_t
                                                 st));
     }
     const char *env = getenv("KMIG_TUPLE_FUSED");
-    const bool use_fused = packed && tuple_fused_ok(a) && !(env && env[0] == '0');
+    const bool use_fused = (po.packed || po.packed16) && tuple_fused_ok(a) &&
+                           !(env && env[0] == '0');
     if (fused) *fused = use_fused;
+    if (use_fused && po.packed) {
+        HIPCHK(c, (hipError_t)launch_tuple_fused(a, o, po.packed, c->dt, po.th_scale, st));
+        return KMIG_OK;
+    }
     if (use_fused) {
-        HIPCHK(c, (hipError_t)launch_tuple_fused(a, o, packed, c->dt, th_scale, st));
+        HIPCHK(c, (hipError_t)launch_tuple_fused16(a, o, po.packed16, po.scale, c->dt,
+                                                   po.th_scale, po.precision, st));
         return KMIG_OK;
     }
     HIPCHK(c, (hipError_t)launch_tuple_finish(a, o, c->tmp1, c->tmp2, st));
@@ -738,8 +752,15 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
     A.nslots = nsb;
     A.dpos = c->side[0].d;
     A.dside = ds;
-    int rc = side_tuples(c, 0, A, c->ts, spec_s_off, st, pack32 ? c->pk_s : nullptr, th_scale,
-                         &fused_s);
+    PackOut po;
+    po.th_scale = th_scale;
+    po.precision = p.precision;
+    if (pack32) po.packed = c->pk_s;
+    if (resident && !pack32) {
+        po.packed16 = c->pk16_s;
+        po.scale = c->sc_s;
+    }
+    int rc = side_tuples(c, 0, A, c->ts, spec_s_off, st, po, &fused_s);
     if (rc) return rc;
     // receiver side
     A.tab = c->side[1].T;
@@ -750,9 +771,12 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
     A.nslots = b.nrcv;
     A.dpos = c->side[1].d;
     A.dside = dh;
-    if ((rc = side_tuples(c, 1, A, c->tr, b.spec_r0, st, pack32 ? c->pk_r : nullptr, th_scale,
-                          &fused_r)))
-        return rc;
+    if (pack32) po.packed = c->pk_r;
+    if (resident && !pack32) {
+        po.packed16 = c->pk16_r;
+        po.scale = c->sc_r;
+    }
+    if ((rc = side_tuples(c, 1, A, c->tr, b.spec_r0, st, po, &fused_r))) return rc;
     if (pack32) {
         if (!fused_s)
             HIPCHK(c, (hipError_t)launch_pack(c->ts, c->pk_s, nsb, c->nzx, c->dt, th_scale,
@@ -761,10 +785,12 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
             HIPCHK(c, (hipError_t)launch_pack(c->tr, c->pk_r, b.nrcv, c->nzx, c->dt, th_scale,
                                               st));
     } else if (resident) {
-        HIPCHK(c, (hipError_t)launch_pack16(c->ts, c->pk16_s, c->sc_s, nsb, c->nzx, c->dt,
-                                            th_scale, p.precision, st));
-        HIPCHK(c, (hipError_t)launch_pack16(c->tr, c->pk16_r, c->sc_r, b.nrcv, c->nzx, c->dt,
-                                            th_scale, p.precision, st));
+        if (!fused_s)
+            HIPCHK(c, (hipError_t)launch_pack16(c->ts, c->pk16_s, c->sc_s, nsb, c->nzx, c->dt,
+                                                th_scale, p.precision, st));
+        if (!fused_r)
+            HIPCHK(c, (hipError_t)launch_pack16(c->tr, c->pk16_r, c->sc_r, b.nrcv, c->nzx,
+                                                c->dt, th_scale, p.precision, st));
     }
     return KMIG_OK;
 }

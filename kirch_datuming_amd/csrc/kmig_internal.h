@@ -183,6 +183,12 @@ int launch_tuple_finish(const TuplePrepArgs &a, Tuples &out, float *tmp1, float 
 bool tuple_fused_ok(const TuplePrepArgs &a);
 int launch_tuple_fused(const TuplePrepArgs &a, Tuples &o, PackedTuple *out, float dt,
                        float th_scale, void *stream);
+// the same for the 16-bit formats; the per-slot anti-alias scale is the
+// maximum over all points of the slot (the unfused path takes it over the
+// points with W > 0)
+int launch_tuple_fused16(const TuplePrepArgs &a, Tuples &o, PackedTuple16 *out,
+                         SlotScale *scale, float dt, float th_scale, int precision,
+                         void *stream);
 int launch_migrate(const MigArgs &a, int tile, void *stream);
 int migrate_lds_bytes(int ng, int tile);
 int launch_fill(float *p, size_t n, float v, void *stream);

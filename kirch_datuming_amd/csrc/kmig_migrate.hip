@@ -545,13 +545,14 @@ static int launch_ph_t(const MigResArgs &a, hipStream_t st)
 }
 
 // Presets: tile shape (TZ x TX points), receivers per chunk R, group G.  The
-// 16-bit tuple formats are built for the four presets the autotuner keeps
-// for them (16x8 and 8x8 points, R = 8 or 16).
+// 16-bit tuple formats are built for the five presets the autotuner keeps
+// for them (16x8 and 8x8 points, R = 8 or 16, G = 4, and 16x8, R = 8, G = 8).
 template <bool Probe, int Fmt>
 static int launch_res_preset(const MigResArgs &a, int tile, hipStream_t st)
 {
     if constexpr (Fmt != 0) {
         switch (tile) {
+        case 11: return launch_ph_t<16, 8, 8, 8, Probe, Fmt>(a, st);
         case 10: return launch_ph_t<16, 8, 16, 4, Probe, Fmt>(a, st);
         case 13: return launch_ph_t<8, 8, 16, 4, Probe, Fmt>(a, st);
         case 15: return launch_ph_t<8, 8, 8, 4, Probe, Fmt>(a, st);

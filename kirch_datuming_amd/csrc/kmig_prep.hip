@@ -481,8 +481,20 @@ __device__ inline uint32_t to16(float v, int precision)
     return (uint32_t)__builtin_bit_cast(unsigned short, h);
 }
 
-// Per-slot maximum of the anti-alias term over points with W > 0: the scale
-// of its 16-bit field.
+__device__ inline PackedTuple16 tuple16(const PackFields &f, float a_scale, int precision)
+{
+    PackedTuple16 q;
+    q.iw = f.iw;
+    const uint32_t fq = precision == KMIG_PREC_U16
+                            ? (uint32_t)fminf(65535.0f, rintf(f.frac * 65536.0f))
+                            : to16(f.frac, precision);
+    q.fa = fq | (to16(f.a / a_scale, precision) << 16);
+    q.th = __float_as_uint(f.th);
+    return q;
+}
+
+// Per-slot maximum of the anti-alias term over points with W > 0 (over all
+// points when W is null): the scale of its 16-bit field.
 __global__ __launch_bounds__(256) void k_slot_range(const float *A, const float *W,
                                                     SlotScale *out, size_t nzx, double inv_dt)
 {
@@ -491,7 +503,7 @@ __global__ __launch_bounds__(256) void k_slot_range(const float *A, const float 
     float amax = 0.0f;
     for (size_t i = threadIdx.x; i < nzx; i += blockDim.x) {
         const size_t k = (size_t)slot * nzx + i;
-        if (W[k] > 0.0f) amax = fmaxf(amax, (float)((double)A[k] * inv_dt));
+        if (!W || W[k] > 0.0f) amax = fmaxf(amax, (float)((double)A[k] * inv_dt));
     }
     samax[threadIdx.x] = amax;
     __syncthreads();
@@ -511,17 +523,9 @@ __global__ __launch_bounds__(256) void k_pack16(const float *T, const float *A, 
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const SlotScale sc = scale[k / nzx];
     const PackFields f = pack_fields(T[k], A[k], th_scale != 0.0f ? GX[k] : 0.0f,
                                      th_scale != 0.0f ? GZ[k] : 0.0f, W[k], inv_dt, th_scale);
-    PackedTuple16 q;
-    q.iw = f.iw;
-    const uint32_t fq = precision == KMIG_PREC_U16
-                            ? (uint32_t)fminf(65535.0f, rintf(f.frac * 65536.0f))
-                            : to16(f.frac, precision);
-    q.fa = fq | (to16(f.a / sc.a_scale, precision) << 16);
-    q.th = __float_as_uint(f.th);
-    out[k] = q;
+    out[k] = tuple16(f, scale[k / nzx].a_scale, precision);
 }
 
 // Fused tuple finish for the resident FP32 path: FD gradients (or the
@@ -549,9 +553,12 @@ __device__ inline float box_sum(const float *buf, int stride, int l, int g, int 
     return s / (float)(hi - lo + 1);
 }
 
+// Out = PackedTuple (FP32) or PackedTuple16 (16-bit formats: `precision`,
+// per-slot anti-alias scales in `scale`).
+template <class Out>
 __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, const Tuples o,
-                                                     PackedTuple *out, double inv_dt,
-                                                     float th_scale)
+                                                     Out *out, double inv_dt, float th_scale,
+                                                     const SlotScale *scale, int precision)
 {
     extern __shared__ float sm[];
     const int slot = blockIdx.y;
@@ -645,12 +652,16 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
         if (sqrtf(gx * gx + gz * gz) <= 1.0e-12f) w = 0.0f;
         const size_t oi = base + (size_t)ix * nz + iz;
         const PackFields f = pack_fields(o.T[oi], o.A[oi], gx, gz, w, inv_dt, th_scale);
-        PackedTuple q;
-        q.frac = f.frac;
-        q.a = f.a;
-        q.th = f.th;
-        q.iw = f.iw;
-        out[oi] = q;
+        if constexpr (sizeof(Out) == sizeof(PackedTuple16)) {
+            out[oi] = tuple16(f, scale[slot].a_scale, precision);
+        } else {
+            PackedTuple q;
+            q.frac = f.frac;
+            q.a = f.a;
+            q.th = f.th;
+            q.iw = f.iw;
+            out[oi] = q;
+        }
     }
 }
 
@@ -750,8 +761,9 @@ bool tuple_fused_ok(const TuplePrepArgs &a)
            a.nslots <= 65535;
 }
 
-int launch_tuple_fused(const TuplePrepArgs &a, Tuples &o, PackedTuple *out, float dt,
-                       float th_scale, void *stream)
+template <class Out>
+static int launch_fused_t(const TuplePrepArgs &a, Tuples &o, Out *out, float dt, float th_scale,
+                          const SlotScale *scale, int precision, hipStream_t st)
 {
     if (a.nslots <= 0) return 0;
     const int rf = a.radius < 1 ? 1 : a.radius, rs = a.smooth > 0 ? a.smooth : 0;
@@ -761,14 +773,35 @@ int launch_tuple_fused(const TuplePrepArgs &a, Tuples &o, PackedTuple *out, floa
     const size_t lds = (n0 + 4 * n1) * sizeof(float);
     const unsigned tiles = (unsigned)(((a.nz + FZ - 1) / FZ) * ((a.nx + FX - 1) / FX));
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_tuple_fused,
+        const hipError_t e = hipFuncSetAttribute((const void *)k_tuple_fused<Out>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL(k_tuple_fused, dim3(tiles, (unsigned)a.nslots), dim3(256), lds,
-                       (hipStream_t)stream, a, o, out, 1.0 / (double)dt, th_scale);
+    hipLaunchKernelGGL(k_tuple_fused<Out>, dim3(tiles, (unsigned)a.nslots), dim3(256), lds, st,
+                       a, o, out, 1.0 / (double)dt, th_scale, scale, precision);
     return (int)hipGetLastError();
+}
+
+int launch_tuple_fused(const TuplePrepArgs &a, Tuples &o, PackedTuple *out, float dt,
+                       float th_scale, void *stream)
+{
+    return launch_fused_t(a, o, out, dt, th_scale, (const SlotScale *)nullptr, 0,
+                          (hipStream_t)stream);
+}
+
+int launch_tuple_fused16(const TuplePrepArgs &a, Tuples &o, PackedTuple16 *out,
+                         SlotScale *scale, float dt, float th_scale, int precision,
+                         void *stream)
+{
+    if (a.nslots <= 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    // the scale from the interpolated anti-alias term of every point (the
+    // aperture weights are computed inside the fused pass)
+    hipLaunchKernelGGL(k_slot_range, dim3((unsigned)a.nslots), dim3(256), 0, st,
+                       (const float *)o.A, (const float *)nullptr, scale, a.nzx,
+                       1.0 / (double)dt);
+    return launch_fused_t(a, o, out, dt, th_scale, (const SlotScale *)scale, precision, st);
 }
 
 int launch_tuple_prep(const TuplePrepArgs &a, Tuples &o, float *tmp1, float *tmp2, void *stream)

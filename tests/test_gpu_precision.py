@@ -58,3 +58,22 @@ def test_16bit_modes_need_the_resident_variant():
     kv, run, _ = gc.case("angle_cmp1")
     with pytest.raises(kmig.KmigError, match="applies to the resident kernel"):
         kmig.migrate(dict(kv, precision="u16"), *_golden_args("angle_cmp1"))
+
+
+@pytest.mark.parametrize("prec", ["u16", "fp16", "bf16"])
+def test_fused_16bit_pass_matches_the_separate_passes(monkeypatch, prec):
+    """The fused tuple pass takes the 16-bit anti-alias scale over all points
+    of a slot (the separate passes over the points with W > 0): the two CIGs
+    agree to well within the mode's own error."""
+    from kirch_datuming_amd import kmig
+    kv, args = _vz_case()
+    kv = dict(kv, precision=prec, variant="resident")
+    monkeypatch.setenv("KMIG_TUPLE_FUSED", "0")
+    sep = kmig.migrate(kv, *args)[0]
+    monkeypatch.setenv("KMIG_TUPLE_FUSED", "1")
+    fus = kmig.migrate(kv, *args)[0]
+    ref = O.migrate(dict(kv, precision="fp32"), *args, precision="f64")[0]
+    d, r = O.rel_l2(fus, sep), O.rel_l2(fus, ref)
+    print(f"{prec}: fused vs separate {d:.2e}, fused vs FP64 {r:.2e}")
+    assert d < 0.5 * BOUND[prec]
+    assert r < BOUND[prec]
