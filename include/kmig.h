@@ -186,6 +186,20 @@ int kmig_batch_sync(kmig_batch *b);
 int kmig_batch_stats(const kmig_batch *b, double *updates, double *ms, int *graph_nodes);
 const char *kmig_batch_last_error(const kmig_batch *b);
 void kmig_batch_destroy(kmig_batch *b);
+/* After kmig_batch_sync: the depth-layer stitch of the batch's angle gathers
+   into one (z, x, angle) volume, replacing src/layercom_cig.cpp:412-557 (its
+   checks and messages, :245-305 and :424-426) and the angle window of
+   run_salt_adcig.sh:234-236.  The contexts are the layers, top to bottom;
+   layer k covers full depth samples [start[k], end[k]] (its nz), each layer
+   overlaps the one above and extends deeper.  scale (nullable: 1) multiplies
+   layer k; layers after the first are also scaled by the lateral edge ramp
+   scalex_aper / scalex_scale; the last stitch_blend overlap samples (-1: all)
+   blend with complementary raised-cosine weights.  Angle samples
+   [g_first, g_first + g_count) are written (g_count <= 0: all) to out
+   (host, [g_count][nx][end[n-1] + 1]). */
+int kmig_batch_stitch(kmig_batch *b, const int *start, const int *end, const float *scale,
+                      int stitch_blend, int scalex_aper, double scalex_scale, int g_first,
+                      int g_count, float *out);
 
 /* ---- learned traveltime surrogate (include/kmig_nn.h) ----
    No counterpart in the reference, whose traveltimes always come from the

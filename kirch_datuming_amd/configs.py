@@ -78,6 +78,7 @@ class Workload:
     rder: np.ndarray
     r_axes: list
     vel: np.ndarray = field(default=None)
+    z_range: tuple = field(default=None)  # full-model depth samples [start, end] of a layer
 
     @property
     def updates(self) -> float:
@@ -163,15 +164,20 @@ LAYERS_SLOPE = (50.0, 40.0, 30.0, 20.0, 10.0)
 LAYERS_TAPER = (0.05, 0.0625, 0.075, 0.0875, 0.10)
 
 
-def layers(threads=0, seed=20260303, scale=1.0):
+def layers(threads=0, seed=20260303, scale=1.0, overlap=0):
     """The five depth layers as separate Workloads (see module docstring).
-    scale < 1 shrinks shot and receiver counts (for tests)."""
+    scale < 1 shrinks shot and receiver counts (for tests).  overlap > 0
+    extends every layer after the first upward by that many samples (the
+    overlapping layers the depth-layer stitch blends); each Workload's
+    z_range is its [start, end] in full-model depth samples."""
     L = synth()
     nx, d = 1000, 0.01
     full = np.empty((nx, 400), np.float32)
     L.kms_velocity_salt(_p(full), 400, nx, d, d)
-    out, z0 = [], 0
-    for k, nz in enumerate(LAYERS_NZ):
+    out, zb = [], 0
+    for k, nz0 in enumerate(LAYERS_NZ):
+        z0 = zb - (overlap if k > 0 else 0)
+        nz = nz0 + (zb - z0)
         vel = np.ascontiguousarray(full[:, z0:z0 + nz])
         ns = max(2, int(round(LAYERS_NS[k] * scale)))
         nh = max(2, int(round(LAYERS_NH[k] * scale)))
@@ -188,8 +194,9 @@ def layers(threads=0, seed=20260303, scale=1.0):
         tax = [[nz, 0.0, d], [nx, 0.0, d]]
         out.append(Workload(f"layer {k + 1}: {nz}x{nx}, {ns} shots x {nh} receivers", kv, data,
                             [[nt, 0.0, dt], [nh, 0.0, dh], [ns, 0.0, ds]], ST, SD,
-                            tax + [[ns, 0.0, ds]], RT, RD, tax + [[nh, 0.0, dh]], vel))
-        z0 += nz
+                            tax + [[ns, 0.0, ds]], RT, RD, tax + [[nh, 0.0, dh]], vel,
+                            (z0, z0 + nz - 1)))
+        zb += nz0
     return out
 
 

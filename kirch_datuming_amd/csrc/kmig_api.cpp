@@ -1431,6 +1431,91 @@ int kmig_batch_stats(const kmig_batch *b, double *updates, double *ms, int *grap
 
 const char *kmig_batch_last_error(const kmig_batch *b) { return b ? b->err.c_str() : "null batch"; }
 
+int kmig_batch_stitch(kmig_batch *b, const int *start, const int *end, const float *scale,
+                      int stitch_blend, int scalex_aper, double scalex_scale, int g_first,
+                      int g_count, float *out)
+{
+    if (!b || !start || !end || !out) return b ? bfail(b, KMIG_E_PARAM, "null argument")
+                                               : KMIG_E_PARAM;
+    const int n = (int)b->ctx.size();
+    // the checks and messages of src/layercom_cig.cpp:245-305 and :424-426
+    if (stitch_blend < -1) return bfail(b, KMIG_E_PARAM, "stitch_blend must be -1 or nonnegative");
+    if (scalex_aper < 0) return bfail(b, KMIG_E_PARAM, "scalex_aper must be nonnegative");
+    if (!std::isfinite(scalex_scale)) return bfail(b, KMIG_E_PARAM, "Invalid scalex_scale");
+    if (n < 1) return bfail(b, KMIG_E_PARAM, "No layers found");
+    if (start[0] != 0) return bfail(b, KMIG_E_PARAM, "First layer must start at depth sample 0");
+    for (int i = 0; i < n; i++) {
+        if (start[i] < 0 || end[i] < start[i])
+            return bfail(b, KMIG_E_PARAM, "Invalid layer range [%d,%d]", start[i], end[i]);
+        if (i > 0 && start[i] > end[i - 1])
+            return bfail(b, KMIG_E_PARAM, "Layer %d does not overlap previous layer", i + 1);
+        if (i > 0 && end[i] <= end[i - 1])
+            return bfail(b, KMIG_E_PARAM, "Layer %d must extend deeper than previous layer",
+                         i + 1);
+    }
+    const kmig_ctx *c0 = b->ctx[0];
+    const int nx = c0->nx, ng = c0->p.cig ? c0->p.ng : 1;
+    if (g_count <= 0) {
+        g_first = 0;
+        g_count = ng;
+    }
+    if (g_first < 0 || g_first + g_count > ng)
+        return bfail(b, KMIG_E_PARAM, "angle window [%d, %d) outside the %d angle samples",
+                     g_first, g_first + g_count, ng);
+    StitchArgs s{};
+    for (int i = 0; i < n; i++) {
+        const kmig_ctx *c = b->ctx[i];
+        const int ngi = c->p.cig ? c->p.ng : 1;
+        if (!c->ran || !c->cig) return bfail(b, KMIG_E_STATE, "layer %d has no CIG yet", i + 1);
+        if (c->nz != end[i] - start[i] + 1 || c->nx != nx || ngi != ng)
+            return bfail(b, KMIG_E_DIM,
+                         "layer %d dimensions are %d,%d,%d; expected %d,%d,%d", i + 1, c->nz,
+                         c->nx, ngi, end[i] - start[i] + 1, nx, ng);
+        s.layer[i] = c->cig;
+        s.start[i] = start[i];
+        s.end[i] = end[i];
+        s.scale[i] = scale ? scale[i] : 1.0f;
+    }
+    // lateral edge scale (src/layercom_cig.cpp:445-456)
+    std::vector<float> lat(nx, 1.0f);
+    for (int i2 = 0; i2 < nx && scalex_aper > 0; i2++) {
+        const int edge = std::min(i2, nx - 1 - i2);
+        if (edge < scalex_aper) {
+            const double taper =
+                scalex_aper == 1 ? 1.0 : 1.0 - (double)edge / (double)(scalex_aper - 1);
+            lat[i2] = (float)(1.0 + (scalex_scale - 1.0) * taper);
+        }
+    }
+    s.nlayers = n;
+    s.nz = end[n - 1] + 1;
+    s.nx = nx;
+    s.g0 = g_first;
+    s.ng_out = g_count;
+    s.stitch_blend = stitch_blend;
+    const size_t nout = (size_t)s.nz * nx * g_count;
+    float *dlat = nullptr, *dout = nullptr;
+    unsigned *dbad = nullptr;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    hipError_t e = hipMalloc((void **)&dlat, nx * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void **)&dout, nout * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void **)&dbad, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemcpy(dlat, lat.data(), nx * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(dbad, 0, sizeof(unsigned));
+    s.lat = dlat;
+    if (e == hipSuccess) e = (hipError_t)launch_stitch(s, dout, dbad, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    unsigned bad = 0;
+    if (e == hipSuccess) e = hipMemcpy(&bad, dbad, sizeof(unsigned), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(out, dout, nout * sizeof(float), hipMemcpyDeviceToHost);
+    dfree(dlat);
+    dfree(dout);
+    dfree(dbad);
+    if (e != hipSuccess)
+        return bfail(b, KMIG_E_DEVICE, "stitch failed: %s", hipGetErrorString(e));
+    if (bad) return bfail(b, KMIG_E_PARAM, "layers contain non-finite samples after scaling");
+    return KMIG_OK;
+}
+
 void kmig_batch_destroy(kmig_batch *b)
 {
     if (!b) return;

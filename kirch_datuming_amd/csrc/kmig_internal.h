@@ -159,6 +159,17 @@ struct MigResArgs {
 
 constexpr int KMIG_MAX_PROBLEMS = 8;
 
+// Depth-layer stitch of angle gathers (layercom_cig): layer il covers full
+// depth samples [start[il], end[il]] with CIG layer[il] ([ng][nx][n1_il]).
+struct StitchArgs {
+    const float *layer[KMIG_MAX_PROBLEMS];
+    int start[KMIG_MAX_PROBLEMS], end[KMIG_MAX_PROBLEMS];
+    float scale[KMIG_MAX_PROBLEMS];
+    const float *lat;   // [nx] lateral edge scale (layers after the first)
+    int nlayers, nz, nx, g0, ng_out;
+    int stitch_blend;   // -1: blend over the whole overlap
+};
+
 struct MigMultiArgs {
     int nprob;
     int tile0[KMIG_MAX_PROBLEMS + 1];  // first block of each problem; tile0[nprob] = grid
@@ -198,6 +209,7 @@ int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, floa
 int launch_pack16(const Tuples &in, PackedTuple16 *out, SlotScale *scale, int nslots,
                   size_t nzx, float dt, float th_scale, int precision, void *stream);
 int launch_pairs(const float *prep, float *pairs, long ntraces, int nt, void *stream);
+int launch_stitch(const StitchArgs &a, float *out, unsigned *nonfinite, void *stream);
 // tile: preset 9..21 (kmig_migrate.hip); precision: KMIG_PREC_* tuple format
 int launch_migrate_resident(const MigResArgs &a, int tile, int precision, void *stream);
 int launch_migrate_multi(const MigMultiArgs &m, int tile, int precision, void *stream);

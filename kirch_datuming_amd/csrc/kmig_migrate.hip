@@ -448,7 +448,12 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 float2 v0[G], vm[G], vp[G];
 #pragma unroll
                 for (int k = 0; k < G; k++) {
-                    const char *tr = (const char *)(trow + (size_t)(g0 + k) * a.nt);
+                    // receivers past nh (the last chunk's padding) are
+                    // rejected and read the sentinel pair of the shot's last
+                    // real trace: their own row would lie past this shot
+                    // (past the array for the last shot)
+                    const int kr = min(g0 + k, a.nh - 1 - r0);
+                    const char *tr = (const char *)(trow + (size_t)kr * a.nt);
                     v0[k] = *(const float2 *)(tr + q[k].o0);
                     vm[k] = *(const float2 *)(tr + q[k].om);
                     vp[k] = *(const float2 *)(tr + q[k].op);

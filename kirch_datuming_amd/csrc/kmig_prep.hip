@@ -683,6 +683,63 @@ __global__ void k_fill(float *p, size_t n, float v)
     if (i < n) p[i] = v;
 }
 
+// ---------------------------------------------------------------- stitch --
+
+// Depth-layer stitching of angle gathers (src/layercom_cig.cpp:412-557): one
+// thread per output sample (z, x, g) of the window [g0, g0 + ng_out) walks
+// the layers top to bottom exactly as the reference's loops visit that
+// sample: outside the previous layer's range the layer's value is written;
+// inside the overlap the upper value is kept for the first keep_upper
+// samples and then blended with complementary raised-cosine weights
+// w = 0.5 (1 - cos(pi u)), u = bi / (blend - 1) (double, then float), as
+// out (1 - w) + v w.  Values are the layer's sample times scale[layer] times
+// the lateral edge scale (layers after the first).  Non-finite values count
+// into *nonfinite and are written as 0, as the reference does before it fails.
+__global__ __launch_bounds__(256) void k_stitch(const StitchArgs s, float *out,
+                                                unsigned *nonfinite)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)s.nz * s.nx * s.ng_out;
+    if (idx >= n) return;
+    const int g1 = (int)(idx % s.nz);
+    const int i2 = (int)((idx / s.nz) % s.nx);
+    const int i3 = (int)(idx / ((size_t)s.nz * s.nx)) + s.g0;
+    float o = 0.0f;
+    unsigned bad = 0;
+    int previous_end = -1;
+    for (int il = 0; il < s.nlayers; il++) {
+        const int start = s.start[il], end = s.end[il];
+        if (g1 >= start && g1 <= end) {
+            const int n1 = end - start + 1;
+            const float sx = il > 0 ? s.lat[i2] : 1.0f;
+            const float scale = s.scale[il] * sx;
+            float v = s.layer[il][((size_t)i3 * s.nx + i2) * n1 + (g1 - start)] * scale;
+            if (!isfinite(v)) {
+                bad++;
+                v = 0.0f;
+            }
+            const int overlap = il > 0 ? previous_end - start + 1 : 0;
+            int blend = overlap;
+            if (overlap > 0 && s.stitch_blend >= 0) blend = min(s.stitch_blend, overlap);
+            const int keep_upper = overlap - blend;
+            if (overlap > 0 && g1 <= previous_end) {
+                const int k = g1 - start;
+                if (!(k < keep_upper || blend == 0)) {
+                    const int bi = k - keep_upper;
+                    const double u = blend > 1 ? (double)bi / (double)(blend - 1) : 1.0;
+                    const float w = (float)(0.5 * (1.0 - cos(KPI * u)));
+                    o = o * (1.0f - w) + v * w;
+                }
+            } else {
+                o = v;
+            }
+        }
+        previous_end = end;
+    }
+    out[idx] = o;
+    if (bad) atomicAdd(nonfinite, bad);
+}
+
 // -------------------------------------------------------------- launchers --
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
@@ -831,6 +888,15 @@ int launch_pack16(const Tuples &in, PackedTuple16 *out, SlotScale *scale, int ns
     hipLaunchKernelGGL(k_pack16, dim3(nblk(n, 256)), dim3(256), 0, st, in.T, in.A, in.GX, in.GZ,
                        in.W, (const SlotScale *)scale, out, nzx, n, 1.0 / (double)dt, th_scale,
                        precision);
+    return (int)hipGetLastError();
+}
+
+int launch_stitch(const StitchArgs &a, float *out, unsigned *nonfinite, void *stream)
+{
+    const size_t n = (size_t)a.nz * a.nx * a.ng_out;
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_stitch, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, a, out,
+                       nonfinite);
     return (int)hipGetLastError();
 }
 

@@ -31,7 +31,7 @@ EXPORTS = ("kmig_params_default", "kmig_params_parse", "kmig_params_check", "kmi
            "kmig_sync", "kmig_get_cig", "kmig_get_qc", "kmig_get_stats", "kmig_migrate",
            "kmig_stream", "kmig_version", "kmig_batch_create", "kmig_batch_run",
            "kmig_batch_sync", "kmig_batch_stats", "kmig_batch_last_error", "kmig_batch_destroy",
-           "kmig_set_surrogate", "kmig_get_surrogate_stats")
+           "kmig_set_surrogate", "kmig_get_surrogate_stats", "kmig_batch_stitch")
 
 
 class KmigError(RuntimeError):
@@ -113,6 +113,9 @@ def lib():
         L.kmig_batch_last_error.restype = C.c_char_p
         L.kmig_batch_destroy.argtypes = [vp]
         L.kmig_batch_destroy.restype = None
+        L.kmig_batch_stitch.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                        C.POINTER(C.c_float), C.c_int, C.c_int, C.c_double,
+                                        C.c_int, C.c_int, fp]
         L.kmig_set_surrogate.argtypes = [vp, C.c_int, vp, C.c_float]
         L.kmig_get_surrogate_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
                                                C.POINTER(C.c_double), C.POINTER(C.c_float)]
@@ -279,6 +282,24 @@ class Batch:
         self._chk(self.L.kmig_batch_stats(self.b, C.byref(u), C.byref(ms), C.byref(nodes)))
         return {"updates": u.value, "ms": ms.value, "graph_nodes": nodes.value}
 
+    def stitch(self, start, end, scale=None, stitch_blend=15, scalex_aper=0, scalex_scale=1.0,
+               angles=None):
+        """Depth-layer stitch of the batch's gathers (layercom_cig): layer k
+        covers full depth samples [start[k], end[k]]; angles = (first, count)
+        windows the angle axis.  Returns (count, nx, end[-1] + 1) float32."""
+        n = len(self.m)
+        st = (C.c_int * n)(*[int(v) for v in start])
+        en = (C.c_int * n)(*[int(v) for v in end])
+        sc = (C.c_float * n)(*[float(v) for v in scale]) if scale is not None else None
+        m0 = self.m[0]
+        ng = m0.p.ng if m0.p.cig else 1
+        g0, gn = angles if angles is not None else (0, ng)
+        out = np.empty((gn, m0.nx, int(end[-1]) + 1), np.float32)
+        self._chk(self.L.kmig_batch_stitch(self.b, st, en, sc, int(stitch_blend),
+                                           int(scalex_aper), float(scalex_scale), int(g0),
+                                           int(gn), _fp(out)))
+        return out
+
     def close(self):
         if self.b:
             self.L.kmig_batch_destroy(self.b)
@@ -295,6 +316,15 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def extract_adcig(adcig, ix):
+    """One lateral position of a (angle, x, z) gather volume as an (angle, z)
+    gather (src/extract_adcig.cpp:126-281; ix bounds as there)."""
+    ng, nx, nz = adcig.shape
+    if not 0 <= ix < nx:
+        raise KmigError(-1, f"ix must be in [0,{nx - 1}]")
+    return np.ascontiguousarray(adcig[:, ix, :])
 
 
 def migrate(kv, data, data_axes, stab, sder, s_axes, rtab, rder, r_axes, grads=None,
