@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -396,6 +397,57 @@ hipEvent_t next_event(kmig_ctx *c)
 }
 
 
+// Angle thresholds of the linear-bin range test of the resident kernel,
+// which evaluates u = fmaf(angle, inv_dg, g0_inv_dg) (correctly rounded on
+// host and device alike).  u is non-decreasing in the angle and non-negative
+// floats order as their bit patterns, so a bisection over the bits of
+// [0, amax] (amax = wrap / 2, the largest folded angle) finds the exact
+// boundaries:
+//   u <= ng - 1 + 1e-5  <=>  angle <= ang_hi  (-1: no angle passes)
+//   u >= -1e-5          <=>  angle >= ang_lo  (lo_check = 0: all of [0, amax])
+static void bin_thresholds(float amax, float inv_dg, float g0_inv_dg, int ng, float *ang_hi,
+                           float *ang_lo, int *lo_check)
+{
+    auto val = [](uint32_t bits) {
+        float f;
+        std::memcpy(&f, &bits, 4);
+        return f;
+    };
+    auto u_of = [&](uint32_t bits) { return std::fmaf(val(bits), inv_dg, g0_inv_dg); };
+    uint32_t hb;
+    std::memcpy(&hb, &amax, 4);
+    const float u_hi = (float)(ng - 1) + 1.0e-5f, u_lo = -1.0e-5f;
+    // largest angle with u <= u_hi
+    if (u_of(hb) <= u_hi) {
+        *ang_hi = amax;
+    } else if (!(u_of(0) <= u_hi)) {
+        *ang_hi = -1.0f;
+    } else {
+        uint32_t lo = 0, hi = hb;  // u(lo) <= u_hi < u(hi)
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            (u_of(mid) <= u_hi ? lo : hi) = mid;
+        }
+        *ang_hi = val(lo);
+    }
+    // smallest angle with u >= u_lo
+    if (u_of(0) >= u_lo) {
+        *ang_lo = 0.0f;
+        *lo_check = 0;
+    } else if (!(u_of(hb) >= u_lo)) {
+        *ang_lo = std::numeric_limits<float>::infinity();
+        *lo_check = 1;
+    } else {
+        uint32_t lo = 0, hi = hb;  // u(lo) < u_lo <= u(hi)
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            (u_of(mid) >= u_lo ? hi : lo) = mid;
+        }
+        *ang_lo = val(hi);
+        *lo_check = 1;
+    }
+}
+
 MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
 {
     const kmig_params &p = c->p;
@@ -423,9 +475,10 @@ MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
     R.c_frac = (float)(cst - ci);
     R.aal = p.antialias;
     R.wrap = p.half_angle ? 180.0f : 360.0f;
-    R.g0 = p.g0;
     R.inv_dg = 1.0f / p.dg;
     R.g0_inv_dg = -p.g0 / p.dg;
+    bin_thresholds(0.5f * R.wrap, R.inv_dg, R.g0_inv_dg, ng, &R.ang_hi, &R.ang_lo, &R.lo_check);
+    R.ang_hi_up = std::nextafter(R.ang_hi, std::numeric_limits<float>::infinity());
     R.ng = ng;
     R.oazmin = p.oazmin;
     R.oazmax = p.oazmax;
@@ -595,7 +648,12 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
         if (variant == KMIG_VARIANT_RESIDENT) {
             if ((rc = ensure(c, &c->pairs, &c->pairs_cap, (size_t)c->ns * c->nh * c->nt * 2)))
                 return rc;
-            if ((rc = ensure(c, &c->wap, &c->wap_cap, (size_t)c->ns * c->nh))) return rc;
+            // padded: the resident kernels read a whole receiver chunk of
+            // wap entries, past nh in the last chunk of the last shot; the
+            // pad is zero (finite) and meets zero tuple weights there
+            const size_t nwap = (size_t)c->ns * c->nh;
+            if ((rc = ensure(c, &c->wap, &c->wap_cap, nwap + RES_WAP_PAD))) return rc;
+            HIPCHK(c, hipMemset(c->wap + nwap, 0, RES_WAP_PAD * sizeof(float)));
             if ((rc = ensure(c, &c->rslot, &c->rslot_cap, (size_t)c->nh))) return rc;
             if (!wapv.empty())
                 HIPCHK(c, hipMemcpy(c->wap + (size_t)sb * c->nh, wapv.data(),

@@ -151,11 +151,21 @@ struct MigResArgs {
     float c_frac;                // fractional part of (tau - t0)/dt
     float aal;
     float wrap;                  // 360 (full opening angle) or 180 (half)
-    float g0, inv_dg, g0_inv_dg; // g0_inv_dg = -g0 / dg
+    // Bin coordinate u = fmaf(angle, inv_dg, g0_inv_dg), g0_inv_dg = -g0/dg.
+    // u is non-decreasing in the angle (dg > 0), so u <= ng - 1 + 1e-5 holds
+    // exactly for angle <= ang_hi (ang_hi_up: the next float above), and
+    // u >= -1e-5 for angle >= ang_lo (tested only when lo_check; otherwise
+    // every angle in [0, wrap / 2] passes).
+    float inv_dg, g0_inv_dg;
+    float ang_hi, ang_hi_up, ang_lo;
+    int lo_check;
     int ng;
     float oazmin, oazmax, oaztaper, inv_oaztaper;
     float taper_start, taper_end, inv_taper;
 };
+
+// wap entries past the last trace: at least the largest receiver chunk (R)
+constexpr int RES_WAP_PAD = 64;
 
 constexpr int KMIG_MAX_PROBLEMS = 8;
 

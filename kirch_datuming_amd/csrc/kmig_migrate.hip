@@ -209,7 +209,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
 // chunk, shot, receiver) order.
 struct Pend {
     float coef, w1, ft, ftm, ftp;
-    int ig0, ig1;          // bins; ig1 = min(ig0 + 1, ng - 1)
+    int ig0;               // first bin; the second is ig0 + 1 (row ng is a spare)
     unsigned o0, om, op;   // byte offsets of the three taps in the trace row
     bool ok;
 };
@@ -287,11 +287,6 @@ __device__ __forceinline__ float pair_angle(const PackedTuple &s, const PackedTu
     return fminf(d, wrap - d);
 }
 
-// Everything of one update except the opening angle and the depth-mute
-// weight, which the caller evaluates first (wd = 0 rejects the update).
-// s_int8, s_frac: the source tuple's integer samples (as a byte offset of
-// 8-byte tap pairs) and fractional samples, with the constant (tau - t0)/dt
-// already added (once per shot).
 // floor(x) as an integer in one instruction (the compiler emits v_floor and
 // v_cvt_i32 for (int)floorf(x)); x is finite here
 __device__ __forceinline__ int cvt_floor(float x)
@@ -301,52 +296,53 @@ __device__ __forceinline__ int cvt_floor(float x)
     return r;
 }
 
+// Everything of one update after the weights: aw is the product of the
+// trace-aperture weight, both tuple weights and the angle weights (depth
+// mute, taper, bin range), 0 or negative (skipped trace) where rejected;
+// u is the bin coordinate.  s_int8, s_frac: the source tuple's integer
+// samples (as a byte offset of 8-byte tap pairs) and fractional samples, with
+// the constant (tau - t0)/dt already added (once per shot).
 __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int8, float s_frac,
-                                           const PackedTuple &r, float wap, float angle,
-                                           float wd, const MigResArgs &a)
+                                           const PackedTuple &r, float aw, float u,
+                                           const MigResArgs &a)
 {
     Pend q;
-    const float inv32k = 1.0f / 32768.0f;
-    const float apw = (float)(s.iw & 0xFFFFu) * inv32k * ((float)(r.iw & 0xFFFFu) * inv32k);
-    // wd (the depth-mute weight times the angle taper) and apw are >= 0, so
-    // aw > 0 also rejects skipped traces (wap < 0)
-    const float aw = wd * wap * apw;
-    const float u = fmaf(angle, a.inv_dg, a.g0_inv_dg);  // (angle - g0) / dg
-    const float ngm1 = (float)(a.ng - 1);
-    bool ok = aw > 0.0f && u >= -1.0e-5f && u <= ngm1 + 1.0e-5f;
     // linear bins: u clamped to [0, ng-1] gives the edge bin with w1 = 0 at
     // either end and (floor(u), u - floor(u)) inside
-    const float uc = __builtin_amdgcn_fmed3f(u, 0.0f, ngm1);
+    const float uc = __builtin_amdgcn_fmed3f(u, 0.0f, (float)(a.ng - 1));
     const int ig0 = cvt_floor(uc);
     const float w1 = __builtin_amdgcn_fractf(uc);
     // split-sample time: integer parts add as integers (in bytes of 8-byte
-    // tap pairs), fractions in float
+    // tap pairs), fractions in float.  Taps at ff and ff -/+ (delta + 1)
+    // samples; both a fields are finite and >= 0 and antialias >= 0 (the
+    // resident variant requires it), so delta + 1 >= 1 and itm < it < itp:
+    // the outer taps bound the middle one.  Non-negative floats order as
+    // their bit patterns: one integer max.
+    int b8 = s_int8 + (((int)r.iw >> 16) << 3);
+    // kept as one value: re-associated, the three tap offsets cost two
+    // integer instructions each instead of one
+    asm("" : "+v"(b8));
     const float ff = s_frac + r.frac;
-    const int it8 = s_int8 + (((int)r.iw >> 16) << 3) + (cvt_floor(ff) << 3);
-    const float ft = __builtin_amdgcn_fractf(ff);
-    // taps at ft -/+ (delta + 1) samples; both a fields are finite and >= 0
-    // and antialias >= 0 (the resident variant requires it), so delta + 1 >= 1
-    // and itm < it < itp: the outer taps bound the middle one.  Non-negative
-    // floats order as their bit patterns: one integer max.
     const float amax = __int_as_float(max(__float_as_int(s.a), __float_as_int(r.a)));
     const float dA1 = fmaf(amax, a.aal, 1.0f);
-    const float fm = ft - dA1, fp = ft + dA1;
-    const int itm8 = it8 + (cvt_floor(fm) << 3), itp8 = it8 + (cvt_floor(fp) << 3);
+    const float fm = ff - dA1, fp = ff + dA1;
+    const int it8 = b8 + (cvt_floor(ff) << 3);
+    const int itm8 = b8 + (cvt_floor(fm) << 3), itp8 = b8 + (cvt_floor(fp) << 3);
     const unsigned lim8 = (unsigned)(a.nt - 1) << 3;
-    ok = ok && (unsigned)itm8 < lim8 && (unsigned)itp8 < lim8;
+    const bool ok = (aw > 0.0f) & ((unsigned)itm8 < lim8) & ((unsigned)itp8 < lim8);
     const float rr = __builtin_amdgcn_rcpf(fmaf(2.0f, dA1, 1.0f));  // 1 / (3 + 2 delta)
     // A rejected update reads the trace's last pair, (0, 0), at all three taps:
     // with its finite fractions and coefficient its picked value is an exact
-    // zero, added to a bin inside [0, ng) (uc is clamped, and w1 > 0 only
-    // below ng - 1), so only the tap indices are selected.
+    // zero, added to bins inside [0, ng] (uc is clamped; row ng is a spare
+    // that only ever receives w1 = 0 shares), so only the tap indices are
+    // selected.
     q.ok = ok;
     q.coef = aw * (rr * rr);
     q.w1 = w1;
-    q.ft = ft;
+    q.ft = __builtin_amdgcn_fractf(ff);
     q.ftm = __builtin_amdgcn_fractf(fm);
     q.ftp = __builtin_amdgcn_fractf(fp);
     q.ig0 = ig0;
-    q.ig1 = min(ig0 + 1, a.ng - 1);  // w1 = 0 wherever this clamps
     q.o0 = ok ? (unsigned)it8 : lim8;
     q.om = ok ? (unsigned)itm8 : lim8;
     q.op = ok ? (unsigned)itp8 : lim8;
@@ -367,11 +363,22 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
     const bool valid = iz < a.nz && ix < a.nx;
     const size_t i = valid ? (size_t)ix * a.nz + iz : 0;
     float *my = acc + lid;
-    for (int g = 0; g < a.ng; g++) my[g * P] = valid ? a.cig[(size_t)g * a.nzx + i] : 0.0f;
+    // ng bins and a spare row ng, which takes the zero second share of
+    // updates in the last bin, so both bins of an update are one LDS pair
+    for (int g = 0; g <= a.ng; g++)
+        my[g * P] = valid && g < a.ng ? a.cig[(size_t)g * a.nzx + i] : 0.0f;
     const float zf = (a.nz > 1) ? (float)iz / (float)(a.nz - 1) : 0.0f;
     const float cutoff = a.oazmin + zf * (a.oazmax - a.oazmin);
+    // depth mute: full weight up to cutoff, the ramp below cutoff_hi, zero
+    // above; every update also needs angle <= ang_hi (bin range), so the
+    // full-weight test is one compare against plim, and the ramp band is
+    // (cutoff, bhi) with bhi also capped by the bin range (the band is empty
+    // wherever the cutoff lies above the last bin)
     const float cutoff_hi = cutoff + a.oaztaper;
+    const float plim = fminf(cutoff, a.ang_hi);
+    const float bhi = fminf(cutoff_hi, a.ang_hi_up);
     const bool taper_on = !(a.taper_start < 0.0f || a.taper_end <= a.taper_start);
+    const float inv32k = 1.0f / 32768.0f;
     const float2 *__restrict__ pairs = (const float2 *)a.pairs;
 
     for (int r0 = 0; r0 < a.nh; r0 += R) {
@@ -395,34 +402,46 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             if (__ballot(live) == 0) continue;
             const int s_int8 = (((int)st.iw >> 16) + a.c_int) << 3;
             const float s_frac = st.frac + a.c_frac;
+            const float s_w = (float)(st.iw & 0xFFFFu) * (inv32k * inv32k);
             const int is = a.shot0 + js;
             const float *__restrict__ wrow = a.wap + (size_t)is * a.nh + r0;
             const float2 *__restrict__ trow = pairs + ((size_t)is * a.nh + r0) * a.nt;
 #pragma unroll
             for (int g0 = 0; g0 < R; g0 += G) {
-                // stage 1: opening angle and depth-mute weight (a wave-level
-                // skip of rejected groups measured slower: few groups are empty)
-                // The depth-mute ramp is evaluated only when some lane of the
-                // wave has an angle inside the taper band (cutoff, cutoff +
-                // oaztaper); outside it the weight is 1 or 0.
-                float ang[G], wap[G], wd[G];
+                // stage 1: opening angle, bin coordinate and weight.  aw =
+                // trace-aperture weight x both tuple weights where the angle
+                // passes the depth mute and the bin range, else 0; skipped
+                // traces (wap < 0) leave aw < 0, and the receivers past nh
+                // that pad the last chunk have zero tuple weight (their wap
+                // entries are finite: the array is padded).  The depth-mute
+                // ramp is evaluated only when some lane of the wave has an
+                // angle inside the band; outside it the weight is 1 or 0.  (A
+                // wave-level skip of rejected groups measured slower: few
+                // groups are empty.)
+                float ang[G], u[G], aw[G], base[G];
                 bool band = false;
 #pragma unroll
                 for (int k = 0; k < G; k++) {
-                    wap[k] = (r0 + g0 + k < a.nh) ? wrow[g0 + k] : -1.0f;
+                    base[k] = wrow[g0 + k] * s_w * (float)(rt[g0 + k].iw & 0xFFFFu);
                     ang[k] = pair_angle(st, rt[g0 + k], a.wrap);
-                    wd[k] = ang[k] <= cutoff ? 1.0f : 0.0f;
+                    u[k] = fmaf(ang[k], a.inv_dg, a.g0_inv_dg);  // (angle - g0) / dg
+                    aw[k] = ang[k] <= plim ? base[k] : 0.0f;
                     // bitwise, not short-circuit: no per-update branches
-                    band |= (ang[k] > cutoff) & (ang[k] < cutoff_hi);
+                    band |= (ang[k] > cutoff) & (ang[k] < bhi);
                 }
                 if (a.oaztaper > 0.0f && __ballot(band) != 0) {
 #pragma unroll
                     for (int k = 0; k < G; k++) {
                         const float rd = ramp_down(
                             fminf(fmaxf((ang[k] - cutoff) * a.inv_oaztaper, 0.0f), 1.0f));
-                        wd[k] = ang[k] <= cutoff ? 1.0f
-                                                 : (ang[k] >= cutoff + a.oaztaper ? 0.0f : rd);
+                        const float wd =
+                            ang[k] <= cutoff ? 1.0f : (ang[k] >= cutoff_hi ? 0.0f : rd);
+                        aw[k] = ang[k] <= a.ang_hi ? wd * base[k] : 0.0f;
                     }
+                }
+                if (a.lo_check) {
+#pragma unroll
+                    for (int k = 0; k < G; k++) aw[k] = ang[k] >= a.ang_lo ? aw[k] : 0.0f;
                 }
                 // the angle taper (angle_taper_start/end) multiplies in here
                 if (taper_on) {
@@ -430,16 +449,16 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                     for (int k = 0; k < G; k++) {
                         const float rt_ = ramp_down(fminf(
                             fmaxf((ang[k] - a.taper_start) * a.inv_taper, 0.0f), 1.0f));
-                        wd[k] *= ang[k] >= a.taper_end ? 0.0f
+                        aw[k] *= ang[k] >= a.taper_end ? 0.0f
                                                        : (ang[k] > a.taper_start ? rt_ : 1.0f);
                     }
                 }
-                // stage 2: weights, bins and tap indices
+                // stage 2: bins and tap indices
                 Pend q[G];
                 bool any = false;
 #pragma unroll
                 for (int k = 0; k < G; k++) {
-                    q[k] = res_phase1(st, s_int8, s_frac, rt[g0 + k], wap[k], ang[k], wd[k], a);
+                    q[k] = res_phase1(st, s_int8, s_frac, rt[g0 + k], aw[k], u[k], a);
                     any |= q[k].ok;
                 }
                 if (__ballot(any) == 0) continue;
@@ -469,8 +488,9 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                                      (vm[k].x + q[k].ftm * vm[k].y) -
                                      (vp[k].x + q[k].ftp * vp[k].y);
                     const float picked = q[k].coef * sm;
-                    my[q[k].ig0 * P] += (1.0f - q[k].w1) * picked;
-                    my[q[k].ig1 * P] += q[k].w1 * picked;
+                    float *cell = my + q[k].ig0 * P;
+                    cell[0] += (1.0f - q[k].w1) * picked;
+                    cell[P] += q[k].w1 * picked;
                 }
             }
         }
@@ -510,11 +530,13 @@ static void tile_dims(int tile, int *tz, int *tx)
     }
 }
 
+// LDS of a migrate block: ng bin rows plus the resident kernels' spare row
+// (one row more than the stream kernel needs; the bound stays conservative)
 int migrate_lds_bytes(int ng, int tile)
 {
     int tz, tx;
     tile_dims(tile, &tz, &tx);
-    return ng * tz * tx * (int)sizeof(float);
+    return (ng + 1) * tz * tx * (int)sizeof(float);
 }
 
 template <int TZ, int TX>
@@ -537,7 +559,7 @@ template <int TZ, int TX, int R, int G, bool Probe = false, int Fmt = 0>
 static int launch_ph_t(const MigResArgs &a, hipStream_t st)
 {
     const int ntz = (a.nz + TZ - 1) / TZ, ntx = (a.nx + TX - 1) / TX;
-    const size_t lds = (size_t)a.ng * TZ * TX * sizeof(float);
+    const size_t lds = (size_t)(a.ng + 1) * TZ * TX * sizeof(float);
     if (lds > 64 * 1024) {
         const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_ph<TZ, TX, R, G, Probe, Fmt>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -610,7 +632,7 @@ int migrate_tiles(int nz, int nx, int tile)
 template <int TZ, int TX, int R, int Fmt>
 static int launch_multi_t(const MigMultiArgs &m, hipStream_t st)
 {
-    const size_t lds = (size_t)m.prob[0].ng * TZ * TX * sizeof(float);
+    const size_t lds = (size_t)(m.prob[0].ng + 1) * TZ * TX * sizeof(float);
     if (lds > 64 * 1024) {
         const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_multi<TZ, TX, R, 4, Fmt>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
