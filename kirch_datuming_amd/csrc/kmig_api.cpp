@@ -84,6 +84,8 @@ struct kmig_ctx {
     size_t pk16_s_cap = 0, pk16_r_cap = 0;
     SlotScale *sc_s = nullptr, *sc_r = nullptr;
     size_t sc_s_cap = 0, sc_r_cap = 0;
+    ThBound *bnd_s = nullptr, *bnd_r = nullptr;  // direction bounds per (slot, cell)
+    size_t bnd_s_cap = 0, bnd_r_cap = 0;
     float *pairs = nullptr;
     size_t pairs_cap = 0;
     float *tmpd = nullptr;          // diff2 scratch (diff=1 only)
@@ -468,6 +470,15 @@ MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
     R.rslot = c->rslot;
     R.wap = c->wap;
     R.pairs = c->pairs;
+    // KMIG_CULL=0 switches the group culling off (same results, for timing)
+    const char *cull_env = std::getenv("KMIG_CULL");
+    const bool cull = !(cull_env && std::atoi(cull_env) == 0);
+    R.sbnd = cull ? c->bnd_s : nullptr;
+    R.rbnd = cull ? c->bnd_r : nullptr;
+    R.ncz = (c->nz + BND_CZ - 1) / BND_CZ;
+    R.ncx = (c->nx + BND_CX - 1) / BND_CX;
+    R.sbnd_stride = nshots;
+    R.nrslots = c->batches.empty() ? 0 : c->batches[0].nrcv;
     R.cig = c->cig;
     const double cst = ((double)p.tau - (double)c->t0) / (double)c->dt;
     const double ci = std::floor(cst);
@@ -506,14 +517,24 @@ int autotune(kmig_ctx *c, const MigResArgs &full)
     const int npresets = p16 ? 5 : 7;
     const double per_shot = (double)c->nh * (double)c->nzx;
     int S = (int)std::max(1.0, std::min((double)full.nshots, 5e9 / std::max(per_shot, 1.0)));
-    // a slice from the middle of the batch (source tuples are batch-local)
-    const int off = (full.nshots - S) / 2;
-    MigResArgs R = full;
-    R.nshots = S;
-    R.shot0 = full.shot0 + off;
-    if (full.stup) R.stup = full.stup + (size_t)off * c->nzx;
-    if (full.stup16) R.stup16 = full.stup16 + (size_t)off * c->nzx;
-    if (full.sscale) R.sscale = full.sscale + off;
+    // two half slices, centred at 1/4 and 3/4 of the batch (source tuples are
+    // batch-local): how much of the work the group culling removes varies
+    // along the shot line, so one middle slice can misjudge the presets
+    const int S1 = std::max(1, S / 2), S2 = std::max(0, S - S1);
+    MigResArgs Rs[2];
+    for (int h = 0; h < 2; h++) {
+        const int n = h ? S2 : S1;
+        const int off = std::min(std::max(0, (full.nshots * (1 + 2 * h)) / 4 - n / 2),
+                                 std::max(0, full.nshots - n));
+        MigResArgs &R = Rs[h];
+        R = full;
+        R.nshots = n;
+        R.shot0 = full.shot0 + off;
+        if (full.stup) R.stup = full.stup + (size_t)off * c->nzx;
+        if (full.stup16) R.stup16 = full.stup16 + (size_t)off * c->nzx;
+        if (full.sscale) R.sscale = full.sscale + off;
+        if (full.sbnd) R.sbnd = full.sbnd + off;  // rows keep their full stride
+    }
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
@@ -526,8 +547,10 @@ int autotune(kmig_ctx *c, const MigResArgs &full)
         float tmin = 1e30f;
         for (int rep = 0; rep < 2; rep++) {
             HIPCHK(c, hipEventRecord(e0, c->stream));
-            HIPCHK(c, (hipError_t)launch_migrate_resident_probe(R, t, c->p.precision,
-                                                                c->stream));
+            for (int h = 0; h < 2; h++)
+                if (Rs[h].nshots > 0)
+                    HIPCHK(c, (hipError_t)launch_migrate_resident_probe(Rs[h], t, c->p.precision,
+                                                                        c->stream));
             HIPCHK(c, hipEventRecord(e1, c->stream));
             HIPCHK(c, hipEventSynchronize(e1));
             float ms = 0.0f;
@@ -646,6 +669,12 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
                 return rc;
         }
         if (variant == KMIG_VARIANT_RESIDENT) {
+            const size_t ncell = (size_t)((c->nz + BND_CZ - 1) / BND_CZ) *
+                                 ((c->nx + BND_CX - 1) / BND_CX);
+            if ((rc = ensure(c, &c->bnd_s, &c->bnd_s_cap, std::max<size_t>(1, maxs) * ncell)))
+                return rc;
+            if ((rc = ensure(c, &c->bnd_r, &c->bnd_r_cap, std::max<size_t>(1, maxr) * ncell)))
+                return rc;
             if ((rc = ensure(c, &c->pairs, &c->pairs_cap, (size_t)c->ns * c->nh * c->nt * 2)))
                 return rc;
             // padded: the resident kernels read a whole receiver chunk of
@@ -850,6 +879,14 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
             HIPCHK(c, (hipError_t)launch_pack16(c->tr, c->pk16_r, c->sc_r, b.nrcv, c->nzx,
                                                 c->dt, th_scale, p.precision, st));
     }
+    if (resident) {
+        HIPCHK(c, (hipError_t)launch_th_bounds(pack32 ? c->pk_s : nullptr,
+                                               pack32 ? nullptr : c->pk16_s, nsb, c->nz, c->nx,
+                                               c->bnd_s, st));
+        HIPCHK(c, (hipError_t)launch_th_bounds(pack32 ? c->pk_r : nullptr,
+                                               pack32 ? nullptr : c->pk16_r, b.nrcv, c->nz,
+                                               c->nx, c->bnd_r, st));
+    }
     return KMIG_OK;
 }
 
@@ -918,6 +955,8 @@ void kmig_destroy(kmig_ctx *c)
     dfree(c->pk_s);
     dfree(c->pk_r);
     dfree(c->pk16_s);
+    dfree(c->bnd_s);
+    dfree(c->bnd_r);
     dfree(c->pk16_r);
     dfree(c->sc_s);
     dfree(c->sc_r);

@@ -133,6 +133,22 @@ struct SlotScale {
     float a_scale;
 };
 
+// Bounds of the packed directions th (output-angle degrees) of one slot over
+// one cell of BND_CZ x BND_CX image points, counting only points of nonzero
+// tuple weight: lo = floor(min th * BND_Q), hi = ceil(max th * BND_Q), exact
+// integers (|th| <= 180).  A cell with no such point has lo = BND_EMPTY > hi.
+struct ThBound {
+    int lo, hi;
+};
+constexpr int BND_CZ = 8, BND_CX = 4;
+constexpr float BND_Q = 65536.0f;
+constexpr int BND_EMPTY = 1 << 28;
+
+// Bounds of every (slot, cell) of FP32 (pk) or 16-bit (pk16) packed tuples,
+// cell-major: out[cell * nslots + slot], cell = cx * ncz + cz.
+int launch_th_bounds(const PackedTuple *pk, const PackedTuple16 *pk16, int nslots, int nz,
+                     int nx, ThBound *out, void *stream);
+
 struct MigResArgs {
     int nz, nx;
     size_t nzx;
@@ -146,6 +162,12 @@ struct MigResArgs {
     const int *rslot;            // [nh] receiver slot of each receiver index (-1: unused)
     const float *wap;            // [ns][nh] trace-aperture weight, < 0: trace skipped
     const float *pairs;          // [ns][nh][nt][2] (tr[k], tr[k+1] - tr[k])
+    const ThBound *sbnd;         // [ncz * ncx][sbnd_stride] direction bounds of the
+                                 // shots (entry js of a row: batch-local shot js),
+                                 // or null (no culling)
+    const ThBound *rbnd;         // [ncz * ncx][nrslots]
+    int ncz, ncx;                // cells of BND_CZ x BND_CX points
+    int sbnd_stride, nrslots;    // shots per sbnd row, receiver slots
     float *cig;
     int c_int;                   // integer part of (tau - t0)/dt
     float c_frac;                // fractional part of (tau - t0)/dt

@@ -900,6 +900,81 @@ int launch_stitch(const StitchArgs &a, float *out, unsigned *nonfinite, void *st
     return (int)hipGetLastError();
 }
 
+// Direction bounds per (slot, cell of BND_CZ x BND_CX points), over the
+// points of nonzero tuple weight (the resident migrate kernel culls whole
+// groups of updates with them), stored cell-major: out[cell * nslots + slot],
+// so one cell's bounds for consecutive shots share cache lines.  One wave per
+// (slot, cell column, 64 rows): lane = row, so each tuple load is one
+// contiguous 1 KB run; groups of BND_CZ lanes then reduce to their cell.
+template <bool Half>
+__global__ __launch_bounds__(64) void k_th_bounds(const PackedTuple *__restrict__ pk,
+                                                  const PackedTuple16 *__restrict__ pk16,
+                                                  int nslots, int nz, int nx, int ncz, int ncx,
+                                                  ThBound *__restrict__ out)
+{
+    const int nzb = (nz + 63) / 64;
+    const size_t blk = blockIdx.x;
+    const size_t slot = blk / ((size_t)ncx * nzb);
+    const int rem = (int)(blk - slot * ncx * nzb);
+    const int cx = rem / nzb, zb = rem - cx * nzb;
+    const int lane = threadIdx.x;
+    const int iz = zb * 64 + lane;
+    const size_t nzx = (size_t)nz * nx;
+    float lo = INFINITY, hi = -INFINITY;
+    if (iz < nz) {
+        for (int jx = 0; jx < BND_CX; jx++) {
+            const int ix = cx * BND_CX + jx;
+            if (ix >= nx) break;
+            const size_t i = slot * nzx + (size_t)ix * nz + iz;
+            float th;
+            uint32_t w;
+            if constexpr (Half) {
+                th = __uint_as_float(pk16[i].th);
+                w = pk16[i].iw & 0xFFFFu;
+            } else {
+                th = pk[i].th;
+                w = pk[i].iw & 0xFFFFu;
+            }
+            if (w != 0u) {
+                lo = fminf(lo, th);
+                hi = fmaxf(hi, th);
+            }
+        }
+    }
+    for (int o = 1; o < BND_CZ; o <<= 1) {
+        lo = fminf(lo, __shfl_xor(lo, o, 64));
+        hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+    }
+    const int cz = zb * (64 / BND_CZ) + lane / BND_CZ;
+    if (lane % BND_CZ != 0 || cz >= ncz) return;
+    ThBound b;
+    if (lo > hi) {
+        b.lo = BND_EMPTY;
+        b.hi = -BND_EMPTY;
+    } else {
+        // |th| <= 180, so th * 2^16 is exact and below 2^24
+        b.lo = (int)floorf(lo * BND_Q);
+        b.hi = (int)ceilf(hi * BND_Q);
+    }
+    out[(size_t)(cx * ncz + cz) * nslots + slot] = b;
+}
+
+int launch_th_bounds(const PackedTuple *pk, const PackedTuple16 *pk16, int nslots, int nz,
+                     int nx, ThBound *out, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    const int ncz = (nz + BND_CZ - 1) / BND_CZ, ncx = (nx + BND_CX - 1) / BND_CX;
+    const size_t nblocks = (size_t)nslots * ncx * ((nz + 63) / 64);
+    if (nblocks == 0) return 0;
+    if (pk16)
+        hipLaunchKernelGGL(k_th_bounds<true>, dim3((unsigned)nblocks), dim3(64), 0, st, pk, pk16,
+                           nslots, nz, nx, ncz, ncx, out);
+    else
+        hipLaunchKernelGGL(k_th_bounds<false>, dim3((unsigned)nblocks), dim3(64), 0, st, pk,
+                           pk16, nslots, nz, nx, ncz, ncx, out);
+    return (int)hipGetLastError();
+}
+
 int launch_pairs(const float *prep, float *pairs, long ntraces, int nt, void *stream)
 {
     const size_t n = (size_t)ntraces * nt;
