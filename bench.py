@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--cpu-shots", type=int, default=2,
                     help="shots of the workload timed on the CPU oracle (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--overlap", type=int, default=0,
+                    help="layers: extend layers 2..5 upward by this many depth samples and "
+                         "stitch the gathers on the GPU (layercom_cig blend)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain device copy of this many MiB (profiler calibration)")
     args = ap.parse_args()
@@ -221,7 +224,7 @@ def bench_layers(args, cores):
     from kirch_datuming_amd import configs, kmig
     t0 = time.time()
     log(f"building the five-layer workload on {cores} host threads")
-    ws = configs.layers(threads=cores)
+    ws = configs.layers(threads=cores, overlap=args.overlap)
     total = sum(w.updates for w in ws)
     log(f"workload ready in {time.time() - t0:.1f} s: {total:.4e} updates in 5 layers")
     ms = []
@@ -252,7 +255,21 @@ def bench_layers(args, cores):
     value = total * args.steps / wall
     log(f"timed {args.steps} steps in {wall:.3f} s ({st['graph_nodes']} graph nodes)")
     cigs = [m.cig() for m in ms]
-    adcig = configs.stack_layers(cigs)
+    stitch = None
+    if args.overlap > 0:
+        # the layers overlap: blend them into the full ADCIG on the GPU
+        # (layercom_cig's default 15-sample cosine blend); the wall time
+        # includes the device scratch and the copy of the volume to the host
+        start, end = [w.z_range[0] for w in ws], [w.z_range[1] for w in ws]
+        b.stitch(start, end)
+        ts = time.perf_counter()
+        adcig = b.stitch(start, end)
+        stitch = {"ms_with_copy_out": (time.perf_counter() - ts) * 1e3, "blend": 15,
+                  "ranges": [list(r) for r in zip(start, end)]}
+        log(f"stitched {adcig.shape} in {stitch['ms_with_copy_out']:.1f} ms (with copy-out)")
+    else:
+        # C5's layers abut (nz 80/70/90/100/60 = 400): the gathers concatenate
+        adcig = configs.stack_layers(cigs)
     cpu = parity = None
     if not args.no_cpu and args.cpu_shots > 0:
         from oracle import oracle as O
@@ -275,9 +292,11 @@ def bench_layers(args, cores):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else f"{args.precision} tables, f32 math",
         "data": "synthetic",
-        "config": {"workload": "five salt layers nz 80/70/90/100/60 x 1000, shots "
-                               "768/640/512/384/256, receivers 401/361/321/281/241, nt=2001, "
-                               "one graph-captured step", "config": "layers",
+        "config": {"workload": "five salt layers nz 80/70/90/100/60 x 1000"
+                               + (f" (layers 2-5 extended {args.overlap} samples up)"
+                                  if args.overlap else "") +
+                               ", shots 768/640/512/384/256, receivers 401/361/321/281/241, "
+                               "nt=2001, one graph-captured step", "config": "layers",
                    "updates_per_step": total, "precision": args.precision,
                    "graph_nodes": st["graph_nodes"]},
         "roofline": {"bound": "hbm", "achieved": total * BYTES_PER_UPDATE[args.precision] /
@@ -288,6 +307,7 @@ def bench_layers(args, cores):
                      "algorithmic_bytes_per_update": BYTES_PER_UPDATE[args.precision]},
         "cpu_baseline": cpu, "parity": parity, "upload_s": upload_s,
         "adcig_shape": list(adcig.shape), "adcig_abs_max": float(np.abs(adcig).max()),
+        "stitch": stitch,
     }
     b.close()
     for m in ms:

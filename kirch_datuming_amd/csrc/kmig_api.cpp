@@ -1399,7 +1399,7 @@ int batch_capture(kmig_batch *b, int tile)
 // from zero, so the timing replays leave no trace in the results.
 int batch_tune(kmig_batch *b)
 {
-    static const int presets[] = {9, 10, 15, 13};
+    static const int presets[] = {9, 10, 11, 15, 13};
     const int ng = b->ctx[0]->p.ng;
     b->tune_log.clear();
     int best = 0;

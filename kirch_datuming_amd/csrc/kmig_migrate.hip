@@ -707,31 +707,33 @@ int migrate_tiles(int nz, int nx, int tile)
     return ((nz + tz - 1) / tz) * ((nx + tx - 1) / tx);
 }
 
-template <int TZ, int TX, int R, int Fmt>
+template <int TZ, int TX, int R, int G, int Fmt>
 static int launch_multi_t(const MigMultiArgs &m, hipStream_t st)
 {
     const size_t lds = (size_t)(m.prob[0].ng + 1) * TZ * TX * sizeof(float);
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_multi<TZ, TX, R, 4, Fmt>,
+        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_multi<TZ, TX, R, G, Fmt>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL((k_migrate_multi<TZ, TX, R, 4, Fmt>), dim3((unsigned)m.tile0[m.nprob]),
+    hipLaunchKernelGGL((k_migrate_multi<TZ, TX, R, G, Fmt>), dim3((unsigned)m.tile0[m.nprob]),
                        dim3(TZ * TX), lds, st, m);
     return (int)hipGetLastError();
 }
 
-// presets 9 (16x8 points, R = 8), 10 (16x8, R = 16), 15 (8x8, R = 8) and
-// 13 (8x8, R = 16); the grid in m.tile0 must use the same tile shape
+// presets 9 (16x8 points, R = 8, G = 4), 10 (16x8, R = 16, G = 4), 11 (16x8,
+// R = 8, G = 8), 15 (8x8, R = 8, G = 4) and 13 (8x8, R = 16, G = 4); the grid
+// in m.tile0 must use the same tile shape
 template <int Fmt>
 static int launch_multi_fmt(const MigMultiArgs &m, int tile, hipStream_t st)
 {
     switch (tile) {
-    case 10: return launch_multi_t<16, 8, 16, Fmt>(m, st);
-    case 13: return launch_multi_t<8, 8, 16, Fmt>(m, st);
-    case 15: return launch_multi_t<8, 8, 8, Fmt>(m, st);
-    default: return launch_multi_t<16, 8, 8, Fmt>(m, st);
+    case 10: return launch_multi_t<16, 8, 16, 4, Fmt>(m, st);
+    case 11: return launch_multi_t<16, 8, 8, 8, Fmt>(m, st);
+    case 13: return launch_multi_t<8, 8, 16, 4, Fmt>(m, st);
+    case 15: return launch_multi_t<8, 8, 8, 4, Fmt>(m, st);
+    default: return launch_multi_t<16, 8, 8, 4, Fmt>(m, st);
     }
 }
 

@@ -318,10 +318,24 @@ class Batch:
             pass
 
 
-def extract_adcig(adcig, ix):
+def extract_adcig(adcig, ix=None, x=None, o2=0.0, d2=1.0):
     """One lateral position of a (angle, x, z) gather volume as an (angle, z)
-    gather (src/extract_adcig.cpp:126-281; ix bounds as there)."""
+    gather (src/extract_adcig.cpp:126-281): by index ix, or by position x on
+    the lateral axis o2 + i * d2, rounded to the nearest sample as there
+    (:163-177); out-of-range requests raise with the reference's messages."""
     ng, nx, nz = adcig.shape
+    if ix is None:
+        if x is None:
+            raise KmigError(-1, "ix= or x= is required")
+        if d2 == 0.0:
+            raise KmigError(-1, "Cannot use x= because d2=0")
+        if not np.isfinite(x):
+            raise KmigError(-1, "Invalid x= value")
+        q = (float(x) - o2) / d2
+        nearest = int(np.floor(abs(q) + 0.5)) * (1 if q >= 0 else -1)  # llround
+        if not 0 <= nearest < nx:
+            raise KmigError(-1, f"x={x:g} is outside lateral axis [{o2:g}, {o2 + (nx - 1) * d2:g}]")
+        ix = nearest
     if not 0 <= ix < nx:
         raise KmigError(-1, f"ix must be in [0,{nx - 1}]")
     return np.ascontiguousarray(adcig[:, ix, :])

@@ -1,5 +1,6 @@
 # Compiles the reference's kirchmig2d_cig path (and the table/shot/compare
-# helpers used to write golden fixtures) directly from its sources under
+# helpers and the layer stitch/extract programs used to write golden
+# fixtures) directly from its sources under
 # REF, with the reference's release flags (-O3 -DNDEBUG + OpenMP).  Nothing
 # is copied, generated or stubbed: the vendored fftw3.h header and sqlite3.c
 # source are the reference's own.  Outputs go to _ref/ only.
@@ -14,7 +15,7 @@ SR := kirmig.cpp tinterpsr.cpp eikods.cpp neighbors.cpp pqueue.cpp decart.cpp ef
 OB := $(patsubst %.cpp,$(OUT)/obj/SEBASIC/%.o,$(SB)) $(OUT)/obj/sqlite3.o
 OF := $(patsubst %.cpp,$(OUT)/obj/SEFILESYSTEM/%.o,$(SF))
 OR := $(patsubst %.cpp,$(OUT)/obj/SERECKIRCH/src/%.o,$(SR))
-PROGS := kirchmig2d_cig eikods2d shotgen rsf_l2cmp
+PROGS := kirchmig2d_cig eikods2d shotgen rsf_l2cmp layercom_cig extract_adcig
 
 all: $(addprefix $(OUT)/,$(PROGS))
 
