@@ -2,15 +2,15 @@
 
 small   v(z) = 1.5 + 0.5 z km/s, nz=201, nx=301, dz=dx=0.01 km; 61 shots every
         0.05 km at z=0; 201 receivers at 0.015 km (cmp=0, absolute receiver
-        axis); tables by fast marching at the 61 shot and 201 receiver
-        positions; nt=1001, dt=2 ms; 15 Hz Ricker at the two-way times of
+        axis); eikods2d tables (fast marching, box 2) at the 61 shot and 201
+        receiver positions; nt=1001, dt=2 ms; 15 Hz Ricker at the two-way times of
         flat reflectors z = 0.6 / 1.2 / 1.7 km (r = +0.1 / -0.1 / +0.1) plus
         Gaussian noise at 20 dB SNR; angle axis 0..60 deg, 61 bins.
 layers  the salt model cut into five depth layers migrated as separate
         problems (nz 80/70/90/100/60 from the top; shots 768/640/512/384/256 and
         receivers 401/361/321/281/241 spread over the same 0..9.59 / 0..10 km;
         per-layer mutes, slopes 50/40/30/20/10 s/km, tapers 0.05..0.10 s);
-        tables by fast marching inside each layer from its top (the datum the
+        eikods2d tables inside each layer from its top (the datum the
         recursive workflow moves the acquisition to), Born data from scatterers
         inside each layer; angle axis 0..60 deg for every layer, so the five
         CIGs stack into one (400, 1000, 61) angle gather volume.
@@ -21,8 +21,9 @@ salt    five-layer model 1.5 / 2.0 / 2.6 / 4.5 (salt, Gaussian-bump top) /
         (Born-style) events plus a direct arrival; direct_mute=1 with slope
         50 s/km, intercept 0, taper 0.05 s; angle axis 0..60 deg, 61 bins.
 
-Inputs are generated by libkmsynth (own C++: seeded RNG, fast-marching
-tables), so they are identical on every machine.
+Velocity models and data come from libkmsynth (own C++, seeded RNG) and the
+traveltime tables from libkmeik (kmig_eikods2d, bitwise the reference's
+eikods2d), so the inputs are identical on every machine.
 """
 
 from __future__ import annotations
@@ -48,7 +49,6 @@ def synth():
         f, i = C.c_float, C.c_int
         L.kms_velocity_vz.argtypes = [fp, i, i, f, f, f]
         L.kms_velocity_salt.argtypes = [fp, i, i, f, f]
-        L.kms_tables.argtypes = [fp, i, i, f, f, f, i, f, f, i, i, fp, fp]
         L.kms_time_vz.restype = C.c_double
         L.kms_time_vz.argtypes = [C.c_double] * 5
         L.kms_data_reflectors.argtypes = [fp, i, f, f, i, f, f, i, f, f, i, f, f, i, fp, fp, f,
@@ -56,7 +56,6 @@ def synth():
         L.kms_data_scatterers.argtypes = [fp, i, f, f, i, f, f, i, f, f, i, fp, i, f, f, fp, i,
                                           f, f, i, i, i, f, f, C.c_uint64]
         L.kms_velocity_grf.argtypes = [fp, i, i, f, f, f, f, f, C.c_uint64]
-        L.kms_nn_records.argtypes = [fp, i, i, f, f, f, i, f, f, i, i, i, i, C.c_uint64, i, fp]
         _SYN = L
     return _SYN
 
@@ -106,11 +105,17 @@ class Workload:
 
 
 def tables(vel, dz, dx, npos, o, d, box=2, threads=0):
-    nx, nz = vel.shape
-    T = np.empty((npos, nx, nz), np.float32)
-    D = np.empty_like(T)
-    synth().kms_tables(_p(vel), nz, nx, dz, dx, 0.0, npos, o, d, box, threads, _p(T), _p(D))
-    return T, D
+    """Traveltime tables and their source derivatives at npos surface
+    positions, made as the reference's workflow makes them
+    (run_salt_adcig.sh:108-119): shotgen positions o + k d in float, then
+    eikods2d with a box of `box` samples (kmig_eikods2d, bitwise the
+    program's time= and tds1= outputs).  vel: (nx, nz), z fastest."""
+    from . import eik
+    shots = np.zeros((npos, 3), np.float32)
+    shots[:, 1] = np.float32(o) + np.arange(npos, dtype=np.float32) * np.float32(d)
+    r = eik.eikods2d(vel, np.float32(dz), np.float32(dx), shots, b1=box, b2=box,
+                     threads=threads, grads=False, tdl1=False)
+    return r["time"], r["tds1"]
 
 
 def small(threads=0, seed=20260101) -> Workload:
@@ -232,18 +237,25 @@ def table_records(T, D, pos, nper, seed, dz=0.01, dx=0.01):
 
 def grf_records(nmodels=256, npos=8, nper=2048, seed=20260505, threads=0, corr=0.3,
                 window=(201, 301), s_hi=3.0):
-    """Records from fast-marching tables of nmodels seeded Gaussian-random-field
-    velocity models (nz=400, nx=1000, dz=dx=0.01 km, v in 1.5..4.5 km/s,
-    correlation length corr km): npos random surface sources in [0, s_hi] km
-    per model (tables at s and s -/+ one cell for dT/ds), nper records per
-    source inside the top-left window (nz, nx) of the small workload."""
+    """Records from eikods2d tables (kmig_eikods2d, box 2) of nmodels seeded
+    Gaussian-random-field velocity models (nz=400, nx=1000, dz=dx=0.01 km, v in
+    1.5..4.5 km/s, correlation length corr km): npos random surface sources in
+    [0, s_hi] km per model, nper records per source inside the top-left window
+    (nz, nx) of the small workload; dT/ds is the table's source derivative."""
+    from . import eik
     L = synth()
     nz, nx = 400, 1000
     v = np.empty((nx, nz), np.float32)
-    out = np.empty((nmodels * npos * nper, 7), np.float32)
+    rng = np.random.default_rng(seed)
+    out = []
     for m in range(nmodels):
         L.kms_velocity_grf(_p(v), nz, nx, 0.01, 0.01, corr, 1.5, 4.5, seed + 7919 * m)
-        blk = out[m * npos * nper:(m + 1) * npos * nper]
-        L.kms_nn_records(_p(v), nz, nx, 0.01, 0.01, 0.0, npos, 0.0, s_hi, window[0], window[1],
-                         nper, 2, seed + 104729 * (m + 1), threads, _p(blk))
-    return out
+        pos = np.sort(rng.uniform(0.0, s_hi, npos)).astype(np.float32)
+        shots = np.zeros((npos, 3), np.float32)
+        shots[:, 1] = pos
+        r = eik.eikods2d(v, np.float32(0.01), np.float32(0.01), shots, b1=2, b2=2,
+                         threads=threads, grads=False, tdl1=False)
+        T = r["time"][:, :window[1], :window[0]]
+        D = r["tds1"][:, :window[1], :window[0]]
+        out.append(table_records(T, D, pos, nper, seed + 104729 * (m + 1)))
+    return np.concatenate(out)

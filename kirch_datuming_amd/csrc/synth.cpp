@@ -1,19 +1,10 @@
-// libkmsynth: synthetic inputs for the adj=1 imaging path (velocity models,
-// traveltime tables, seismic data), all seeded and deterministic.
+// libkmsynth: synthetic inputs for the adj=1 imaging path (velocity models
+// and seismic data), all seeded and deterministic.  The traveltime tables
+// come from the fast-marching table source (eikonal.cpp, libkmeik).
 //
-// Traveltime tables: first-arrival times from a fast-marching eikonal solve
-// (second-order upwind, binary heap, double precision) per table position, with
-// a straight-ray box of radius `box` cells around the source like eikods2d's
-// b1=b2 box (src/eikods2d.cpp:251-281).  dT/ds at each position node is the
-// centred difference of neighbouring position tables (one-sided at the ends),
-// the derivative the Hermite interpolation along the position axis consumes.
-//
-// Surrogate training data: Gaussian-random-field velocity models (white noise
+// Surrogate training models: Gaussian-random-field velocities (white noise
 // smoothed by a separable Gaussian filter, rescaled, added to a random linear
-// v(z) trend, clipped) and records (s, z, x, T, dT/dx, dT/dz, dT/ds) sampled
-// from fast-marching tables: dT/dx, dT/dz are centred differences of the
-// table on the grid (one-sided at edges), dT/ds the centred difference of
-// the tables at s -/+ one grid step.
+// v(z) trend, clipped).
 //
 // Data: 15-20 Hz Ricker wavelets at analytic two-way times (flat reflectors
 // in v(z) = v0 + g z, or point scatterers with table times), plus seeded
@@ -54,147 +45,6 @@ double ricker(double t, double f)
 {
     const double a = (M_PI * f * t) * (M_PI * f * t);
     return (1.0 - 2.0 * a) * std::exp(-a);
-}
-
-// Indexed binary min-heap over grid nodes.
-struct Heap {
-    std::vector<int> h;      // node ids
-    std::vector<int> where;  // position in h, -1 if absent
-    const double *key;
-    explicit Heap(size_t n, const double *k) : where(n, -1), key(k) { h.reserve(4096); }
-    void swap_(int a, int b)
-    {
-        std::swap(h[a], h[b]);
-        where[h[a]] = a;
-        where[h[b]] = b;
-    }
-    void up(int i)
-    {
-        while (i > 0) {
-            int p = (i - 1) / 2;
-            if (key[h[p]] <= key[h[i]]) break;
-            swap_(p, i);
-            i = p;
-        }
-    }
-    void down(int i)
-    {
-        const int n = (int)h.size();
-        for (;;) {
-            int l = 2 * i + 1, r = l + 1, m = i;
-            if (l < n && key[h[l]] < key[h[m]]) m = l;
-            if (r < n && key[h[r]] < key[h[m]]) m = r;
-            if (m == i) break;
-            swap_(m, i);
-            i = m;
-        }
-    }
-    void push_or_decrease(int node)
-    {
-        if (where[node] < 0) {
-            h.push_back(node);
-            where[node] = (int)h.size() - 1;
-            up((int)h.size() - 1);
-        } else {
-            up(where[node]);
-        }
-    }
-    int pop()
-    {
-        int top = h[0];
-        where[top] = -1;
-        h[0] = h.back();
-        h.pop_back();
-        if (!h.empty()) {
-            where[h[0]] = 0;
-            down(0);
-        }
-        return top;
-    }
-    bool empty() const { return h.empty(); }
-};
-
-// First-arrival times on an (nx, nz) grid (z fastest) for a point source.
-void fmm(const double *slow, int nz, int nx, double dz, double dx, double zs, double xs, int box,
-         double *T, std::vector<uint8_t> &state, bool order2)
-{
-    const size_t n = (size_t)nz * nx;
-    const double INF = std::numeric_limits<double>::infinity();
-    std::fill(T, T + n, INF);
-    state.assign(n, 0);  // 0 far, 1 trial, 2 known
-    Heap heap(n, T);
-    const int izs = (int)std::floor(zs / dz + 0.5), ixs = (int)std::floor(xs / dx + 0.5);
-    const int izc = std::min(std::max(izs, 0), nz - 1), ixc = std::min(std::max(ixs, 0), nx - 1);
-    const double s0 = slow[(size_t)ixc * nz + izc];
-    for (int ix = std::max(0, ixc - box); ix <= std::min(nx - 1, ixc + box); ix++)
-        for (int iz = std::max(0, izc - box); iz <= std::min(nz - 1, izc + box); iz++) {
-            const size_t k = (size_t)ix * nz + iz;
-            T[k] = s0 * std::hypot(iz * dz - zs, ix * dx - xs);
-            state[k] = 1;
-            heap.push_or_decrease((int)k);
-        }
-    const int dzi[4] = {-1, 1, 0, 0}, dxi[4] = {0, 0, -1, 1};
-    while (!heap.empty()) {
-        const int k = heap.pop();
-        state[k] = 2;
-        const int iz = k % nz, ix = k / nz;
-        for (int q = 0; q < 4; q++) {
-            const int jz = iz + dzi[q], jx = ix + dxi[q];
-            if (jz < 0 || jz >= nz || jx < 0 || jx >= nx) continue;
-            const size_t j = (size_t)jx * nz + jz;
-            if (state[j] == 2) continue;
-            // upwind neighbours along each axis: second-order one-sided
-            // stencil (3T - 4T1 + T2) / 2h where the next node is known and
-            // earlier, first order otherwise (Sethian 1999, sec. 8.3)
-            double qa[2], qb[2];
-            int nq = 0;
-            for (int ax = 0; ax < 2; ax++) {
-                const int step = ax == 0 ? 1 : nz;
-                const int pos = ax == 0 ? jz : jx, lim = ax == 0 ? nz : nx;
-                const double hh = ax == 0 ? dz : dx;
-                double best = INF, best2 = INF;
-                for (int sg = -1; sg <= 1; sg += 2) {
-                    const int p1 = pos + sg, p2 = pos + 2 * sg;
-                    if (p1 < 0 || p1 >= lim) continue;
-                    const size_t j1 = (size_t)((long)j + (long)sg * step);
-                    if (state[j1] != 2 || T[j1] >= best) continue;
-                    best = T[j1];
-                    best2 = INF;
-                    if (p2 >= 0 && p2 < lim) {
-                        const size_t j2 = (size_t)((long)j + 2L * sg * step);
-                        if (state[j2] == 2 && T[j2] <= T[j1]) best2 = T[j2];
-                    }
-                }
-                if (best == INF) continue;
-                if (order2 && best2 < INF) {
-                    qa[nq] = 9.0 / (4.0 * hh * hh);
-                    qb[nq] = (4.0 * best - best2) / 3.0;
-                } else {
-                    qa[nq] = 1.0 / (hh * hh);
-                    qb[nq] = best;
-                }
-                nq++;
-            }
-            const double s = slow[j];
-            double t = INF;
-            if (nq == 2) {
-                // qa0 (t - qb0)^2 + qa1 (t - qb1)^2 = s^2
-                const double a = qa[0] + qa[1];
-                const double b = -2.0 * (qa[0] * qb[0] + qa[1] * qb[1]);
-                const double c = qa[0] * qb[0] * qb[0] + qa[1] * qb[1] * qb[1] - s * s;
-                const double disc = b * b - 4.0 * a * c;
-                if (disc >= 0.0) t = (-b + std::sqrt(disc)) / (2.0 * a);
-                if (!(t >= std::max(qb[0], qb[1]))) t = INF;  // not upwind: 1-D update
-            }
-            if (t == INF)
-                for (int q2 = 0; q2 < nq; q2++) t = std::min(t, qb[q2] + s / std::sqrt(qa[q2]));
-            if (t < T[j]) {
-                T[j] = t;
-                state[j] = 1;
-                heap.push_or_decrease((int)j);
-            }
-        }
-    }
 }
 
 void smooth5(std::vector<double> &v, int nz, int nx)
@@ -291,62 +141,6 @@ void kms_velocity_grf(float *v, int nz, int nx, float dz, float dx, float corr, 
         }
 }
 
-// Training records of one model: for each of npos source positions drawn
-// uniformly in [s_lo, s_hi] (sources at depth zs), nper records at grid
-// nodes drawn uniformly in the window iz < nz_win, ix < nx_win.  out holds
-// npos * nper records of 7 floats.  Returns 0.
-int kms_nn_records(const float *vel, int nz, int nx, float dz, float dx, float zs, int npos,
-                   float s_lo, float s_hi, int nz_win, int nx_win, int nper, int box,
-                   uint64_t seed, int nthreads, float *out)
-{
-    const size_t n = (size_t)nz * nx;
-    std::vector<double> slow(n);
-    for (size_t k = 0; k < n; k++) slow[k] = 1.0 / (double)vel[k];
-    nz_win = std::min(std::max(nz_win, 1), nz);
-    nx_win = std::min(std::max(nx_win, 1), nx);
-#ifdef _OPENMP
-    if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel
-#endif
-    {
-        std::vector<uint8_t> state;
-        std::vector<double> T0(n), Tm(n), Tp(n);
-#ifdef _OPENMP
-#pragma omp for schedule(dynamic)
-#endif
-        for (int k = 0; k < npos; k++) {
-            Rng rng(seed ^ (0x2545F4914F6CDD1Dull * (uint64_t)(k + 1)));
-            const double xs = s_lo + (s_hi - s_lo) * rng.uniform();
-            const double h = dx;
-            fmm(slow.data(), nz, nx, dz, dx, zs, xs, box, T0.data(), state, true);
-            fmm(slow.data(), nz, nx, dz, dx, zs, xs - h, box, Tm.data(), state, true);
-            fmm(slow.data(), nz, nx, dz, dx, zs, xs + h, box, Tp.data(), state, true);
-            float *o = out + (size_t)k * nper * 7;
-            for (int q = 0; q < nper; q++) {
-                const int iz = std::min(nz_win - 1, (int)(nz_win * rng.uniform()));
-                const int ix = std::min(nx_win - 1, (int)(nx_win * rng.uniform()));
-                const size_t c = (size_t)ix * nz + iz;
-                const int za = std::max(iz - 1, 0), zb = std::min(iz + 1, nz - 1);
-                const int xa = std::max(ix - 1, 0), xb = std::min(ix + 1, nx - 1);
-                const double tz = (T0[(size_t)ix * nz + zb] - T0[(size_t)ix * nz + za]) /
-                                  ((zb - za) * (double)dz);
-                const double tx = (T0[(size_t)xb * nz + iz] - T0[(size_t)xa * nz + iz]) /
-                                  ((xb - xa) * (double)dx);
-                const double ts = (Tp[c] - Tm[c]) / (2.0 * h);
-                float *r = o + (size_t)q * 7;
-                r[0] = (float)xs;
-                r[1] = (float)(iz * (double)dz);
-                r[2] = (float)(ix * (double)dx);
-                r[3] = (float)T0[c];
-                r[4] = (float)tx;
-                r[5] = (float)tz;
-                r[6] = (float)ts;
-            }
-        }
-    }
-    return 0;
-}
-
 // v(z) = v0 + g z on (nx, nz), z fastest.
 void kms_velocity_vz(float *v, int nz, int nx, float dz, float v0, float g)
 {
@@ -378,44 +172,6 @@ void kms_velocity_salt(float *v, int nz, int nx, float dz, float dx)
     }
     smooth5(m, nz, nx);
     for (size_t k = 0; k < m.size(); k++) v[k] = (float)m[k];
-}
-
-// Tables T and dT/ds (npos, nx, nz) for sources at depth zs and lateral
-// positions o + k d (k < npos).  Returns 0.
-int kms_tables(const float *vel, int nz, int nx, float dz, float dx, float zs, int npos, float o,
-               float d, int box, int nthreads, float *T, float *D)
-{
-    const size_t n = (size_t)nz * nx;
-    std::vector<double> slow(n);
-    for (size_t k = 0; k < n; k++) slow[k] = 1.0 / (double)vel[k];
-    std::vector<double> Td((size_t)npos * n);
-#ifdef _OPENMP
-    if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel
-#endif
-    {
-        std::vector<uint8_t> state;
-#ifdef _OPENMP
-#pragma omp for schedule(dynamic)
-#endif
-        for (int k = 0; k < npos; k++) {
-            const double xs = (double)o + k * (double)d;
-            fmm(slow.data(), nz, nx, dz, dx, zs, xs, box, Td.data() + (size_t)k * n, state, true);
-        }
-    }
-    for (size_t k = 0; k < Td.size(); k++) T[k] = (float)Td[k];
-    for (int k = 0; k < npos; k++) {
-        const int a = std::max(0, k - 1), b = std::min(npos - 1, k + 1);
-        float *Dk = D + (size_t)k * n;
-        if (a == b) {
-            std::fill(Dk, Dk + n, 0.0f);
-            continue;
-        }
-        const double inv = 1.0 / ((b - a) * (double)d);
-        const double *ta = Td.data() + (size_t)a * n, *tb = Td.data() + (size_t)b * n;
-        for (size_t q = 0; q < n; q++) Dk[q] = (float)((tb[q] - ta[q]) * inv);
-    }
-    return 0;
 }
 
 // Analytic first-arrival time in v(z) = v0 + g z between (0, xs) and (z, x).

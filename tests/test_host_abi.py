@@ -32,7 +32,14 @@ def test_library_exports_every_declared_symbol():
     for n in names + nn_names:
         assert hasattr(L, n), n
     assert b"gfx950" in L.kmig_version()
-    assert set(os.listdir(os.path.join(ROOT, "include"))) == {"kmig.h", "kmig_nn.h"}
+    from kirch_datuming_amd import eik
+    E = eik.lib()
+    eik_names = _header_functions("kmig_eik.h")
+    assert set(eik_names) == set(eik.EXPORTS)
+    for n in eik_names:
+        assert hasattr(E, n), n
+    assert set(os.listdir(os.path.join(ROOT, "include"))) == {"kmig.h", "kmig_nn.h",
+                                                               "kmig_eik.h"}
 
 
 @pytest.mark.parametrize("name", gc.case_names())
