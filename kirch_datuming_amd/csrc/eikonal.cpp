@@ -148,6 +148,34 @@ struct Marcher {
         return left;
     }
 
+    // one axis of the quadratic update: the smaller neighbour along the axis
+    // (coordinate c of n, stride st), second-order stencil where the point
+    // two steps out is known and not later
+    void leg(Leg &l, int i, int c, int nc, int st, float rd) const
+    {
+        const float a = c > 0 ? t[i - st] : FLT_MAX;
+        const float b = c < nc - 1 ? t[i + st] : FLT_MAX;
+        l.delta = rd;
+        l.stencil = l.value = a < b ? a : b;
+        if (order > 1) {
+            float tt;
+            if (a < b && c - 2 >= 0) {
+                const int k = i - 2 * st;
+                if (flag[k] != kOut && a >= (tt = t[k])) {
+                    l.delta *= 2.25;
+                    l.stencil = (4.0 * l.value - tt) / 3.0;
+                }
+            }
+            if (a > b && c + 2 <= nc - 1) {
+                const int k = i + 2 * st;
+                if (flag[k] != kOut && b >= (tt = t[k])) {
+                    l.delta *= 2.25;
+                    l.stencil = (4.0 * l.value - tt) / 3.0;
+                }
+            }
+        }
+    }
+
     bool solve(int m, float *res, Leg *const *v) const
     {
         double a = 0.0, b = 0.0, c = 0.0;
@@ -165,35 +193,15 @@ struct Marcher {
         return true;
     }
 
-    // new time at grid point i from its upwind neighbours
-    float quad(int i)
+    // new time at grid point i = (iz, ix) from its upwind neighbours; the
+    // models are 2-D (n[2] = 1), so the third leg is always empty
+    float quad(int i, int iz, int ix)
     {
         Leg x[3];
-        for (int j = 0; j < 3; j++) {
-            const int ix = (i / s[j]) % n[j];
-            const float a = ix > 0 ? t[i - s[j]] : FLT_MAX;
-            const float b = ix < n[j] - 1 ? t[i + s[j]] : FLT_MAX;
-            Leg &l = x[j];
-            l.delta = rdx[j];
-            l.stencil = l.value = a < b ? a : b;
-            if (order > 1) {
-                float tt;
-                if (a < b && ix - 2 >= 0) {
-                    const int k = i - 2 * s[j];
-                    if (flag[k] != kOut && a >= (tt = t[k])) {
-                        l.delta *= 2.25;
-                        l.stencil = (4.0 * l.value - tt) / 3.0;
-                    }
-                }
-                if (a > b && ix + 2 <= n[j] - 1) {
-                    const int k = i + 2 * s[j];
-                    if (flag[k] != kOut && b >= (tt = t[k])) {
-                        l.delta *= 2.25;
-                        l.stencil = (4.0 * l.value - tt) / 3.0;
-                    }
-                }
-            }
-        }
+        leg(x[0], i, iz, n[0], 1, rdx[0]);
+        leg(x[1], i, ix, n[1], s[1], rdx[1]);
+        x[2].stencil = x[2].value = FLT_MAX;
+        x[2].delta = rdx[2];
         // the legs in ascending value, ties as the reference's network
         Leg *v[3];
         if (x[0].value <= x[1].value) {
@@ -238,20 +246,17 @@ struct Marcher {
         return 0;
     }
 
+    // the four neighbours in the reference's order (+z, -z, +x, -x)
     int update_neighbours(int i)
     {
+        const int ix = i / n[0], iz = i - ix * n[0];
         int added = 0;
-        for (int j = 0; j < 3; j++) {
-            const int ix = (i / s[j]) % n[j];
-            if (ix + 1 <= n[j] - 1) {
-                const int k = i + s[j];
-                if (flag[k] != kIn) added += relax(quad(k), k);
-            }
-            if (ix - 1 >= 0) {
-                const int k = i - s[j];
-                if (flag[k] != kIn) added += relax(quad(k), k);
-            }
-        }
+        if (iz + 1 <= n[0] - 1 && flag[i + 1] != kIn) added += relax(quad(i + 1, iz + 1, ix), i + 1);
+        if (iz - 1 >= 0 && flag[i - 1] != kIn) added += relax(quad(i - 1, iz - 1, ix), i - 1);
+        const int sx = s[1];
+        if (ix + 1 <= n[1] - 1 && flag[i + sx] != kIn)
+            added += relax(quad(i + sx, iz, ix + 1), i + sx);
+        if (ix - 1 >= 0 && flag[i - sx] != kIn) added += relax(quad(i - sx, iz, ix - 1), i - sx);
         return added;
     }
 
@@ -275,32 +280,28 @@ struct Marcher {
     }
 };
 
-// ascending order of non-negative float keys (radix sort on the bits;
-// FLT_MAX and +0 included), ties in index order
-void time_order(const float *t, int n, std::vector<int> &ord, std::vector<int> &tmp)
+// ascending order of the times (radix sort on the float bits, stable, so ties
+// stay in index order): ord[k] is the k-th point.  buf is reused.
+void time_order(const float *t, int n, std::vector<uint64_t> &buf, std::vector<int> &ord)
 {
-    ord.resize(n);
-    tmp.resize(n);
-    std::vector<uint32_t> key(n), key2(n);
+    buf.resize(2 * (size_t)n);
+    uint64_t *a = buf.data(), *b = buf.data() + n;
     for (int i = 0; i < n; i++) {
         uint32_t u;
         std::memcpy(&u, &t[i], 4);
-        // negative keys (never produced) would sort last; map them below
-        key[i] = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-        ord[i] = i;
+        // order-preserving map of the float bits (negative times never occur)
+        u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+        a[i] = ((uint64_t)u << 32) | (uint32_t)i;
     }
-    for (int shift = 0; shift < 32; shift += 11) {
+    for (int shift = 32; shift < 64; shift += 11) {
         int cnt[2049] = {0};
-        for (int i = 0; i < n; i++) cnt[((key[i] >> shift) & 2047u) + 1]++;
-        for (int b = 0; b < 2048; b++) cnt[b + 1] += cnt[b];
-        for (int i = 0; i < n; i++) {
-            const int p = cnt[(key[i] >> shift) & 2047u]++;
-            tmp[p] = ord[i];
-            key2[p] = key[i];
-        }
-        ord.swap(tmp);
-        key.swap(key2);
+        for (int i = 0; i < n; i++) cnt[((a[i] >> shift) & 2047u) + 1]++;
+        for (int k = 0; k < 2048; k++) cnt[k + 1] += cnt[k];
+        for (int i = 0; i < n; i++) b[cnt[(a[i] >> shift) & 2047u]++] = a[i];
+        std::swap(a, b);
     }
+    ord.resize(n);
+    for (int i = 0; i < n; i++) ord[i] = (int)(a[i] & 0xFFFFFFFFu);
 }
 
 struct Derivs {
@@ -311,13 +312,9 @@ struct Derivs {
     void first(int l, float *dl1, float *ds1, int i0) const
     {
         int ii[3];
-        {
-            int q = i0;
-            for (int a = 0; a < 3; a++) {
-                ii[a] = q % nn[a];
-                q /= nn[a];
-            }
-        }
+        ii[1] = i0 / nn[0];
+        ii[0] = i0 - ii[1] * nn[0];
+        ii[2] = 0;  // 2-D
         double rhs = 0.0, den = 0.0, dw = 0.0, dt = 0.0;
         for (int j = 0; j < ndim; j++) {
             const int a = i0 - ss[j], b = i0 + ss[j];
@@ -428,7 +425,8 @@ int kmig_eikods2d(const kmig_eik_config *cfg, const float *vel, int n1, int n2, 
 #endif
     {
         Marcher m(n1, n2, n3, d1, d2, d3, cfg->order);
-        std::vector<int> ord, tmp;
+        std::vector<int> ord;
+        std::vector<uint64_t> sortbuf;
         std::vector<float> dl1_scratch(tdl1 ? 0 : n123);  // the recursion needs dT/dl
         Derivs dv;
         dv.nn[0] = n1; dv.nn[1] = n2; dv.nn[2] = n3;
@@ -455,7 +453,7 @@ int kmig_eikods2d(const kmig_eik_config *cfg, const float *vel, int n1, int n2, 
                 status = KMIG_EIK_W_FRONT;
             }
             dv.t0 = t;
-            time_order(t, n123, ord, tmp);
+            time_order(t, n123, sortbuf, ord);
             float *dl1 = tdl1 ? tdl1 + off : dl1_scratch.data(), *ds1 = tds1 + off;
             for (int k = 0; k < n123; k++) dv.first(cfg->l, dl1, ds1, ord[k]);
             if (gradz || gradx) {
@@ -465,7 +463,7 @@ int kmig_eikods2d(const kmig_eik_config *cfg, const float *vel, int n1, int n2, 
                 for (int q = 0; q < n123; q++) {
                     if (gradz) gradz[off + q] = upwind(t, q, 0, nn, ss, dd);
                     if (gradx) gradx[off + q] = upwind(t, q, 1, nn, ss, dd);
-                }
+                }  // (a small share of the time: one pass of two stencils)
             }
         }
     }
