@@ -71,7 +71,7 @@ extern "C" {
 /* Every key of kirchmig2d_cig's adj=1 path, same names, units and defaults
    (src/kirchmig2d_cig.cpp:520-560, :568-598, :707-717, :816-862, :952). */
 typedef struct kmig_params {
-    int adj;                   /* 1 (migration); adj=0 is not provided      */
+    int adj;                   /* 1 migration (kmig_run), 0 modelling (kmig_model) */
     int cig;                   /* 1: common-image gathers, 0: stacked image */
     int cig_is_angle;          /* mode=angle (1) or mode=offset (0)         */
     int cmp;                   /* 1: data axis 2 is offset, 0: receiver x   */
@@ -161,6 +161,20 @@ int kmig_get_stats(const kmig_ctx *ctx, kmig_stats *st);
 int kmig_migrate(kmig_ctx *ctx, const float *data, int nt, float t0, float dt, int nh,
                  float h0, float dh, int ns, float s0, float ds, float *cig,
                  float *qc_before, float *qc_after);
+
+/* adj=0 Kirchhoff modelling, replacing the adj=0 branch of
+   src/kirchmig2d_cig.cpp:979-1351 (data axes from its nt/t0/dt/nh/h0/dh/ns/s0/
+   ds keys, :663-701): the image (ng*nx*nz floats, the migration= file) is
+   sprayed with the transposed kirmig_pick (SERECKIRCH/src/kirmig.cpp:59-67)
+   and the migration's weights, bins and apertures into every trace, each
+   trace is doubly integrated, and data receives ns*nh*nt floats (zeros
+   outside the trace aperture).  It is the adjoint of kmig_run without mute
+   and diff.  The context needs adj=0 and the resident variant's conditions
+   (mode=angle, angle_interp=1, antialias >= 0, cmp=0, precision=fp32).
+   Synchronous.  The reference's own adj=0 run ends in a segmentation fault
+   after its axis message (the output header's ndim is never set, :875-920). */
+int kmig_model(kmig_ctx *ctx, const float *image, int nt, float t0, float dt, int nh,
+               float h0, float dh, int ns, float s0, float ds, float *data);
 
 /* The HIP stream of the context (a hipStream_t), for callers that time or
    order their own work around kmig_run. */

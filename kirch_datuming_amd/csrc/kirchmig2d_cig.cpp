@@ -1,4 +1,5 @@
 // kirchmig2d_cig — drop-in command line for the adj=1 path of the reference
+// (and its adj=0 modelling)
 // program of the same name (src/kirchmig2d_cig.cpp:494-1401): same key=value
 // parameters and RSF inputs/outputs, migration on the GPU through libkmig.
 //
@@ -55,18 +56,39 @@ int main(int argc, char **argv)
     char err[1024] = {0};
     if (kmig::collect_kv(kv, argc - 1, (const char *const *)(argv + 1), err, sizeof(err)))
         die(err);
-    if (kv.count("adj") && atoi(kv["adj"].c_str()) != 1)
-        die("adj=0 (Kirchhoff modelling) is not provided by this build; use adj=1.");
+    const int adj = kv.count("adj") ? atoi(kv["adj"].c_str()) : 1;
     const std::string dat_f = need(kv, "seismic_data"), mig_f = need(kv, "migration");
     const std::string stim_f = need(kv, "stable"), sder_f = need(kv, "sderiv");
     const std::string rtim_f = need(kv, "rtable"), rder_f = need(kv, "rderiv");
 
-    Vol dat = load(dat_f);
-    if (dat.h.ndim < 3) die("Need 3D data volume for seismic_data= (t,h/r,s)");
-    const int nt = dat.h.n[0], nh = dat.h.n[1], ns = dat.h.n[2];
-    const float t0 = (float)dat.h.o[0], dt = (float)dat.h.d[0];
-    const float h0 = (float)dat.h.o[1], dh = (float)dat.h.d[1];
-    const float s0 = (float)dat.h.o[2], ds = (float)dat.h.d[2];
+    Vol dat;
+    int nt, nh, ns;
+    float t0, dt, h0, dh, s0, ds;
+    if (adj) {
+        dat = load(dat_f);
+        if (dat.h.ndim < 3) die("Need 3D data volume for seismic_data= (t,h/r,s)");
+        nt = dat.h.n[0], nh = dat.h.n[1], ns = dat.h.n[2];
+        t0 = (float)dat.h.o[0], dt = (float)dat.h.d[0];
+        h0 = (float)dat.h.o[1], dh = (float)dat.h.d[1];
+        s0 = (float)dat.h.o[2], ds = (float)dat.h.d[2];
+    } else {
+        // the data axes of the modelled volume (src/kirchmig2d_cig.cpp:662-699)
+        auto geti = [&](const char *k, int def) {
+            return kv.count(k) ? atoi(kv[k].c_str()) : def;
+        };
+        auto getf = [&](const char *k, float def) {
+            return kv.count(k) ? (float)atof(kv[k].c_str()) : def;
+        };
+        nt = atoi(need(kv, "nt").c_str());
+        nh = geti("nh", 1);
+        ns = geti("ns", 1);
+        t0 = getf("t0", 0.0f);
+        dt = (float)atof(need(kv, "dt").c_str());
+        h0 = getf("h0", 0.0f);
+        dh = (float)atof(need(kv, "dh").c_str());
+        s0 = getf("s0", 0.0f);
+        ds = (float)atof(need(kv, "ds").c_str());
+    }
 
     kmig_params p;
     int rc = kmig::parse_kv(&p, kv, nh, h0, dh, err, sizeof(err));
@@ -146,12 +168,48 @@ int main(int argc, char **argv)
     chk(kmig_set_tables(ctx, 1, rtim.v.data(), rder.v.data(), use_grad ? g[2].v.data() : nullptr,
                         use_grad ? g[3].v.data() : nullptr, nz, nx, rny, z0, dz, x0, dx,
                         (float)rtim.h.o[2], (float)rtim.h.d[2]));
+    const int ng = p.cig ? p.ng : 1;
+    if (!adj) {
+        // modelling: image in (migration=), data out (seismic_data=)
+        Vol img = load(mig_f);
+        if (img.h.n[0] != nz || img.h.n[1] != nx || img.h.n[2] != ng) {
+            snprintf(err, sizeof(err),
+                     "Dimension mismatch in migration=. Expected n1=%d n2=%d n3=%d", nz, nx, ng);
+            die(err);
+        }
+        std::vector<float> data((size_t)nt * nh * ns);
+        auto m0 = std::chrono::steady_clock::now();
+        chk(kmig_model(ctx, img.v.data(), nt, t0, dt, nh, h0, dh, ns, s0, ds, data.data()));
+        auto m1 = std::chrono::steady_clock::now();
+        kmig_stats st;
+        kmig_get_stats(ctx, &st);
+        printf("INFO: Done. Elapsed time: %.3f s. (model kernels %.3f ms)\n",
+               std::chrono::duration<double>(m1 - m0).count(), st.ms_migrate);
+        rsf::Header dh_;
+        dh_.ndim = 3;
+        dh_.n[0] = nt;
+        dh_.o[0] = t0;
+        dh_.d[0] = dt;
+        dh_.n[1] = nh;
+        dh_.o[1] = h0;
+        dh_.d[1] = nh == 1 ? 0.0f : dh;
+        dh_.n[2] = ns;
+        dh_.o[2] = s0;
+        dh_.d[2] = ns == 1 ? 0.0f : ds;
+        dh_.extra["label1"] = "Time";
+        dh_.extra["unit1"] = "s";
+        dh_.extra["label2"] = p.cmp ? "Offset" : "Receiver";
+        dh_.extra["label3"] = "Shot";
+        std::string werr;
+        if (!rsf::write(dat_f, data.data(), dh_, werr)) die(werr);
+        kmig_destroy(ctx);
+        return 0;
+    }
     auto t_start = std::chrono::steady_clock::now();
     chk(kmig_set_data(ctx, dat.v.data(), nt, t0, dt, nh, h0, dh, ns, s0, ds));
     chk(kmig_run(ctx, 0, ns));
     chk(kmig_sync(ctx));
     auto t_end = std::chrono::steady_clock::now();
-    const int ng = p.cig ? p.ng : 1;
     std::vector<float> out((size_t)ng * nz * nx);
     chk(kmig_get_cig(ctx, out.data()));
     kmig_stats st;

@@ -1115,6 +1115,9 @@ int kmig_run(kmig_ctx *c, int sb, int se)
     if (!c->side[0].set || !c->side[1].set)
         return fail(c, KMIG_E_STATE, "source and receiver tables must be set before kmig_run");
     if (!c->data_set) return fail(c, KMIG_E_STATE, "data must be set before kmig_run");
+    if (c->p.adj != 1)
+        return fail(c, KMIG_E_PARAM, "kmig_run migrates (adj=1); an adj=0 context models with "
+                                     "kmig_model");
     const kmig_params &p = c->p;
     const bool angle = p.cig && p.cig_is_angle;
     const bool gfiles = c->side[0].GX != nullptr;
@@ -1281,6 +1284,105 @@ int kmig_get_stats(const kmig_ctx *c, kmig_stats *st)
 {
     if (!c || !st) return KMIG_E_PARAM;
     *st = c->st;
+    return KMIG_OK;
+}
+
+int kmig_model(kmig_ctx *c, const float *image, int nt, float t0, float dt, int nh, float h0,
+               float dh, int ns, float s0, float ds, float *data)
+{
+    if (!c || !image || !data) return KMIG_E_PARAM;
+    if (!c->stream) return fail(c, KMIG_E_STATE, "context has no device");
+    const kmig_params &p = c->p;
+    if (p.adj != 0) return fail(c, KMIG_E_PARAM, "kmig_model needs adj=0");
+    if (!c->side[0].set || !c->side[1].set)
+        return fail(c, KMIG_E_STATE, "source and receiver tables must be set before kmig_model");
+    if (nt < 2 || nh <= 0 || ns <= 0)
+        return fail(c, KMIG_E_DIM, "adj=0 needs nt>1, nh>0 and ns>0 for the data volume");
+    char err[512] = {0};
+    int rc = kmig_params_check(&p, ns, err, sizeof(err));
+    if (rc) return fail(c, rc, "%s", err);
+    const bool angle = p.cig && p.cig_is_angle;
+    if (angle && (c->side[0].GX != nullptr) != (c->side[1].GX != nullptr))
+        return fail(c, KMIG_E_PARAM,
+                    "For mode=angle, please provide all of sgradx=, sgradz=, rgradx=, and "
+                    "rgradz=, or provide none of them.");
+    // the data axes of the adj=0 keys (src/kirchmig2d_cig.cpp:663-701) and the
+    // trace buffers: raw takes the spray, prep the doubly integrated traces
+    const size_t n = (size_t)nt * nh * ns;
+    size_t cap = c->raw_cap;
+    if ((rc = ensure(c, &c->raw, &c->raw_cap, n))) return rc;
+    if ((rc = ensure(c, &c->prep, &cap, n))) return rc;
+    c->nt = nt;
+    c->t0 = t0;
+    c->dt = dt;
+    c->nh = nh;
+    c->h0 = h0;
+    c->dh = dh;
+    c->ns = ns;
+    c->s0 = s0;
+    c->ds = ds;
+    c->data_set = false;  // no input traces: kmig_run stays refused
+    c->geo_version++;
+    if ((rc = plan_if_needed(c, 0, ns))) return rc;
+    if (c->variant_used != KMIG_VARIANT_RESIDENT || p.precision != KMIG_PREC_FP32)
+        return fail(c, KMIG_E_PARAM,
+                    "adj=0 on the GPU needs mode=angle, angle_interp=1, antialias >= 0, "
+                    "receivers that do not move with the shot (cmp=0), tuples that fit one "
+                    "batch, and precision=fp32");
+    if ((rc = ensure_outputs(c, 0, ns))) return rc;
+    const int ng = p.ng;
+    HIPCHK(c, hipMemcpy(c->cig, image, (size_t)ng * c->nzx * sizeof(float),
+                        hipMemcpyHostToDevice));
+    HIPCHK(c, (hipError_t)launch_fill(c->raw, n, 0.0f, c->stream));
+    c->ev_used = 0;
+    c->ev_kind.clear();
+    hipEvent_t e0 = next_event(c);
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    size_t spec_s_off = 0;
+    int tile = p.tile == 11 || p.tile == 15 ? p.tile : 9;
+    if (migrate_lds_bytes(ng, tile) > 64 * 1024) tile = 15;
+    if (migrate_lds_bytes(ng, tile) > 160 * 1024)
+        return fail(c, KMIG_E_PARAM, "output axis of %d samples does not fit in LDS", ng);
+    for (const Batch &b : c->batches) {
+        const int nsb = b.s1 - b.s0;
+        if ((rc = enqueue_tuple_prep(c, c->stream, b, spec_s_off))) return rc;
+        if (nsb > 0) {
+            MigResArgs R = res_args(c, b.s0, nsb, ng);
+            R.spray = c->raw;
+            HIPCHK(c, (hipError_t)launch_model_resident(R, tile, c->stream));
+        }
+        spec_s_off += (size_t)nsb;
+    }
+    // doubint of every trace (:1341-1349): the trace kernels with no mute,
+    // no diff and no tap pairs
+    TracePrepArgs ta{};
+    ta.raw = c->raw;
+    ta.prep = c->prep;
+    ta.meta = c->meta;
+    ta.nt = nt;
+    ta.nh = nh;
+    ta.shot_begin = 0;
+    ta.shot_end = ns;
+    ta.t0 = t0;
+    ta.dt = dt;
+    ta.qc_shot = -1;
+    HIPCHK(c, (hipError_t)launch_trace_prep2(ta, nullptr, nullptr, c->stream));
+    hipEvent_t e1 = next_event(c);
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipMemcpyAsync(data, c->prep, n * sizeof(float), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    c->st = kmig_stats{};
+    c->st.updates = (double)ns * (double)nh * (double)c->nzx;
+    c->st.migrate_launches = (int)c->batches.size();
+    c->st.shot_batches = (int)c->batches.size();
+    c->st.variant = c->variant_used;
+    c->st.tile = tile;
+    c->st.ms_total = ms;
+    c->st.ms_migrate = ms;
+    c->ev_used = 0;
     return KMIG_OK;
 }
 

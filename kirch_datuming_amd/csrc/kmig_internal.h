@@ -168,6 +168,7 @@ struct MigResArgs {
     const ThBound *rbnd;         // [ncz * ncx][nrslots]
     int ncz, ncx;                // cells of BND_CZ x BND_CX points
     int sbnd_stride, nrslots;    // shots per sbnd row, receiver slots
+    float *spray;                // adj=0: [ns][nh][nt] traces the image is sprayed into
     float *cig;
     int c_int;                   // integer part of (tau - t0)/dt
     float c_frac;                // fractional part of (tau - t0)/dt
@@ -244,6 +245,9 @@ int launch_pairs(const float *prep, float *pairs, long ntraces, int nt, void *st
 int launch_stitch(const StitchArgs &a, float *out, unsigned *nonfinite, void *stream);
 // tile: preset 9..21 (kmig_migrate.hip); precision: KMIG_PREC_* tuple format
 int launch_migrate_resident(const MigResArgs &a, int tile, int precision, void *stream);
+// adj=0: sprays the image held in a.cig into a.spray (FP32 tuples; presets
+// 9, 11 and 15)
+int launch_model_resident(const MigResArgs &a, int tile, void *stream);
 int launch_migrate_multi(const MigMultiArgs &m, int tile, int precision, void *stream);
 int migrate_tiles(int nz, int nx, int tile);
 // the same kernels under a distinct symbol, for the autotuner's timing launches

@@ -352,7 +352,10 @@ __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int8, flo
 // Probe = true only renames the kernel for the autotuner's timing launches,
 // so profiler statistics of the production kernel cover production launches.
 // One tile of TZ x TX image points of one problem.
-template <int TZ, int TX, int R, int G, int Fmt>
+// Model = true is the adj=0 transpose: the LDS column holds the image (read
+// only) and every accepted update sprays its six weighted samples into the
+// traces of a.spray instead of reading its taps.
+template <int TZ, int TX, int R, int G, int Fmt, bool Model = false>
 __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, float *acc)
 {
     constexpr int P = TZ * TX;
@@ -540,6 +543,30 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                     any |= q[k].ok;
                 }
                 if (__ballot(any) == 0) continue;
+                if constexpr (Model) {
+                    // kirmig_pick(adj=0), SERECKIRCH/src/kirmig.cpp:59-67: the two
+                    // bins' image values, weighted, sprayed with the transposed
+                    // three-tap weights (offsets are in 8-byte pair units: >> 3
+                    // gives samples, and ok keeps every tap in [0, nt - 1])
+#pragma unroll
+                    for (int k = 0; k < G; k++) {
+                        if (q[k].ok & valid) {
+                            const float *cell = my + q[k].ig0 * P;
+                            const float v =
+                                q[k].coef * ((1.0f - q[k].w1) * cell[0] + q[k].w1 * cell[P]);
+                            float *tr = a.spray + ((size_t)is * a.nh + r0 + g0 + k) * a.nt;
+                            const int it = (int)(q[k].o0 >> 3), itm = (int)(q[k].om >> 3),
+                                      itp = (int)(q[k].op >> 3);
+                            atomicAdd(tr + it, 2.0f * (1.0f - q[k].ft) * v);
+                            atomicAdd(tr + it + 1, 2.0f * q[k].ft * v);
+                            atomicAdd(tr + itm, -(1.0f - q[k].ftm) * v);
+                            atomicAdd(tr + itm + 1, -q[k].ftm * v);
+                            atomicAdd(tr + itp, -(1.0f - q[k].ftp) * v);
+                            atomicAdd(tr + itp + 1, -q[k].ftp * v);
+                        }
+                    }
+                    continue;
+                }
                 // tap offsets are 32-bit byte offsets from the wave-uniform
                 // trace row (indices are in [0, nt - 1])
                 float2 v0[G], vm[G], vp[G];
@@ -573,7 +600,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             }
         }
     }
-    if (valid)
+    if (!Model && valid)
         for (int g = 0; g < a.ng; g++) a.cig[(size_t)g * a.nzx + i] = my[g * P];
 }
 
@@ -582,6 +609,13 @@ __global__ __launch_bounds__(TZ *TX, 3) void k_migrate_ph(const MigResArgs a)
 {
     extern __shared__ float acc[];
     migrate_tile<TZ, TX, R, G, Fmt>(a, xcd_tile(blockIdx.x, gridDim.x), acc);
+}
+
+template <int TZ, int TX, int R, int G>
+__global__ __launch_bounds__(TZ *TX, 3) void k_model_ph(const MigResArgs a)
+{
+    extern __shared__ float acc[];
+    migrate_tile<TZ, TX, R, G, 0, true>(a, xcd_tile(blockIdx.x, gridDim.x), acc);
 }
 
 // Several independent problems (e.g. the depth layers of a recursive
@@ -693,6 +727,32 @@ static int launch_res_prec(const MigResArgs &a, int tile, int precision, hipStre
 int launch_migrate_resident(const MigResArgs &a, int tile, int precision, void *stream)
 {
     return launch_res_prec<false>(a, tile, precision, (hipStream_t)stream);
+}
+
+template <int TZ, int TX, int R, int G>
+static int launch_model_t(const MigResArgs &a, hipStream_t st)
+{
+    const int ntz = (a.nz + TZ - 1) / TZ, ntx = (a.nx + TX - 1) / TX;
+    const size_t lds = (size_t)(a.ng + 1) * TZ * TX * sizeof(float);
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void *)k_model_ph<TZ, TX, R, G>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)lds);
+        if (e != hipSuccess) return (int)e;
+    }
+    hipLaunchKernelGGL((k_model_ph<TZ, TX, R, G>), dim3((unsigned)(ntz * ntx)), dim3(TZ * TX), lds,
+                       st, a);
+    return (int)hipGetLastError();
+}
+
+int launch_model_resident(const MigResArgs &a, int tile, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    switch (tile) {
+    case 11: return launch_model_t<16, 8, 8, 8>(a, st);
+    case 15: return launch_model_t<8, 8, 8, 4>(a, st);
+    default: return launch_model_t<16, 8, 8, 4>(a, st);  // preset 9
+    }
 }
 
 int launch_migrate_resident_probe(const MigResArgs &a, int tile, int precision, void *stream)

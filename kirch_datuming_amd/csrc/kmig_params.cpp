@@ -281,8 +281,13 @@ extern "C" int kmig_params_check(const kmig_params *p, int ns, char *err, int er
 {
     using kmig::set_err;
     if (!p) return KMIG_E_PARAM;
-    if (p->adj != 1) {
-        set_err(err, errlen, "adj=%d is not provided: this library implements adj=1.", p->adj);
+    if (p->adj != 0 && p->adj != 1) {
+        set_err(err, errlen, "adj=%d: use adj=1 (migration) or adj=0 (modelling).", p->adj);
+        return KMIG_E_PARAM;
+    }
+    if (p->adj == 0 && p->direct_mute) { /* src/kirchmig2d_cig.cpp:719-720 */
+        set_err(err, errlen,
+                "direct_mute=1 is an input-data preprocessing option and requires adj=1.");
         return KMIG_E_PARAM;
     }
     if (p->interp_type != 'l' && p->interp_type != 'p' && p->interp_type != 'h') {

@@ -31,7 +31,8 @@ EXPORTS = ("kmig_params_default", "kmig_params_parse", "kmig_params_check", "kmi
            "kmig_sync", "kmig_get_cig", "kmig_get_qc", "kmig_get_stats", "kmig_migrate",
            "kmig_stream", "kmig_version", "kmig_batch_create", "kmig_batch_run",
            "kmig_batch_sync", "kmig_batch_stats", "kmig_batch_last_error", "kmig_batch_destroy",
-           "kmig_set_surrogate", "kmig_get_surrogate_stats", "kmig_batch_stitch")
+           "kmig_set_surrogate", "kmig_get_surrogate_stats", "kmig_batch_stitch",
+           "kmig_model")
 
 
 class KmigError(RuntimeError):
@@ -97,6 +98,8 @@ def lib():
         L.kmig_set_data.argtypes = [vp, fp, C.c_int, C.c_float, C.c_float, C.c_int, C.c_float,
                                     C.c_float, C.c_int, C.c_float, C.c_float]
         L.kmig_run.argtypes = [vp, C.c_int, C.c_int]
+        L.kmig_model.argtypes = [vp, fp, C.c_int, C.c_float, C.c_float, C.c_int, C.c_float,
+                                 C.c_float, C.c_int, C.c_float, C.c_float, fp]
         L.kmig_sync.argtypes = [vp]
         L.kmig_get_cig.argtypes = [vp, fp]
         L.kmig_get_qc.argtypes = [vp, fp, fp]
@@ -196,6 +199,17 @@ class Migrator:
     def run(self, shots=None):
         sb, se = (0, self.ns) if shots is None else shots
         self._chk(self.L.kmig_run(self.ctx, int(sb), int(se)))
+
+    def model(self, image, axes):
+        """adj=0: the image (ng, nx, nz) modelled into data (ns, nh, nt) on the
+        data axes [[nt, t0, dt], [nh, h0, dh], [ns, s0, ds]] (synchronous)."""
+        img = _f32(image)
+        (nt, t0, dt), (nh, h0, dh), (ns, s0, ds) = axes[:3]
+        out = np.empty((int(ns), int(nh), int(nt)), np.float32)
+        self._chk(self.L.kmig_model(self.ctx, _fp(img), int(nt), t0, dt, int(nh), h0, dh,
+                                    int(ns), s0, ds, _fp(out)))
+        self.nt, self.nh, self.ns = int(nt), int(nh), int(ns)
+        return out
 
     def sync(self):
         self._chk(self.L.kmig_sync(self.ctx))
@@ -339,6 +353,19 @@ def extract_adcig(adcig, ix=None, x=None, o2=0.0, d2=1.0):
     if not 0 <= ix < nx:
         raise KmigError(-1, f"ix must be in [0,{nx - 1}]")
     return np.ascontiguousarray(adcig[:, ix, :])
+
+
+def model(kv, image, data_axes, stab, sder, s_axes, rtab, rder, r_axes, grads=None):
+    """Full adj=0 run on the GPU: the image (ng, nx, nz) modelled into data
+    (ns, nh, nt) on data_axes."""
+    kv = dict(kv, adj=0)
+    p = parse_params(kv, int(data_axes[1][0]), float(data_axes[1][1]), float(data_axes[1][2]))
+    check_params(p, int(data_axes[2][0]))
+    with Migrator(p) as m:
+        g = grads or [None] * 4
+        m.set_tables(0, stab, sder, s_axes, g[0], g[1])
+        m.set_tables(1, rtab, rder, r_axes, g[2], g[3])
+        return m.model(image, data_axes)
 
 
 def migrate(kv, data, data_axes, stab, sder, s_axes, rtab, rder, r_axes, grads=None,

@@ -10,7 +10,8 @@
  *   src/kirchmig2d_cig.cpp:294-322  gradient-volume linear interpolation
  *   src/kirchmig2d_cig.cpp:381-394  diff2
  *   src/kirchmig2d_cig.cpp:979-1339 shot / receiver / image-point loops (adj=1)
- *   SERECKIRCH/src/kirmig.cpp:8-58  doubint, kirmig_pick (adj=1)
+ *   src/kirchmig2d_cig.cpp:979-1351 the same loops with adj=0 (modelling)
+ *   SERECKIRCH/src/kirmig.cpp:8-67  doubint, kirmig_pick (adj=1 and adj=0)
  *   SERECKIRCH/src/tinterpsr.cpp:37-188  linear / partial / Hermite
  *
  * How each part is pinned (tests/test_oracle_*.py):
@@ -95,6 +96,31 @@ int kmo_migrate_f64(const kmo_params *p, const kmo_geom *g, const float *data,
 {
     return migrate_impl_f64(p, g, data, stab, sder, rtab, rder, sgx, sgz, rgx, rgz,
                             shot_begin, shot_end, out, qc_before, qc_after);
+}
+
+int kmo_model_f32(const kmo_params *p, const kmo_geom *g, const float *image,
+                  const float *stab, const float *sder, const float *rtab, const float *rder,
+                  const float *sgx, const float *sgz, const float *rgx, const float *rgz,
+                  int shot_begin, int shot_end, float *data)
+{
+    return model_impl_f32(p, g, image, stab, sder, rtab, rder, sgx, sgz, rgx, rgz, shot_begin,
+                          shot_end, data);
+}
+
+int kmo_model_f64(const kmo_params *p, const kmo_geom *g, const float *image,
+                  const float *stab, const float *sder, const float *rtab, const float *rder,
+                  const float *sgx, const float *sgz, const float *rgx, const float *rgz,
+                  int shot_begin, int shot_end, double *data)
+{
+    return model_impl_f64(p, g, image, stab, sder, rtab, rder, sgx, sgz, rgx, rgz, shot_begin,
+                          shot_end, data);
+}
+
+/* the transposed pick: sample sprayed into trace (added) */
+void kmo_pick0_f64(int nt, double t0, double dt, double ti, double deltat, double sample,
+                   double *trace)
+{
+    pick0_f64(nt, t0, dt, ti, deltat, sample, trace);
 }
 
 void kmo_doubint_f32(int nt, float *trace) { doubint__f32(nt, trace); }

@@ -27,7 +27,7 @@ extern "C" {
 /* Parameters of one migration run, already parsed from key=value form.
    Field names follow the reference's variables in src/kirchmig2d_cig.cpp. */
 typedef struct kmo_params {
-    int adj;                  /* only 1 (migration) is restated */
+    int adj;                  /* 1 migration (kmo_migrate), 0 modelling (kmo_model) */
     int cig;                  /* 0: stacked image, 1: CIG */
     int cig_is_angle;         /* mode=angle */
     int cmp;                  /* 1: data axis 2 is offset, 0: absolute receiver */
@@ -78,6 +78,20 @@ int kmo_migrate_f64(const kmo_params *p, const kmo_geom *g, const float *data,
                     const float *rgx, const float *rgz,
                     int shot_begin, int shot_end,
                     double *out, float *qc_before, float *qc_after);
+
+/* adj=0 modelling: image [ng][nx][nz] (float, as read from the reference's
+   migration= file) to data [ns][nh][nt] (adj=0 outputs every trace; traces
+   outside the trace aperture or the offset axis are zeros). */
+void kmo_pick0_f64(int nt, double t0, double dt, double ti, double deltat, double sample,
+                   double *trace);
+int kmo_model_f32(const kmo_params *p, const kmo_geom *g, const float *image,
+                  const float *stab, const float *sder, const float *rtab, const float *rder,
+                  const float *sgx, const float *sgz, const float *rgx, const float *rgz,
+                  int shot_begin, int shot_end, float *data);
+int kmo_model_f64(const kmo_params *p, const kmo_geom *g, const float *image,
+                  const float *stab, const float *sder, const float *rtab, const float *rder,
+                  const float *sgx, const float *sgz, const float *rgx, const float *rgz,
+                  int shot_begin, int shot_end, double *data);
 
 /* Building blocks, exported so that tests can pin each one separately. */
 void kmo_doubint_f32(int nt, float *trace);

@@ -243,6 +243,32 @@ static void F(pick)(int nt, R t0, R dt, R ti, R deltat, R *sample, const R *trac
                         (1. - ftp) * trace[itp] - ftp * trace[itp + 1]);
 }
 
+/* kirmig_pick(adj=0), SERECKIRCH/src/kirmig.cpp:34-53, :59-67: the transpose
+   of the pick above.  Each trace update is formed in double (the reference's
+   2. and 1. literals) and rounded once into the float trace sample. */
+static void F(pick0)(int nt, R t0, R dt, R ti, R deltat, R sample, R *trace)
+{
+    int it, itm, itp;
+    R ft, tm, tp, ftm, ftp, imp, s;
+    ft = (ti - t0) / dt; it = (int)R_FLOOR(ft); ft -= (R)it;
+    if (it < 0 || it >= nt - 1) return;
+    tm = ti - deltat - dt;
+    ftm = (tm - t0) / dt; itm = (int)R_FLOOR(ftm); ftm -= (R)itm;
+    if (itm < 0) return;
+    tp = ti + deltat + dt;
+    ftp = (tp - t0) / dt; itp = (int)R_FLOOR(ftp); ftp -= (R)itp;
+    if (itp >= nt - 1) return;
+    imp = dt / (dt + tp - tm);
+    imp *= imp;
+    s = imp * sample;
+    trace[it] += 2. * (1. - ft) * s;
+    trace[it + 1] += 2. * ft * s;
+    trace[itm] -= (1. - ftm) * s;
+    trace[itm + 1] -= ftm * s;
+    trace[itp] -= (1. - ftp) * s;
+    trace[itp + 1] -= ftp * s;
+}
+
 /* tinterp_linear / dinterp_linear, SERECKIRCH/src/tinterpsr.cpp:37-72 */
 static void F(lin)(size_t nn, float dx_, float x, const float *p0, const float *p1, R *T, R *Tx)
 {
@@ -556,6 +582,161 @@ static int F(migrate_impl)(const kmo_params *p, const kmo_geom *g_in, const floa
     free(stable); free(stablex); free(rtable); free(rtablex);
     free(gsx); free(gsz); free(grx); free(grz); free(tmp1); free(tmp2);
     free(trace); free(ttmp);
+    return 0;
+}
+
+/* adj=0 modelling, src/kirchmig2d_cig.cpp:979-1351 with adj = 0: for every
+   trace the image values are sprayed with the transposed pick and the
+   weights, bins and apertures of the migration, then the trace is doubly
+   integrated (:1341-1350) and written; traces outside the trace aperture or
+   the offset axis are written as zeros (:1106-1139).  No mute, no diff (the
+   reference applies both to input traces only).  The point loop is
+   sequential, the order of the reference's own run on one thread (it sums
+   per-thread trace buffers in thread order, :1342-1347). */
+static int F(model_impl)(const kmo_params *p, const kmo_geom *g_in, const float *image,
+                         const float *stab, const float *sder, const float *rtab,
+                         const float *rder, const float *sgxv, const float *sgzv,
+                         const float *rgxv, const float *rgzv, int shot_begin, int shot_end,
+                         R *data)
+{
+    kmo_geom g = *g_in;
+    if (g.nh == 1) g.dh = 0;
+    if (g.ns == 1) g.ds = 0;
+    const int nz = g.nz, nx = g.nx, nt = g.nt, nh = g.nh, ns = g.ns;
+    const size_t nzx = (size_t)nz * nx;
+    const int use_grad = (sgxv && sgzv && rgxv && rgzv);
+    const int angle_mode = p->cig && p->cig_is_angle;
+    const int ng = p->cig ? p->ng : 1;
+    const R cig0 = p->cig ? (R)p->g0 : 0, dcig = p->cig ? (R)p->dg : 1;
+    if ((sgxv || sgzv || rgxv || rgzv) && !use_grad) return -2;
+    if (nz <= 0 || nx <= 0 || nt < 2 || nh <= 0 || ns <= 0 || ng <= 0) return -3;
+    if (shot_begin < 0) shot_begin = 0;
+    if (shot_end > ns) shot_end = ns;
+
+    R *stable = (R *)malloc(sizeof(R) * nzx), *stablex = (R *)malloc(sizeof(R) * nzx);
+    R *rtable = (R *)malloc(sizeof(R) * nzx), *rtablex = (R *)malloc(sizeof(R) * nzx);
+    R *gsx = (R *)malloc(sizeof(R) * nzx), *gsz = (R *)malloc(sizeof(R) * nzx);
+    R *grx = (R *)malloc(sizeof(R) * nzx), *grz = (R *)malloc(sizeof(R) * nzx);
+    R *tmp1 = (R *)malloc(sizeof(R) * nzx), *tmp2 = (R *)malloc(sizeof(R) * nzx);
+    R *img = (R *)malloc(sizeof(R) * nzx * (size_t)ng);
+    for (size_t k = 0; k < nzx * (size_t)ng; k++) img[k] = (R)image[k];
+
+    const float h0 = g.h0, s0 = g.s0;
+    const float sy0 = g.sy0, sdy = g.sdy, ry0 = g.ry0, rdy = g.rdy;
+    const R t0 = g.t0, dt = g.dt, dh = g.dh, ds = g.ds;
+    const R dz = g.dz, x0 = g.x0, dx = g.dx;
+    const int sny = g.sny, rny = g.rny;
+    const R tau = p->tau, aal = p->antialias, aper = p->aperture;
+    const R ap_taper = p->aperture_taper, ap_rmin = p->aperture_rmin, ap_rt = p->aperture_rtaper;
+
+    for (int is = shot_begin; is < shot_end; is++) {
+        const float s = s0 + (float)is * g.ds;
+        int ist = F(interp_side)(p->interp_type, nz, nx, g.dx, sny, sy0, sdy, s, stab, sder,
+                                 stable, stablex);
+        if (angle_mode) {
+            if (use_grad) {
+                F(interp_volume_linear)(gsx, sgxv, sny, nzx, s, sy0, sdy);
+                F(interp_volume_linear)(gsz, sgzv, sny, nzx, s, sy0, sdy);
+            } else {
+                F(grad_plane)(stable, nz, nx, dz, dx, p->angle_grad_radius, gsx, gsz);
+            }
+            if (p->angle_smooth > 0) {
+                F(smooth)(gsx, nz, nx, p->angle_smooth, tmp1, tmp2);
+                F(smooth)(gsz, nz, nx, p->angle_smooth, tmp1, tmp2);
+            }
+        }
+        for (int ih = 0; ih < nh; ih++) {
+            const float h = h0 + (float)ih * g.dh;
+            const float receiver_pos = p->cmp ? (s + h) : h;
+            float rf = 0.0f;
+            if (rny > 1 && rdy != 0.0f) rf = (receiver_pos - ry0) / rdy;
+            const int iht = (rny > 1 && rdy != 0.0f) ? (int)floorf(rf) : 0;
+            R *trace = data + ((size_t)is * nh + ih) * (size_t)nt;
+            for (int it = 0; it < nt; it++) trace[it] = 0;
+            R trace_ap_weight = 1;
+            if (p->aperture_trace != -1) {
+                const R ad = (R)fabsf((float)(iht - ist));
+                if (ad >= (R)p->aperture_trace) continue;
+                if (p->aperture_trace_taper > 0) {
+                    const R taper_begin =
+                        MAX((R)0, (R)(p->aperture_trace - p->aperture_trace_taper));
+                    if (ad > taper_begin) {
+                        const R denom = (R)p->aperture_trace - taper_begin;
+                        if (denom > 0) {
+                            const R x = (ad - taper_begin) / denom;
+                            trace_ap_weight = LIT(0.5) * (LIT(1.0) + R_COS((R)M_PI * x));
+                        }
+                    }
+                }
+            }
+            int ig_offset = 0;
+            if (p->cig && !p->cig_is_angle) {
+                ig_offset = F(nearest_bin)((R)h, cig0, dcig, ng);
+                if (ig_offset < 0) continue;
+            }
+            F(interp_side)(p->interp_type, nz, nx, g.dx, rny, ry0, rdy, receiver_pos, rtab, rder,
+                           rtable, rtablex);
+            if (angle_mode) {
+                if (use_grad) {
+                    F(interp_volume_linear)(grx, rgxv, rny, nzx, receiver_pos, ry0, rdy);
+                    F(interp_volume_linear)(grz, rgzv, rny, nzx, receiver_pos, ry0, rdy);
+                } else {
+                    F(grad_plane)(rtable, nz, nx, dz, dx, p->angle_grad_radius, grx, grz);
+                }
+                if (p->angle_smooth > 0) {
+                    F(smooth)(grx, nz, nx, p->angle_smooth, tmp1, tmp2);
+                    F(smooth)(grz, nz, nx, p->angle_smooth, tmp1, tmp2);
+                }
+            }
+            const R rec_surface_x = (R)(p->cmp ? (s + h) : h);
+            for (size_t i = 0; i < nzx; i++) {
+                const int iz = (int)(i % nz);
+                const int ix = (int)((i - iz) / nz);
+                const R zdist = MAX(R_FABS((R)iz * dz), MAX(LIT(1.0e-12), LIT(1.0e-6) * R_FABS(dz)));
+                const R ximg = x0 + (R)ix * dx;
+                const R dsx = R_FABS(s - ximg);
+                const R drx = R_FABS(rec_surface_x - ximg);
+                const R as_deg = R_ATAN(dsx / zdist) * LIT(180.0) / (R)M_PI;
+                const R ar_deg = R_ATAN(drx / zdist) * LIT(180.0) / (R)M_PI;
+                const R ws_leg = MAX(F(leg_weight)(as_deg, aper, ap_taper),
+                                     F(floor_weight)(dsx, ap_rmin, ap_rt));
+                const R wr_leg = MAX(F(leg_weight)(ar_deg, aper, ap_taper),
+                                     F(floor_weight)(drx, ap_rmin, ap_rt));
+                const R apw = ws_leg * wr_leg;
+                if (apw <= 0) continue;
+                const R ti = stable[i] + rtable[i] + tau;
+                const R tx = MAX(R_FABS(stablex[i] * ds), R_FABS(rtablex[i] * dh));
+                if (angle_mode) {
+                    const R angle = F(angle)(gsx[i], gsz[i], grx[i], grz[i], p->half_angle);
+                    const R aw = F(angle_taper)(angle, p->angle_taper_start, p->angle_taper_end) *
+                                 F(depth_mute)(angle, iz, nz, p->oazmin, p->oazmax, p->oaztaper) *
+                                 trace_ap_weight * apw;
+                    if (aw <= 0) continue;
+                    if (p->angle_interp) {
+                        int ig0 = -1, ig1 = -1;
+                        R w0 = 0, w1 = 0;
+                        if (!F(linear_bins)(angle, cig0, dcig, ng, &ig0, &ig1, &w0, &w1)) continue;
+                        F(pick0)(nt, t0, dt, ti, tx * aal, aw * w0 * img[(size_t)ig0 * nzx + i],
+                                 trace);
+                        if (ig1 != ig0 && w1 > 0)
+                            F(pick0)(nt, t0, dt, ti, tx * aal,
+                                     aw * w1 * img[(size_t)ig1 * nzx + i], trace);
+                    } else {
+                        const int ig = F(nearest_bin)(angle, cig0, dcig, ng);
+                        if (ig < 0) continue;
+                        F(pick0)(nt, t0, dt, ti, tx * aal, aw * img[(size_t)ig * nzx + i], trace);
+                    }
+                } else {
+                    const int ig = p->cig ? ig_offset : 0;
+                    F(pick0)(nt, t0, dt, ti, tx * aal,
+                             trace_ap_weight * apw * img[(size_t)ig * nzx + i], trace);
+                }
+            }
+            F(doubint_)(nt, trace);
+        }
+    }
+    free(stable); free(stablex); free(rtable); free(rtablex);
+    free(gsx); free(gsz); free(grx); free(grz); free(tmp1); free(tmp2); free(img);
     return 0;
 }
 

@@ -66,8 +66,16 @@ def lib():
             f.restype = C.c_int
             f.argtypes = [C.POINTER(KmoParams), C.POINTER(KmoGeom), fp, fp, fp, fp, fp,
                           fp, fp, fp, fp, C.c_int, C.c_int, outp, fp, fp]
+        for nm, outp in (("kmo_model_f32", fp), ("kmo_model_f64", dp)):
+            f = getattr(L, nm)
+            f.restype = C.c_int
+            f.argtypes = [C.POINTER(KmoParams), C.POINTER(KmoGeom), fp, fp, fp, fp, fp,
+                          fp, fp, fp, fp, C.c_int, C.c_int, outp]
         L.kmo_pick_f32.restype = C.c_float
         L.kmo_pick_f32.argtypes = [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp]
+        L.kmo_pick0_f64.restype = None
+        L.kmo_pick0_f64.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, dp]
         L.kmo_pick_f64.restype = C.c_double
         L.kmo_pick_f64.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.kmo_doubint_f32.argtypes = [C.c_int, fp]
@@ -223,6 +231,27 @@ def migrate(kv, data, data_axes, stab, sder, s_axes, rtab, rder, r_axes, grads=N
     if rc != 0:
         raise RuntimeError(f"oracle returned {rc}")
     return out, qcb, qca
+
+
+def model(kv, image, data_axes, stab, sder, s_axes, rtab, rder, r_axes, grads=None,
+          shots=None, precision="f32"):
+    """adj=0 modelling of the image (ng, nx, nz) into data (ns, nh, nt), the
+    reference's run on one thread (its point order)."""
+    L = lib()
+    g = geom(data_axes, s_axes, r_axes)
+    p = params_from_kv(dict(kv, adj=0), g.nh, g.h0, g.dh)
+    arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (image, stab, sder, rtab, rder)]
+    gr = [np.ascontiguousarray(a, dtype=np.float32) for a in grads] if grads else [None] * 4
+    dtype = np.float32 if precision == "f32" else np.float64
+    out = np.zeros((g.ns, g.nh, g.nt), dtype=dtype)
+    sb, se = (0, g.ns) if shots is None else shots
+    fn = L.kmo_model_f32 if precision == "f32" else L.kmo_model_f64
+    outp = out.ctypes.data_as(C.POINTER(C.c_float if precision == "f32" else C.c_double))
+    rc = fn(C.byref(p), C.byref(g), *[_fp(a) for a in arrs], *[_fp(a) for a in gr], sb, se,
+            outp)
+    if rc != 0:
+        raise RuntimeError(f"oracle returned {rc}")
+    return out
 
 
 def rel_l2(a, b) -> float:

@@ -123,8 +123,16 @@ def test_cli_argument_errors(tmp_path):
     assert r.returncode == 1 and "provide all of sgradx=" in r.stderr
     r = _cli(tmp_path, *base, "sderiv=s.rsf", "direct_mute=1", "direct_mute_qc_shot=1")
     assert r.returncode == 1 and "requires direct_mute_qc_before=" in r.stderr
+    # adj=0 (modelling) takes its data axes from keys, with the reference's messages
     r = _cli(tmp_path, *base, "sderiv=s.rsf", "adj=0")
-    assert r.returncode == 1 and "adj=0" in r.stderr
+    assert r.returncode == 1 and "ERROR: Need nt=" in r.stderr
+    r = _cli(tmp_path, *base, "sderiv=s.rsf", "adj=0", "nt=100")
+    assert r.returncode == 1 and "ERROR: Need dt=" in r.stderr
+    r = _cli(tmp_path, *base, "sderiv=s.rsf", "adj=0", "nt=100", "dt=0.004", "dh=0.1",
+             "ds=0.1", "direct_mute=1", "direct_mute_slope=1")
+    assert r.returncode == 1 and "direct_mute=1 is an input-data preprocessing option" in r.stderr
+    r = _cli(tmp_path, *base, "sderiv=s.rsf", "adj=2")
+    assert r.returncode == 1 and "adj=2" in r.stderr
     r = _cli(tmp_path, *base, "sderiv=missing.rsf")
     assert r.returncode == 1 and "Cannot open" in r.stderr
 
