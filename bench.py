@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--overlap", type=int, default=0,
                     help="layers: extend layers 2..5 upward by this many depth samples and "
                          "stitch the gathers on the GPU (layercom_cig blend)")
+    ap.add_argument("--adj", type=int, default=1, choices=[0, 1],
+                    help="0: time adj=0 modelling of the workload's migrated image instead")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain device copy of this many MiB (profiler calibration)")
     args = ap.parse_args()
@@ -88,6 +90,8 @@ def main():
     log(f"building the {args.config} workload on {cores} host threads")
     w = configs.salt(threads=cores) if args.config == "salt" else configs.small(threads=cores)
     log(f"workload ready in {time.time() - t0:.1f} s: {w.name}, {w.updates:.4e} updates")
+    if args.adj == 0:
+        return bench_model(args, cores, w)
 
     kv = dict(w.kv, precision=args.precision, variant=args.variant)
     if args.tile:
@@ -213,6 +217,84 @@ def main():
         "cig_abs_max": float(np.abs(cig_gpu).max()),
     }
     m.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_model(args, cores, w):
+    """adj=0: one step models the workload's data from its migrated image (the
+    CIG of one migration of the workload's data); the steps run kmig_model,
+    which takes the image from host memory and returns the data there, so the
+    kernel time (spray + double integration, HIP events) gives the value and
+    the wall time the PCIe-inclusive rate."""
+    import numpy as np
+
+    from kirch_datuming_amd import kmig
+
+    kv = {k: v for k, v in w.kv.items() if not k.startswith(("direct_mute", "diff"))}
+    kv = dict(kv, variant=args.variant)
+    if args.tile:
+        kv["tile"] = args.tile
+    ax = w.data_axes
+    img = kmig.migrate(kv, w.data, ax, w.stab, w.sder, w.s_axes, w.rtab, w.rder, w.r_axes)[0]
+    log(f"image: the workload's CIG {img.shape}")
+    p = kmig.parse_params(dict(kv, adj=0), int(ax[1][0]), ax[1][1], ax[1][2])
+    m = kmig.Migrator(p)
+    m.set_tables(0, w.stab, w.sder, w.s_axes)
+    m.set_tables(1, w.rtab, w.rder, w.r_axes)
+    for i in range(args.warmup):
+        m.model(img, ax)
+        log(f"warmup {i + 1}/{args.warmup}: {m.stats()['ms_total']:.1f} ms")
+    ker_ms = []
+    tstart = time.perf_counter()
+    for _ in range(args.steps):
+        data = m.model(img, ax)
+        ker_ms.append(m.stats()["ms_total"])
+    wall = time.perf_counter() - tstart
+    st = m.stats()
+    m.close()
+    ker = float(np.mean(ker_ms))
+    value = w.updates / (ker * 1e-3)
+    log(f"timed {args.steps} steps: kernels {ker:.1f} ms, wall {wall / args.steps * 1e3:.1f} ms")
+
+    cpu = parity = None
+    if not args.no_cpu and args.cpu_shots > 0:
+        from oracle import oracle as O
+        ns = int(ax[2][0])
+        k = 1  # the oracle models on one thread: one shot is 1e8 .. 2e8 updates
+        shots = [ns // 2]
+        sub = w.shot_subset(shots)
+        sa = sub.args()
+        log(f"CPU oracle modelling (float) of shots {shots}")
+        tc = time.perf_counter()
+        d_cpu = O.model(kv, img, sa[1], *sa[2:], precision="f32")
+        tcpu = time.perf_counter() - tc
+        cpu = {"value": sub.updates / tcpu, "unit": "updates/s", "cores": 1, "kind": "port",
+               "sample": f"{k} shot ({sub.updates:.3e} updates), {tcpu:.1f} s, "
+                         "the reference's single-thread point order"}
+        d_gpu = kmig.model(kv, img, sa[1], *sa[2:])
+        log("FP64 oracle modelling of the same shots")
+        d64 = O.model(kv, img, sa[1], *sa[2:], precision="f64")
+        parity = {"sample": f"shots {shots}", "rel_l2_gpu_vs_f64": O.rel_l2(d_gpu, d64),
+                  "rel_l2_cpu_f32_vs_f64": O.rel_l2(d_cpu, d64)}
+        log(f"relL2 GPU vs f64 {parity['rel_l2_gpu_vs_f64']:.2e}, CPU f32 vs f64 "
+            f"{parity['rel_l2_cpu_f32_vs_f64']:.2e}")
+    (nt, _, _), (nh, _, _), (ns, _, _) = ax
+    kern = "k_model_ph" if st["variant"] == 2 else "k_migrate<Model>"
+    out = {
+        "metric": "adj=0 modelling updates/s (image point x trace)",
+        "value": value, "unit": "updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ker, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": w.name + ", adj=0 from its migrated CIG", "config": args.config,
+                   "ns": int(ns), "nh": int(nh), "nt": int(nt), "ng": int(p.ng),
+                   "updates_per_step": w.updates, "kernel": f"{kern} (tile {st['tile']})",
+                   "variant": {0: "auto", 1: "stream", 2: "resident"}[st["variant"]]},
+        "roofline": None,
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "pcie_inclusive_updates_per_s": w.updates * args.steps / wall,
+        "data_abs_max": float(np.abs(data).max()),
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -169,9 +169,8 @@ int kmig_migrate(kmig_ctx *ctx, const float *data, int nt, float t0, float dt, i
    and the migration's weights, bins and apertures into every trace, each
    trace is doubly integrated, and data receives ns*nh*nt floats (zeros
    outside the trace aperture).  It is the adjoint of kmig_run without mute
-   and diff.  The context needs adj=0 and the resident variant's conditions
-   (mode=angle, angle_interp=1, antialias >= 0, cmp=0, precision=fp32).
-   Synchronous.  The reference's own adj=0 run ends in a segmentation fault
+   and diff, for every output mode (angle, offset, stack) and cmp; the
+   context needs adj=0 and precision=fp32.  Synchronous.  The reference's own adj=0 run ends in a segmentation fault
    after its axis message (the output header's ndim is never set, :875-920). */
 int kmig_model(kmig_ctx *ctx, const float *image, int nt, float t0, float dt, int nh,
                float h0, float dh, int ns, float s0, float ds, float *data);

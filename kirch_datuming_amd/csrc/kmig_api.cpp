@@ -1108,6 +1108,50 @@ int kmig_set_data(kmig_ctx *c, const float *data, int nt, float t0, float dt, in
     return KMIG_OK;
 }
 
+// Arguments of the stream migrate kernel for one shot batch.
+static MigArgs stream_args(const kmig_ctx *c, const Batch &b)
+{
+    const kmig_params &p = c->p;
+    const bool angle = p.cig && p.cig_is_angle;
+    MigArgs M{};
+    M.nz = c->nz;
+    M.nx = c->nx;
+    M.nzx = c->nzx;
+    M.nt = c->nt;
+    M.t0 = c->t0;
+    M.dt = c->dt;
+    M.inv_dt = 1.0f / c->dt;
+    M.nh = c->nh;
+    M.shot0 = b.s0;
+    M.nshots = b.s1 - b.s0;
+    M.sT = c->ts.T;
+    M.sA = c->ts.A;
+    M.sGX = c->ts.GX;
+    M.sGZ = c->ts.GZ;
+    M.sW = c->ts.W;
+    M.rT = c->tr.T;
+    M.rA = c->tr.A;
+    M.rGX = c->tr.GX;
+    M.rGZ = c->tr.GZ;
+    M.rW = c->tr.W;
+    M.meta = c->meta;
+    M.traces = c->prep;
+    M.cig = c->cig;
+    M.tau = p.tau;
+    M.aal = p.antialias;
+    M.mode = angle ? (p.angle_interp ? 0 : 1) : 2;
+    M.half_angle = p.half_angle;
+    M.g0 = p.cig ? p.g0 : 0.0f;
+    M.dg = p.cig ? p.dg : 1.0f;
+    M.oazmin = p.oazmin;
+    M.oazmax = p.oazmax;
+    M.oaztaper = p.oaztaper;
+    M.taper_start = p.angle_taper_start;
+    M.taper_end = p.angle_taper_end;
+    M.ng = p.cig ? p.ng : 1;
+    return M;
+}
+
 int kmig_run(kmig_ctx *c, int sb, int se)
 {
     if (!c) return KMIG_E_PARAM;
@@ -1154,42 +1198,7 @@ int kmig_run(kmig_ctx *c, int sb, int se)
         HIPCHK(c, hipEventRecord(a1, c->stream));
         c->ev_kind.push_back(1);
 
-        MigArgs M{};
-        M.nz = c->nz;
-        M.nx = c->nx;
-        M.nzx = c->nzx;
-        M.nt = c->nt;
-        M.t0 = c->t0;
-        M.dt = c->dt;
-        M.inv_dt = 1.0f / c->dt;
-        M.nh = c->nh;
-        M.shot0 = b.s0;
-        M.nshots = nsb;
-        M.sT = c->ts.T;
-        M.sA = c->ts.A;
-        M.sGX = c->ts.GX;
-        M.sGZ = c->ts.GZ;
-        M.sW = c->ts.W;
-        M.rT = c->tr.T;
-        M.rA = c->tr.A;
-        M.rGX = c->tr.GX;
-        M.rGZ = c->tr.GZ;
-        M.rW = c->tr.W;
-        M.meta = c->meta;
-        M.traces = c->prep;
-        M.cig = c->cig;
-        M.tau = p.tau;
-        M.aal = p.antialias;
-        M.mode = angle ? (p.angle_interp ? 0 : 1) : 2;
-        M.half_angle = p.half_angle;
-        M.g0 = p.cig ? p.g0 : 0.0f;
-        M.dg = p.cig ? p.dg : 1.0f;
-        M.ng = ng;
-        M.oazmin = p.oazmin;
-        M.oazmax = p.oazmax;
-        M.oaztaper = p.oaztaper;
-        M.taper_start = p.angle_taper_start;
-        M.taper_end = p.angle_taper_end;
+        const MigArgs M = stream_args(c, b);
         int tile = p.tile;
         if (tile <= 0) {
             tile = resident ? 9 : 2;  // 16 x 8 points
@@ -1324,13 +1333,11 @@ int kmig_model(kmig_ctx *c, const float *image, int nt, float t0, float dt, int 
     c->data_set = false;  // no input traces: kmig_run stays refused
     c->geo_version++;
     if ((rc = plan_if_needed(c, 0, ns))) return rc;
-    if (c->variant_used != KMIG_VARIANT_RESIDENT || p.precision != KMIG_PREC_FP32)
-        return fail(c, KMIG_E_PARAM,
-                    "adj=0 on the GPU needs mode=angle, angle_interp=1, antialias >= 0, "
-                    "receivers that do not move with the shot (cmp=0), tuples that fit one "
-                    "batch, and precision=fp32");
+    if (p.precision != KMIG_PREC_FP32)
+        return fail(c, KMIG_E_PARAM, "adj=0 on the GPU computes with precision=fp32 tables");
+    const bool resident = c->variant_used == KMIG_VARIANT_RESIDENT;
     if ((rc = ensure_outputs(c, 0, ns))) return rc;
-    const int ng = p.ng;
+    const int ng = p.cig ? p.ng : 1;
     HIPCHK(c, hipMemcpy(c->cig, image, (size_t)ng * c->nzx * sizeof(float),
                         hipMemcpyHostToDevice));
     HIPCHK(c, (hipError_t)launch_fill(c->raw, n, 0.0f, c->stream));
@@ -1339,17 +1346,22 @@ int kmig_model(kmig_ctx *c, const float *image, int nt, float t0, float dt, int 
     hipEvent_t e0 = next_event(c);
     HIPCHK(c, hipEventRecord(e0, c->stream));
     size_t spec_s_off = 0;
-    int tile = p.tile == 11 || p.tile == 15 ? p.tile : 9;
-    if (migrate_lds_bytes(ng, tile) > 64 * 1024) tile = 15;
+    // resident presets 9 (16 x 8), 11, 15 (8 x 8); stream tiles 2 (16 x 8), 1 (8 x 8)
+    int tile = resident ? (p.tile == 11 || p.tile == 15 ? p.tile : 9) : (p.tile == 1 ? 1 : 2);
+    if (migrate_lds_bytes(ng, tile) > 64 * 1024) tile = resident ? 15 : 1;
     if (migrate_lds_bytes(ng, tile) > 160 * 1024)
         return fail(c, KMIG_E_PARAM, "output axis of %d samples does not fit in LDS", ng);
     for (const Batch &b : c->batches) {
         const int nsb = b.s1 - b.s0;
         if ((rc = enqueue_tuple_prep(c, c->stream, b, spec_s_off))) return rc;
-        if (nsb > 0) {
+        if (nsb > 0 && resident) {
             MigResArgs R = res_args(c, b.s0, nsb, ng);
             R.spray = c->raw;
             HIPCHK(c, (hipError_t)launch_model_resident(R, tile, c->stream));
+        } else if (nsb > 0) {
+            MigArgs M = stream_args(c, b);
+            M.spray = c->raw;
+            HIPCHK(c, (hipError_t)launch_model(M, tile, c->stream));
         }
         spec_s_off += (size_t)nsb;
     }

@@ -94,6 +94,7 @@ struct MigArgs {
     const TraceMeta *meta;
     const float *traces;
     float *cig;
+    float *spray;        // adj=0: traces the image is sprayed into (cig is read only)
     float tau, aal;
     int mode;            // 0 angle/linear bins, 1 angle/nearest bin, 2 offset CIG or stack
     int half_angle;
@@ -234,6 +235,8 @@ int launch_tuple_fused16(const TuplePrepArgs &a, Tuples &o, PackedTuple16 *out,
                          SlotScale *scale, float dt, float th_scale, int precision,
                          void *stream);
 int launch_migrate(const MigArgs &a, int tile, void *stream);
+// adj=0 with the same kernel: a.cig is the image, a.spray the zeroed traces
+int launch_model(const MigArgs &a, int tile, void *stream);
 int migrate_lds_bytes(int ng, int tile);
 int launch_fill(float *p, size_t n, float v, void *stream);
 // th_scale: degrees per radian of the packed direction (0: no direction)

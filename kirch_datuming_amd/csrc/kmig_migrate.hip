@@ -51,9 +51,46 @@ __device__ __forceinline__ float pick(const float *__restrict__ tr, int nt, floa
     return imp * s;
 }
 
+// kirmig_pick(adj=0): the transpose of pick above, the same window checks,
+// the six weighted samples of v added into the trace.  Traces are summed by
+// many points at once, so the adds are atomic.
+__device__ __forceinline__ void spray(float *tr, int nt, float t0, float dt, float inv_dt,
+                                      float ti, float deltat, float v)
+{
+    const float lim = (float)(nt - 1);
+    float ft = (ti - t0) * inv_dt;
+    const float fl = floorf(ft);
+    if (!(fl >= 0.0f && fl < lim)) return;
+    const float tm = ti - deltat - dt;
+    float ftm = (tm - t0) * inv_dt;
+    const float flm = floorf(ftm);
+    if (!(flm >= 0.0f && flm < lim)) return;
+    const float tp = ti + deltat + dt;
+    float ftp = (tp - t0) * inv_dt;
+    const float flp = floorf(ftp);
+    if (!(flp >= 0.0f && flp < lim)) return;
+    ft -= fl;
+    ftm -= flm;
+    ftp -= flp;
+    const int it = (int)fl, itm = (int)flm, itp = (int)flp;
+    const float imp0 = dt / (dt + tp - tm);
+    const float s = imp0 * imp0 * v;
+    atomicAdd(tr + it, 2.0f * (1.0f - ft) * s);
+    atomicAdd(tr + it + 1, 2.0f * ft * s);
+    atomicAdd(tr + itm, -(1.0f - ftm) * s);
+    atomicAdd(tr + itm + 1, -ftm * s);
+    atomicAdd(tr + itp, -(1.0f - ftp) * s);
+    atomicAdd(tr + itp + 1, -ftp * s);
+}
+
 __device__ __forceinline__ float cos_ramp_up(float u) { return 0.5f * (1.0f + cosf((float)KPI * u)); }
 
-template <int TZ, int TX>
+// Model = true is adj=0: the LDS column holds the image (read only), and each
+// accepted update sprays its weighted image value into the trace of a.spray
+// instead of picking from it (src/kirchmig2d_cig.cpp:1300-1312, :1319-1323,
+// :1333-1337; the two bins of a linear update are one spray of their
+// weighted sum, the reference's two sprays added).
+template <int TZ, int TX, bool Model = false>
 __global__ __launch_bounds__(TZ *TX) void k_migrate(const MigArgs a)
 {
     constexpr int P = TZ * TX;
@@ -95,7 +132,13 @@ __global__ __launch_bounds__(TZ *TX) void k_migrate(const MigArgs a)
             const float rA = a.rA[ri];
             const float tx = sA > rA ? sA : rA;
             const float *__restrict__ tr = trow + (size_t)ih * a.nt;
+            float *str = Model ? a.spray + ((size_t)is * a.nh + ih) * a.nt : nullptr;
             if (a.mode == 2) {
+                if constexpr (Model) {
+                    spray(str, a.nt, a.t0, a.dt, a.inv_dt, ti, tx * a.aal,
+                          wap * apw * my[mrow[ih].ig * P]);
+                    continue;
+                }
                 const float picked = pick(tr, a.nt, a.t0, a.dt, a.inv_dt, ti, tx * a.aal);
                 my[mrow[ih].ig * P] += wap * apw * picked;
                 continue;
@@ -143,18 +186,28 @@ __global__ __launch_bounds__(TZ *TX) void k_migrate(const MigArgs a)
                     w1 = u - fu;
                     w0 = 1.0f - w1;
                 }
+                if constexpr (Model) {
+                    float m = w0 * my[ig0 * P];
+                    if (ig1 != ig0 && w1 > 0.0f) m += w1 * my[ig1 * P];
+                    spray(str, a.nt, a.t0, a.dt, a.inv_dt, ti, tx * a.aal, aw * m);
+                    continue;
+                }
                 const float picked = aw * pick(tr, a.nt, a.t0, a.dt, a.inv_dt, ti, tx * a.aal);
                 my[ig0 * P] += w0 * picked;
                 if (ig1 != ig0 && w1 > 0.0f) my[ig1 * P] += w1 * picked;
             } else {
                 const int ig = (int)floorf(u + 0.5f);
                 if (ig < 0 || ig >= a.ng) continue;
+                if constexpr (Model) {
+                    spray(str, a.nt, a.t0, a.dt, a.inv_dt, ti, tx * a.aal, aw * my[ig * P]);
+                    continue;
+                }
                 const float picked = pick(tr, a.nt, a.t0, a.dt, a.inv_dt, ti, tx * a.aal);
                 my[ig * P] += aw * picked;
             }
         }
     }
-    if (valid)
+    if (!Model && valid)
         for (int g = 0; g < a.ng; g++) a.cig[(size_t)g * a.nzx + i] = my[g * P];
 }
 
@@ -651,19 +704,19 @@ int migrate_lds_bytes(int ng, int tile)
     return (ng + 1) * tz * tx * (int)sizeof(float);
 }
 
-template <int TZ, int TX>
+template <int TZ, int TX, bool Model = false>
 static int launch_mig_t(const MigArgs &a, hipStream_t st)
 {
     const int ntz = (a.nz + TZ - 1) / TZ, ntx = (a.nx + TX - 1) / TX;
     const size_t lds = (size_t)a.ng * TZ * TX * sizeof(float);
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate<TZ, TX>,
+        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate<TZ, TX, Model>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL((k_migrate<TZ, TX>), dim3((unsigned)(ntz * ntx)), dim3(TZ * TX), lds, st,
-                       a);
+    hipLaunchKernelGGL((k_migrate<TZ, TX, Model>), dim3((unsigned)(ntz * ntx)), dim3(TZ * TX),
+                       lds, st, a);
     return (int)hipGetLastError();
 }
 
@@ -816,6 +869,15 @@ int launch_migrate(const MigArgs &a, int tile, void *stream)
     case 3: return launch_mig_t<16, 16>(a, st);
     case 4: return launch_mig_t<32, 4>(a, st);
     default: return launch_mig_t<16, 8>(a, st);
+    }
+}
+
+int launch_model(const MigArgs &a, int tile, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    switch (tile) {
+    case 1: return launch_mig_t<8, 8, true>(a, st);
+    default: return launch_mig_t<16, 8, true>(a, st);
     }
 }
 

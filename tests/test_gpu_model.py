@@ -22,11 +22,13 @@ TOL = 3e-4
 REPEAT = 2e-4
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "kirch_datuming_amd", "bin", "kirchmig2d_cig")
-# the golden cases the resident kernel takes (angle gathers, two-bin
-# interpolation, receivers fixed in space); type_p is left out because the
-# reference reads outside its table slice at ix < 2 there
-CASES = ["angle_base", "mute_qc", "aperture_trace", "type_l", "half", "gradfiles",
-         "salt_like", "default_axis"]
+# every golden geometry but type_p, where the reference reads outside its
+# table slice at ix < 2; the first group also runs on the resident kernel
+# (angle gathers, two-bin interpolation, receivers fixed in space), the
+# second (offset gathers, stacks, cmp=1, nearest bins) only on the stream one
+RESIDENT = ["angle_base", "mute_qc", "aperture_trace", "type_l", "half", "gradfiles",
+            "salt_like", "default_axis"]
+STREAM_ONLY = ["angle_cmp1", "mute_invert", "nearest", "hard_cone", "offset_cig", "stack"]
 
 
 def _kmig():
@@ -48,11 +50,12 @@ def _inputs(name, seed):
     return kv, ax, m, d, tabs, run["grads"]
 
 
-@pytest.mark.parametrize("name", CASES)
-def test_model_against_fp64_oracle(name):
+@pytest.mark.parametrize("name,variant", [(n, v) for n in RESIDENT for v in ("auto", "stream")]
+                         + [(n, "auto") for n in STREAM_ONLY])
+def test_model_against_fp64_oracle(name, variant):
     kv, ax, m, _, tabs, grads = _inputs(name, 5)
-    print(f"model case {name}", flush=True)
-    got = _kmig().model(kv, m, ax, *tabs, grads=grads)
+    print(f"model case {name} variant {variant}", flush=True)
+    got = _kmig().model(dict(kv, variant=variant), m, ax, *tabs, grads=grads)
     ref = O.model(kv, m, ax, *tabs, grads=grads, precision="f64")
     assert got.shape == ref.shape
     r = O.rel_l2(got, ref)
@@ -60,14 +63,17 @@ def test_model_against_fp64_oracle(name):
     assert np.abs(ref).max() > 0 and r < TOL
 
 
-@pytest.mark.parametrize("name", ["angle_base", "salt_like", "gradfiles"])
-def test_gpu_model_is_the_transpose_of_gpu_migration(name):
+@pytest.mark.parametrize("name,variant", [("angle_base", "resident"), ("salt_like", "resident"),
+                                          ("gradfiles", "resident"), ("angle_base", "stream"),
+                                          ("offset_cig", "stream"), ("stack", "stream"),
+                                          ("nearest", "stream")])
+def test_gpu_model_is_the_transpose_of_gpu_migration(name, variant):
     kv, ax, m, d, tabs, grads = _inputs(name, 11)
-    print(f"dot-product case {name}", flush=True)
+    kv = dict(kv, variant=variant)
+    print(f"dot-product case {name} variant {variant}", flush=True)
     kmig = _kmig()
     Lm = kmig.model(kv, m, ax, *tabs, grads=grads).astype(np.float64)
-    LTd = kmig.migrate(dict(kv, variant="resident"), d, ax, *tabs,
-                       grads=grads)[0].astype(np.float64)
+    LTd = kmig.migrate(kv, d, ax, *tabs, grads=grads)[0].astype(np.float64)
     lhs = float(np.dot(Lm.ravel(), d.astype(np.float64).ravel()))
     rhs = float(np.dot(m.astype(np.float64).ravel(), LTd.ravel()))
     # each side is a float sum of ~1e5 terms of both signs: a few 1e-5 of
@@ -105,12 +111,12 @@ def test_repeat_model_runs_and_context_reuse():
 def test_model_refusals():
     kmig = _kmig()
     kv, ax, m, _, tabs, _ = _inputs("angle_base", 1)
-    for bad in (dict(kv, cmp=1), dict(kv, angle_interp=0), dict(kv, precision="fp16"),
-                {"cmp": 0}):
-        with pytest.raises(kmig.KmigError, match="adj=0 on the GPU needs"):
-            p_ng = O.params_from_kv(bad, int(ax[1][0]), ax[1][1], ax[1][2])
-            ng = p_ng.ng if p_ng.cig else 1
-            kmig.model(bad, m[:ng], ax, *tabs)
+    for prec in ("fp16", "bf16", "u16"):
+        with pytest.raises(kmig.KmigError, match="precision=fp32"):
+            kmig.model(dict(kv, precision=prec), m, ax, *tabs)
+    with pytest.raises(kmig.KmigError, match="adj=0"):
+        kmig.Migrator(kmig.parse_params(dict(kv, adj=1), int(ax[1][0]), ax[1][1],
+                                        ax[1][2])).model(m, ax)
 
 
 def test_cli_adj0_writes_the_modelled_data(tmp_path):
