@@ -30,7 +30,12 @@
  *   angle              closed-form opening angle of two known vectors;
  *   bins/tapers/mute   closed-form scalar functions;
  *   _f64 path          agrees with _f32 to float rounding on the golden cases
- *                      and is linear in the data.
+ *                      and is linear in the data;
+ *   model (adj=0)      the reference's adj=0 run crashes, so no golden data:
+ *                      pinned as the exact transpose of the pinned migration,
+ *                      <L m, d> = <m, L^T d> to 1e-10 in FP64 on ten golden
+ *                      geometries (angle, offset, stack, cmp 0/1, nearest,
+ *                      half angle), and pick0 against pick sample by sample.
  */
 #include "kmig_oracle.h"
 

@@ -15,7 +15,8 @@ SR := kirmig.cpp tinterpsr.cpp eikods.cpp neighbors.cpp pqueue.cpp decart.cpp ef
 OB := $(patsubst %.cpp,$(OUT)/obj/SEBASIC/%.o,$(SB)) $(OUT)/obj/sqlite3.o
 OF := $(patsubst %.cpp,$(OUT)/obj/SEFILESYSTEM/%.o,$(SF))
 OR := $(patsubst %.cpp,$(OUT)/obj/SERECKIRCH/src/%.o,$(SR))
-PROGS := kirchmig2d_cig eikods2d shotgen rsf_l2cmp layercom_cig extract_adcig
+PROGS := kirchmig2d_cig eikods2d shotgen rsf_l2cmp layercom_cig extract_adcig kirchdat2d_auto_cs \
+  greensol_auto
 
 all: $(addprefix $(OUT)/,$(PROGS))
 
