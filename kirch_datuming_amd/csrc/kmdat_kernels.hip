@@ -1,0 +1,309 @@
+// Device kernels of the frequency-domain Kirchhoff datuming (kirchdat2d_freq).
+//
+// Spectra are stored trace-major, frequency fastest: U[trace][iw], float2
+// (re, im), trace = is * nh + ih.  Every kernel puts 64 consecutive
+// frequencies on the 64 lanes of a wave, so each load and store of a
+// spectrum row is one coalesced 512-byte access; everything that depends
+// only on the summation pair (indices, weights, table keys) is wave-uniform.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+
+#include "kmdat_internal.h"
+
+namespace kmdat {
+
+#define KPI_F 3.14159265358979323846f
+
+// Consecutive block ids go to the 8 XCDs in turn; remap so that each XCD
+// walks a contiguous range of tiles (neighbouring tiles share inputs in L2).
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+    constexpr int NX = 8;
+    if (nb < NX) return b;
+    const int x = b % NX, k = b / NX;
+    const int per = nb / NX, extra = nb % NX;
+    return x * per + min(x, extra) + k;
+}
+
+// ---------------------------------------------------------------- traces --
+
+// read_traces_to_freq (src/kirchdat2d_frequency_cs.cpp:281-313) for traces
+// [tr0, tr0 + ntr): time reversal, the data-edge taper weight of the trace
+// and zero padding to nfft samples.
+__global__ void k_pad(const float *__restrict__ data, const float *__restrict__ wtr,
+                      float *__restrict__ pad, int tr0, int ntr, int nt, int nfft)
+{
+    const size_t n = (size_t)ntr * nfft;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int t = (int)(e / nfft), k = (int)(e % nfft);
+        float v = 0.0f;
+        if (k < nt) v = wtr[tr0 + t] * data[(size_t)(tr0 + t) * nt + (nt - 1 - k)];
+        pad[e] = v;
+    }
+}
+
+// write_traces_from_freq (:470-491): the first nt samples of the inverse
+// transform divided by nfft, reversed when asked.
+__global__ void k_post(const float *__restrict__ pad, float *__restrict__ out, int tr0, int ntr,
+                       int nt, int nfft, int reverse)
+{
+    const size_t n = (size_t)ntr * nt;
+    const float fn = (float)nfft;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int t = (int)(e / nt), it = (int)(e % nt);
+        const float v = pad[(size_t)t * nfft + it] / fn;
+        out[(size_t)(tr0 + t) * nt + (reverse ? nt - 1 - it : it)] = v;
+    }
+}
+
+__global__ void k_zero2(float2 *__restrict__ p, size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        p[e] = make_float2(0.0f, 0.0f);
+}
+
+// -------------------------------------------------------- operator table --
+
+// One row per table key (a (summation, output) pair of table positions):
+// ktab[key][iw] = antialias(iw, slope) * H(tau, iw), with H the spectrum of
+// the datuming filter delayed by tau (build_filter_F, build_H_iw, :100-184)
+// and the antialias weight (:203-233) of the key's traveltime slope.  Keys
+// with tau <= 0 or non-finite tau have H = 0.  The filter taps of the key
+// are formed once per block in LDS.
+__global__ __launch_bounds__(256) void k_ktab(float2 *__restrict__ ktab,
+                                              const float *__restrict__ tau,
+                                              const float *__restrict__ aarg, int nkey, int nw,
+                                              int nchunk, int nfft, float dt, int nsam,
+                                              float antialias)
+{
+    __shared__ float Fk[KMDAT_MAX_NSAM];
+    const int key = blockIdx.x / nchunk, chunk = blockIdx.x % nchunk;
+    const float t = tau[key];
+    const bool live = t > 0.0f && isfinite(t);
+    const int ntap = nsam - 1;
+    // F[k] = f(k+1) - 2 f(k) + f(k-1), f(0) = 0, f(m) = sqrt(((tau + m dt)/tau)^2 - 1)
+    if (live) {
+        for (int k = threadIdx.x; k < ntap; k += blockDim.x) {
+            auto f = [&](int m) -> float {
+                if (m == 0) return 0.0f;
+                const float q = (t + (float)m * dt) / t;
+                return sqrtf(q * q - 1.0f);
+            };
+            Fk[k] = k == 0 ? f(1) - 0.0f : f(k + 1) - 2.0f * f(k) + f(k - 1);
+        }
+    }
+    __syncthreads();
+    const int iw = chunk * blockDim.x + threadIdx.x;
+    if (iw >= nw) return;
+    float2 K = make_float2(0.0f, 0.0f);
+    if (live) {
+        const float omega = 2.0f * KPI_F * (float)iw / (float)nfft;
+        float step_i, step_r;
+        sincosf(-omega, &step_i, &step_r);
+        float z_r = 1.0f, z_i = 0.0f, sum_r = 0.0f, sum_i = 0.0f;
+        for (int k = 0; k < ntap; k++) {
+            const float fk = Fk[k] / dt;
+            sum_r += fk * z_r;
+            sum_i += fk * z_i;
+            const float nr = z_r * step_r - z_i * step_i;
+            const float ni = z_r * step_i + z_i * step_r;
+            z_r = nr;
+            z_i = ni;
+        }
+        const float q = t / dt;
+        const int m = (int)ceilf(q);
+        const float delta = (float)m - q;
+        float sin_m, cos_m, sin_m1, cos_m1;
+        sincosf(-omega * (float)m, &sin_m, &cos_m);
+        sincosf(-omega * (float)(m - 1), &sin_m1, &cos_m1);
+        const float lin_r = (1.0f - delta) * cos_m + delta * cos_m1;
+        const float lin_i = (1.0f - delta) * sin_m + delta * sin_m1;
+        K.x = lin_r * sum_r - lin_i * sum_i;
+        K.y = lin_r * sum_i + lin_i * sum_r;
+        // antialias_weight_iw
+        float aa = 1.0f;
+        if (antialias > 0.0f && iw > 0) {
+            const float dtau = fabsf(aarg[key]) * antialias;
+            if (dtau > 1e-12f && isfinite(dtau)) {
+                const float freq = (float)iw / ((float)nfft * dt);
+                const float fnyq = 0.5f / dt;
+                const float fmax = 0.5f / dtau;
+                if (fmax < fnyq) {
+                    if (fmax <= 0.0f) {
+                        aa = 0.0f;
+                    } else {
+                        const float fpass = 0.8f * fmax;
+                        if (freq >= fmax) {
+                            aa = 0.0f;
+                        } else if (freq > fpass) {
+                            const float x = (freq - fpass) / (fmax - fpass);
+                            aa = 0.5f * (1.0f + cosf(KPI_F * x));
+                        }
+                    }
+                }
+            }
+        }
+        K.x *= aa;
+        K.y *= aa;
+    }
+    ktab[(size_t)key * nw + iw] = K;
+}
+
+// ------------------------------------------------------- banded operator --
+
+// out[b][o] = sum_{|i - o| <= aper} w(o, i) ktab[key(o, i)] in[b][i], for
+// operators whose pairs do not depend on the batch index b (cmp=0: the
+// receiver side sums receivers ic of one shot, b = shot; the source side sums
+// shots of one receiver, b = receiver).  A wave holds a TB x TO register tile
+// of (batch, output) at its 64 frequencies: per summation index it loads TB
+// input rows and, per output whose band holds the index, one table row.
+template <int TB, int TO>
+__global__ __launch_bounds__(256) void k_band(const BandArgs a)
+{
+    const int nbt = (a.n_b + TB - 1) / TB, not_ = (a.n_o + TO - 1) / TO;
+    const int tile = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int ot = tile % not_, rest = tile / not_;
+    const int bt = rest % nbt, chunk = rest / nbt;
+    const int iw = chunk * 256 + (int)threadIdx.x;
+    const bool ok = iw < a.nw;
+    const int iwc = ok ? iw : a.nw - 1;  // keeps the tail lanes' loads in bounds
+    const int o0 = ot * TO, b0 = bt * TB;
+    float2 acc[TB][TO];
+#pragma unroll
+    for (int tb = 0; tb < TB; tb++)
+#pragma unroll
+        for (int to = 0; to < TO; to++) acc[tb][to] = make_float2(0.0f, 0.0f);
+    const int i_lo = max(0, o0 - a.aper), i_hi = min(a.n_i - 1, o0 + TO - 1 + a.aper);
+    for (int i = i_lo; i <= i_hi; i++) {
+        float2 u[TB];
+#pragma unroll
+        for (int tb = 0; tb < TB; tb++) {
+            const int b = min(b0 + tb, a.n_b - 1);
+            u[tb] = a.in[(size_t)b * a.in_b + (size_t)i * a.in_i + iwc];
+        }
+#pragma unroll
+        for (int to = 0; to < TO; to++) {
+            const int o = o0 + to, j = i - o + a.aper;
+            if (o >= a.n_o || j < 0 || j >= a.nb) continue;
+            const int key = a.pkey[(size_t)o * a.nb + j];
+            if (key < 0) continue;
+            const float w = a.pw[(size_t)o * a.nb + j];
+            const float2 k = a.ktab[(size_t)key * a.nw + iwc];
+            const float kr = w * k.x, ki = w * k.y;
+#pragma unroll
+            for (int tb = 0; tb < TB; tb++) {
+                acc[tb][to].x += kr * u[tb].x - ki * u[tb].y;
+                acc[tb][to].y += kr * u[tb].y + ki * u[tb].x;
+            }
+        }
+    }
+    if (!ok) return;
+#pragma unroll
+    for (int tb = 0; tb < TB; tb++) {
+        const int b = b0 + tb;
+        if (b >= a.n_b) continue;
+#pragma unroll
+        for (int to = 0; to < TO; to++) {
+            const int o = o0 + to;
+            if (o < a.n_o) a.out[(size_t)b * a.out_b + (size_t)o * a.out_o + iw] = acc[tb][to];
+        }
+    }
+}
+
+// ---------------------------------------------------- pair-list operator --
+
+// out[o] = sum_q w[q] ktab[key[q]] in[tr[q]], q over the pairs of output
+// trace o (rowptr[o] .. rowptr[o+1]): the cmp=1 operators, whose pairs
+// change with the shot (receiver side) or follow the common-receiver
+// regrouping (source side).
+__global__ __launch_bounds__(256) void k_pairs(const PairArgs a)
+{
+    const int nchunk = (a.nw + 255) / 256;
+    const int o = (int)blockIdx.x / nchunk, chunk = (int)blockIdx.x % nchunk;
+    const int iw = chunk * 256 + (int)threadIdx.x;
+    if (iw >= a.nw) return;
+    float2 acc = make_float2(0.0f, 0.0f);
+    const int q0 = a.rowptr[o], q1 = a.rowptr[o + 1];
+    for (int q = q0; q < q1; q++) {
+        const PairRec p = a.pairs[q];
+        const float2 k = a.ktab[(size_t)p.key * a.nw + iw];
+        const float2 u = a.in[(size_t)p.tr * a.nw + iw];
+        const float kr = p.w * k.x, ki = p.w * k.y;
+        acc.x += kr * u.x - ki * u.y;
+        acc.y += kr * u.y + ki * u.x;
+    }
+    a.out[(size_t)o * a.nw + iw] = acc;
+}
+
+// ------------------------------------------------------------- launchers --
+
+static unsigned grid_for(size_t n, int block)
+{
+    size_t g = (n + block - 1) / block;
+    if (g > 65536 * 4) g = 65536 * 4;
+    return (unsigned)(g ? g : 1);
+}
+
+int launch_pad(const float *data, const float *wtr, float *pad, int tr0, int ntr, int nt, int nfft,
+               void *stream)
+{
+    hipLaunchKernelGGL(k_pad, dim3(grid_for((size_t)ntr * nfft, 256)), dim3(256), 0,
+                       (hipStream_t)stream, data, wtr, pad, tr0, ntr, nt, nfft);
+    return (int)hipGetLastError();
+}
+
+int launch_post(const float *pad, float *out, int tr0, int ntr, int nt, int nfft, int reverse,
+                void *stream)
+{
+    hipLaunchKernelGGL(k_post, dim3(grid_for((size_t)ntr * nt, 256)), dim3(256), 0,
+                       (hipStream_t)stream, pad, out, tr0, ntr, nt, nfft, reverse);
+    return (int)hipGetLastError();
+}
+
+int launch_zero2(float2 *p, size_t n, void *stream)
+{
+    hipLaunchKernelGGL(k_zero2, dim3(grid_for(n, 256)), dim3(256), 0, (hipStream_t)stream, p, n);
+    return (int)hipGetLastError();
+}
+
+int launch_ktab(float2 *ktab, const float *tau, const float *aarg, int nkey, int nw, int nfft,
+                float dt, int nsam, float antialias, void *stream)
+{
+    if (nkey <= 0) return 0;
+    if (nsam - 1 > KMDAT_MAX_NSAM) return (int)hipErrorInvalidValue;
+    const int nchunk = (nw + 255) / 256;
+    const size_t nblk = (size_t)nkey * nchunk;
+    if (nblk > 0x7fffffffu) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_ktab, dim3((unsigned)nblk), dim3(256), 0, (hipStream_t)stream, ktab,
+                       tau, aarg, nkey, nw, nchunk, nfft, dt, nsam, antialias);
+    return (int)hipGetLastError();
+}
+
+int launch_band(const BandArgs &a, void *stream)
+{
+    constexpr int TB = 8, TO = 4;
+    const size_t nblk = (size_t)((a.n_o + TO - 1) / TO) * ((a.n_b + TB - 1) / TB) *
+                        ((a.nw + 255) / 256);
+    if (nblk == 0) return 0;
+    if (nblk > 0x7fffffffu) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_band<TB, TO>), dim3((unsigned)nblk), dim3(256), 0, (hipStream_t)stream,
+                       a);
+    return (int)hipGetLastError();
+}
+
+int launch_pairs(const PairArgs &a, int n_out, void *stream)
+{
+    const size_t nblk = (size_t)n_out * ((a.nw + 255) / 256);
+    if (nblk == 0) return 0;
+    if (nblk > 0x7fffffffu) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pairs, dim3((unsigned)nblk), dim3(256), 0, (hipStream_t)stream, a);
+    return (int)hipGetLastError();
+}
+
+}  // namespace kmdat

@@ -16,10 +16,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "kirch_datuming_amd", "bin", "kirchmig2d_cig")
 
 
-def _header_functions(name="kmig.h"):
+def _header_functions(name="kmig.h", prefix="kmig_"):
     txt = open(os.path.join(ROOT, "include", name)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(kmig_[a-z_0-9]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"[a-z_0-9]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -38,8 +38,14 @@ def test_library_exports_every_declared_symbol():
     assert set(eik_names) == set(eik.EXPORTS)
     for n in eik_names:
         assert hasattr(E, n), n
+    from kirch_datuming_amd import dat
+    Dl = dat.lib()
+    dat_names = _header_functions("kmig_dat.h", "kmdat_")
+    assert set(dat_names) == set(dat.EXPORTS)
+    for n in dat_names:
+        assert hasattr(Dl, n), n
     assert set(os.listdir(os.path.join(ROOT, "include"))) == {"kmig.h", "kmig_nn.h",
-                                                               "kmig_eik.h"}
+                                                               "kmig_eik.h", "kmig_dat.h"}
 
 
 @pytest.mark.parametrize("name", gc.case_names())
