@@ -37,7 +37,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="salt",
-                    choices=["salt", "small", "layers", "surrogate"])
+                    choices=["salt", "small", "layers", "surrogate", "datum"])
     ap.add_argument("--grf-models", type=int, default=256,
                     help="surrogate: velocity models of the pretraining set")
     ap.add_argument("--fit-steps", type=int, default=8000,
@@ -86,6 +86,8 @@ def main():
         return bench_layers(args, cores)
     if args.config == "surrogate":
         return bench_surrogate(args, cores)
+    if args.config == "datum":
+        return bench_datum(args, cores)
     t0 = time.time()
     log(f"building the {args.config} workload on {cores} host threads")
     w = configs.salt(threads=cores) if args.config == "salt" else configs.small(threads=cores)
@@ -296,6 +298,111 @@ def bench_model(args, cores, w):
         "data_abs_max": float(np.abs(data).max()),
     }
     print(json.dumps(out), flush=True)
+
+
+FP32_PEAK_TFLOPS = 157.3
+
+
+def bench_datum(args, cores):
+    """Frequency-domain datuming of the workflow's datuming step (one step =
+    forward transforms, both operators with their tables, inverse transforms,
+    data resident in HBM).  The operators are complex multiply-adds (8 flops
+    each) on the vector FP32 units."""
+    import numpy as np
+
+    from kirch_datuming_amd import configs, dat
+
+    t0 = time.time()
+    log(f"building the datuming workload on {cores} host threads")
+    w = configs.datum(threads=cores)
+    log(f"workload ready in {time.time() - t0:.1f} s: {w.name}")
+    cfg = dat.config(w.kv, w.datum)
+    dat.check(cfg)
+    dm = dat.Datum(cfg)
+    dm.set_green(0, w.green, *w.pos_axis)
+    dm.set_green(1, w.green, *w.pos_axis)
+    tu = time.perf_counter()
+    dm.set_data(w.data, w.data_axes)
+    upload_s = time.perf_counter() - tu
+    for i in range(args.warmup):
+        dm.run()
+        dm.sync()
+        log(f"warmup {i + 1}/{args.warmup}: {dm.stats()['ms_total']:.1f} ms")
+    rec, fwd, inv, tab, rcv, src = [], [], [], [], [], []
+    tstart = time.perf_counter()
+    for _ in range(args.steps):
+        dm.run()
+        dm.sync()
+        st = dm.stats()
+        rec.append(st["ms_total"])
+        fwd.append(st["ms_forward"])
+        inv.append(st["ms_inverse"])
+        tab.append(st["ms_tables"])
+        rcv.append(st["ms_receiver"])
+        src.append(st["ms_source"])
+    wall = time.perf_counter() - tstart
+    out = dm.output()
+    dm.close()
+    ms_step = wall / args.steps * 1e3
+    value = st["cmacs"] * args.steps / wall
+    log(f"timed {args.steps} steps: {ms_step:.1f} ms/step, nfft {st['nfft']}, "
+        f"{st['cmacs']:.3e} complex MACs per step")
+    rcv_ms = float(np.mean(rcv))
+    achieved = 8.0 * st["pairs_receiver"] * st["nw"] / (rcv_ms * 1e-3) / 1e12
+
+    cpu = parity = None
+    if not args.no_cpu:
+        from oracle import datum as D
+        from oracle import oracle as O
+        sub = w.corner(96)
+        log("CPU oracle (float, direct path) on a 96 x 96 corner of the cube")
+        tc = time.perf_counter()
+        ref32 = D.datum(sub.kv, sub.data, sub.data_axes, sub.green, sub.pos_axis, sub.green,
+                        sub.pos_axis, sub.datum)
+        tcpu = time.perf_counter() - tc
+        with dat.Datum(cfg) as ds:
+            ds.set_green(0, sub.green, *sub.pos_axis)
+            ds.set_green(1, sub.green, *sub.pos_axis)
+            ds.set_data(sub.data, sub.data_axes)
+            ds.run()
+            ds.sync()
+            gpu_sub = ds.output()
+            sst = ds.stats()
+        log("FP64 oracle on the same corner")
+        ref64 = D.datum(sub.kv, sub.data, sub.data_axes, sub.green, sub.pos_axis, sub.green,
+                        sub.pos_axis, sub.datum, precision="f64")
+        cpu = {"value": sst["cmacs"] / tcpu, "unit": "complex MAC/s", "cores": cores,
+               "kind": "port",
+               "sample": f"96 shots x 96 receivers of the same cube, tables and keys "
+                         f"({sst['cmacs']:.3e} complex MACs, nfft {sst['nfft']}), {tcpu:.1f} s, "
+                         f"the reference's direct path with OpenMP over frequencies"}
+        parity = {"sample": "96 x 96 corner", "rel_l2_gpu_vs_f64": O.rel_l2(gpu_sub, ref64),
+                  "rel_l2_cpu_f32_vs_f64": O.rel_l2(ref32, ref64)}
+        log(f"CPU {cpu['value']:.3e} complex MAC/s; relL2 GPU vs f64 "
+            f"{parity['rel_l2_gpu_vs_f64']:.2e}, CPU f32 vs f64 {parity['rel_l2_cpu_f32_vs_f64']:.2e}")
+    (nt, _, _), (nh, _, _), (ns, _, _) = w.data_axes
+    line = {
+        "metric": "datuming operator complex MACs/s (kirchdat2d_freq, direct sums)",
+        "value": value, "unit": "complex MAC/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (complex)", "data": "synthetic",
+        "config": {"workload": w.name, "config": "datum", "ns": int(ns), "nh": int(nh),
+                   "nt": int(nt), "nfft": st["nfft"], "keys": w.kv,
+                   "pairs_receiver": st["pairs_receiver"], "pairs_source": st["pairs_source"],
+                   "kernel": "k_band<8,4>" if st["banded"] else "k_pairs"},
+        "roofline": {"bound": "valu-fp32", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_band<8,4> (receiver side)", "flops_per_cmac": 8,
+                     "avg_launch_ms": rcv_ms},
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "breakdown_ms": {"forward": float(np.mean(fwd)), "tables": float(np.mean(tab)),
+                         "receiver": rcv_ms, "source": float(np.mean(src)),
+                         "inverse": float(np.mean(inv)), "events_total": float(np.mean(rec))},
+        "upload_s": upload_s,
+        "out_abs_max": float(np.abs(out).max()),
+    }
+    print(json.dumps(line), flush=True)
 
 
 def bench_layers(args, cores):

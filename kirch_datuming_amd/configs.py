@@ -162,6 +162,53 @@ def salt(threads=0, seed=20260202, nshots=768) -> Workload:
                     RT, RD, tax + [[nh, h0, dh]], vel)
 
 
+@dataclass
+class DatumWorkload:
+    name: str
+    kv: dict
+    data: np.ndarray            # (ns, nh, nt)
+    data_axes: list             # [[nt,t0,dt],[nh,h0,dh],[ns,s0,ds]]
+    green: np.ndarray           # (npos, npos) datum table (sources and receivers)
+    pos_axis: tuple             # (o, d) of the table positions
+    datum: float                # datum depth, km
+
+    def corner(self, n):
+        """The first n shots x n receivers of the cube (same tables, datum
+        and keys): a bounded sample of the same operator."""
+        (nt, t0, dt), (_, h0, dh), (_, s0, ds) = self.data_axes
+        return DatumWorkload(self.name + f"[{n}x{n} corner]", self.kv,
+                             np.ascontiguousarray(self.data[:n, :n]),
+                             [[nt, t0, dt], [n, h0, dh], [n, s0, ds]], self.green,
+                             self.pos_axis, self.datum)
+
+
+def datum(threads=0, seed=20260404, npos=768, nt=2001, should_datum=80) -> DatumWorkload:
+    """The datuming step of the recursive workflow (run_salt_adcig.sh:155-166)
+    on the salt model: 768 shots and receivers at the model's 10 m sampling
+    (shotgen nshot=nx=768, dy=dx=0.01, :18-20, :111-112), absolute receiver
+    coordinates (cmp=0, :20), the datum at the base of the first layer
+    (80 samples, 0.8 km), the workflow's keys (aperture=200 data_taper=20
+    taper=0 length=0.05 antialias=1 use_bf=1); datum tables cut from
+    fast-marching traveltimes as greensol_auto does, synthetic diffraction
+    data of 2001 samples at 2 ms."""
+    from . import dat
+    L = synth()
+    nz, nx, d = 400, 1000, 0.01
+    vel = np.empty((nx, nz), np.float32)
+    L.kms_velocity_salt(_p(vel), nz, nx, d, d)
+    T, _ = tables(vel, d, d, npos, 0.0, d, 2, threads)
+    green = dat.greensol(T, [[nz, 0.0, d], [nx, 0.0, d]], (0.0, d), should_datum * d)
+    dt = 0.002
+    data = np.empty((npos, npos, nt), np.float32)
+    L.kms_data_scatterers(_p(data), nt, 0.0, dt, npos, 0.0, d, npos, 0.0, d, 0, _p(T), npos,
+                          0.0, d, _p(T), npos, 0.0, d, nz, nx, 60, 20.0, 1.5, seed)
+    kv = dict(cmp=0, aperture=200, data_taper=20, taper=0, length=0.05, antialias=1, use_bf=1)
+    return DatumWorkload(f"datum salt {npos} shots x {npos} receivers, nt={nt}, datum "
+                         f"{should_datum * d:.2f} km", kv, data,
+                         [[nt, 0.0, dt], [npos, 0.0, d], [npos, 0.0, d]], green, (0.0, d),
+                         should_datum * d)
+
+
 LAYERS_NZ = (80, 70, 90, 100, 60)
 LAYERS_NS = (768, 640, 512, 384, 256)
 LAYERS_NH = (401, 361, 321, 281, 241)
