@@ -250,6 +250,8 @@ int launch_stitch(const StitchArgs &a, float *out, unsigned *nonfinite, void *st
 int launch_migrate_resident(const MigResArgs &a, int tile, int precision, void *stream);
 // adj=0: sprays the image held in a.cig into a.spray (FP32 tuples; presets
 // 9, 11 and 15)
+// Trace window (samples) one wave of the resident model kernel sums in LDS.
+constexpr int MODEL_WIN = 256;
 int launch_model_resident(const MigResArgs &a, int tile, void *stream);
 int launch_migrate_multi(const MigMultiArgs &m, int tile, int precision, void *stream);
 int migrate_tiles(int nz, int nx, int tile);

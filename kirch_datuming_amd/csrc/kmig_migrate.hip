@@ -402,6 +402,15 @@ __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int8, flo
     return q;
 }
 
+// LDS ordering among the lanes of one wave: the wave's LDS operations run in
+// order, so only the compiler must keep them so.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Probe = true only renames the kernel for the autotuner's timing launches,
 // so profiler statistics of the production kernel cover production launches.
 // One tile of TZ x TX image points of one problem.
@@ -600,22 +609,57 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                     // kirmig_pick(adj=0), SERECKIRCH/src/kirmig.cpp:59-67: the two
                     // bins' image values, weighted, sprayed with the transposed
                     // three-tap weights (offsets are in 8-byte pair units: >> 3
-                    // gives samples, and ok keeps every tap in [0, nt - 1])
+                    // gives samples, and ok keeps every tap in [0, nt - 1]).  The
+                    // wave's updates of one trace land in a narrow window of it:
+                    // they are summed in the wave's LDS window first and the
+                    // window is added to the trace with one atomic per touched
+                    // sample (windows wider than MODEL_WIN add directly).
+                    float *win = acc + (a.ng + 1) * P + (lid >> 6) * MODEL_WIN;
+                    const int ln = lid & 63;
 #pragma unroll
                     for (int k = 0; k < G; k++) {
-                        if (q[k].ok & valid) {
-                            const float *cell = my + q[k].ig0 * P;
-                            const float v =
-                                q[k].coef * ((1.0f - q[k].w1) * cell[0] + q[k].w1 * cell[P]);
-                            float *tr = a.spray + ((size_t)is * a.nh + r0 + g0 + k) * a.nt;
-                            const int it = (int)(q[k].o0 >> 3), itm = (int)(q[k].om >> 3),
-                                      itp = (int)(q[k].op >> 3);
-                            atomicAdd(tr + it, 2.0f * (1.0f - q[k].ft) * v);
-                            atomicAdd(tr + it + 1, 2.0f * q[k].ft * v);
-                            atomicAdd(tr + itm, -(1.0f - q[k].ftm) * v);
-                            atomicAdd(tr + itm + 1, -q[k].ftm * v);
-                            atomicAdd(tr + itp, -(1.0f - q[k].ftp) * v);
-                            atomicAdd(tr + itp + 1, -q[k].ftp * v);
+                        const bool sp = q[k].ok & valid;
+                        int lo = sp ? (int)(q[k].om >> 3) : 0x7fffffff;
+                        int hi = sp ? (int)(q[k].op >> 3) + 1 : -1;
+                        for (int o = 32; o > 0; o >>= 1) {
+                            lo = min(lo, __shfl_xor(lo, o, 64));
+                            hi = max(hi, __shfl_xor(hi, o, 64));
+                        }
+                        if (hi < lo) continue;  // no lane sprays this trace
+                        const float *cell = my + q[k].ig0 * P;
+                        const float v =
+                            q[k].coef * ((1.0f - q[k].w1) * cell[0] + q[k].w1 * cell[P]);
+                        float *tr = a.spray + ((size_t)is * a.nh + r0 + g0 + k) * a.nt;
+                        const int it = (int)(q[k].o0 >> 3), itm = (int)(q[k].om >> 3),
+                                  itp = (int)(q[k].op >> 3);
+                        const float w0 = 2.0f * (1.0f - q[k].ft) * v, w1 = 2.0f * q[k].ft * v;
+                        const float wm0 = -(1.0f - q[k].ftm) * v, wm1 = -q[k].ftm * v;
+                        const float wp0 = -(1.0f - q[k].ftp) * v, wp1 = -q[k].ftp * v;
+                        const int span = hi - lo + 1;
+                        if (span <= MODEL_WIN) {
+                            for (int e = ln; e < span; e += 64) win[e] = 0.0f;
+                            wave_lds_sync();
+                            if (sp) {
+                                atomicAdd(win + (it - lo), w0);
+                                atomicAdd(win + (it + 1 - lo), w1);
+                                atomicAdd(win + (itm - lo), wm0);
+                                atomicAdd(win + (itm + 1 - lo), wm1);
+                                atomicAdd(win + (itp - lo), wp0);
+                                atomicAdd(win + (itp + 1 - lo), wp1);
+                            }
+                            wave_lds_sync();
+                            for (int e = ln; e < span; e += 64) {
+                                const float sv = win[e];
+                                if (sv != 0.0f) atomicAdd(tr + lo + e, sv);
+                            }
+                            wave_lds_sync();
+                        } else if (sp) {
+                            atomicAdd(tr + it, w0);
+                            atomicAdd(tr + it + 1, w1);
+                            atomicAdd(tr + itm, wm0);
+                            atomicAdd(tr + itm + 1, wm1);
+                            atomicAdd(tr + itp, wp0);
+                            atomicAdd(tr + itp + 1, wp1);
                         }
                     }
                     continue;
@@ -786,7 +830,8 @@ template <int TZ, int TX, int R, int G>
 static int launch_model_t(const MigResArgs &a, hipStream_t st)
 {
     const int ntz = (a.nz + TZ - 1) / TZ, ntx = (a.nx + TX - 1) / TX;
-    const size_t lds = (size_t)(a.ng + 1) * TZ * TX * sizeof(float);
+    // the image column plus one trace window per wave
+    const size_t lds = ((size_t)(a.ng + 1) * TZ * TX + (TZ * TX / 64) * MODEL_WIN) * sizeof(float);
     if (lds > 64 * 1024) {
         const hipError_t e = hipFuncSetAttribute((const void *)k_model_ph<TZ, TX, R, G>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
