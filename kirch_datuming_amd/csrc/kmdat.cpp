@@ -80,6 +80,7 @@ struct kmdat_ctx {
     size_t work_cap = 0;
     hipEvent_t ev[8] = {};
     kmdat_stats st{};
+    bool mfma = false;  // KMDAT_MFMA=1: the matrix-core banded kernel
 };
 
 static int fail(kmdat_ctx *c, int code, const char *fmt, ...)
@@ -595,6 +596,7 @@ int kmdat_create(kmdat_ctx **out, const kmdat_config *cfg)
     if (rc) return fail(c, rc, "%s", err);
     static std::once_flag once;
     std::call_once(once, [] { rocfft_setup(); });
+    if (const char *v = getenv("KMDAT_MFMA")) c->mfma = atoi(v) != 0;
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
     return KMDAT_OK;
@@ -716,7 +718,8 @@ static int apply_op(kmdat_ctx *c, int k, const float2 *in, float2 *out)
         a.in_i = op.in_i * nw;
         a.out_b = op.out_b * nw;
         a.out_o = op.out_o * nw;
-        HIPCHK(c, (hipError_t)launch_band(a, c->stream));
+        HIPCHK(c, (hipError_t)(c->mfma ? launch_band_mfma(a, c->stream)
+                                        : launch_band(a, c->stream)));
     } else {
         PairArgs a{};
         a.in = in;

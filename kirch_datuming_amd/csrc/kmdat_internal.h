@@ -50,6 +50,8 @@ int launch_zero2(float2 *p, size_t n, void *stream);
 int launch_ktab(float2 *ktab, const float *tau, const float *aarg, int nkey, int nw, int nfft,
                 float dt, int nsam, float antialias, void *stream);
 int launch_band(const BandArgs &a, void *stream);
+// the same operator on the matrix cores (v_mfma_f32_32x32x2f32)
+int launch_band_mfma(const BandArgs &a, void *stream);
 int launch_pairs(const PairArgs &a, int n_out, void *stream);
 
 }  // namespace kmdat

@@ -223,6 +223,101 @@ __global__ __launch_bounds__(256) void k_band(const BandArgs a)
     }
 }
 
+// The banded operator on the matrix cores.  Per frequency the operator is a
+// complex product C[b][o] = sum_i A[b][i] K[o][i]; a block owns a 32 (batch)
+// x 32 (output) tile at MF = 16 consecutive frequencies and walks the
+// summation index in stages of MK = 8.  Each stage brings the 32 input rows
+// and the 32 pre-weighted table rows (w(o, i) ktab[key(o, i)], zero outside
+// the band) of its 8 indices into LDS as [index][frequency][row] (rows
+// fastest, padded to 33, so the operand reads of a wave and the staging
+// writes are free of bank conflicts); two blocks per CU overlap one's
+// staging with the other's products.  Wave w takes frequencies 4w .. 4w+3: per pair of indices and
+// frequency four v_mfma_f32_32x32x2f32 form the real and imaginary parts
+// (Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br), 32 accumulators per frequency.
+constexpr int MF = 16, MB = 32, MO = 32, MK = 8, MPAD = 33;
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(256, 2) void k_band_mfma(const BandArgs a)
+{
+    __shared__ float2 su[MK][MF][MPAD];
+    __shared__ float2 sk[MK][MF][MPAD];
+    const int t = (int)threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int sf = t % MF, sg = t / MF;  // staging: frequency, row group of 16
+    const int nbt = (a.n_b + MB - 1) / MB, not_ = (a.n_o + MO - 1) / MO;
+    const int tile = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int ot = tile % not_, rest = tile / not_;
+    const int bt = rest % nbt, chunk = rest / nbt;
+    const int o0 = ot * MO, b0 = bt * MB, f0 = chunk * MF;
+    const int siw = min(f0 + sf, a.nw - 1);  // staging frequency, kept in bounds
+    const int i_lo = max(0, o0 - a.aper);
+    const int i_hi = min(a.n_i - 1, min(o0 + MO, a.n_o) - 1 + a.aper);
+
+    floatx16 cr[4], ci[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            cr[q][j] = 0.0f;
+            ci[q][j] = 0.0f;
+        }
+    const int row = lane % 32, kh = lane / 32;
+    for (int is = i_lo; is <= i_hi; is += MK) {
+        // stage: rows r = 16 p + sg of the 8 x 32 input and table rows
+#pragma unroll 4
+        for (int p = 0; p < (MK * MB) / 16; p++) {
+            const int r = p * 16 + sg;
+            const int ii = r / MB, rr = r % MB;
+            const int i = is + ii;
+            const int b = b0 + rr;
+            float2 v = make_float2(0.0f, 0.0f);
+            if (i <= i_hi && b < a.n_b) v = a.in[(size_t)b * a.in_b + (size_t)i * a.in_i + siw];
+            su[ii][sf][rr] = v;
+            const int o = o0 + rr, j = i - o + a.aper;
+            float2 k = make_float2(0.0f, 0.0f);
+            if (i <= i_hi && o < a.n_o && j >= 0 && j < a.nb) {
+                const int key = a.pkey[(size_t)o * a.nb + j];
+                if (key >= 0) {
+                    const float w = a.pw[(size_t)o * a.nb + j];
+                    const float2 kv = a.ktab[(size_t)key * a.nw + siw];
+                    k = make_float2(w * kv.x, w * kv.y);
+                }
+            }
+            sk[ii][sf][rr] = k;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < MK / 2; ks++) {
+            const int ii = 2 * ks + kh;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int f = wave * 4 + q;
+                const float2 av = su[ii][f][row];
+                const float2 bv = sk[ii][f][row];
+                cr[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bv.x, cr[q], 0, 0, 0);
+                cr[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(-av.y, bv.y, cr[q], 0, 0, 0);
+                ci[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bv.y, ci[q], 0, 0, 0);
+                ci[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bv.x, ci[q], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // accumulator j of lane l holds C[8 (j / 4) + 4 (l / 32) + j % 4][l % 32]
+    const int o = o0 + row;
+    if (o >= a.n_o) return;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int iw = f0 + wave * 4 + q;
+        if (iw >= a.nw) continue;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int b = b0 + 8 * (j / 4) + 4 * kh + (j % 4);
+            if (b < a.n_b)
+                a.out[(size_t)b * a.out_b + (size_t)o * a.out_o + iw] = make_float2(cr[q][j], ci[q][j]);
+        }
+    }
+}
+
 // ---------------------------------------------------- pair-list operator --
 
 // out[o] = sum_q w[q] ktab[key[q]] in[tr[q]], q over the pairs of output
@@ -301,6 +396,16 @@ static int launch_band_t(const BandArgs &a, hipStream_t st)
     if (nblk == 0) return 0;
     if (nblk > 0x7fffffffu) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL((k_band<TB, TO, WB>), dim3((unsigned)nblk), dim3(256), 0, st, a);
+    return (int)hipGetLastError();
+}
+
+int launch_band_mfma(const BandArgs &a, void *stream)
+{
+    const size_t nblk = (size_t)((a.n_o + MO - 1) / MO) * ((a.n_b + MB - 1) / MB) *
+                        ((a.nw + MF - 1) / MF);
+    if (nblk == 0) return 0;
+    if (nblk > 0x7fffffffu) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_band_mfma, dim3((unsigned)nblk), dim3(256), 0, (hipStream_t)stream, a);
     return (int)hipGetLastError();
 }
 

@@ -68,6 +68,20 @@ def test_options_off_against_the_time_domain_reference(name):
     assert O.rel_l2(out, GOLD[name + "/out"]) < 1e-5 + TOL
 
 
+@pytest.mark.parametrize("name", ["cmp0", "cmp0_wide"])
+def test_matrix_core_kernel_against_fp64_oracle(name, monkeypatch):
+    """The banded operator on the matrix cores (KMDAT_MFMA=1, read when a
+    context is created) against the same FP64 oracle."""
+    monkeypatch.setenv("KMDAT_MFMA", "1")
+    kv, data, ax = _case(name)
+    kv.update(antialias=1.0, data_taper=2, taper=2)
+    g = GOLD["green"]
+    print(f"datum case {name} on the matrix cores", flush=True)
+    out, mid = _dat().datum(kv, data, ax, g, POS, g, POS, SDATUM, interm=True)
+    ref, rmid = D.datum(kv, data, ax, g, POS, g, POS, SDATUM, precision="f64", interm=True)
+    assert O.rel_l2(out, ref) < TOL and O.rel_l2(mid, rmid) < TOL
+
+
 def test_repeat_runs_are_bitwise_and_stats():
     dat = _dat()
     kv, data, ax = _case("cmp0")
