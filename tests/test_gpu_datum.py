@@ -82,6 +82,47 @@ def test_matrix_core_kernel_against_fp64_oracle(name, monkeypatch):
     assert O.rel_l2(out, ref) < TOL and O.rel_l2(mid, rmid) < TOL
 
 
+EDGE = {
+    # name: (ns, s0, ds, nh, h0, dh, nt, keys); 256 samples hold the datumed
+    # energy (the two datum delays add up to about 0.35 s)
+    "one_shot": (1, 0.5, 0.05, 12, 0.2, 0.05, 256, {"cmp": 0, "aperture": 4, "taper": 0}),
+    "one_receiver": (9, 0.3, 0.05, 1, 0.6, 0.05, 256, {"cmp": 0, "aperture": 3, "taper": 0}),
+    "aperture_0": (6, 0.3, 0.05, 7, 0.3, 0.05, 256, {"cmp": 0, "aperture": 0, "taper": 0}),
+    # the default taper (10) gives a lone pair at the aperture edge weight 0
+    "aperture_0_tapered": (6, 0.3, 0.05, 7, 0.3, 0.05, 256, {"cmp": 0, "aperture": 0}),
+    "short_traces": (5, 0.3, 0.05, 6, 0.3, 0.05, 7, {"cmp": 0, "aperture": 2}),
+    "taper_over_aperture": (8, 0.3, 0.05, 9, 0.3, 0.05, 256,
+                            {"cmp": 0, "aperture": 3, "taper": 9, "data_taper": 50}),
+    "cmp1_tie_jump": (10, 0.3, 0.025, 8, 0.0, 0.05, 256, {"cmp": 1, "aperture": 3}),
+    "cmp1_tie_coarse": (8, 0.05, 0.1, 14, 0.0, 0.05, 256, {"cmp": 1, "aperture": 6}),
+}
+
+
+@pytest.mark.parametrize("name", sorted(EDGE))
+def test_edge_geometries_against_the_oracle(name):
+    """Degenerate axes, zero aperture, traces shorter than the filter, tapers
+    wider than the aperture, and cmp=1 geometries whose regrouping indices
+    fall on exact .5 ties (the GPU plans them with the reference's float
+    arithmetic, as the float oracle does: same traces, so the two agree to
+    rounding)."""
+    ns, s0, ds, nh, h0, dh, nt, kv = EDGE[name]
+    rng = np.random.default_rng(abs(hash(name)) % 2**32)
+    data = rng.normal(size=(ns, nh, nt)).astype(np.float32)
+    ax = [[nt, 0.0, 0.004], [nh, h0, dh], [ns, s0, ds]]
+    g = GOLD["green"]
+    print(f"datum edge case {name}", flush=True)
+    out = _dat().datum(kv, data, ax, g, POS, g, POS, SDATUM)
+    ref32 = D.datum(kv, data, ax, g, POS, g, POS, SDATUM, precision="f32")
+    assert np.isfinite(out).all()
+    scale = float(np.abs(data).max())
+    if np.abs(ref32).max() > 1e-3 * scale:
+        assert O.rel_l2(out, ref32) < TOL
+    else:
+        # (almost) all energy is delayed out of the traces: what is left is
+        # rounding-level, held in absolute terms
+        assert np.abs(out - ref32).max() <= 1e-6 * scale
+
+
 def test_repeat_runs_are_bitwise_and_stats():
     dat = _dat()
     kv, data, ax = _case("cmp0")
