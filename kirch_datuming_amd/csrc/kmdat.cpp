@@ -81,6 +81,7 @@ struct kmdat_ctx {
     hipEvent_t ev[8] = {};
     kmdat_stats st{};
     bool mfma = false;  // KMDAT_MFMA=1: the matrix-core banded kernel
+    int super_o = 16, super_b = 8;  // KMDAT_SUPER=o,b: tile grouping of the VALU kernel
 };
 
 static int fail(kmdat_ctx *c, int code, const char *fmt, ...)
@@ -597,6 +598,13 @@ int kmdat_create(kmdat_ctx **out, const kmdat_config *cfg)
     static std::once_flag once;
     std::call_once(once, [] { rocfft_setup(); });
     if (const char *v = getenv("KMDAT_MFMA")) c->mfma = atoi(v) != 0;
+    if (const char *v = getenv("KMDAT_SUPER")) {
+        int so = 1, sb = 1;
+        if (sscanf(v, "%d,%d", &so, &sb) == 2 && so >= 1 && sb >= 1) {
+            c->super_o = so;
+            c->super_b = sb;
+        }
+    }
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) HIPCHK(c, hipEventCreate(&e));
     return KMDAT_OK;
@@ -718,6 +726,8 @@ static int apply_op(kmdat_ctx *c, int k, const float2 *in, float2 *out)
         a.in_i = op.in_i * nw;
         a.out_b = op.out_b * nw;
         a.out_o = op.out_o * nw;
+        a.super_o = c->super_o;
+        a.super_b = c->super_b;
         HIPCHK(c, (hipError_t)(c->mfma ? launch_band_mfma(a, c->stream)
                                         : launch_band(a, c->stream)));
     } else {

@@ -22,6 +22,7 @@ struct BandArgs {
     const float *pw;
     int n_o, n_i, n_b, nb, aper, nw;
     size_t in_b, in_i, out_b, out_o;
+    int super_o = 16, super_b = 8;  // tile grouping of the VALU kernel (1 x 1: output-fastest)
 };
 
 struct PairRec {

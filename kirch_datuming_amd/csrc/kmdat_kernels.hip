@@ -175,8 +175,15 @@ __global__ __launch_bounds__(256) void k_band(const BandArgs a)
     const int wb = wave % WB, wf = wave / WB;
     const int nbt = (a.n_b + TB * WB - 1) / (TB * WB), not_ = (a.n_o + TO - 1) / TO;
     const int tile = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int ot = tile % not_, rest = tile / not_;
-    const int bt = rest % nbt, chunk = rest / nbt;
+    // super-tiles of so output tiles x sb batch tiles (output tiles fastest
+    // inside): the blocks an XCD runs together share table rows and input rows
+    const int so = a.super_o, sb = a.super_b;
+    const int nog = (not_ + so - 1) / so, nbg = (nbt + sb - 1) / sb;
+    const int per_chunk = nog * nbg * so * sb;
+    const int chunk = tile / per_chunk, r = tile % per_chunk;
+    const int grp = r / (so * sb), w_ = r % (so * sb);
+    const int ot = (grp % nog) * so + w_ % so, bt = (grp / nog) * sb + w_ / so;
+    if (ot >= not_ || bt >= nbt) return;
     const int iw = (chunk * WF + wf) * 64 + lane;
     const bool ok = iw < a.nw;
     const int iwc = ok ? iw : a.nw - 1;  // keeps the tail lanes' loads in bounds
@@ -391,8 +398,10 @@ template <int TB, int TO, int WB>
 static int launch_band_t(const BandArgs &a, hipStream_t st)
 {
     constexpr int WF = 4 / WB;
-    const size_t nblk = (size_t)((a.n_o + TO - 1) / TO) *
-                        ((a.n_b + TB * WB - 1) / (TB * WB)) * ((a.nw + 64 * WF - 1) / (64 * WF));
+    const int not_ = (a.n_o + TO - 1) / TO, nbt = (a.n_b + TB * WB - 1) / (TB * WB);
+    const int so = a.super_o, sb = a.super_b;
+    const size_t nblk = (size_t)((not_ + so - 1) / so) * ((nbt + sb - 1) / sb) * so * sb *
+                        ((a.nw + 64 * WF - 1) / (64 * WF));
     if (nblk == 0) return 0;
     if (nblk > 0x7fffffffu) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL((k_band<TB, TO, WB>), dim3((unsigned)nblk), dim3(256), 0, st, a);
@@ -413,8 +422,10 @@ int launch_band(const BandArgs &a, void *stream)
 {
     // 8 x 4 tile per wave, the block's four waves along the batch (measured on
     // the 768 x 768 datuming workload: 148 ms per side, against 175 ms with the
-    // waves along frequency, 265 ms with 8 x 8, 174 ms with 16 x 4, and 207 ms
-    // for an LDS-staged 32 x 32 block tile at two waves per SIMD)
+    // waves along frequency, 265 ms with 8 x 8, 174 ms with 16 x 4, 207 ms for
+    // an LDS-staged 32 x 32 block tile at two waves per SIMD, and 187 ms with
+    // the summation loop unrolled twice, three waves per SIMD); super-tiles of
+    // 16 x 8 tiles 145 ms (8 x 8: 146, 32 x 4: 149, output-fastest: 148)
     return launch_band_t<8, 4, 4>(a, (hipStream_t)stream);
 }
 
