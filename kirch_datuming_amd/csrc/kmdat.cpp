@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -266,6 +267,8 @@ static int plan_receiver(kmdat_ctx *c)
                 q.w = w;
                 op.pairs.push_back(q);
             }
+            if (op.pairs.size() > (size_t)INT32_MAX)
+                return fail(c, KMDAT_E_DIM, "more than 2^31 receiver-side summation pairs");
             op.rowptr[(size_t)o + 1] = (int)op.pairs.size();
         }
     }
@@ -373,6 +376,8 @@ static int plan_source(kmdat_ctx *c)
     size_t tot = 0;
     for (size_t o = 0; o < rows.size(); o++) {
         tot += rows[o].size();
+        if (tot > (size_t)INT32_MAX)
+            return fail(c, KMDAT_E_DIM, "more than 2^31 source-side summation pairs");
         op.rowptr[o + 1] = (int)tot;
     }
     op.pairs.reserve(tot);
@@ -654,6 +659,8 @@ int kmdat_set_data(kmdat_ctx *c, const float *data, int nt, float dt, int nh, fl
 {
     if (!c || !data) return KMDAT_E_PARAM;
     if (nt < 1 || nh < 1 || ns < 1) return fail(c, KMDAT_E_DIM, "Input must be 3D.");
+    if ((long long)ns * nh > INT32_MAX)
+        return fail(c, KMDAT_E_DIM, "more than 2^31 traces in the cube");
     if (!(dt > 0.0f)) return fail(c, KMDAT_E_PARAM, "time sampling must be > 0");
     if (c->cfg.cmp == 1 && (!(dh != 0.0f) || !(ds != 0.0f)))
         return fail(c, KMDAT_E_PARAM, "cmp=1 needs nonzero shot and offset spacings");
