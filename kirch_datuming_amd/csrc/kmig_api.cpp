@@ -92,6 +92,8 @@ struct kmig_ctx {
     size_t tmpd_cap = 0;
     float *wap = nullptr;
     size_t wap_cap = 0;
+    float *part = nullptr;  // shot groups' partial CIGs [KMIG_MAX_SPLIT][ng][nzx]
+    size_t part_cap = 0;
     int *rslot = nullptr;
     size_t rslot_cap = 0;
     bool rcv_invariant = false;     // every shot uses receiver slot rslot[ih]
@@ -963,6 +965,7 @@ void kmig_destroy(kmig_ctx *c)
     dfree(c->pairs);
     dfree(c->tmpd);
     dfree(c->wap);
+    dfree(c->part);
     dfree(c->rslot);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1208,6 +1211,14 @@ int kmig_run(kmig_ctx *c, int sb, int se)
             return fail(c, KMIG_E_PARAM, "output axis of %d samples does not fit in LDS", ng);
         if (nsb > 0 && resident) {
             MigResArgs R = res_args(c, b.s0, nsb, ng);
+            if (nsb >= 16) {
+                // shot groups for the launch's tail (the launcher picks the count)
+                if ((rc = ensure(c, &c->part, &c->part_cap,
+                                 (size_t)KMIG_MAX_SPLIT * ng * c->nzx)))
+                    return rc;
+                R.part = c->part;
+                R.part_splits = KMIG_MAX_SPLIT;
+            }
             if (p.tile <= 0) {
                 if ((rc = autotune(c, R))) return rc;
                 tile = c->tuned_tile;

@@ -171,6 +171,13 @@ struct MigResArgs {
     int sbnd_stride, nrslots;    // shots per sbnd row, receiver slots
     float *spray;                // adj=0: [ns][nh][nt] traces the image is sprayed into
     float *cig;
+    // Shot groups: the launch runs nsplit blocks per tile, group sg taking
+    // batch-local shots [sg nshots / nsplit, (sg + 1) nshots / nsplit) and
+    // writing its partial CIG to part[sg] ([nsplit][ng][nzx], group 0 starting
+    // from the CIG); the partials are then summed into cig in group order.
+    // part_splits = groups part holds (0: no buffer, one group).
+    float *part;
+    int part_splits;
     int c_int;                   // integer part of (tau - t0)/dt
     float c_frac;                // fractional part of (tau - t0)/dt
     float aal;
@@ -190,6 +197,9 @@ struct MigResArgs {
 
 // wap entries past the last trace: at least the largest receiver chunk (R)
 constexpr int RES_WAP_PAD = 64;
+
+// Most shot groups of one resident launch.
+constexpr int KMIG_MAX_SPLIT = 8;
 
 constexpr int KMIG_MAX_PROBLEMS = 8;
 

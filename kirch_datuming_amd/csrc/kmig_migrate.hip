@@ -11,7 +11,9 @@
 // shots over several launches keeps that order too.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "kmig_internal.h"
 
@@ -418,7 +420,8 @@ __device__ __forceinline__ void wave_lds_sync()
 // only) and every accepted update sprays its six weighted samples into the
 // traces of a.spray instead of reading its taps.
 template <int TZ, int TX, int R, int G, int Fmt, bool Model = false>
-__device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, float *acc)
+__device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, float *acc,
+                                             int sg = 0, int nsplit = 1)
 {
     constexpr int P = TZ * TX;
     const int lid = threadIdx.x;
@@ -432,8 +435,11 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
     float *my = acc + lid;
     // ng bins and a spare row ng, which takes the zero second share of
     // updates in the last bin, so both bins of an update are one LDS pair
+    // the batch's earlier sums enter through shot group 0 only
     for (int g = 0; g <= a.ng; g++)
-        my[g * P] = valid && g < a.ng ? a.cig[(size_t)g * a.nzx + i] : 0.0f;
+        my[g * P] = valid && g < a.ng && sg == 0 ? a.cig[(size_t)g * a.nzx + i] : 0.0f;
+    const int jb = (int)((long)sg * a.nshots / nsplit);
+    const int je = (int)((long)(sg + 1) * a.nshots / nsplit);
     const float zf = (a.nz > 1) ? (float)iz / (float)(a.nz - 1) : 0.0f;
     const float cutoff = a.oazmin + zf * (a.oazmax - a.oazmin);
     // depth mute: full weight up to cutoff, the ramp below cutoff_hi, zero
@@ -530,17 +536,17 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
         };
         auto next_shot = [&](int j) {
             if (cull)
-                while (j < a.nshots && culled(j)) j++;
+                while (j < je && culled(j)) j++;
             return j;
         };
-        int js = next_shot(0);
+        int js = next_shot(jb);
         typename RawOf<Fmt>::type nxt{};
-        if (js < a.nshots) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
-        while (js < a.nshots) {
+        if (js < je) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
+        while (js < je) {
             const PackedTuple st = decode<Fmt>(nxt);
             const int jc = js;
             js = next_shot(js + 1);
-            if (js < a.nshots) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
+            if (js < je) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
             const bool live = valid && (st.iw & 0xFFFFu) != 0u;
             if (__ballot(live) == 0) continue;
             const int s_int8 = (((int)st.iw >> 16) + a.c_int) << 3;
@@ -697,15 +703,30 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             }
         }
     }
-    if (!Model && valid)
-        for (int g = 0; g < a.ng; g++) a.cig[(size_t)g * a.nzx + i] = my[g * P];
+    if (!Model && valid) {
+        float *dst = nsplit > 1 ? a.part + (size_t)sg * a.ng * a.nzx : a.cig;
+        for (int g = 0; g < a.ng; g++) dst[(size_t)g * a.nzx + i] = my[g * P];
+    }
+}
+
+// cig = part[0] + part[1] + ... (group order), the shot groups' partial CIGs
+__global__ __launch_bounds__(256) void k_sum_parts(const float *__restrict__ part,
+                                                   float *__restrict__ cig, size_t n, int nsplit)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        float v = part[e];
+        for (int s = 1; s < nsplit; s++) v += part[(size_t)s * n + e];
+        cig[e] = v;
+    }
 }
 
 template <int TZ, int TX, int R, int G, bool Probe, int Fmt>
 __global__ __launch_bounds__(TZ *TX, 3) void k_migrate_ph(const MigResArgs a)
 {
     extern __shared__ float acc[];
-    migrate_tile<TZ, TX, R, G, Fmt>(a, xcd_tile(blockIdx.x, gridDim.x), acc);
+    migrate_tile<TZ, TX, R, G, Fmt>(a, xcd_tile(blockIdx.x, gridDim.x), acc, (int)blockIdx.y,
+                                    (int)gridDim.y);
 }
 
 template <int TZ, int TX, int R, int G>
@@ -775,8 +796,26 @@ static int launch_ph_t(const MigResArgs &a, hipStream_t st)
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL((k_migrate_ph<TZ, TX, R, G, Probe, Fmt>), dim3((unsigned)(ntz * ntx)), dim3(TZ * TX),
-                       lds, st, a);
+    // Shot groups: all points x all shots in one block per tile leaves about
+    // 2.4 rounds of long blocks on the salt workload (LDS allows five 16 x 8
+    // tiles per CU), and the last, partial round idles most of the GPU; a
+    // few thousand blocks per round keep the tail short.  KMIG_SPLIT forces a
+    // count (tests).
+    const int ntiles = ntz * ntx;
+    int nsplit = 1;
+    if (a.part_splits > 1) {
+        nsplit = std::min(a.part_splits, std::max(1, (10240 + ntiles - 1) / ntiles));
+        nsplit = std::min(nsplit, std::max(1, a.nshots / 8));
+        if (const char *v = std::getenv("KMIG_SPLIT"))
+            nsplit = std::min(std::max(1, std::atoi(v)), std::min(a.part_splits, a.nshots));
+    }
+    hipLaunchKernelGGL((k_migrate_ph<TZ, TX, R, G, Probe, Fmt>), dim3((unsigned)ntiles,
+                       (unsigned)nsplit), dim3(TZ * TX), lds, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nsplit == 1) return (int)e;
+    const size_t n = (size_t)a.ng * a.nzx;
+    hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<size_t>((n + 255) / 256, 65536)),
+                       dim3(256), 0, st, a.part, a.cig, n, nsplit);
     return (int)hipGetLastError();
 }
 

@@ -179,11 +179,13 @@ def test_resident_variant_is_used_and_matches_stream():
         kmig.migrate(dict(kv1, variant="resident"), *_args(run1))
 
 
-def test_batch_graph_matches_individual_runs_and_replays():
+def test_batch_graph_matches_individual_runs_and_replays(monkeypatch):
     """Five small depth layers in one graph-captured step: each layer's CIG is
-    bitwise the one its own context computes with the same preset, and every
-    graph replay (with eager work and syncs in between) repeats it."""
+    bitwise the one its own context computes with the same preset (one shot
+    group, as the graph's multi-problem launch runs), and every graph replay
+    (with eager work and syncs in between) repeats it."""
     from kirch_datuming_amd import configs, kmig
+    monkeypatch.setenv("KMIG_SPLIT", "1")
     ws = configs.layers(threads=8, scale=0.05)
     ms = []
     singles = []
