@@ -1482,6 +1482,12 @@ int batch_capture(kmig_batch *b, int tile)
     for (int q = 0; q < n; q++) {
         kmig_ctx *c = b->ctx[order[q]];
         m.prob[q] = res_args(c, 0, c->ns, ng);
+        if (c->ns >= 16) {  // shot groups (the launcher picks the count)
+            const int rc0 = ensure(c, &c->part, &c->part_cap, (size_t)KMIG_MAX_SPLIT * ng * c->nzx);
+            if (rc0) return bfail(b, rc0, "%s", c->err.c_str());
+            m.prob[q].part = c->part;
+            m.prob[q].part_splits = KMIG_MAX_SPLIT;
+        }
         m.tile0[q + 1] = m.tile0[q] + migrate_tiles(c->nz, c->nx, tile);
         b->updates += (double)c->ns * c->nh * (double)c->nzx;
     }

@@ -199,7 +199,7 @@ struct MigResArgs {
 constexpr int RES_WAP_PAD = 64;
 
 // Most shot groups of one resident launch.
-constexpr int KMIG_MAX_SPLIT = 8;
+constexpr int KMIG_MAX_SPLIT = 16;
 
 constexpr int KMIG_MAX_PROBLEMS = 8;
 

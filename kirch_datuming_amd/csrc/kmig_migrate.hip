@@ -709,6 +709,12 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
     }
 }
 
+// Blocks a resident launch aims at when it splits the shots into groups:
+// about 32 rounds of five 16 x 8 tiles per CU (measured on the salt workload:
+// 4 groups 128 ms, 8 121 ms, 16 115 ms, 24 115 ms, 32 116 ms; groups keep at
+// least 8 shots, as more groups of fewer shots lose on the small workload).
+constexpr int kSplitBlocks = 40960;
+
 // cig = part[0] + part[1] + ... (group order), the shot groups' partial CIGs
 __global__ __launch_bounds__(256) void k_sum_parts(const float *__restrict__ part,
                                                    float *__restrict__ cig, size_t n, int nsplit)
@@ -746,7 +752,8 @@ __global__ __launch_bounds__(TZ *TX, 3) void k_migrate_multi(const MigMultiArgs 
     extern __shared__ float acc[];
     int p = 0;
     while (p + 1 < m.nprob && (int)blockIdx.x >= m.tile0[p + 1]) p++;
-    migrate_tile<TZ, TX, R, G, Fmt>(m.prob[p], (int)blockIdx.x - m.tile0[p], acc);
+    migrate_tile<TZ, TX, R, G, Fmt>(m.prob[p], (int)blockIdx.x - m.tile0[p], acc,
+                                    (int)blockIdx.y, (int)gridDim.y);
 }
 
 static void tile_dims(int tile, int *tz, int *tx)
@@ -804,7 +811,7 @@ static int launch_ph_t(const MigResArgs &a, hipStream_t st)
     const int ntiles = ntz * ntx;
     int nsplit = 1;
     if (a.part_splits > 1) {
-        nsplit = std::min(a.part_splits, std::max(1, (10240 + ntiles - 1) / ntiles));
+        nsplit = std::min(a.part_splits, std::max(1, (kSplitBlocks + ntiles - 1) / ntiles));
         nsplit = std::min(nsplit, std::max(1, a.nshots / 8));
         if (const char *v = std::getenv("KMIG_SPLIT"))
             nsplit = std::min(std::max(1, std::atoi(v)), std::min(a.part_splits, a.nshots));
@@ -914,9 +921,31 @@ static int launch_multi_t(const MigMultiArgs &m, hipStream_t st)
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL((k_migrate_multi<TZ, TX, R, G, Fmt>), dim3((unsigned)m.tile0[m.nprob]),
-                       dim3(TZ * TX), lds, st, m);
-    return (int)hipGetLastError();
+    // shot groups as in launch_ph_t, one count for every problem (bounded by
+    // the smallest problem's shots and partial buffers)
+    const int ntiles = m.tile0[m.nprob];
+    int splits = KMIG_MAX_SPLIT, min_shots = 1 << 30;
+    for (int q = 0; q < m.nprob; q++) {
+        splits = std::min(splits, m.prob[q].part_splits);
+        min_shots = std::min(min_shots, m.prob[q].nshots);
+    }
+    int nsplit = 1;
+    if (splits > 1) {
+        nsplit = std::min(splits, std::max(1, (kSplitBlocks + ntiles - 1) / ntiles));
+        nsplit = std::min(nsplit, std::max(1, min_shots / 8));
+        if (const char *v = std::getenv("KMIG_SPLIT"))
+            nsplit = std::min(std::max(1, std::atoi(v)), std::min(splits, min_shots));
+    }
+    hipLaunchKernelGGL((k_migrate_multi<TZ, TX, R, G, Fmt>), dim3((unsigned)ntiles,
+                       (unsigned)nsplit), dim3(TZ * TX), lds, st, m);
+    hipError_t e = hipGetLastError();
+    for (int q = 0; q < m.nprob && e == hipSuccess && nsplit > 1; q++) {
+        const size_t n = (size_t)m.prob[q].ng * m.prob[q].nzx;
+        hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)std::min<size_t>((n + 255) / 256, 65536)),
+                           dim3(256), 0, st, m.prob[q].part, m.prob[q].cig, n, nsplit);
+        e = hipGetLastError();
+    }
+    return (int)e;
 }
 
 // presets 9 (16x8 points, R = 8, G = 4), 10 (16x8, R = 16, G = 4), 11 (16x8,
