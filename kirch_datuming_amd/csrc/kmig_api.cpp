@@ -54,6 +54,7 @@ struct kmig_ctx {
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // trace preparation, overlapped with the tuple preparation
     int nz = 0, nx = 0;
     float z0 = 0, dz = 0, x0 = 0, dx = 0;
     size_t nzx = 0;
@@ -917,7 +918,8 @@ int kmig_create(kmig_ctx **out, const kmig_params *p)
         return fail(c, KMIG_E_DEVICE, "no HIP device is available");
     }
     (void)hipGetDevice(&c->device);
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
         *out = c;
         return fail(c, KMIG_E_DEVICE, "hipStreamCreate failed");
     }
@@ -968,6 +970,7 @@ void kmig_destroy(kmig_ctx *c)
     dfree(c->part);
     dfree(c->rslot);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1094,6 +1097,9 @@ int kmig_set_data(kmig_ctx *c, const float *data, int nt, float t0, float dt, in
     if (rc) return fail(c, rc, "%s", err);
     const size_t n = (size_t)nt * nh * ns;
     size_t cap = c->raw_cap;
+    // a run still in flight may read the traces being replaced
+    HIPCHK(c, hipStreamSynchronize(c->stream2));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     if ((rc = ensure(c, &c->raw, &c->raw_cap, n))) return rc;
     if ((rc = ensure(c, &c->prep, &cap, n))) return rc;
     HIPCHK(c, hipMemcpy(c->raw, data, n * sizeof(float), hipMemcpyHostToDevice));
@@ -1183,12 +1189,18 @@ int kmig_run(kmig_ctx *c, int sb, int se)
     if ((rc = plan_if_needed(c, sb, se))) return rc;
     const bool resident = c->variant_used == KMIG_VARIANT_RESIDENT;
     if ((rc = ensure_outputs(c, sb, se))) return rc;
+    // The trace preparation (data) runs on stream2 while the tuple preparation
+    // (tables) runs on the context stream; the first migrate launch waits for
+    // both.  stream2 starts after everything earlier on the context stream,
+    // so a new run never rewrites traces an earlier launch still reads.
     hipEvent_t e0 = next_event(c);
     HIPCHK(c, hipEventRecord(e0, c->stream));
-    if ((rc = enqueue_trace_prep(c, c->stream, sb, se))) return rc;
+    HIPCHK(c, hipStreamWaitEvent(c->stream2, e0, 0));
+    if ((rc = enqueue_trace_prep(c, c->stream2, sb, se))) return rc;
     hipEvent_t e1 = next_event(c);
-    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventRecord(e1, c->stream2));
     c->ev_kind.push_back(0);
+    bool joined = false;
 
     size_t spec_s_off = 0;
     int launches = 0;
@@ -1209,6 +1221,10 @@ int kmig_run(kmig_ctx *c, int sb, int se)
         }
         if (migrate_lds_bytes(ng, tile) > 160 * 1024)
             return fail(c, KMIG_E_PARAM, "output axis of %d samples does not fit in LDS", ng);
+        if (!joined) {
+            HIPCHK(c, hipStreamWaitEvent(c->stream, e1, 0));
+            joined = true;
+        }
         if (nsb > 0 && resident) {
             MigResArgs R = res_args(c, b.s0, nsb, ng);
             if (nsb >= 16) {
@@ -1234,6 +1250,7 @@ int kmig_run(kmig_ctx *c, int sb, int se)
         c->ev_kind.push_back(2);
         spec_s_off += (size_t)nsb;
     }
+    if (!joined) HIPCHK(c, hipStreamWaitEvent(c->stream, e1, 0));
     c->st = kmig_stats{};
     c->st.updates = (double)(se - sb) * (double)c->nh * (double)c->nzx;
     c->st.migrate_launches = launches;
