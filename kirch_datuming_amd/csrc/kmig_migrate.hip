@@ -435,9 +435,10 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
     float *my = acc + lid;
     // ng bins and a spare row ng, which takes the zero second share of
     // updates in the last bin, so both bins of an update are one LDS pair
-    // the batch's earlier sums enter through shot group 0 only
+    // the batch's earlier sums enter through shot group 0 only; in the
+    // transpose every group reads the image
     for (int g = 0; g <= a.ng; g++)
-        my[g * P] = valid && g < a.ng && sg == 0 ? a.cig[(size_t)g * a.nzx + i] : 0.0f;
+        my[g * P] = valid && g < a.ng && (Model || sg == 0) ? a.cig[(size_t)g * a.nzx + i] : 0.0f;
     const int jb = (int)((long)sg * a.nshots / nsplit);
     const int je = (int)((long)(sg + 1) * a.nshots / nsplit);
     const float zf = (a.nz > 1) ? (float)iz / (float)(a.nz - 1) : 0.0f;
@@ -739,7 +740,8 @@ template <int TZ, int TX, int R, int G>
 __global__ __launch_bounds__(TZ *TX, 3) void k_model_ph(const MigResArgs a)
 {
     extern __shared__ float acc[];
-    migrate_tile<TZ, TX, R, G, 0, true>(a, xcd_tile(blockIdx.x, gridDim.x), acc);
+    migrate_tile<TZ, TX, R, G, 0, true>(a, xcd_tile(blockIdx.x, gridDim.x), acc, (int)blockIdx.y,
+                                        (int)gridDim.y);
 }
 
 // Several independent problems (e.g. the depth layers of a recursive
@@ -884,8 +886,15 @@ static int launch_model_t(const MigResArgs &a, hipStream_t st)
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL((k_model_ph<TZ, TX, R, G>), dim3((unsigned)(ntz * ntx)), dim3(TZ * TX), lds,
-                       st, a);
+    // shot groups as in the migration (no partial buffers: each group sprays
+    // into the traces of its own shots)
+    const int ntiles = ntz * ntx;
+    int nsplit = std::min(KMIG_MAX_SPLIT, std::max(1, (kSplitBlocks + ntiles - 1) / ntiles));
+    nsplit = std::min(nsplit, std::max(1, a.nshots / 8));
+    if (const char *v = std::getenv("KMIG_SPLIT"))
+        nsplit = std::min(std::max(1, std::atoi(v)), std::min(KMIG_MAX_SPLIT, a.nshots));
+    hipLaunchKernelGGL((k_model_ph<TZ, TX, R, G>), dim3((unsigned)ntiles, (unsigned)nsplit),
+                       dim3(TZ * TX), lds, st, a);
     return (int)hipGetLastError();
 }
 
