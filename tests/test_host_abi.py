@@ -77,6 +77,10 @@ def test_defaults_are_the_reference_defaults():
     p = kmig.parse_params({"angle_grad_radius": 0, "angle_smooth": -3,
                            "aperture_trace_taper": -1, "cmp": "0.9"}, 3, 0, 1)
     assert (p.angle_grad_radius, p.angle_smooth, p.aperture_trace_taper, p.cmp) == (1, 0, 0, 0)
+    for name, code in kmig.PRECISION.items():
+        assert kmig.parse_params({"precision": name.upper()}, 3, 0, 1).precision == code
+    with pytest.raises(kmig.KmigError, match="Supported are fp32, fp16, bf16, u16"):
+        kmig.parse_params({"precision": "fp8"}, 3, 0, 1)
 
 
 def test_parameter_errors_carry_reference_messages():
