@@ -487,6 +487,7 @@ MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
     const double ci = std::floor(cst);
     R.c_int = (int)ci;
     R.c_frac = (float)(cst - ci);
+    R.t_cull = R.nt - 2 - R.c_int;
     R.aal = p.antialias;
     R.wrap = p.half_angle ? 180.0f : 360.0f;
     R.inv_dg = 1.0f / p.dg;

@@ -137,9 +137,11 @@ struct SlotScale {
 // Bounds of the packed directions th (output-angle degrees) of one slot over
 // one cell of BND_CZ x BND_CX image points, counting only points of nonzero
 // tuple weight: lo = floor(min th * BND_Q), hi = ceil(max th * BND_Q), exact
-// integers (|th| <= 180).  A cell with no such point has lo = BND_EMPTY > hi.
+// integers (|th| <= 180); tmin = the smallest integer sample (iw >> 16) of
+// those points.  A cell with no such point has lo = tmin = BND_EMPTY > hi.
 struct ThBound {
     int lo, hi;
+    int tmin, pad;
 };
 constexpr int BND_CZ = 8, BND_CX = 4;
 constexpr float BND_Q = 65536.0f;
@@ -179,6 +181,9 @@ struct MigResArgs {
     float *part;
     int part_splits;
     int c_int;                   // integer part of (tau - t0)/dt
+    int t_cull;                  // nt - 2 - c_int: a group whose smallest source plus
+                                 // receiver integer samples reach it has every
+                                 // update's last tap past the trace
     float c_frac;                // fractional part of (tau - t0)/dt
     float aal;
     float wrap;                  // 360 (full opening angle) or 180 (half)

@@ -488,7 +488,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
 
     for (int r0 = 0; r0 < a.nh; r0 += R) {
         // bounds of the chunk's receivers over the wave's cells
-        int rlo = BND_EMPTY, rhi = -BND_EMPTY;
+        int rlo = BND_EMPTY, rhi = -BND_EMPTY, rtm = BND_EMPTY;
         if (cull) {
             for (int r = 0; r < R; r++) {
                 const int ih = r0 + r;
@@ -499,6 +499,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                     const ThBound b = a.rbnd[(size_t)cell[c] * a.nrslots + slot];
                     rlo = min(rlo, b.lo);
                     rhi = max(rhi, b.hi);
+                    rtm = min(rtm, b.tmin);
                 }
             }
             if (rlo > rhi) continue;  // no receiver weight at any point of the wave
@@ -520,14 +521,21 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
         // costs a few scalar instructions (its tuple is never loaded).  The
         // bounds rows are cell-major, so consecutive shots share cache lines.
         auto culled = [&](int j) -> bool {
-            int slo = BND_EMPTY, shi = -BND_EMPTY;
+            int slo = BND_EMPTY, shi = -BND_EMPTY, stm = BND_EMPTY;
 #pragma unroll
             for (int c = 0; c < NC; c++) {
                 const ThBound b = a.sbnd[(size_t)cell[c] * a.sbnd_stride + j];
                 slo = min(slo, b.lo);
                 shi = max(shi, b.hi);
+                stm = min(stm, b.tmin);
             }
             if (slo > shi) return true;  // no source weight at any point of the wave
+            // Time window: an update's last tap is at s_int + r_int + c_int +
+            // floor(ff + delta + 1) >= s_int + r_int + c_int + 1 (ff >= 0,
+            // delta >= 0), so where the group's smallest integer samples reach
+            // nt - 2 - c_int every tap pair lies past the trace's last pair
+            // and every update is rejected.
+            if (stm + rtm >= a.t_cull) return true;
             // th_s - th_r lies in [dlo, dhi]; the folded angle min(|d|,
             // wrap - |d|) is smallest at an end of the |d| interval
             const int dlo = slo - rhi, dhi = shi - rlo;

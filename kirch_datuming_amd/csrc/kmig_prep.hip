@@ -921,36 +921,43 @@ __global__ __launch_bounds__(64) void k_th_bounds(const PackedTuple *__restrict_
     const int iz = zb * 64 + lane;
     const size_t nzx = (size_t)nz * nx;
     float lo = INFINITY, hi = -INFINITY;
+    int tmin = BND_EMPTY;
     if (iz < nz) {
         for (int jx = 0; jx < BND_CX; jx++) {
             const int ix = cx * BND_CX + jx;
             if (ix >= nx) break;
             const size_t i = slot * nzx + (size_t)ix * nz + iz;
             float th;
-            uint32_t w;
+            uint32_t iwv;
             if constexpr (Half) {
                 th = __uint_as_float(pk16[i].th);
-                w = pk16[i].iw & 0xFFFFu;
+                iwv = pk16[i].iw;
             } else {
                 th = pk[i].th;
-                w = pk[i].iw & 0xFFFFu;
+                iwv = pk[i].iw;
             }
+            const uint32_t w = iwv & 0xFFFFu;
             if (w != 0u) {
                 lo = fminf(lo, th);
                 hi = fmaxf(hi, th);
+                tmin = min(tmin, (int)iwv >> 16);
             }
         }
     }
     for (int o = 1; o < BND_CZ; o <<= 1) {
         lo = fminf(lo, __shfl_xor(lo, o, 64));
         hi = fmaxf(hi, __shfl_xor(hi, o, 64));
+        tmin = min(tmin, __shfl_xor(tmin, o, 64));
     }
     const int cz = zb * (64 / BND_CZ) + lane / BND_CZ;
     if (lane % BND_CZ != 0 || cz >= ncz) return;
     ThBound b;
+    b.tmin = tmin;
+    b.pad = 0;
     if (lo > hi) {
         b.lo = BND_EMPTY;
         b.hi = -BND_EMPTY;
+        b.tmin = BND_EMPTY;
     } else {
         // |th| <= 180, so th * 2^16 is exact and below 2^24
         b.lo = (int)floorf(lo * BND_Q);

@@ -2,8 +2,9 @@
 
 The kernel skips a (shot, receiver chunk) group of a wave when per-cell bounds
 of the packed directions show that every opening angle of the group lies
-above the largest angle that receives weight; each skipped update would have
-added an exact zero.  The CIG with culling (default) is therefore bitwise the
+above the largest angle that receives weight, or when the per-cell smallest
+integer samples put every update's last tap past the end of the trace; each
+skipped update would have added an exact zero.  The CIG with culling (default) is therefore bitwise the
 CIG without it (KMIG_CULL=0) at the same preset, which these tests check on
 workloads where many groups are culled and many are partly rejected: the v(z)
 workload at its full geometry (61 shots x 201 receivers, 201 x 301 points),
@@ -38,6 +39,25 @@ def test_culling_is_bitwise_neutral(small, monkeypatch, tile, precision):
     on = _run(small, kv, monkeypatch, True)
     assert np.abs(off).max() > 0
     assert np.array_equal(off.view(np.uint32), on.view(np.uint32))
+
+
+@pytest.mark.parametrize("nt,tau", [(300, 0.0), (450, 0.1), (450, -0.04)])
+def test_time_window_culling_is_bitwise_neutral(small, monkeypatch, nt, tau):
+    """Short traces (and a time shift tau) put the deep points' taps past the
+    trace end, where whole groups are culled by the per-cell smallest
+    integer samples; the CIG is still bitwise the unculled one."""
+    from kirch_datuming_amd import kmig
+    args = list(small.args())
+    args[0] = np.ascontiguousarray(args[0][..., :nt])
+    args[1] = [[nt] + list(args[1][0][1:])] + [list(a) for a in args[1][1:]]
+    kv = dict(small.kv, variant="resident", tile=11, tau=tau)
+    print(f"cull: small workload cut to nt={nt}, tau={tau}", flush=True)
+    out = {}
+    for cull in (False, True):
+        monkeypatch.setenv("KMIG_CULL", "1" if cull else "0")
+        out[cull] = kmig.migrate(kv, *args)[0]
+    assert np.abs(out[False]).max() > 0
+    assert np.array_equal(out[False].view(np.uint32), out[True].view(np.uint32))
 
 
 def test_culling_with_other_angle_options(small, monkeypatch):
