@@ -539,6 +539,23 @@ __global__ __launch_bounds__(256) void k_pack16(const float *T, const float *A, 
 // tile border read a window clipped by the tile and are never used).
 constexpr int FZ = 64, FX = 16, FUSED_MAX_R = 4;
 
+// (k % w, k / w) for k = tid, tid + 256, tid + 512, ...: one division up
+// front, then steps of (256 % w, 256 / w) with a carry (the tile loops of
+// k_tuple_fused run over regions whose height w is known only at run time)
+struct TileWalk {
+    int l, m, dl, dm, w;
+    __device__ TileWalk(int tid, int w_) : l(tid % w_), m(tid / w_), dl(256 % w_), dm(256 / w_), w(w_) {}
+    __device__ void next()
+    {
+        l += dl;
+        m += dm;
+        if (l >= w) {
+            l -= w;
+            m++;
+        }
+    }
+};
+
 __device__ inline float box_sum(const float *buf, int stride, int l, int g, int r, int n_img,
                                 int l_lo, int l_hi)
 {
@@ -575,16 +592,18 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
     const size_t base = (size_t)slot * a.nzx;
     const int tid = threadIdx.x;
     if (!a.use_grad_files) {
-        for (int k = tid; k < w0z * w0x; k += blockDim.x) {
-            const int iz = z0 - H0 + k % w0z, ix = x0 - H0 + k / w0z;
+        TileWalk t(tid, w0z);
+        for (int k = tid; k < w0z * w0x; k += 256, t.next()) {
+            const int iz = z0 - H0 + t.l, ix = x0 - H0 + t.m;
             sT[k] = (iz >= 0 && iz < nz && ix >= 0 && ix < nx) ? o.T[base + (size_t)ix * nz + iz]
                                                                  : 0.0f;
         }
         __syncthreads();
     }
     // gradients on the region with halo H1
-    for (int k = tid; k < n1; k += blockDim.x) {
-        const int lz = k % w1z, lx = k / w1z;
+    TileWalk tg(tid, w1z);
+    for (int k = tid; k < n1; k += 256, tg.next()) {
+        const int lz = tg.l, lx = tg.m;
         const int iz = z0 - H1 + lz, ix = x0 - H1 + lx;
         float vx = 0.0f, vz = 0.0f;
         if (iz >= 0 && iz < nz && ix >= 0 && ix < nx) {
@@ -617,8 +636,9 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
     float *src_x = ax, *src_z = az, *dst_x = bx, *dst_z = bz;
     for (int pass = 0; rs > 0 && pass < 4; pass++) {
         const bool zdir = (pass % 2) == 0;
-        for (int k = tid; k < n1; k += blockDim.x) {
-            const int lz = k % w1z, lx = k / w1z;
+        TileWalk tp(tid, w1z);
+        for (int k = tid; k < n1; k += 256, tp.next()) {
+            const int lz = tp.l, lx = tp.m;
             const int iz = z0 - H1 + lz, ix = x0 - H1 + lx;
             if (!(iz >= 0 && iz < nz && ix >= 0 && ix < nx)) continue;
             if (zdir) {
@@ -635,7 +655,7 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
     }
     // weights and packing of the output tile
     const float dzabs = fabsf(a.dz);
-    for (int k = tid; k < FZ * FX; k += blockDim.x) {
+    for (int k = tid; k < FZ * FX; k += 256) {
         const int iz = z0 + k % FZ, ix = x0 + k / FZ;
         if (iz >= nz || ix >= nx) continue;
         const int l = (ix - x0 + H1) * w1z + (iz - z0 + H1);
