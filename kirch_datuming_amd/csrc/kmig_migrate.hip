@@ -404,6 +404,29 @@ __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int8, flo
     return q;
 }
 
+// Wave-wide integer min / max for full waves: four DPP steps inside each row
+// of 16 lanes (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
+// row_mirror: every lane ends with its row's extremum), then the four rows'
+// values read as scalars.  No LDS round trips, unlike a shuffle chain.
+__device__ __forceinline__ int wave_min_i32(int v)
+{
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, true));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, true));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, true));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, true));
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+__device__ __forceinline__ int wave_max_i32(int v)
+{
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, true));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, true));
+    return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
 // LDS ordering among the lanes of one wave: the wave's LDS operations run in
 // order, so only the compiler must keep them so.
 __device__ __forceinline__ void wave_lds_sync()
@@ -634,13 +657,14 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
 #pragma unroll
                     for (int k = 0; k < G; k++) {
                         const bool sp = q[k].ok & valid;
-                        int lo = sp ? (int)(q[k].om >> 3) : 0x7fffffff;
-                        int hi = sp ? (int)(q[k].op >> 3) + 1 : -1;
-                        for (int o = 32; o > 0; o >>= 1) {
-                            lo = min(lo, __shfl_xor(lo, o, 64));
-                            hi = max(hi, __shfl_xor(hi, o, 64));
-                        }
+                        // every lane of the wave runs this (uniform control
+                        // flow, full waves of 64)
+                        int lo = wave_min_i32(sp ? (int)(q[k].om >> 3) : 0x7fffffff);
+                        int hi = wave_max_i32(sp ? (int)(q[k].op >> 3) + 1 : -1);
                         if (hi < lo) continue;  // no lane sprays this trace
+                        // taps lie in [0, nt - 1]: the window stays inside the trace
+                        lo = max(lo, 0);
+                        hi = min(hi, a.nt - 1);
                         const float *cell = my + q[k].ig0 * P;
                         const float v =
                             q[k].coef * ((1.0f - q[k].w1) * cell[0] + q[k].w1 * cell[P]);
