@@ -462,6 +462,11 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
     // transpose every group reads the image
     for (int g = 0; g <= a.ng; g++)
         my[g * P] = valid && g < a.ng && (Model || sg == 0) ? a.cig[(size_t)g * a.nzx + i] : 0.0f;
+    if constexpr (Model) {
+        float *win = acc + (a.ng + 1) * P + (lid >> 6) * MODEL_WIN;
+        for (int e = lid & 63; e < MODEL_WIN; e += 64) win[e] = 0.0f;
+        wave_lds_sync();
+    }
     const int jb = (int)((long)sg * a.nshots / nsplit);
     const int je = (int)((long)(sg + 1) * a.nshots / nsplit);
     const float zf = (a.nz > 1) ? (float)iz / (float)(a.nz - 1) : 0.0f;
@@ -676,8 +681,8 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                         const float wp0 = -(1.0f - q[k].ftp) * v, wp1 = -q[k].ftp * v;
                         const int span = hi - lo + 1;
                         if (span <= MODEL_WIN) {
-                            for (int e = ln; e < span; e += 64) win[e] = 0.0f;
-                            wave_lds_sync();
+                            // the window is all zero here: cleared at the start
+                            // and by every flush
                             if (sp) {
                                 atomicAdd(win + (it - lo), w0);
                                 atomicAdd(win + (it + 1 - lo), w1);
@@ -689,6 +694,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                             wave_lds_sync();
                             for (int e = ln; e < span; e += 64) {
                                 const float sv = win[e];
+                                win[e] = 0.0f;
                                 if (sv != 0.0f) atomicAdd(tr + lo + e, sv);
                             }
                             wave_lds_sync();
