@@ -513,6 +513,13 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
         tq = __builtin_amdgcn_readfirstlane((int)ceilf(tl * BND_Q)) + 64;
     }
     const int wrapq = (int)(a.wrap * BND_Q);
+    // smallest source sample of the wave's cells over all shots: with a
+    // chunk's smallest receiver sample it bounds every group of the chunk
+    int sall = BND_EMPTY;
+    if (cull && a.scell_tmin) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) sall = min(sall, a.scell_tmin[cell[c]]);
+    }
 
     for (int r0 = 0; r0 < a.nh; r0 += R) {
         // bounds of the chunk's receivers over the wave's cells
@@ -531,6 +538,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 }
             }
             if (rlo > rhi) continue;  // no receiver weight at any point of the wave
+            if (sall + rtm >= a.t_cull) continue;  // every shot's group is past the trace
         }
         PackedTuple rt[R];
 #pragma unroll

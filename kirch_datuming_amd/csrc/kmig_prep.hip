@@ -986,6 +986,24 @@ __global__ __launch_bounds__(64) void k_th_bounds(const PackedTuple *__restrict_
     out[(size_t)(cx * ncz + cz) * nslots + slot] = b;
 }
 
+__global__ __launch_bounds__(64) void k_cell_tmin(const ThBound *__restrict__ bnd, int nslots,
+                                                  int *__restrict__ out)
+{
+    const ThBound *row = bnd + (size_t)blockIdx.x * nslots;
+    int m = BND_EMPTY;
+    for (int s = threadIdx.x; s < nslots; s += 64) m = min(m, row[s].tmin);
+    for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
+    if (threadIdx.x == 0) out[blockIdx.x] = m;
+}
+
+int launch_cell_tmin(const ThBound *bnd, int nslots, int ncell, int *out, void *stream)
+{
+    if (ncell <= 0) return 0;
+    hipLaunchKernelGGL(k_cell_tmin, dim3((unsigned)ncell), dim3(64), 0, (hipStream_t)stream, bnd,
+                       nslots, out);
+    return (int)hipGetLastError();
+}
+
 int launch_th_bounds(const PackedTuple *pk, const PackedTuple16 *pk16, int nslots, int nz,
                      int nx, ThBound *out, void *stream)
 {

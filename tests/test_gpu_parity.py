@@ -239,7 +239,7 @@ def test_fused_tuple_pass_is_bitwise_the_separate_passes(monkeypatch, radius, sm
     tax = [[nz, 0.0, d], [nx, 0.0, d], [npos, 0.0, dpos]]
     kv = dict(cmp=0, mode="angle", angle_min=0, angle_max=60, angle_step=1,
               angle_grad_radius=radius, angle_smooth=smooth, half_angle=half, aperture=60,
-              aperture_rmin=0.3, variant="resident")
+              aperture_rmin=0.3, variant="resident", tile=9)
     print(f"fused tuple pass: radius {radius} smooth {smooth} half {half}", flush=True)
     monkeypatch.setenv("KMIG_TUPLE_FUSED", "0")
     sep = _kmig().migrate(kv, data, data_axes, T, D, tax, T, D, tax)[0]
@@ -256,8 +256,10 @@ def test_fused_tuple_pass_is_bitwise_the_separate_passes(monkeypatch, radius, sm
 
 
 def test_fused_tuple_pass_with_gradient_files(monkeypatch):
+    # one preset for both runs: the autotuner picks by timing, and presets of
+    # other receiver-chunk sizes sum each point in another order
     kv, run, _ = gc.case("gradfiles")
-    kv = dict(kv, variant="resident")
+    kv = dict(kv, variant="resident", tile=9)
     monkeypatch.setenv("KMIG_TUPLE_FUSED", "0")
     sep = _kmig().migrate(kv, *_args(run), grads=run["grads"])[0]
     monkeypatch.setenv("KMIG_TUPLE_FUSED", "1")
