@@ -481,6 +481,8 @@ MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
     R.sbnd = cull ? c->bnd_s : nullptr;
     R.rbnd = cull ? c->bnd_r : nullptr;
     R.scell_tmin = cull ? c->cell_tmin : nullptr;
+    R.scell_nblk = (nshots + TMIN_BLK - 1) / TMIN_BLK;
+    R.scell_off = 0;
     R.ncz = (c->nz + BND_CZ - 1) / BND_CZ;
     R.ncx = (c->nx + BND_CX - 1) / BND_CX;
     R.sbnd_stride = nshots;
@@ -541,6 +543,7 @@ int autotune(kmig_ctx *c, const MigResArgs &full)
         if (full.stup16) R.stup16 = full.stup16 + (size_t)off * c->nzx;
         if (full.sscale) R.sscale = full.sscale + off;
         if (full.sbnd) R.sbnd = full.sbnd + off;  // rows keep their full stride
+        R.scell_off = full.scell_off + off;
     }
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
@@ -680,7 +683,9 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
                                  ((c->nx + BND_CX - 1) / BND_CX);
             if ((rc = ensure(c, &c->bnd_s, &c->bnd_s_cap, std::max<size_t>(1, maxs) * ncell)))
                 return rc;
-            if ((rc = ensure(c, &c->cell_tmin, &c->cell_tmin_cap, ncell))) return rc;
+            if ((rc = ensure(c, &c->cell_tmin, &c->cell_tmin_cap,
+                             ncell * ((std::max<size_t>(1, maxs) + TMIN_BLK - 1) / TMIN_BLK))))
+                return rc;
             if ((rc = ensure(c, &c->bnd_r, &c->bnd_r_cap, std::max<size_t>(1, maxr) * ncell)))
                 return rc;
             if ((rc = ensure(c, &c->pairs, &c->pairs_cap, (size_t)c->ns * c->nh * c->nt * 2)))

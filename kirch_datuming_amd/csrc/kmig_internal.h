@@ -147,7 +147,9 @@ constexpr int BND_CZ = 8, BND_CX = 4;
 constexpr float BND_Q = 65536.0f;
 constexpr int BND_EMPTY = 1 << 28;
 
-// out[cell] = min over slots of bnd[cell * nslots + slot].tmin
+// out[cell * nblk + b] = min over slots [b TMIN_BLK, (b + 1) TMIN_BLK) of
+// bnd[cell * nslots + slot].tmin, nblk = ceil(nslots / TMIN_BLK)
+constexpr int TMIN_BLK = 32;
 int launch_cell_tmin(const ThBound *bnd, int nslots, int ncell, int *out, void *stream);
 
 // Bounds of every (slot, cell) of FP32 (pk) or 16-bit (pk16) packed tuples,
@@ -172,9 +174,11 @@ struct MigResArgs {
                                  // shots (entry js of a row: batch-local shot js),
                                  // or null (no culling)
     const ThBound *rbnd;         // [ncz * ncx][nrslots]
-    const int *scell_tmin;       // [ncz * ncx] smallest source tmin of each cell over
-                                 // the batch's shots (a receiver chunk whose smallest
-                                 // sample reaches t_cull with it has no live shot)
+    const int *scell_tmin;       // [ncz * ncx][scell_nblk] smallest source tmin of each
+                                 // cell over each block of TMIN_BLK batch shots (a
+                                 // receiver chunk whose smallest sample reaches t_cull
+                                 // with the group's blocks has no live shot)
+    int scell_nblk, scell_off;   // blocks per cell; batch index of local shot 0
     int ncz, ncx;                // cells of BND_CZ x BND_CX points
     int sbnd_stride, nrslots;    // shots per sbnd row, receiver slots
     float *spray;                // adj=0: [ns][nh][nt] traces the image is sprayed into

@@ -513,12 +513,16 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
         tq = __builtin_amdgcn_readfirstlane((int)ceilf(tl * BND_Q)) + 64;
     }
     const int wrapq = (int)(a.wrap * BND_Q);
-    // smallest source sample of the wave's cells over all shots: with a
-    // chunk's smallest receiver sample it bounds every group of the chunk
+    // smallest source sample of the wave's cells over the shot group's
+    // blocks of shots: with a chunk's smallest receiver sample it bounds
+    // every (shot, chunk) group of the chunk
     int sall = BND_EMPTY;
-    if (cull && a.scell_tmin) {
+    if (cull && a.scell_tmin && je > jb) {
+        const int b0 = (a.scell_off + jb) / TMIN_BLK, b1 = (a.scell_off + je - 1) / TMIN_BLK;
 #pragma unroll
-        for (int c = 0; c < NC; c++) sall = min(sall, a.scell_tmin[cell[c]]);
+        for (int c = 0; c < NC; c++)
+            for (int b = b0; b <= b1; b++)
+                sall = min(sall, a.scell_tmin[(size_t)cell[c] * a.scell_nblk + b]);
     }
 
     for (int r0 = 0; r0 < a.nh; r0 += R) {

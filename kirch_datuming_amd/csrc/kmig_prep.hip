@@ -987,20 +987,22 @@ __global__ __launch_bounds__(64) void k_th_bounds(const PackedTuple *__restrict_
 }
 
 __global__ __launch_bounds__(64) void k_cell_tmin(const ThBound *__restrict__ bnd, int nslots,
-                                                  int *__restrict__ out)
+                                                  int nblk, int *__restrict__ out)
 {
     const ThBound *row = bnd + (size_t)blockIdx.x * nslots;
-    int m = BND_EMPTY;
-    for (int s = threadIdx.x; s < nslots; s += 64) m = min(m, row[s].tmin);
-    for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
-    if (threadIdx.x == 0) out[blockIdx.x] = m;
+    for (int b = threadIdx.x; b < nblk; b += 64) {
+        int m = BND_EMPTY;
+        for (int s = b * TMIN_BLK; s < min(nslots, (b + 1) * TMIN_BLK); s++)
+            m = min(m, row[s].tmin);
+        out[(size_t)blockIdx.x * nblk + b] = m;
+    }
 }
 
 int launch_cell_tmin(const ThBound *bnd, int nslots, int ncell, int *out, void *stream)
 {
     if (ncell <= 0) return 0;
     hipLaunchKernelGGL(k_cell_tmin, dim3((unsigned)ncell), dim3(64), 0, (hipStream_t)stream, bnd,
-                       nslots, out);
+                       nslots, (nslots + TMIN_BLK - 1) / TMIN_BLK, out);
     return (int)hipGetLastError();
 }
 
