@@ -86,8 +86,8 @@ struct kmig_ctx {
     SlotScale *sc_s = nullptr, *sc_r = nullptr;
     size_t sc_s_cap = 0, sc_r_cap = 0;
     ThBound *bnd_s = nullptr, *bnd_r = nullptr;  // direction bounds per (slot, cell)
-    int *cell_tmin = nullptr;                    // smallest source sample per cell
-    size_t cell_tmin_cap = 0;
+    ThBound *bnd_sblk = nullptr;  // source bounds per (cell, block of TMIN_BLK shots)
+    size_t bnd_sblk_cap = 0;
     size_t bnd_s_cap = 0, bnd_r_cap = 0;
     float *pairs = nullptr;
     size_t pairs_cap = 0;
@@ -480,7 +480,7 @@ MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
     const bool cull = !(cull_env && std::atoi(cull_env) == 0);
     R.sbnd = cull ? c->bnd_s : nullptr;
     R.rbnd = cull ? c->bnd_r : nullptr;
-    R.scell_tmin = cull ? c->cell_tmin : nullptr;
+    R.sblk = cull ? c->bnd_sblk : nullptr;
     R.scell_nblk = (nshots + TMIN_BLK - 1) / TMIN_BLK;
     R.scell_off = 0;
     R.ncz = (c->nz + BND_CZ - 1) / BND_CZ;
@@ -683,7 +683,7 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
                                  ((c->nx + BND_CX - 1) / BND_CX);
             if ((rc = ensure(c, &c->bnd_s, &c->bnd_s_cap, std::max<size_t>(1, maxs) * ncell)))
                 return rc;
-            if ((rc = ensure(c, &c->cell_tmin, &c->cell_tmin_cap,
+            if ((rc = ensure(c, &c->bnd_sblk, &c->bnd_sblk_cap,
                              ncell * ((std::max<size_t>(1, maxs) + TMIN_BLK - 1) / TMIN_BLK))))
                 return rc;
             if ((rc = ensure(c, &c->bnd_r, &c->bnd_r_cap, std::max<size_t>(1, maxr) * ncell)))
@@ -897,7 +897,7 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
                                                pack32 ? nullptr : c->pk16_s, nsb, c->nz, c->nx,
                                                c->bnd_s, st));
         const int ncell = ((c->nz + BND_CZ - 1) / BND_CZ) * ((c->nx + BND_CX - 1) / BND_CX);
-        HIPCHK(c, (hipError_t)launch_cell_tmin(c->bnd_s, nsb, ncell, c->cell_tmin, st));
+        HIPCHK(c, (hipError_t)launch_cell_blocks(c->bnd_s, nsb, ncell, c->bnd_sblk, st));
         HIPCHK(c, (hipError_t)launch_th_bounds(pack32 ? c->pk_r : nullptr,
                                                pack32 ? nullptr : c->pk16_r, b.nrcv, c->nz,
                                                c->nx, c->bnd_r, st));
@@ -972,7 +972,7 @@ void kmig_destroy(kmig_ctx *c)
     dfree(c->pk_r);
     dfree(c->pk16_s);
     dfree(c->bnd_s);
-    dfree(c->cell_tmin);
+    dfree(c->bnd_sblk);
     dfree(c->bnd_r);
     dfree(c->pk16_r);
     dfree(c->sc_s);

@@ -147,10 +147,11 @@ constexpr int BND_CZ = 8, BND_CX = 4;
 constexpr float BND_Q = 65536.0f;
 constexpr int BND_EMPTY = 1 << 28;
 
-// out[cell * nblk + b] = min over slots [b TMIN_BLK, (b + 1) TMIN_BLK) of
-// bnd[cell * nslots + slot].tmin, nblk = ceil(nslots / TMIN_BLK)
+// out[cell * nblk + b] = union over slots [b TMIN_BLK, (b + 1) TMIN_BLK) of
+// bnd[cell * nslots + slot] (min lo, max hi, min tmin), nblk = ceil(nslots /
+// TMIN_BLK)
 constexpr int TMIN_BLK = 32;
-int launch_cell_tmin(const ThBound *bnd, int nslots, int ncell, int *out, void *stream);
+int launch_cell_blocks(const ThBound *bnd, int nslots, int ncell, ThBound *out, void *stream);
 
 // Bounds of every (slot, cell) of FP32 (pk) or 16-bit (pk16) packed tuples,
 // cell-major: out[cell * nslots + slot], cell = cx * ncz + cz.
@@ -174,10 +175,10 @@ struct MigResArgs {
                                  // shots (entry js of a row: batch-local shot js),
                                  // or null (no culling)
     const ThBound *rbnd;         // [ncz * ncx][nrslots]
-    const int *scell_tmin;       // [ncz * ncx][scell_nblk] smallest source tmin of each
-                                 // cell over each block of TMIN_BLK batch shots (a
-                                 // receiver chunk whose smallest sample reaches t_cull
-                                 // with the group's blocks has no live shot)
+    const ThBound *sblk;         // [ncz * ncx][scell_nblk] union of the source bounds of
+                                 // each cell over each block of TMIN_BLK batch shots
+                                 // (smallest lo and tmin, largest hi): a block culled
+                                 // with them has every shot culled
     int scell_nblk, scell_off;   // blocks per cell; batch index of local shot 0
     int ncz, ncx;                // cells of BND_CZ x BND_CX points
     int sbnd_stride, nrslots;    // shots per sbnd row, receiver slots

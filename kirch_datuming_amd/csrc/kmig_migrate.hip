@@ -517,12 +517,12 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
     // blocks of shots: with a chunk's smallest receiver sample it bounds
     // every (shot, chunk) group of the chunk
     int sall = BND_EMPTY;
-    if (cull && a.scell_tmin && je > jb) {
+    if (cull && a.sblk && je > jb) {
         const int b0 = (a.scell_off + jb) / TMIN_BLK, b1 = (a.scell_off + je - 1) / TMIN_BLK;
 #pragma unroll
         for (int c = 0; c < NC; c++)
             for (int b = b0; b <= b1; b++)
-                sall = min(sall, a.scell_tmin[(size_t)cell[c] * a.scell_nblk + b]);
+                sall = min(sall, a.sblk[(size_t)cell[c] * a.scell_nblk + b].tmin);
     }
 
     for (int r0 = 0; r0 < a.nh; r0 += R) {
@@ -560,11 +560,13 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
         // one is loaded while the current one is processed, and a culled shot
         // costs a few scalar instructions (its tuple is never loaded).  The
         // bounds rows are cell-major, so consecutive shots share cache lines.
-        auto culled = [&](int j) -> bool {
+        // the same test for one shot (bnd = sbnd, stride sbnd_stride, index
+        // j) or for a block of TMIN_BLK shots (their union bounds)
+        auto culled_by = [&](const ThBound *bnd, size_t stride, int j) -> bool {
             int slo = BND_EMPTY, shi = -BND_EMPTY, stm = BND_EMPTY;
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                const ThBound b = a.sbnd[(size_t)cell[c] * a.sbnd_stride + j];
+                const ThBound b = bnd[(size_t)cell[c] * stride + j];
                 slo = min(slo, b.lo);
                 shi = max(shi, b.hi);
                 stm = min(stm, b.tmin);
@@ -583,9 +585,26 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             const int bmax = max(abs(dlo), abs(dhi));
             return min(amin, wrapq - bmax) > tq;
         };
+        auto culled = [&](int j) { return culled_by(a.sbnd, a.sbnd_stride, j); };
+        // Shots in blocks of TMIN_BLK: a block whose union bounds cull it is
+        // passed in one test; a block that may hold live shots is scanned shot
+        // by shot (and not tested again).
+        int blk_open = -1;
         auto next_shot = [&](int j) {
-            if (cull)
-                while (j < je && culled(j)) j++;
+            if (cull) {
+                while (j < je) {
+                    const int bj = (a.scell_off + j) / TMIN_BLK;
+                    if (a.sblk && bj != blk_open) {
+                        if (culled_by(a.sblk, (size_t)a.scell_nblk, bj)) {
+                            j = min(je, (bj + 1) * TMIN_BLK - a.scell_off);
+                            continue;
+                        }
+                        blk_open = bj;
+                    }
+                    if (!culled(j)) break;
+                    j++;
+                }
+            }
             return j;
         };
         int js = next_shot(jb);
