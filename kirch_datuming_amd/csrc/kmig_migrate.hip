@@ -780,10 +780,12 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
 }
 
 // Blocks a resident launch aims at when it splits the shots into groups:
-// about 32 rounds of five 16 x 8 tiles per CU (measured on the salt workload:
-// 4 groups 128 ms, 8 121 ms, 16 115 ms, 24 115 ms, 32 116 ms; groups keep at
-// least 8 shots, as more groups of fewer shots lose on the small workload).
-constexpr int kSplitBlocks = 40960;
+// about 38 rounds of five 16 x 8 tiles per CU, 16 groups on the salt workload
+// (measured there: 4 groups 128 ms, 8 121 ms, 16 115 ms, 24 115 ms, 32 116 ms;
+// with the time-window and shot-block culling 8 94.8 ms, 11 93.4, 14 91.4,
+// 16 90.1, 20 90.4, 24 90.2, 32 92.9); groups keep at least 8 shots, as more
+// groups of fewer shots lose on the small workload.
+constexpr int kSplitBlocks = 49152;
 
 // cig = part[0] + part[1] + ... (group order), the shot groups' partial CIGs
 __global__ __launch_bounds__(256) void k_sum_parts(const float *__restrict__ part,
