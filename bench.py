@@ -47,9 +47,13 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "u16"])
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--variant", default="auto", choices=["auto", "stream", "resident"])
-    ap.add_argument("--cpu-shots", type=int, default=2,
+    ap.add_argument("--cpu-shots", type=int, default=4,
                     help="shots of the workload timed on the CPU oracle (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-stream-check", action="store_true",
+                    help="skip the stream-kernel run that checks the timed run's CIG")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the 16-bit table forms of the default line")
     ap.add_argument("--overlap", type=int, default=0,
                     help="layers: extend layers 2..5 upward by this many depth samples and "
                          "stitch the gathers on the GPU (layercom_cig blend)")
@@ -108,41 +112,43 @@ def main():
     m.set_data(w.data, ax)
     upload_s = time.perf_counter() - tu
     log(f"tables and data resident in HBM ({upload_s:.2f} s host-to-device)")
-
-    for i in range(args.warmup):
-        m.run()
-        m.sync()
-        log(f"warmup {i + 1}/{args.warmup}: {m.stats()['ms_total']:.1f} ms")
-    mig_ms, tot_ms, prep_ms, tup_ms = [], [], [], []
-    m.sync()
-    tstart = time.perf_counter()
-    for i in range(args.steps):
-        m.run()
-        m.sync()
-        st = m.stats()
-        mig_ms.append(st["ms_migrate"] / max(1, st["migrate_launches"]))
-        tot_ms.append(st["ms_total"])
-        prep_ms.append(st["ms_trace_prep"])
-        tup_ms.append(st["ms_tuple_prep"])
-    m.sync()
-    wall = time.perf_counter() - tstart
-    log(f"timed {args.steps} steps in {wall:.3f} s")
+    r = timed_steps(m, args.warmup, args.steps, w.updates)
+    st = r["stats"]
+    value = r["value"]
     launches = st["migrate_launches"]
-    value = w.updates * args.steps / wall
-    mig_avg = float(np.mean(mig_ms))
+    mig_avg = r["migrate_ms"]
     upd_per_launch = w.updates / max(1, launches)
-    bpu = BYTES_PER_UPDATE[args.precision]
-    achieved = upd_per_launch * bpu / (mig_avg * 1e-3) / 1e9
-    cig_gpu = m.cig()
-
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"migrate_{args.config}_{args.precision}_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    # the CIG of the timed run itself: finite, and against the stream kernel
+    # (the reference's per-point order, pinned on the golden cases) on the
+    # same full workload
+    cig_timed = m.cig()
+    counts = None
+    if st["variant"] == 2 and args.precision == "fp32":
+        counts = m.count_updates()
+        log(f"updates evaluated {counts['evaluated'] / counts['nominal']:.3f}, accepted "
+            f"{counts['accepted'] / counts['nominal']:.3f} of nominal")
+    m.close()
+    parity = {"timed_run_finite": bool(np.all(np.isfinite(cig_timed))),
+              "timed_run_abs_max": float(np.abs(cig_timed).max())}
+    if st["variant"] == 2 and not args.no_stream_check:
+        log("stream kernel (reference order) on the full workload")
+        with kmig.Migrator(kmig.parse_params(dict(kv, variant="stream", precision="fp32"),
+                                             int(ax[1][0]), ax[1][1], ax[1][2])) as ms:
+            ms.set_tables(0, w.stab, w.sder, w.s_axes)
+            ms.set_tables(1, w.rtab, w.rder, w.r_axes)
+            ms.set_data(w.data, ax)
+            ms.run()
+            ms.sync()
+            cig_stream = ms.cig()
+            t_stream = ms.stats()["ms_total"]
+        parity["timed_run_vs_stream_kernel_full"] = rel_l2(cig_timed, cig_stream)
+        parity["stream_kernel_ms"] = t_stream
+        log(f"timed run vs stream kernel: relL2 {parity['timed_run_vs_stream_kernel_full']:.2e}")
+        del cig_stream
+    del cig_timed
 
     cpu = None
-    parity = None
+    forms = {}
     if not args.no_cpu and args.cpu_shots > 0:
         from oracle import oracle as O
         ns = int(ax[2][0])
@@ -160,30 +166,39 @@ def main():
                "sample": f"{k} evenly spaced shots ({sub.updates:.3e} updates) of the same "
                          f"workload, {tcpu:.1f} s, OpenMP over image points with the serial "
                          f"per-trace preparation of the reference"}
-        with kmig.Migrator(p) as ms:
-            ms.set_tables(0, w.stab, w.sder, w.s_axes)
-            ms.set_tables(1, w.rtab, w.rder, w.r_axes)
-            ms.set_data(sub.data, sub.data_axes)
-            ms.run()
-            ms.sync()
-            cig_sub = ms.cig()
         log("FP64 oracle on the same sample")
         cig64 = O.migrate(w.kv, *sub.args(), precision="f64")[0]
-        # worst angle bin: its error norm over the largest bin norm of the
-        # FP64 CIG (a relative L2 per bin is meaningless for near-empty bins)
-        bnorm = np.sqrt(np.sum(cig64.astype(np.float64) ** 2, axis=(1, 2)))
-        berr = np.sqrt(np.sum((cig_sub.astype(np.float64) - cig64) ** 2, axis=(1, 2)))
-        bins = [float(berr.max() / bnorm.max())] if bnorm.max() > 0 else []
-        parity = {"sample": f"shots {shots}",
-                  "rel_l2_gpu_vs_f64": O.rel_l2(cig_sub, cig64),
-                  "worst_angle_bin_err_over_max_bin_norm": bins[0] if bins else None,
-                  "rel_l2_cpu_f32_vs_f64": O.rel_l2(cig_cpu, cig64),
-                  "rel_l2_gpu_vs_cpu_f32": O.rel_l2(cig_sub, cig_cpu)}
+        # the production path on the sample, two shot groups forced so the
+        # partial-CIG sums are in it
+        cig_sub = sample_run(kmig, kv, w, sub, split=2)
+        parity.update({"sample": f"shots {shots}, 2 shot groups forced",
+                       **accuracy(cig_sub, cig64, "gpu"),
+                       "rel_l2_cpu_f32_vs_f64": rel_l2(cig_cpu, cig64),
+                       "rel_l2_gpu_vs_cpu_f32": rel_l2(cig_sub, cig_cpu)})
         log(f"CPU {cpu['value']:.3e} updates/s; relL2 on the sample: GPU vs f64 "
             f"{parity['rel_l2_gpu_vs_f64']:.2e}, CPU f32 vs f64 "
             f"{parity['rel_l2_cpu_f32_vs_f64']:.2e}")
+        # the 16-bit table forms of the same workload: timed the same way,
+        # accuracy on the same sample
+        if args.precision == "fp32" and not args.no_forms:
+            for prec in ("fp16", "bf16"):
+                kvp = dict(kv, precision=prec)
+                with kmig.Migrator(kmig.parse_params(kvp, int(ax[1][0]), ax[1][1],
+                                                     ax[1][2])) as mp:
+                    mp.set_tables(0, w.stab, w.sder, w.s_axes)
+                    mp.set_tables(1, w.rtab, w.rder, w.r_axes)
+                    mp.set_data(w.data, ax)
+                    rp = timed_steps(mp, args.warmup, min(args.steps, 10), w.updates)
+                cp = sample_run(kmig, kvp, w, sub, split=2)
+                forms[prec] = {"value": rp["value"], "ms_per_step": rp["ms_per_step"],
+                               "migrate_ms": rp["migrate_ms"], "tile": rp["stats"]["tile"],
+                               "vs_fp32": rp["value"] / value, **accuracy(cp, cig64, "gpu")}
+                log(f"{prec}: {rp['value']:.3e} updates/s ({forms[prec]['vs_fp32']:.3f} x fp32), "
+                    f"relL2 vs f64 {forms[prec]['rel_l2_gpu_vs_f64']:.2e}")
 
     (nt, _, dt), (nh, _, _), (ns, _, _) = ax
+    kernel = (f"k_migrate_ph (preset {st['tile']})" if st["variant"] == 2 else "k_migrate")
+    roof = roofline(args.config, args.precision, st, mig_avg, upd_per_launch, counts, kernel)
     out = {
         "metric": METRIC,
         "value": value,
@@ -191,7 +206,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -199,27 +214,148 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "config": args.config, "nz": int(w.s_axes[0][0]),
                    "nx": int(w.s_axes[1][0]), "ns": int(ns), "nh": int(nh), "nt": int(nt),
-                   "ng": int(p.ng), "updates_per_step": w.updates,
-                   "precision": args.precision,
+                   "ng": int(p.ng), "cmp": int(w.kv.get("cmp", 1)),
+                   "updates_per_step": w.updates, "precision": args.precision,
                    "tile": p.tile or f"auto (preset {st['tile']})",
                    "variant": {0: "auto", 1: "stream", 2: "resident"}[st["variant"]]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": (f"k_migrate_ph (preset {st['tile']})" if st["variant"] == 2
-                                else "k_migrate"),
-                     "algorithmic_bytes_per_update": bpu,
-                     "updates_per_launch": upd_per_launch, "avg_launch_ms": mig_avg},
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "breakdown_ms": {"trace_prep": float(np.mean(prep_ms)), "tuple_prep": float(np.mean(tup_ms)),
-                         "migrate": float(np.mean(mig_ms)) * launches,
-                         "events_total": float(np.mean(tot_ms))},
+        "breakdown_ms": {"trace_prep": r["trace_prep_ms"], "tuple_prep": r["tuple_prep_ms"],
+                         "migrate": mig_avg * launches, "events_total": r["events_ms"]},
         "parity": parity,
+        "forms": forms or None,
         "upload_s": upload_s,
-        "pcie_inclusive_updates_per_s": w.updates / (wall / args.steps + upload_s),
-        "cig_abs_max": float(np.abs(cig_gpu).max()),
+        "pcie_inclusive_updates_per_s": w.updates / (r["ms_per_step"] * 1e-3 + upload_s),
     }
-    m.close()
     print(json.dumps(out), flush=True)
+
+
+def rel_l2(a, b):
+    """sqrt(sum (a-b)^2 / sum b^2), rsf_l2cmp's metric (src/rsf_l2cmp.cpp:9-10)."""
+    import numpy as np
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = float(np.sum(b * b))
+    return float(np.sqrt(np.sum((a - b) ** 2) / den)) if den > 0 else float("nan")
+
+
+def accuracy(cig, ref, who):
+    """relL2 and the worst angle bin (its error norm over the largest bin norm
+    of the reference: a relative L2 per bin means nothing for near-empty bins)."""
+    import numpy as np
+    ref = np.asarray(ref, np.float64)
+    bnorm = np.sqrt(np.sum(ref ** 2, axis=(1, 2)))
+    berr = np.sqrt(np.sum((np.asarray(cig, np.float64) - ref) ** 2, axis=(1, 2)))
+    worst = float(berr.max() / bnorm.max()) if bnorm.max() > 0 else None
+    return {f"rel_l2_{who}_vs_f64": rel_l2(cig, ref),
+            "worst_angle_bin_err_over_max_bin_norm": worst}
+
+
+def timed_steps(m, warmup, steps, updates):
+    """W untimed and K timed full migrations (run + sync each), wall clock
+    around the K, kernel times from the context's HIP events."""
+    import numpy as np
+    for i in range(warmup):
+        m.run()
+        m.sync()
+        log(f"warmup {i + 1}/{warmup}: {m.stats()['ms_total']:.1f} ms")
+    mig, tot, prep, tup = [], [], [], []
+    m.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.run()
+        m.sync()
+        st = m.stats()
+        mig.append(st["ms_migrate"] / max(1, st["migrate_launches"]))
+        tot.append(st["ms_total"])
+        prep.append(st["ms_trace_prep"])
+        tup.append(st["ms_tuple_prep"])
+    m.sync()
+    wall = time.perf_counter() - t0
+    log(f"timed {steps} steps in {wall:.3f} s")
+    return {"value": updates * steps / wall, "ms_per_step": wall / steps * 1e3,
+            "migrate_ms": float(np.mean(mig)), "events_ms": float(np.mean(tot)),
+            "trace_prep_ms": float(np.mean(prep)), "tuple_prep_ms": float(np.mean(tup)),
+            "stats": st}
+
+
+def sample_run(kmig, kv, w, sub, split):
+    """The production path (autotuned resident preset) on a shot subset, with
+    `split` shot groups forced."""
+    old = os.environ.get("KMIG_SPLIT")
+    os.environ["KMIG_SPLIT"] = str(split)
+    try:
+        ax = sub.data_axes
+        with kmig.Migrator(kmig.parse_params(kv, int(ax[1][0]), ax[1][1], ax[1][2])) as ms:
+            ms.set_tables(0, w.stab, w.sder, w.s_axes)
+            ms.set_tables(1, w.rtab, w.rder, w.r_axes)
+            ms.set_data(sub.data, ax)
+            ms.run()
+            ms.sync()
+            return ms.cig()
+    finally:
+        if old is None:
+            del os.environ["KMIG_SPLIT"]
+        else:
+            os.environ["KMIG_SPLIT"] = old
+
+
+# VALU issue peak of MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane
+# instructions per second (a wave64 instruction occupies a SIMD for 4 cycles);
+# 2 flops per lane for an FMA give the 78.6 TFLOP/s non-packed FP32 rate.
+VALU_PEAK_LANE_INSTR = 256 * 4 * 16 * 2.4e9
+
+
+def roofline(config, precision, st, mig_ms, updates, counts, kernel):
+    """The resident migrate kernel is bound by instruction issue, not by HBM:
+    its receiver tuples stay in registers and culled groups load nothing, so
+    the per-update streamed tuple of the bytes model (16 B/update FP32) is
+    not its traffic.  `frac` is therefore its VALU issue rate against the
+    VALU peak: the VALU instructions of one launch, from the rocprofv3
+    counters of the same kernel build under profiles/, over the launch time
+    measured here with HIP events.  The HBM bytes per launch (FETCH_SIZE and
+    WRITE_SIZE of the same profile, copy-calibrated) give the HBM fraction
+    beside it, and the bytes model's 16 B/update is reported as a reuse
+    factor over them.  A profile recorded for another build of the kernel
+    source is marked stale and not used."""
+    import hashlib
+    bpu = BYTES_PER_UPDATE.get(precision, 16.0)
+    model = updates * bpu
+    base = {"kernel": kernel, "avg_launch_ms": mig_ms, "updates_per_launch": updates,
+            "bytes_model_per_update": bpu}
+    if counts:
+        base["evaluated_fraction"] = counts["evaluated"] / counts["nominal"]
+        base["accepted_fraction"] = counts["accepted"] / counts["nominal"]
+    prof = os.path.join(ROOT, "profiles", f"migrate_{config}_{precision}_counters.json")
+    if st["variant"] != 2 or not os.path.exists(prof):
+        return {"bound": "hbm", "achieved": model / (mig_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": None, "traffic": None,
+                "note": "no counter profile for this kernel and workload", **base}
+    with open(prof) as f:
+        pc = json.load(f)
+    src = os.path.join(ROOT, "kirch_datuming_amd", "csrc", "kmig_migrate.hip")
+    with open(src, "rb") as f:
+        sha = hashlib.sha256(f.read()).hexdigest()[:16]
+    stale = pc.get("source_sha16") != sha or int(pc.get("preset", -1)) != int(st["tile"])
+    if stale:
+        return {"bound": "valu", "achieved": None, "peak": VALU_PEAK_LANE_INSTR / 1e12,
+                "unit": "T lane-instr/s", "frac": None, "traffic": None, "stale": True,
+                "profile": os.path.relpath(prof, ROOT),
+                "note": "the counter profile was recorded for another kernel build or preset",
+                **base}
+    valu = pc["sq_insts_valu"] * 64.0
+    achieved = valu / (mig_ms * 1e-3)
+    traffic = pc["hbm_bytes_per_launch"]
+    hbm = traffic / (mig_ms * 1e-3) / 1e9
+    return {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK_LANE_INSTR / 1e12,
+            "unit": "T lane-instr/s", "frac": achieved / VALU_PEAK_LANE_INSTR,
+            "traffic": traffic, "stale": False, "profile": os.path.relpath(prof, ROOT),
+            "valu_lane_instr_per_launch": valu,
+            "valu_instr_per_nominal_update": valu / updates,
+            "hbm": {"achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": hbm / HBM_PEAK_GBS},
+            "reuse_factor_bytes_model_over_traffic": model / traffic,
+            "profile_launch_ms": pc.get("avg_launch_ms"), **base}
 
 
 def bench_model(args, cores, w):

@@ -157,6 +157,17 @@ int kmig_get_cig(kmig_ctx *ctx, float *cig);
 int kmig_get_qc(kmig_ctx *ctx, float *qc_before, float *qc_after);
 int kmig_get_stats(const kmig_ctx *ctx, kmig_stats *st);
 
+/* How much of the nominal update grid of the last run survives the culls of
+   the resident kernel (no counterpart in the reference, which evaluates every
+   (shot, receiver, image point) update, src/kirchmig2d_cig.cpp:1203-1339).
+   Re-runs the last run's preset with the same grid and culls, counting
+   instead of accumulating (the CIG is untouched), and waits for it.
+   nominal = ns*nh*nz*nx; evaluated = updates in (shot, receiver group)
+   groups that passed the direction and time-window culls; accepted = updates
+   whose taps lie inside the trace with a nonzero weight.  Needs a previous
+   kmig_run on the resident kernel with FP32 tuples. */
+int kmig_count_updates(kmig_ctx *ctx, double *nominal, double *evaluated, double *accepted);
+
 /* One call: set_data + run(all shots) + sync + get_cig (+ QC when not NULL). */
 int kmig_migrate(kmig_ctx *ctx, const float *data, int nt, float t0, float dt, int nh,
                  float h0, float dh, int ns, float s0, float ds, float *cig,

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -99,6 +100,8 @@ struct kmig_ctx {
     size_t part_cap = 0;
     int *rslot = nullptr;
     size_t rslot_cap = 0;
+    unsigned long long *cnt = nullptr;  // kmig_count_updates: evaluated, accepted
+    size_t cnt_cap = 0;
     bool rcv_invariant = false;     // every shot uses receiver slot rslot[ih]
     int variant_used = 0;
     int tuned_tile = 0;             // autotuned resident preset (0: not tuned yet)
@@ -153,6 +156,17 @@ int ensure(kmig_ctx *c, T **ptr, size_t *cap, size_t n)
                     hipGetErrorString(e));
     }
     *cap = n;
+    // KMIG_POISON=1 fills every new float buffer with NaN bytes, so a read
+    // of memory that no kernel or copy has written shows up as NaN in the
+    // results instead of depending on what the allocator hands back.  Only
+    // float buffers: their contents are values, never indices.
+    if constexpr (std::is_same<T, float>::value) {
+        static const bool poison = [] {
+            const char *v = getenv("KMIG_POISON");
+            return v && atoi(v) != 0;
+        }();
+        if (poison) HIPCHK(c, hipMemset(*ptr, 0xFF, n * sizeof(T)));
+    }
     return KMIG_OK;
 }
 
@@ -690,12 +704,16 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
                 return rc;
             if ((rc = ensure(c, &c->pairs, &c->pairs_cap, (size_t)c->ns * c->nh * c->nt * 2)))
                 return rc;
-            // padded: the resident kernels read a whole receiver chunk of
-            // wap entries, past nh in the last chunk of the last shot; the
-            // pad is zero (finite) and meets zero tuple weights there
+            // The resident kernels read a whole receiver chunk of wap
+            // entries: past nh in the last chunk of a shot, i.e. the first
+            // entries of the next shot's row (of the pad after the last
+            // shot).  Those receivers have zero tuple weight, so any finite
+            // value there adds an exact zero; a run over a shot range writes
+            // only its own rows, so the whole array (pad included) is zeroed
+            // first and never holds uninitialised memory.
             const size_t nwap = (size_t)c->ns * c->nh;
             if ((rc = ensure(c, &c->wap, &c->wap_cap, nwap + RES_WAP_PAD))) return rc;
-            HIPCHK(c, hipMemset(c->wap + nwap, 0, RES_WAP_PAD * sizeof(float)));
+            HIPCHK(c, hipMemset(c->wap, 0, (nwap + RES_WAP_PAD) * sizeof(float)));
             if ((rc = ensure(c, &c->rslot, &c->rslot_cap, (size_t)c->nh))) return rc;
             if (!wapv.empty())
                 HIPCHK(c, hipMemcpy(c->wap + (size_t)sb * c->nh, wapv.data(),
@@ -982,6 +1000,7 @@ void kmig_destroy(kmig_ctx *c)
     dfree(c->wap);
     dfree(c->part);
     dfree(c->rslot);
+    dfree(c->cnt);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1240,7 +1259,10 @@ int kmig_run(kmig_ctx *c, int sb, int se)
         }
         if (nsb > 0 && resident) {
             MigResArgs R = res_args(c, b.s0, nsb, ng);
-            if (nsb >= 16) {
+            // KMIG_SPLIT forces a shot-group count (tests): then short shot
+            // ranges split too, so their partial sums are in the checked path
+            const char *fs = getenv("KMIG_SPLIT");
+            if (nsb >= 16 || (fs && atoi(fs) > 1 && nsb >= 2)) {
                 // shot groups for the launch's tail (the launcher picks the count)
                 if ((rc = ensure(c, &c->part, &c->part_cap,
                                  (size_t)KMIG_MAX_SPLIT * ng * c->nzx)))
@@ -1334,6 +1356,37 @@ int kmig_get_stats(const kmig_ctx *c, kmig_stats *st)
 {
     if (!c || !st) return KMIG_E_PARAM;
     *st = c->st;
+    return KMIG_OK;
+}
+
+int kmig_count_updates(kmig_ctx *c, double *nominal, double *evaluated, double *accepted)
+{
+    if (!c) return KMIG_E_PARAM;
+    if (!c->ran || c->last_sb < 0) return fail(c, KMIG_E_STATE, "kmig_run has not been called");
+    if (c->variant_used != KMIG_VARIANT_RESIDENT || c->batches.size() != 1 ||
+        c->p.precision != KMIG_PREC_FP32)
+        return fail(c, KMIG_E_PARAM,
+                    "update counts are kept for the resident kernel with FP32 tuples");
+    int rc = kmig_sync(c);
+    if (rc) return rc;
+    if ((rc = ensure(c, &c->cnt, &c->cnt_cap, 2))) return rc;
+    const Batch &b = c->batches[0];
+    const int nsb = b.s1 - b.s0;
+    const int ng = c->p.cig ? c->p.ng : 1;
+    MigResArgs R = res_args(c, b.s0, nsb, ng);
+    const char *fs = getenv("KMIG_SPLIT");
+    if (c->part && (nsb >= 16 || (fs && atoi(fs) > 1 && nsb >= 2)))
+        R.part_splits = KMIG_MAX_SPLIT;  // the production launch's shot groups
+    R.counts = c->cnt;
+    HIPCHK(c, (hipError_t)launch_fill((float *)c->cnt, 4, 0.0f, c->stream));
+    if (nsb > 0)
+        HIPCHK(c, (hipError_t)launch_count_resident(R, c->st.tile, c->stream));
+    unsigned long long h[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(h, c->cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (nominal) *nominal = (double)nsb * (double)c->nh * (double)c->nzx;
+    if (evaluated) *evaluated = (double)h[0];
+    if (accepted) *accepted = (double)h[1];
     return KMIG_OK;
 }
 

@@ -191,6 +191,11 @@ struct MigResArgs {
     // part_splits = groups part holds (0: no buffer, one group).
     float *part;
     int part_splits;
+    // counting launches only (launch_count_resident): [0] updates evaluated
+    // (valid points x real receivers of every (shot, receiver chunk) group
+    // that survives the culls), [1] updates accepted (taps inside the trace
+    // and nonzero weight); the counting kernel writes no CIG
+    unsigned long long *counts;
     int c_int;                   // integer part of (tau - t0)/dt
     int t_cull;                  // nt - 2 - c_int: a group whose smallest source plus
                                  // receiver integer samples reach it has every
@@ -283,5 +288,9 @@ int launch_migrate_multi(const MigMultiArgs &m, int tile, int precision, void *s
 int migrate_tiles(int nz, int nx, int tile);
 // the same kernels under a distinct symbol, for the autotuner's timing launches
 int launch_migrate_resident_probe(const MigResArgs &a, int tile, int precision, void *stream);
+// the FP32 resident kernel of preset `tile` with the same grid and culls,
+// counting evaluated and accepted updates into a.counts instead of writing
+// the CIG
+int launch_count_resident(const MigResArgs &a, int tile, void *stream);
 
 }  // namespace kmig

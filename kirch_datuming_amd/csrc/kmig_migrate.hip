@@ -442,7 +442,9 @@ __device__ __forceinline__ void wave_lds_sync()
 // Model = true is the adj=0 transpose: the LDS column holds the image (read
 // only) and every accepted update sprays its six weighted samples into the
 // traces of a.spray instead of reading its taps.
-template <int TZ, int TX, int R, int G, int Fmt, bool Model = false>
+// Count = true: the same work, but the wave counts the updates it evaluates
+// and accepts (a.counts) and writes no CIG.
+template <int TZ, int TX, int R, int G, int Fmt, bool Model = false, bool Count = false>
 __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, float *acc,
                                              int sg = 0, int nsplit = 1)
 {
@@ -525,6 +527,8 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 sall = min(sall, a.sblk[(size_t)cell[c] * a.scell_nblk + b].tmin);
     }
 
+    unsigned long long n_eval = 0, n_acc = 0;  // Count only (wave-uniform)
+    const unsigned long long n_valid = Count ? __popcll(__ballot(valid)) : 0;
     for (int r0 = 0; r0 < a.nh; r0 += R) {
         // bounds of the chunk's receivers over the wave's cells
         int rlo = BND_EMPTY, rhi = -BND_EMPTY, rtm = BND_EMPTY;
@@ -617,6 +621,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             if (js < je) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
             const bool live = valid && (st.iw & 0xFFFFu) != 0u;
             if (__ballot(live) == 0) continue;
+            if constexpr (Count) n_eval += n_valid * (unsigned long long)min(R, a.nh - r0);
             const int s_int8 = (((int)st.iw >> 16) + a.c_int) << 3;
             const float s_frac = st.frac + a.c_frac;
             const float s_w = (float)(st.iw & 0xFFFFu) * (inv32k * inv32k);
@@ -630,7 +635,9 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 // passes the depth mute and the bin range, else 0; skipped
                 // traces (wap < 0) leave aw < 0, and the receivers past nh
                 // that pad the last chunk have zero tuple weight (their wap
-                // entries are finite: the array is padded).  The depth-mute
+                // entries are the next shot's, or the pad after the last
+                // shot: the whole array is zeroed before a run writes its
+                // rows, so they are finite).  The depth-mute
                 // ramp is evaluated only when some lane of the wave has an
                 // angle inside the band; outside it the weight is 1 or 0.  (A
                 // wave-level skip of rejected groups measured slower: few
@@ -677,6 +684,11 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 for (int k = 0; k < G; k++) {
                     q[k] = res_phase1(st, s_int8, s_frac, rt[g0 + k], aw[k], u[k], a);
                     any |= q[k].ok;
+                }
+                if constexpr (Count) {
+#pragma unroll
+                    for (int k = 0; k < G; k++) n_acc += __popcll(__ballot(q[k].ok & valid));
+                    continue;
                 }
                 if (__ballot(any) == 0) continue;
                 if constexpr (Model) {
@@ -773,6 +785,13 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             }
         }
     }
+    if constexpr (Count) {
+        if ((lid & 63) == 0) {
+            atomicAdd(a.counts, n_eval);
+            atomicAdd(a.counts + 1, n_acc);
+        }
+        return;
+    }
     if (!Model && valid) {
         float *dst = nsplit > 1 ? a.part + (size_t)sg * a.ng * a.nzx : a.cig;
         for (int g = 0; g < a.ng; g++) dst[(size_t)g * a.nzx + i] = my[g * P];
@@ -805,6 +824,14 @@ __global__ __launch_bounds__(TZ *TX, 3) void k_migrate_ph(const MigResArgs a)
     extern __shared__ float acc[];
     migrate_tile<TZ, TX, R, G, Fmt>(a, xcd_tile(blockIdx.x, gridDim.x), acc, (int)blockIdx.y,
                                     (int)gridDim.y);
+}
+
+template <int TZ, int TX, int R, int G>
+__global__ __launch_bounds__(TZ *TX, 3) void k_count_ph(const MigResArgs a)
+{
+    extern __shared__ float acc[];
+    migrate_tile<TZ, TX, R, G, 0, false, true>(a, xcd_tile(blockIdx.x, gridDim.x), acc,
+                                               (int)blockIdx.y, (int)gridDim.y);
 }
 
 template <int TZ, int TX, int R, int G>
@@ -849,6 +876,21 @@ int migrate_lds_bytes(int ng, int tile)
     return (ng + 1) * tz * tx * (int)sizeof(float);
 }
 
+// Shot groups of a resident launch over ntiles tiles: enough blocks for about
+// kSplitBlocks in all, at least 8 shots per group, at most the groups the
+// partial buffer holds; KMIG_SPLIT forces a count.
+static int res_splits(const MigResArgs &a, int ntiles)
+{
+    int nsplit = 1;
+    if (a.part_splits > 1) {
+        nsplit = std::min(a.part_splits, std::max(1, (kSplitBlocks + ntiles - 1) / ntiles));
+        nsplit = std::min(nsplit, std::max(1, a.nshots / 8));
+        if (const char *v = std::getenv("KMIG_SPLIT"))
+            nsplit = std::min(std::max(1, std::atoi(v)), std::min(a.part_splits, a.nshots));
+    }
+    return nsplit;
+}
+
 template <int TZ, int TX, bool Model = false>
 static int launch_mig_t(const MigArgs &a, hipStream_t st)
 {
@@ -882,13 +924,7 @@ static int launch_ph_t(const MigResArgs &a, hipStream_t st)
     // few thousand blocks per round keep the tail short.  KMIG_SPLIT forces a
     // count (tests).
     const int ntiles = ntz * ntx;
-    int nsplit = 1;
-    if (a.part_splits > 1) {
-        nsplit = std::min(a.part_splits, std::max(1, (kSplitBlocks + ntiles - 1) / ntiles));
-        nsplit = std::min(nsplit, std::max(1, a.nshots / 8));
-        if (const char *v = std::getenv("KMIG_SPLIT"))
-            nsplit = std::min(std::max(1, std::atoi(v)), std::min(a.part_splits, a.nshots));
-    }
+    const int nsplit = res_splits(a, ntiles);
     hipLaunchKernelGGL((k_migrate_ph<TZ, TX, R, G, Probe, Fmt>), dim3((unsigned)ntiles,
                        (unsigned)nsplit), dim3(TZ * TX), lds, st, a);
     hipError_t e = hipGetLastError();
@@ -982,6 +1018,41 @@ int launch_model_resident(const MigResArgs &a, int tile, void *stream)
 int launch_migrate_resident_probe(const MigResArgs &a, int tile, int precision, void *stream)
 {
     return launch_res_prec<true>(a, tile, precision, (hipStream_t)stream);
+}
+
+template <int TZ, int TX, int R, int G>
+static int launch_count_t(const MigResArgs &a, hipStream_t st)
+{
+    const int ntz = (a.nz + TZ - 1) / TZ, ntx = (a.nx + TX - 1) / TX;
+    const size_t lds = (size_t)(a.ng + 1) * TZ * TX * sizeof(float);
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void *)k_count_ph<TZ, TX, R, G>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)lds);
+        if (e != hipSuccess) return (int)e;
+    }
+    const int ntiles = ntz * ntx;
+    hipLaunchKernelGGL((k_count_ph<TZ, TX, R, G>), dim3((unsigned)ntiles,
+                       (unsigned)res_splits(a, ntiles)), dim3(TZ * TX), lds, st, a);
+    return (int)hipGetLastError();
+}
+
+int launch_count_resident(const MigResArgs &a, int tile, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    switch (tile) {
+    case 10: return launch_count_t<16, 8, 16, 4>(a, st);
+    case 11: return launch_count_t<16, 8, 8, 8>(a, st);
+    case 13: return launch_count_t<8, 8, 16, 4>(a, st);
+    case 15: return launch_count_t<8, 8, 8, 4>(a, st);
+    case 16: return launch_count_t<32, 4, 8, 4>(a, st);
+    case 17: return launch_count_t<64, 2, 8, 4>(a, st);
+    case 18: return launch_count_t<16, 8, 4, 4>(a, st);
+    case 19: return launch_count_t<16, 16, 8, 4>(a, st);
+    case 20: return launch_count_t<16, 8, 8, 2>(a, st);
+    case 21: return launch_count_t<8, 8, 4, 4>(a, st);
+    default: return launch_count_t<16, 8, 8, 4>(a, st);  // preset 9
+    }
 }
 
 int migrate_tiles(int nz, int nx, int tile)

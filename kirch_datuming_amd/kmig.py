@@ -32,7 +32,7 @@ EXPORTS = ("kmig_params_default", "kmig_params_parse", "kmig_params_check", "kmi
            "kmig_stream", "kmig_version", "kmig_batch_create", "kmig_batch_run",
            "kmig_batch_sync", "kmig_batch_stats", "kmig_batch_last_error", "kmig_batch_destroy",
            "kmig_set_surrogate", "kmig_get_surrogate_stats", "kmig_batch_stitch",
-           "kmig_model")
+           "kmig_model", "kmig_count_updates")
 
 
 class KmigError(RuntimeError):
@@ -104,6 +104,8 @@ def lib():
         L.kmig_get_cig.argtypes = [vp, fp]
         L.kmig_get_qc.argtypes = [vp, fp, fp]
         L.kmig_get_stats.argtypes = [vp, C.POINTER(KmigStats)]
+        dp = C.POINTER(C.c_double)
+        L.kmig_count_updates.argtypes = [vp, dp, dp, dp]
         L.kmig_stream.argtypes = [vp]
         L.kmig_stream.restype = vp
         L.kmig_version.restype = C.c_char_p
@@ -233,6 +235,14 @@ class Migrator:
 
     def stream(self):
         return self.L.kmig_stream(self.ctx)
+
+    def count_updates(self) -> dict:
+        """Nominal, evaluated (groups surviving the culls) and accepted
+        (taps inside the trace, nonzero weight) updates of the last run's
+        resident launch, from a counting re-run (synchronous)."""
+        n, e, a = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self.L.kmig_count_updates(self.ctx, C.byref(n), C.byref(e), C.byref(a)))
+        return {"nominal": n.value, "evaluated": e.value, "accepted": a.value}
 
     def set_surrogate(self, side, net, max_dt=1.0e-3):
         """Take traveltimes of `side` from a trained nn.Surrogate wherever they

@@ -138,7 +138,11 @@ def test_surrogate_in_the_migration():
           f"max residual {max(s['max_residual'] for s in st):.2e} s; CIG relL2 vs tables {rel:.3f}")
     assert frac0 < 0.1
     assert frac > 0.85
-    assert rel < 0.5          # network traveltimes within 1 ms, CIG within 50 % (relL2)
+    # Measured 0.40 relL2 (network traveltimes within 1 ms where taken): the
+    # bound is the measured value with margin.  The goal for the surrogate
+    # form is relL2 <= 1e-3, which this network misses by ~400x: a 1 ms
+    # traveltime error is a 0.1 rad phase error at 15 Hz.
+    assert rel < 0.45
     assert np.array_equal(cig_off, cig_t)
     assert 4 * nn.PARAMS < 0.01 * 4 * 2 * (full.stab.size + full.rtab.size)
     untrained.close()

@@ -1,8 +1,15 @@
 """16-bit tuple storage (FP16, BF16, UNORM16 fields in a 12-byte tuple, with
 per-slot scaling of the anti-alias term; the gradient direction stays FP32)
-against the FP64 oracle.  Bounds are the measured errors with margin; the
-precision target of the 16-bit forms is relL2 <= 1e-3 (met by u16 and, on
-these inputs, fp16; bf16 is reported, not held to it)."""
+against the FP64 oracle.
+
+The precision target of the 16-bit forms is relL2 <= 1e-3.  Each bound below
+is the measured error with margin, so a regression shows:
+  u16   measured 2.0e-5 .. 2.3e-5   bound 5e-5   (target met)
+  fp16  measured 2.3e-4 .. 2.5e-4   bound 5e-4   (target met)
+  bf16  measured 1.8e-3 .. 2.0e-3   bound 4e-3   (target NOT met: bfloat16 keeps
+        8 significant bits, so the fractional sample is stored to ~2^-9 of a
+        sample, 2x the error the 1e-3 target allows)
+"""
 import numpy as np
 import pytest
 
@@ -11,7 +18,8 @@ from tests import golden_cases as gc
 
 pytestmark = pytest.mark.gpu
 
-BOUND = {"fp32": 1e-4, "u16": 1e-3, "fp16": 2e-3, "bf16": 1e-2}
+BOUND = {"fp32": 1e-4, "u16": 5e-5, "fp16": 5e-4, "bf16": 4e-3}
+TARGET = 1e-3  # the 16-bit forms' precision goal (bf16 misses it)
 
 
 def _vz_case():
@@ -75,5 +83,5 @@ def test_fused_16bit_pass_matches_the_separate_passes(monkeypatch, prec):
     ref = O.migrate(dict(kv, precision="fp32"), *args, precision="f64")[0]
     d, r = O.rel_l2(fus, sep), O.rel_l2(fus, ref)
     print(f"{prec}: fused vs separate {d:.2e}, fused vs FP64 {r:.2e}")
-    assert d < 0.5 * BOUND[prec]
+    assert d < BOUND[prec]
     assert r < BOUND[prec]
