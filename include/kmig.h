@@ -225,6 +225,24 @@ int kmig_batch_stitch(kmig_batch *b, const int *start, const int *end, const flo
                       int stitch_blend, int scalex_aper, double scalex_scale, int g_first,
                       int g_count, float *out);
 
+/* Allocation-free form of kmig_batch_stitch, for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+   same checks, weights and result as kmig_batch_stitch, bit for bit, except
+   that out is DEVICE memory (caller-owned, on the layers' GPU, at least
+   g_count x nx x (end[n-1] + 1) floats) and that the lateral edge weights and
+   the non-finite counter live in a static device scratch array of the
+   library instead of per-call hipMalloc buffers: the call makes no
+   hipMalloc, hipFree or other allocating call.  Limit: nx <=
+   KMIG_STITCH_NOALLOC_MAX_NX (KMIG_E_PARAM beyond).  The scratch array is
+   shared by every call on one GPU and keeps the last call's contents, so
+   calls that overlap on one GPU (two host threads, two batches) race.
+   Synchronous: returns once out is written. */
+#define KMIG_STITCH_NOALLOC_MAX_NX 65536
+int kmig_batch_stitch_noalloc(kmig_batch *b, const int *start, const int *end,
+                              const float *scale, int stitch_blend, int scalex_aper,
+                              double scalex_scale, int g_first, int g_count, float *out);
+
 /* ---- learned traveltime surrogate (include/kmig_nn.h) ----
    No counterpart in the reference, whose traveltimes always come from the
    tables (load_table_slice and tinterp_hermite, src/kirchmig2d_cig.cpp:274-284,

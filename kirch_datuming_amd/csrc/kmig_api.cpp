@@ -1742,14 +1742,18 @@ int kmig_batch_stats(const kmig_batch *b, double *updates, double *ms, int *grap
 
 const char *kmig_batch_last_error(const kmig_batch *b) { return b ? b->err.c_str() : "null batch"; }
 
-int kmig_batch_stitch(kmig_batch *b, const int *start, const int *end, const float *scale,
-                      int stitch_blend, int scalex_aper, double scalex_scale, int g_first,
-                      int g_count, float *out)
+}  // extern "C"
+
+namespace {
+
+// The checks and messages of src/layercom_cig.cpp:245-305 and :424-426, and
+// the stitch arguments of the batch's layer CIGs (all but the lateral
+// weights).  g_count <= 0 selects every angle sample.
+int stitch_setup(kmig_batch *b, const int *start, const int *end, const float *scale,
+                 int stitch_blend, int scalex_aper, double scalex_scale, int *g_first,
+                 int *g_count, StitchArgs *s)
 {
-    if (!b || !start || !end || !out) return b ? bfail(b, KMIG_E_PARAM, "null argument")
-                                               : KMIG_E_PARAM;
     const int n = (int)b->ctx.size();
-    // the checks and messages of src/layercom_cig.cpp:245-305 and :424-426
     if (stitch_blend < -1) return bfail(b, KMIG_E_PARAM, "stitch_blend must be -1 or nonnegative");
     if (scalex_aper < 0) return bfail(b, KMIG_E_PARAM, "scalex_aper must be nonnegative");
     if (!std::isfinite(scalex_scale)) return bfail(b, KMIG_E_PARAM, "Invalid scalex_scale");
@@ -1766,14 +1770,14 @@ int kmig_batch_stitch(kmig_batch *b, const int *start, const int *end, const flo
     }
     const kmig_ctx *c0 = b->ctx[0];
     const int nx = c0->nx, ng = c0->p.cig ? c0->p.ng : 1;
-    if (g_count <= 0) {
-        g_first = 0;
-        g_count = ng;
+    if (*g_count <= 0) {
+        *g_first = 0;
+        *g_count = ng;
     }
-    if (g_first < 0 || g_first + g_count > ng)
+    if (*g_first < 0 || *g_first + *g_count > ng)
         return bfail(b, KMIG_E_PARAM, "angle window [%d, %d) outside the %d angle samples",
-                     g_first, g_first + g_count, ng);
-    StitchArgs s{};
+                     *g_first, *g_first + *g_count, ng);
+    *s = StitchArgs{};
     for (int i = 0; i < n; i++) {
         const kmig_ctx *c = b->ctx[i];
         const int ngi = c->p.cig ? c->p.ng : 1;
@@ -1782,11 +1786,35 @@ int kmig_batch_stitch(kmig_batch *b, const int *start, const int *end, const flo
             return bfail(b, KMIG_E_DIM,
                          "layer %d dimensions are %d,%d,%d; expected %d,%d,%d", i + 1, c->nz,
                          c->nx, ngi, end[i] - start[i] + 1, nx, ng);
-        s.layer[i] = c->cig;
-        s.start[i] = start[i];
-        s.end[i] = end[i];
-        s.scale[i] = scale ? scale[i] : 1.0f;
+        s->layer[i] = c->cig;
+        s->start[i] = start[i];
+        s->end[i] = end[i];
+        s->scale[i] = scale ? scale[i] : 1.0f;
     }
+    s->nlayers = n;
+    s->nz = end[n - 1] + 1;
+    s->nx = nx;
+    s->g0 = *g_first;
+    s->ng_out = *g_count;
+    s->stitch_blend = stitch_blend;
+    return KMIG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kmig_batch_stitch(kmig_batch *b, const int *start, const int *end, const float *scale,
+                      int stitch_blend, int scalex_aper, double scalex_scale, int g_first,
+                      int g_count, float *out)
+{
+    if (!b || !start || !end || !out) return b ? bfail(b, KMIG_E_PARAM, "null argument")
+                                               : KMIG_E_PARAM;
+    StitchArgs s{};
+    int rc = stitch_setup(b, start, end, scale, stitch_blend, scalex_aper, scalex_scale, &g_first,
+                          &g_count, &s);
+    if (rc) return rc;
+    const int nx = s.nx;
     // lateral edge scale (src/layercom_cig.cpp:445-456)
     std::vector<float> lat(nx, 1.0f);
     for (int i2 = 0; i2 < nx && scalex_aper > 0; i2++) {
@@ -1797,12 +1825,6 @@ int kmig_batch_stitch(kmig_batch *b, const int *start, const int *end, const flo
             lat[i2] = (float)(1.0 + (scalex_scale - 1.0) * taper);
         }
     }
-    s.nlayers = n;
-    s.nz = end[n - 1] + 1;
-    s.nx = nx;
-    s.g0 = g_first;
-    s.ng_out = g_count;
-    s.stitch_blend = stitch_blend;
     const size_t nout = (size_t)s.nz * nx * g_count;
     float *dlat = nullptr, *dout = nullptr;
     unsigned *dbad = nullptr;
@@ -1823,6 +1845,50 @@ int kmig_batch_stitch(kmig_batch *b, const int *start, const int *end, const flo
     dfree(dbad);
     if (e != hipSuccess)
         return bfail(b, KMIG_E_DEVICE, "stitch failed: %s", hipGetErrorString(e));
+    if (bad) return bfail(b, KMIG_E_PARAM, "layers contain non-finite samples after scaling");
+    return KMIG_OK;
+}
+
+int kmig_batch_stitch_noalloc(kmig_batch *b, const int *start, const int *end, const float *scale,
+                              int stitch_blend, int scalex_aper, double scalex_scale, int g_first,
+                              int g_count, float *out)
+{
+    if (!b || !start || !end || !out) return b ? bfail(b, KMIG_E_PARAM, "null argument")
+                                               : KMIG_E_PARAM;
+    StitchArgs s{};
+    int rc = stitch_setup(b, start, end, scale, stitch_blend, scalex_aper, scalex_scale, &g_first,
+                          &g_count, &s);
+    if (rc) return rc;
+    if (s.nx > KMIG_STITCH_NOALLOC_MAX_NX)
+        return bfail(b, KMIG_E_PARAM,
+                     "the allocation-free stitch takes at most %d lateral samples (n2 = %d); "
+                     "use kmig_batch_stitch", KMIG_STITCH_NOALLOC_MAX_NX, s.nx);
+    // the output must be device memory the kernel can write
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, out) != hipSuccess ||
+        !(at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged)) {
+        (void)hipGetLastError();  // clear the lookup's error state
+        return bfail(b, KMIG_E_PARAM,
+                     "the allocation-free stitch writes device memory: out is not a device "
+                     "pointer");
+    }
+    if (at.type == hipMemoryTypeDevice && at.device != b->ctx[0]->device)
+        return bfail(b, KMIG_E_PARAM, "out lies on device %d, the layers on device %d",
+                     at.device, b->ctx[0]->device);
+    // and it must hold the whole window: g_count x nx x nz floats
+    const size_t nout = (size_t)s.nz * s.nx * g_count;
+    void *abase = nullptr;
+    size_t asize = 0;
+    if (at.type == hipMemoryTypeDevice &&
+        hipMemGetAddressRange(&abase, &asize, out) == hipSuccess &&
+        (char *)(out + nout) > (char *)abase + asize)
+        return bfail(b, KMIG_E_DIM,
+                     "out holds %zu bytes past its address, the stitch writes %zu",
+                     (size_t)((char *)abase + asize - (char *)out), nout * sizeof(float));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    unsigned bad = 0;
+    const int e = launch_stitch_noalloc(s, scalex_aper, scalex_scale, out, &bad, b->stream);
+    if (e) return bfail(b, KMIG_E_DEVICE, "stitch failed: %s", hipGetErrorString((hipError_t)e));
     if (bad) return bfail(b, KMIG_E_PARAM, "layers contain non-finite samples after scaling");
     return KMIG_OK;
 }

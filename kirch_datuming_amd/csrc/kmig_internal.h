@@ -277,6 +277,12 @@ int launch_pack16(const Tuples &in, PackedTuple16 *out, SlotScale *scale, int ns
                   size_t nzx, float dt, float th_scale, int precision, void *stream);
 int launch_pairs(const float *prep, float *pairs, long ntraces, int nt, void *stream);
 int launch_stitch(const StitchArgs &a, float *out, unsigned *nonfinite, void *stream);
+// the same with the lateral weights and the counter in the library's static
+// device scratch (no allocation); waits for the stream and returns the count
+// of non-finite samples in *nonfinite_host
+constexpr int KMIG_MAX_DEVICES = 64;
+int launch_stitch_noalloc(StitchArgs a, int scalex_aper, double scalex_scale, float *out,
+                          unsigned *nonfinite_host, void *stream);
 // tile: preset 9..21 (kmig_migrate.hip); precision: KMIG_PREC_* tuple format
 int launch_migrate_resident(const MigResArgs &a, int tile, int precision, void *stream);
 // adj=0: sprays the image held in a.cig into a.spray (FP32 tuples; presets

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "kmig_internal.h"
 
@@ -918,6 +919,79 @@ int launch_stitch(const StitchArgs &a, float *out, unsigned *nonfinite, void *st
     hipLaunchKernelGGL(k_stitch, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, a, out,
                        nonfinite);
     return (int)hipGetLastError();
+}
+
+// ------------------------------------------------ allocation-free stitch --
+//
+// Scratch of the allocation-free stitch (kmig_batch_stitch_noalloc): the
+// lateral edge weights in [0, KMIG_STITCH_NOALLOC_MAX_NX) and the non-finite
+// sample counter right after them.  A file-scope device array: every GPU the
+// library runs on holds its own copy until the process ends, and every call
+// on one GPU shares it, so calls that overlap on one GPU race.  Host code
+// never uses the array's name as an address (on the host it is only a
+// stand-in): stitch_scratch() takes each device's address of it once, with
+// that device current.
+constexpr int kScratchCounter = KMIG_STITCH_NOALLOC_MAX_NX;
+static __device__ __align__(256) float scratch[KMIG_STITCH_NOALLOC_MAX_NX + 64];
+
+static std::mutex g_scratch_mu;
+static float *g_scratch_ptr[KMIG_MAX_DEVICES];
+
+static int stitch_scratch(float **ptr)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= KMIG_MAX_DEVICES) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        if (!p) return (int)hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = (float *)p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return 0;
+}
+
+// Lateral edge scale of src/layercom_cig.cpp:445-456 (the same double
+// arithmetic as the host loop of kmig_batch_stitch, no contraction, so the
+// weights are bitwise the same); thread 0 also clears the counter (a store,
+// so the call holds no memset node under graph capture).
+__global__ __launch_bounds__(256) void k_stitch_lat(float *lat, int nx, int aper, double sscale,
+                                                    unsigned *counter)
+{
+    const int i2 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i2 == 0) *counter = 0u;
+    if (i2 >= nx) return;
+    float v = 1.0f;
+    const int edge = min(i2, nx - 1 - i2);
+    if (aper > 0 && edge < aper) {
+        const double taper = aper == 1 ? 1.0 : 1.0 - (double)edge / (double)(aper - 1);
+        v = (float)(1.0 + (sscale - 1.0) * taper);
+    }
+    lat[i2] = v;
+}
+
+int launch_stitch_noalloc(StitchArgs a, int scalex_aper, double scalex_scale, float *out,
+                          unsigned *nonfinite_host, void *stream)
+{
+    if (a.nx < 1 || a.nx > KMIG_STITCH_NOALLOC_MAX_NX) return (int)hipErrorInvalidValue;
+    float *base = nullptr;
+    int e = stitch_scratch(&base);
+    if (e) return e;
+    hipStream_t st = (hipStream_t)stream;
+    unsigned *counter = (unsigned *)(base + kScratchCounter);
+    hipLaunchKernelGGL(k_stitch_lat, dim3(nblk((size_t)a.nx, 256)), dim3(256), 0, st, base, a.nx,
+                       scalex_aper, scalex_scale, counter);
+    if ((e = (int)hipGetLastError())) return e;
+    a.lat = base;
+    if ((e = launch_stitch(a, out, counter, stream))) return e;
+    if ((e = (int)hipMemcpyAsync(nonfinite_host, counter, sizeof(unsigned),
+                                 hipMemcpyDeviceToHost, st)))
+        return e;
+    return (int)hipStreamSynchronize(st);
 }
 
 // Direction bounds per (slot, cell of BND_CZ x BND_CX points), over the

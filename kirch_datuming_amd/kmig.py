@@ -32,7 +32,7 @@ EXPORTS = ("kmig_params_default", "kmig_params_parse", "kmig_params_check", "kmi
            "kmig_stream", "kmig_version", "kmig_batch_create", "kmig_batch_run",
            "kmig_batch_sync", "kmig_batch_stats", "kmig_batch_last_error", "kmig_batch_destroy",
            "kmig_set_surrogate", "kmig_get_surrogate_stats", "kmig_batch_stitch",
-           "kmig_model", "kmig_count_updates")
+           "kmig_model", "kmig_count_updates", "kmig_batch_stitch_noalloc")
 
 
 class KmigError(RuntimeError):
@@ -121,6 +121,9 @@ def lib():
         L.kmig_batch_stitch.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                         C.POINTER(C.c_float), C.c_int, C.c_int, C.c_double,
                                         C.c_int, C.c_int, fp]
+        L.kmig_batch_stitch_noalloc.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                C.POINTER(C.c_float), C.c_int, C.c_int,
+                                                C.c_double, C.c_int, C.c_int, fp]
         L.kmig_set_surrogate.argtypes = [vp, C.c_int, vp, C.c_float]
         L.kmig_get_surrogate_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
                                                C.POINTER(C.c_double), C.POINTER(C.c_float)]
@@ -323,6 +326,24 @@ class Batch:
                                            int(scalex_aper), float(scalex_scale), int(g0),
                                            int(gn), _fp(out)))
         return out
+
+    def stitch_into(self, out_ptr, start, end, scale=None, stitch_blend=15, scalex_aper=0,
+                    scalex_scale=1.0, angles=None):
+        """The same stitch written to caller-owned DEVICE memory at address
+        out_ptr (an int; at least count * nx * (end[-1] + 1) floats), through
+        kmig_batch_stitch_noalloc: no device allocation in the call."""
+        n = len(self.m)
+        st = (C.c_int * n)(*[int(v) for v in start])
+        en = (C.c_int * n)(*[int(v) for v in end])
+        sc = (C.c_float * n)(*[float(v) for v in scale]) if scale is not None else None
+        m0 = self.m[0]
+        ng = m0.p.ng if m0.p.cig else 1
+        g0, gn = angles if angles is not None else (0, ng)
+        self._chk(self.L.kmig_batch_stitch_noalloc(self.b, st, en, sc, int(stitch_blend),
+                                                   int(scalex_aper), float(scalex_scale),
+                                                   int(g0), int(gn),
+                                                   C.cast(C.c_void_p(int(out_ptr)),
+                                                          C.POINTER(C.c_float))))
 
     def close(self):
         if self.b:
