@@ -37,7 +37,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="salt",
-                    choices=["salt", "small", "layers", "surrogate", "datum"])
+                    choices=["salt", "salt_cmp1", "small", "layers", "surrogate", "datum"])
     ap.add_argument("--grf-models", type=int, default=256,
                     help="surrogate: velocity models of the pretraining set")
     ap.add_argument("--fit-steps", type=int, default=8000,
@@ -94,7 +94,12 @@ def main():
         return bench_datum(args, cores)
     t0 = time.time()
     log(f"building the {args.config} workload on {cores} host threads")
-    w = configs.salt(threads=cores) if args.config == "salt" else configs.small(threads=cores)
+    if args.config == "salt":
+        w = configs.salt(threads=cores)
+    elif args.config == "salt_cmp1":
+        w = configs.salt(threads=cores, cmp=1)
+    else:
+        w = configs.small(threads=cores)
     log(f"workload ready in {time.time() - t0:.1f} s: {w.name}, {w.updates:.4e} updates")
     if args.adj == 0:
         return bench_model(args, cores, w)

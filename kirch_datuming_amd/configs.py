@@ -140,26 +140,38 @@ def small(threads=0, seed=20260101) -> Workload:
                     RT, RD, tax + [[nh, h0, dh]], vel)
 
 
-def salt(threads=0, seed=20260202, nshots=768) -> Workload:
+def salt(threads=0, seed=20260202, nshots=768, cmp=0) -> Workload:
+    """The salt workload.  cmp=0 (the workflow's geometry): 401 fixed
+    receivers every 0.025 km.  cmp=1 (the program's default geometry): 401
+    split-spread offsets -2.5 .. 2.5 km every 0.0125 km moving with the shot,
+    receiver tables at the 800 positions 0 .. 9.9875 km every 0.0125 km
+    (receivers beyond the model edge take the edge table, as the reference's
+    clamped table lookup does)."""
     L = synth()
     nz, nx, d = 400, 1000, 0.01
     vel = np.empty((nx, nz), np.float32)
     L.kms_velocity_salt(_p(vel), nz, nx, d, d)
     ns, s0, ds = nshots, 0.0, 0.0125
-    nh, h0, dh = 401, 0.0, 0.025
     nt, dt = 2001, 0.002
+    if cmp:
+        nh, h0, dh = 401, -2.5, 0.0125
+        nr, r0, dr = 800, 0.0, 0.0125
+    else:
+        nh, h0, dh = 401, 0.0, 0.025
+        nr, r0, dr = nh, h0, dh
     ST, SD = tables(vel, d, d, ns, s0, ds, 2, threads)
-    RT, RD = tables(vel, d, d, nh, h0, dh, 2, threads)
+    RT, RD = tables(vel, d, d, nr, r0, dr, 2, threads)
     data = np.empty((ns, nh, nt), np.float32)
-    L.kms_data_scatterers(_p(data), nt, 0.0, dt, nh, h0, dh, ns, s0, ds, 0, _p(ST), ns, s0, ds,
-                          _p(RT), nh, h0, dh, nz, nx, 60, 20.0, 1.5, seed)
-    kv = dict(cmp=0, mode="angle", angle_min=0, angle_max=60, angle_step=1, angle_n=61,
+    L.kms_data_scatterers(_p(data), nt, 0.0, dt, nh, h0, dh, ns, s0, ds, int(cmp), _p(ST), ns,
+                          s0, ds, _p(RT), nr, r0, dr, nz, nx, 60, 20.0, 1.5, seed)
+    kv = dict(cmp=int(cmp), mode="angle", angle_min=0, angle_max=60, angle_step=1, angle_n=61,
               direct_mute=1, direct_mute_slope=50, direct_mute_intercept=0,
               direct_mute_taper=0.05)
     tax = [[nz, 0.0, d], [nx, 0.0, d]]
-    return Workload("salt 400x1000, 768 shots x 401 receivers, nt=2001", kv, data,
+    geo = "401 offsets -2.5..2.5 km (cmp=1)" if cmp else "401 receivers"
+    return Workload(f"salt 400x1000, {ns} shots x {geo}, nt=2001", kv, data,
                     [[nt, 0.0, dt], [nh, h0, dh], [ns, s0, ds]], ST, SD, tax + [[ns, s0, ds]],
-                    RT, RD, tax + [[nh, h0, dh]], vel)
+                    RT, RD, tax + [[nr, r0, dr]], vel)
 
 
 @dataclass

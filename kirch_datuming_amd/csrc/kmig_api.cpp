@@ -103,6 +103,14 @@ struct kmig_ctx {
     unsigned long long *cnt = nullptr;  // kmig_count_updates: evaluated, accepted
     size_t cnt_cap = 0;
     bool rcv_invariant = false;     // every shot uses receiver slot rslot[ih]
+    // receivers that move with the shot (cmp=1), one batch: receiver columns
+    // (rslot then holds each column's slot; see MigResArgs)
+    bool rcv_columns = false;
+    int ncols = 0;
+    ColTrace *ctab = nullptr;
+    size_t ctab_cap = 0;
+    int *cfirst = nullptr, *clast = nullptr;
+    size_t cfirst_cap = 0, clast_cap = 0;
     int variant_used = 0;
     int tuned_tile = 0;             // autotuned resident preset (0: not tuned yet)
     long long tuned_version = -1;
@@ -272,10 +280,74 @@ size_t free_device_bytes()
     return fr;
 }
 
+// Receiver columns of a run over shots [sb, se) whose receivers move with
+// the shot (one batch): every migrating trace (is, ih) goes to a column that
+// keeps its receiver slot for all shots.  A slot gets one column per trace of
+// one shot at that position (normally one; more where a shot has coincident
+// receivers).  Columns are ordered by receiver position, so the columns of a
+// chunk lie side by side and share their shots.
+struct Columns {
+    std::vector<int> colslot, cfirst, clast;
+    std::vector<ColTrace> ctab;  // [(is - sb) * ncols + c], then RES_WAP_PAD zeros
+};
+
+void build_columns(const kmig_ctx *c, int sb, int se, const std::vector<TraceMeta> &meta,
+                   const std::vector<SlotSpec> &rspec, Columns &out)
+{
+    const int nh = c->nh, nslot = (int)rspec.size(), nsb = se - sb;
+    std::vector<std::vector<int>> cols_of(nslot);
+    std::vector<int> colslot, dup, used(nslot, 0), touched;
+    std::vector<int> col_of((size_t)nsb * nh, -1);
+    for (int js = 0; js < nsb; js++) {
+        for (int ih = 0; ih < nh; ih++) {
+            const int slot = meta[(size_t)js * nh + ih].slot;
+            if (slot < 0) continue;
+            const int k = used[slot]++;
+            if (k == 0) touched.push_back(slot);
+            std::vector<int> &v = cols_of[slot];
+            if (k == (int)v.size()) {
+                v.push_back((int)colslot.size());
+                colslot.push_back(slot);
+                dup.push_back(k);
+            }
+            col_of[(size_t)js * nh + ih] = v[k];
+        }
+        for (int sl : touched) used[sl] = 0;
+        touched.clear();
+    }
+    const int ncols = (int)colslot.size();
+    std::vector<int> order(ncols), rank(ncols);
+    for (int k = 0; k < ncols; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const float px = rspec[colslot[x]].surf_x, py = rspec[colslot[y]].surf_x;
+        if (px != py) return px < py;
+        if (colslot[x] != colslot[y]) return colslot[x] < colslot[y];
+        return dup[x] < dup[y];
+    });
+    out.colslot.resize(ncols);
+    for (int r = 0; r < ncols; r++) {
+        rank[order[r]] = r;
+        out.colslot[r] = colslot[order[r]];
+    }
+    out.ctab.assign((size_t)nsb * ncols, ColTrace{0, -1.0f});
+    out.ctab.resize((size_t)nsb * ncols + RES_WAP_PAD, ColTrace{0, 0.0f});
+    out.cfirst.assign(ncols, 1 << 30);
+    out.clast.assign(ncols, -1);
+    for (int js = 0; js < nsb; js++)
+        for (int ih = 0; ih < nh; ih++) {
+            const int col = col_of[(size_t)js * nh + ih];
+            if (col < 0) continue;
+            const int r = rank[col];
+            out.ctab[(size_t)js * ncols + r] = ColTrace{ih, meta[(size_t)js * nh + ih].wap};
+            out.cfirst[r] = std::min(out.cfirst[r], js);
+            out.clast[r] = std::max(out.clast[r], js);
+        }
+}
+
 // Build TraceMeta for shots [sb, se), the batches and their slot specs.
 int plan_run(kmig_ctx *c, int sb, int se, std::vector<TraceMeta> &meta,
              std::vector<SlotSpec> &sspec, std::vector<SlotSpec> &rspec,
-             std::vector<float> &wapv, std::vector<int> &rslot)
+             std::vector<float> &wapv, std::vector<int> &rslot, Columns &cols)
 {
     const kmig_params &p = c->p;
     const Side &S = c->side[0], &Rt = c->side[1];
@@ -306,6 +378,52 @@ int plan_run(kmig_ctx *c, int sb, int se, std::vector<TraceMeta> &meta,
     Batch cur{sb, sb, 0, 0};
     std::unordered_map<uint32_t, int> slot_of;
     std::vector<SlotSpec> cur_r;
+    // Receivers that move with the shot, planned for the resident kernel:
+    // s + h in float gives one physical receiver position several values a
+    // few ulps apart (0.0125 k + 0.0125 j rounds differently for different
+    // k, j), and each value would be its own tuple slot and receiver column,
+    // splitting the shots of one position over several columns.  Positions
+    // within 8 ulps of the largest |s| + |h| of the run (the size of the
+    // sum's rounding error) of the smallest of their group therefore share
+    // that smallest value's slot: a position change of at most 8 ulps (7.6e-6
+    // km for |s| + |h| up to 16 km), whose traveltime change (about 5e-9 s)
+    // lies far below the float rounding of the traveltimes themselves.  The
+    // trace records (aperture index, mute) keep each trace's own position.
+    std::unordered_map<uint32_t, float> pos_rep;
+    const bool group_pos = p.cmp && angle && p.angle_interp == 1 && p.antialias >= 0.0f &&
+                           p.variant != KMIG_VARIANT_STREAM;
+    if (group_pos) {
+        std::vector<float> all;
+        all.reserve((size_t)(se - sb) * nh);
+        float mag = 0.0f;
+        for (int is = sb; is < se; is++) {
+            const float s = c->s0 + (float)is * ds;
+            for (int ih = 0; ih < nh; ih++) {
+                const float h = c->h0 + (float)ih * dh;
+                all.push_back(s + h);
+                mag = std::max(mag, std::fabs(s) + std::fabs(h));
+            }
+        }
+        const float tol =
+            8.0f * (std::nextafter(mag, std::numeric_limits<float>::infinity()) - mag);
+        std::sort(all.begin(), all.end());
+        all.erase(std::unique(all.begin(), all.end()), all.end());
+        float first = 0.0f;
+        for (size_t k = 0; k < all.size(); k++) {
+            const float x = all[k];
+            if (k == 0 || !(x - first <= tol)) first = x;
+            uint32_t key;
+            memcpy(&key, &x, 4);
+            pos_rep[key] = first;
+        }
+    }
+    auto slot_pos = [&](float pos) {
+        if (!group_pos) return pos;
+        uint32_t key;
+        memcpy(&key, &pos, 4);
+        auto it = pos_rep.find(key);
+        return it == pos_rep.end() ? pos : it->second;
+    };
     auto close = [&](int s1) {
         cur.s1 = s1;
         cur.nrcv = (int)cur_r.size();
@@ -358,7 +476,7 @@ int plan_run(kmig_ctx *c, int sb, int se, std::vector<TraceMeta> &meta,
             }
             m.wap = skip ? -1.0f : wap;
             wapv[(size_t)(is - sb) * nh + ih] = m.wap;
-            if (!skip) need.push_back({ih, receiver_pos});
+            if (!skip) need.push_back({ih, slot_pos(receiver_pos)});
         }
         // new receiver positions of this shot
         int fresh = 0;
@@ -392,6 +510,14 @@ int plan_run(kmig_ctx *c, int sb, int se, std::vector<TraceMeta> &meta,
     }
     close(se);
     c->rcv_invariant = invariant && c->batches.size() == 1;
+    // receivers that move with the shot, all in one batch, for the resident
+    // kernel's bins (angle gathers, linear bins): receiver columns
+    c->rcv_columns = false;
+    cols = Columns{};
+    if (!invariant && c->batches.size() == 1 && p.cig && p.cig_is_angle && p.angle_interp == 1) {
+        build_columns(c, sb, se, meta, rspec, cols);
+        c->rcv_columns = !cols.colslot.empty();
+    }
     return KMIG_OK;
 }
 
@@ -501,6 +627,12 @@ MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
     R.ncx = (c->nx + BND_CX - 1) / BND_CX;
     R.sbnd_stride = nshots;
     R.nrslots = c->batches.empty() ? 0 : c->batches[0].nrcv;
+    if (c->rcv_columns) {
+        R.ctab = c->ctab;
+        R.cfirst = c->cfirst;
+        R.clast = c->clast;
+        R.ncols = c->ncols;
+    }
     R.cig = c->cig;
     const double cst = ((double)p.tau - (double)c->t0) / (double)c->dt;
     const double ci = std::floor(cst);
@@ -535,7 +667,8 @@ int autotune(kmig_ctx *c, const MigResArgs &full)
     if (c->tuned_tile > 0 && c->tuned_version == c->geo_version) return KMIG_OK;
     static const int presets32[] = {9, 15, 16, 20, 11, 10, 18};
     static const int presets16[] = {9, 15, 10, 13, 11};
-    const bool p16 = c->p.precision != KMIG_PREC_FP32;
+    // the 16-bit formats and receiver columns are built for five presets
+    const bool p16 = c->p.precision != KMIG_PREC_FP32 || c->rcv_columns;
     const int *presets = p16 ? presets16 : presets32;
     const int npresets = p16 ? 5 : 7;
     const double per_shot = (double)c->nh * (double)c->nzx;
@@ -617,7 +750,8 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
         std::vector<SlotSpec> sspec, rspec;
         std::vector<float> wapv;
         std::vector<int> rslot;
-        if ((rc = plan_run(c, sb, se, meta, sspec, rspec, wapv, rslot))) return rc;
+        Columns cols;
+        if ((rc = plan_run(c, sb, se, meta, sspec, rspec, wapv, rslot, cols))) return rc;
         if ((rc = check_tables_type_p(c, sspec))) return rc;
         if ((rc = check_tables_type_p(c, rspec))) return rc;
         // meta is indexed by global trace number
@@ -659,15 +793,15 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
         // resident variant: angle CIG, two-bin interpolation, receivers that do
         // not move with the shot, one batch, times within the int16 sample range
         const double cst = ((double)p.tau - (double)c->t0) / (double)c->dt;
-        const bool res_ok = angle && p.angle_interp == 1 && c->rcv_invariant &&
+        const bool res_ok = angle && p.angle_interp == 1 &&
+                            (c->rcv_invariant || c->rcv_columns) &&
                             p.antialias >= 0.0f && std::fabs(cst) + (double)c->nt < 30000.0;
         int variant = KMIG_VARIANT_STREAM;
         if (p.variant == KMIG_VARIANT_RESIDENT) {
             if (!res_ok)
                 return fail(c, KMIG_E_PARAM,
-                            "variant=resident needs mode=angle, angle_interp=1, antialias >= 0, "
-                            "receivers that do not move with the shot (cmp=0) and tuples that "
-                            "fit one batch");
+                            "variant=resident needs mode=angle, angle_interp=1, antialias >= 0 "
+                            "and tuples that fit one batch");
             variant = KMIG_VARIANT_RESIDENT;
         } else if (p.variant == KMIG_VARIANT_AUTO && res_ok) {
             variant = KMIG_VARIANT_RESIDENT;
@@ -675,8 +809,8 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
         if (p.precision != KMIG_PREC_FP32 && variant != KMIG_VARIANT_RESIDENT)
             return fail(c, KMIG_E_PARAM,
                         "precision=fp16/bf16 applies to the resident kernel: it needs "
-                        "mode=angle, angle_interp=1 and receivers that do not move with the "
-                        "shot (cmp=0)");
+                        "mode=angle, angle_interp=1, antialias >= 0 and tuples that fit one "
+                        "batch");
         c->variant_used = variant;
         if (variant == KMIG_VARIANT_RESIDENT && p.precision != KMIG_PREC_FP32) {
             if ((rc = ensure(c, &c->pk16_s, &c->pk16_s_cap, std::max<size_t>(1, maxs) * c->nzx)))
@@ -714,12 +848,27 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
             const size_t nwap = (size_t)c->ns * c->nh;
             if ((rc = ensure(c, &c->wap, &c->wap_cap, nwap + RES_WAP_PAD))) return rc;
             HIPCHK(c, hipMemset(c->wap, 0, (nwap + RES_WAP_PAD) * sizeof(float)));
-            if ((rc = ensure(c, &c->rslot, &c->rslot_cap, (size_t)c->nh))) return rc;
             if (!wapv.empty())
                 HIPCHK(c, hipMemcpy(c->wap + (size_t)sb * c->nh, wapv.data(),
                                     wapv.size() * sizeof(float), hipMemcpyHostToDevice));
-            HIPCHK(c, hipMemcpy(c->rslot, rslot.data(), rslot.size() * sizeof(int),
+            // receiver slot per receiver index (cmp=0) or per receiver column
+            const std::vector<int> &rs = c->rcv_columns ? cols.colslot : rslot;
+            if ((rc = ensure(c, &c->rslot, &c->rslot_cap, std::max<size_t>(1, rs.size()))))
+                return rc;
+            HIPCHK(c, hipMemcpy(c->rslot, rs.data(), rs.size() * sizeof(int),
                                 hipMemcpyHostToDevice));
+            c->ncols = c->rcv_columns ? (int)cols.colslot.size() : 0;
+            if (c->rcv_columns) {
+                if ((rc = ensure(c, &c->ctab, &c->ctab_cap, cols.ctab.size()))) return rc;
+                if ((rc = ensure(c, &c->cfirst, &c->cfirst_cap, cols.cfirst.size()))) return rc;
+                if ((rc = ensure(c, &c->clast, &c->clast_cap, cols.clast.size()))) return rc;
+                HIPCHK(c, hipMemcpy(c->ctab, cols.ctab.data(), cols.ctab.size() * sizeof(ColTrace),
+                                    hipMemcpyHostToDevice));
+                HIPCHK(c, hipMemcpy(c->cfirst, cols.cfirst.data(),
+                                    cols.cfirst.size() * sizeof(int), hipMemcpyHostToDevice));
+                HIPCHK(c, hipMemcpy(c->clast, cols.clast.data(), cols.clast.size() * sizeof(int),
+                                    hipMemcpyHostToDevice));
+            }
         }
         c->plan_version = c->geo_version;
         c->last_sb = sb;
@@ -1001,6 +1150,9 @@ void kmig_destroy(kmig_ctx *c)
     dfree(c->part);
     dfree(c->rslot);
     dfree(c->cnt);
+    dfree(c->ctab);
+    dfree(c->cfirst);
+    dfree(c->clast);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1684,7 +1836,8 @@ int kmig_batch_run(kmig_batch *b)
         int rc = plan_if_needed(c, 0, c->ns);
         if (rc == KMIG_OK) rc = ensure_outputs(c, 0, c->ns);
         if (rc) return bfail(b, rc, "context %d: %s", k, c->err.c_str());
-        if (c->variant_used != KMIG_VARIANT_RESIDENT || c->batches.size() != 1)
+        if (c->variant_used != KMIG_VARIANT_RESIDENT || c->batches.size() != 1 ||
+            c->rcv_columns)
             return bfail(b, KMIG_E_PARAM,
                          "context %d: batches need the resident kernel (mode=angle, "
                          "angle_interp=1, cmp=0) and tuples that fit one shot batch", k);

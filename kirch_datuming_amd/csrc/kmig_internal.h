@@ -158,6 +158,14 @@ int launch_cell_blocks(const ThBound *bnd, int nslots, int ncell, ThBound *out, 
 int launch_th_bounds(const PackedTuple *pk, const PackedTuple16 *pk16, int nslots, int nz,
                      int nx, ThBound *out, void *stream);
 
+// One trace of a receiver column (resident kernel, receivers that move with
+// the shot): its receiver index in the shot and its trace-aperture weight,
+// or (0, -1) where the shot has no trace in the column.
+struct ColTrace {
+    int ih;
+    float wap;
+};
+
 struct MigResArgs {
     int nz, nx;
     size_t nzx;
@@ -196,6 +204,15 @@ struct MigResArgs {
     // that survives the culls), [1] updates accepted (taps inside the trace
     // and nonzero weight); the counting kernel writes no CIG
     unsigned long long *counts;
+    // Receivers that move with the shot (cmp=1; ctab non-null): the kernel
+    // sweeps receiver columns instead of receiver indices.  Column c keeps
+    // receiver tuple slot rslot[c] for every shot; ctab[(scell_off + js) *
+    // ncols + c] is the trace of batch shot scell_off + js in that column;
+    // cfirst[c] .. clast[c] are the first and last batch shots with a trace
+    // in it (clast < cfirst: none).  ctab is padded past its last row.
+    const ColTrace *ctab;
+    const int *cfirst, *clast;
+    int ncols;
     int c_int;                   // integer part of (tau - t0)/dt
     int t_cull;                  // nt - 2 - c_int: a group whose smallest source plus
                                  // receiver integer samples reach it has every

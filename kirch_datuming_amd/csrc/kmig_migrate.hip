@@ -444,10 +444,16 @@ __device__ __forceinline__ void wave_lds_sync()
 // traces of a.spray instead of reading its taps.
 // Count = true: the same work, but the wave counts the updates it evaluates
 // and accepts (a.counts) and writes no CIG.
-template <int TZ, int TX, int R, int G, int Fmt, bool Model = false, bool Count = false>
+// Cols = true: receivers that move with the shot (cmp=1).  The chunks run
+// over receiver columns (a.ncols; column c keeps tuple slot a.rslot[c]) and
+// each chunk sweeps only the shots with a trace in one of its columns, whose
+// receiver index and trace-aperture weight come from a.ctab.
+template <int TZ, int TX, int R, int G, int Fmt, bool Model = false, bool Count = false,
+          bool Cols = false>
 __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, float *acc,
                                              int sg = 0, int nsplit = 1)
 {
+    static_assert(!(Model && Cols), "adj=0 with receiver columns runs on the stream kernel");
     constexpr int P = TZ * TX;
     const int lid = threadIdx.x;
     // tile order z fastest (measured faster than x fastest on the salt
@@ -529,13 +535,28 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
 
     unsigned long long n_eval = 0, n_acc = 0;  // Count only (wave-uniform)
     const unsigned long long n_valid = Count ? __popcll(__ballot(valid)) : 0;
-    for (int r0 = 0; r0 < a.nh; r0 += R) {
+    // receiver indices (cmp=0) or receiver columns (Cols) of the chunks
+    const int nrec = Cols ? a.ncols : a.nh;
+    for (int r0 = 0; r0 < nrec; r0 += R) {
+        // the chunk's shots: the shot group's, and with columns only those
+        // with a trace in one of the chunk's columns
+        int cjb = jb, cje = je;
+        if constexpr (Cols) {
+            int f = 1 << 30, l = -1;
+            for (int r = 0; r < R && r0 + r < nrec; r++) {
+                f = min(f, a.cfirst[r0 + r]);
+                l = max(l, a.clast[r0 + r]);
+            }
+            cjb = max(jb, f - a.scell_off);
+            cje = min(je, l + 1 - a.scell_off);
+            if (cjb >= cje) continue;
+        }
         // bounds of the chunk's receivers over the wave's cells
         int rlo = BND_EMPTY, rhi = -BND_EMPTY, rtm = BND_EMPTY;
         if (cull) {
             for (int r = 0; r < R; r++) {
                 const int ih = r0 + r;
-                const int slot = ih < a.nh ? a.rslot[ih] : -1;
+                const int slot = ih < nrec ? a.rslot[ih] : -1;
                 if (slot < 0) continue;
 #pragma unroll
                 for (int c = 0; c < NC; c++) {
@@ -552,7 +573,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int ih = r0 + r;
-            const int slot = ih < a.nh ? a.rslot[ih] : -1;
+            const int slot = ih < nrec ? a.rslot[ih] : -1;
             if (slot >= 0) {
                 rt[r] = load_tuple<Fmt>(a, false, (size_t)slot, i);
             } else {
@@ -596,11 +617,11 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
         int blk_open = -1;
         auto next_shot = [&](int j) {
             if (cull) {
-                while (j < je) {
+                while (j < cje) {
                     const int bj = (a.scell_off + j) / TMIN_BLK;
                     if (a.sblk && bj != blk_open) {
                         if (culled_by(a.sblk, (size_t)a.scell_nblk, bj)) {
-                            j = min(je, (bj + 1) * TMIN_BLK - a.scell_off);
+                            j = min(cje, (bj + 1) * TMIN_BLK - a.scell_off);
                             continue;
                         }
                         blk_open = bj;
@@ -611,23 +632,28 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             }
             return j;
         };
-        int js = next_shot(jb);
+        int js = next_shot(cjb);
         typename RawOf<Fmt>::type nxt{};
-        if (js < je) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
-        while (js < je) {
+        if (js < cje) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
+        while (js < cje) {
             const PackedTuple st = decode<Fmt>(nxt);
             const int jc = js;
             js = next_shot(js + 1);
-            if (js < je) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
+            if (js < cje) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
             const bool live = valid && (st.iw & 0xFFFFu) != 0u;
             if (__ballot(live) == 0) continue;
-            if constexpr (Count) n_eval += n_valid * (unsigned long long)min(R, a.nh - r0);
+            if constexpr (Count) n_eval += n_valid * (unsigned long long)min(R, nrec - r0);
             const int s_int8 = (((int)st.iw >> 16) + a.c_int) << 3;
             const float s_frac = st.frac + a.c_frac;
             const float s_w = (float)(st.iw & 0xFFFFu) * (inv32k * inv32k);
             const int is = a.shot0 + jc;
             const float *__restrict__ wrow = a.wap + (size_t)is * a.nh + r0;
             const float2 *__restrict__ trow = pairs + ((size_t)is * a.nh + r0) * a.nt;
+            // the chunk's traces of this shot (Cols): receiver index and
+            // trace-aperture weight per column; the rows are padded, and a
+            // padding column has zero tuple weight
+            const ColTrace *__restrict__ crow =
+                Cols ? a.ctab + (size_t)(a.scell_off + jc) * a.ncols + r0 : nullptr;
 #pragma unroll
             for (int g0 = 0; g0 < R; g0 += G) {
                 // stage 1: opening angle, bin coordinate and weight.  aw =
@@ -646,7 +672,8 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 bool band = false;
 #pragma unroll
                 for (int k = 0; k < G; k++) {
-                    base[k] = wrow[g0 + k] * s_w * (float)(rt[g0 + k].iw & 0xFFFFu);
+                    const float wk = Cols ? crow[g0 + k].wap : wrow[g0 + k];
+                    base[k] = wk * s_w * (float)(rt[g0 + k].iw & 0xFFFFu);
                     ang[k] = pair_angle(st, rt[g0 + k], a.wrap);
                     u[k] = fmaf(ang[k], a.inv_dg, a.g0_inv_dg);  // (angle - g0) / dg
                     aw[k] = ang[k] <= plim ? base[k] : 0.0f;
@@ -761,8 +788,15 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                     // rejected and read the sentinel pair of the shot's last
                     // real trace: their own row would lie past this shot
                     // (past the array for the last shot)
-                    const int kr = min(g0 + k, a.nh - 1 - r0);
-                    const char *tr = (const char *)(trow + (size_t)kr * a.nt);
+                    const char *tr;
+                    if constexpr (Cols) {
+                        // a column without a trace in this shot (weight < 0)
+                        // reads receiver 0's sentinel pair
+                        tr = (const char *)(pairs + ((size_t)is * a.nh + crow[g0 + k].ih) * a.nt);
+                    } else {
+                        const int kr = min(g0 + k, a.nh - 1 - r0);
+                        tr = (const char *)(trow + (size_t)kr * a.nt);
+                    }
                     v0[k] = *(const float2 *)(tr + q[k].o0);
                     vm[k] = *(const float2 *)(tr + q[k].om);
                     vp[k] = *(const float2 *)(tr + q[k].op);
@@ -818,20 +852,20 @@ __global__ __launch_bounds__(256) void k_sum_parts(const float *__restrict__ par
     }
 }
 
-template <int TZ, int TX, int R, int G, bool Probe, int Fmt>
+template <int TZ, int TX, int R, int G, bool Probe, int Fmt, bool Cols = false>
 __global__ __launch_bounds__(TZ *TX, 3) void k_migrate_ph(const MigResArgs a)
 {
     extern __shared__ float acc[];
-    migrate_tile<TZ, TX, R, G, Fmt>(a, xcd_tile(blockIdx.x, gridDim.x), acc, (int)blockIdx.y,
-                                    (int)gridDim.y);
+    migrate_tile<TZ, TX, R, G, Fmt, false, false, Cols>(a, xcd_tile(blockIdx.x, gridDim.x), acc,
+                                                        (int)blockIdx.y, (int)gridDim.y);
 }
 
-template <int TZ, int TX, int R, int G>
+template <int TZ, int TX, int R, int G, bool Cols = false>
 __global__ __launch_bounds__(TZ *TX, 3) void k_count_ph(const MigResArgs a)
 {
     extern __shared__ float acc[];
-    migrate_tile<TZ, TX, R, G, 0, false, true>(a, xcd_tile(blockIdx.x, gridDim.x), acc,
-                                               (int)blockIdx.y, (int)gridDim.y);
+    migrate_tile<TZ, TX, R, G, 0, false, true, Cols>(a, xcd_tile(blockIdx.x, gridDim.x), acc,
+                                                     (int)blockIdx.y, (int)gridDim.y);
 }
 
 template <int TZ, int TX, int R, int G>
@@ -907,13 +941,13 @@ static int launch_mig_t(const MigArgs &a, hipStream_t st)
     return (int)hipGetLastError();
 }
 
-template <int TZ, int TX, int R, int G, bool Probe = false, int Fmt = 0>
+template <int TZ, int TX, int R, int G, bool Probe = false, int Fmt = 0, bool Cols = false>
 static int launch_ph_t(const MigResArgs &a, hipStream_t st)
 {
     const int ntz = (a.nz + TZ - 1) / TZ, ntx = (a.nx + TX - 1) / TX;
     const size_t lds = (size_t)(a.ng + 1) * TZ * TX * sizeof(float);
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_ph<TZ, TX, R, G, Probe, Fmt>,
+        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_ph<TZ, TX, R, G, Probe, Fmt, Cols>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
@@ -925,7 +959,7 @@ static int launch_ph_t(const MigResArgs &a, hipStream_t st)
     // count (tests).
     const int ntiles = ntz * ntx;
     const int nsplit = res_splits(a, ntiles);
-    hipLaunchKernelGGL((k_migrate_ph<TZ, TX, R, G, Probe, Fmt>), dim3((unsigned)ntiles,
+    hipLaunchKernelGGL((k_migrate_ph<TZ, TX, R, G, Probe, Fmt, Cols>), dim3((unsigned)ntiles,
                        (unsigned)nsplit), dim3(TZ * TX), lds, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || nsplit == 1) return (int)e;
@@ -941,6 +975,16 @@ static int launch_ph_t(const MigResArgs &a, hipStream_t st)
 template <bool Probe, int Fmt>
 static int launch_res_preset(const MigResArgs &a, int tile, hipStream_t st)
 {
+    if (a.ctab) {
+        // receiver columns (cmp=1): the five presets of the 16-bit formats
+        switch (tile) {
+        case 11: return launch_ph_t<16, 8, 8, 8, Probe, Fmt, true>(a, st);
+        case 10: return launch_ph_t<16, 8, 16, 4, Probe, Fmt, true>(a, st);
+        case 13: return launch_ph_t<8, 8, 16, 4, Probe, Fmt, true>(a, st);
+        case 15: return launch_ph_t<8, 8, 8, 4, Probe, Fmt, true>(a, st);
+        default: return launch_ph_t<16, 8, 8, 4, Probe, Fmt, true>(a, st);
+        }
+    }
     if constexpr (Fmt != 0) {
         switch (tile) {
         case 11: return launch_ph_t<16, 8, 8, 8, Probe, Fmt>(a, st);
@@ -1020,19 +1064,19 @@ int launch_migrate_resident_probe(const MigResArgs &a, int tile, int precision, 
     return launch_res_prec<true>(a, tile, precision, (hipStream_t)stream);
 }
 
-template <int TZ, int TX, int R, int G>
+template <int TZ, int TX, int R, int G, bool Cols = false>
 static int launch_count_t(const MigResArgs &a, hipStream_t st)
 {
     const int ntz = (a.nz + TZ - 1) / TZ, ntx = (a.nx + TX - 1) / TX;
     const size_t lds = (size_t)(a.ng + 1) * TZ * TX * sizeof(float);
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_count_ph<TZ, TX, R, G>,
+        const hipError_t e = hipFuncSetAttribute((const void *)k_count_ph<TZ, TX, R, G, Cols>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
     const int ntiles = ntz * ntx;
-    hipLaunchKernelGGL((k_count_ph<TZ, TX, R, G>), dim3((unsigned)ntiles,
+    hipLaunchKernelGGL((k_count_ph<TZ, TX, R, G, Cols>), dim3((unsigned)ntiles,
                        (unsigned)res_splits(a, ntiles)), dim3(TZ * TX), lds, st, a);
     return (int)hipGetLastError();
 }
@@ -1040,6 +1084,15 @@ static int launch_count_t(const MigResArgs &a, hipStream_t st)
 int launch_count_resident(const MigResArgs &a, int tile, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
+    if (a.ctab) {
+        switch (tile) {
+        case 11: return launch_count_t<16, 8, 8, 8, true>(a, st);
+        case 10: return launch_count_t<16, 8, 16, 4, true>(a, st);
+        case 13: return launch_count_t<8, 8, 16, 4, true>(a, st);
+        case 15: return launch_count_t<8, 8, 8, 4, true>(a, st);
+        default: return launch_count_t<16, 8, 8, 4, true>(a, st);
+        }
+    }
     switch (tile) {
     case 10: return launch_count_t<16, 8, 16, 4>(a, st);
     case 11: return launch_count_t<16, 8, 8, 8>(a, st);

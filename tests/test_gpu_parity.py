@@ -173,8 +173,8 @@ def test_resident_variant_is_used_and_matches_stream():
             out[v] = m.cig()
             assert m.stats()["variant"] == kmig.VARIANT[v]
     assert O.rel_l2(out["resident"], out["stream"]) < TOL
-    # cmp=1 receivers move with the shot: resident is refused, auto streams
-    kv1, run1, _ = gc.case("angle_cmp1")
+    # nearest-bin gathers have no resident form: resident is refused
+    kv1, run1, _ = gc.case("nearest")
     with pytest.raises(kmig.KmigError, match="variant=resident needs"):
         kmig.migrate(dict(kv1, variant="resident"), *_args(run1))
 

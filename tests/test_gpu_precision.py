@@ -62,10 +62,12 @@ def _golden_args(name):
 
 
 def test_16bit_modes_need_the_resident_variant():
+    """Nearest-bin gathers run on the stream kernel only, which keeps FP32
+    tables: a 16-bit precision there is refused."""
     from kirch_datuming_amd import kmig
-    kv, run, _ = gc.case("angle_cmp1")
+    kv, run, _ = gc.case("nearest")
     with pytest.raises(kmig.KmigError, match="applies to the resident kernel"):
-        kmig.migrate(dict(kv, precision="u16"), *_golden_args("angle_cmp1"))
+        kmig.migrate(dict(kv, precision="u16"), *_golden_args("nearest"))
 
 
 @pytest.mark.parametrize("prec", ["u16", "fp16", "bf16"])

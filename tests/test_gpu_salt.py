@@ -107,3 +107,60 @@ def test_salt_full_run_properties(salt, monkeypatch):
     r_str = O.rel_l2(full, stream)
     print(f"salt: resident vs stream kernel, full run: relL2 {r_str:.2e}")
     assert r_str <= 1e-4
+
+
+@pytest.fixture(scope="module")
+def salt_cmp1():
+    from kirch_datuming_amd import configs
+    print("salt cmp=1: building the workload (800 receiver tables)", flush=True)
+    return configs.salt(threads=16, cmp=1)
+
+
+def test_salt_cmp1_on_the_resident_kernel(salt_cmp1, monkeypatch):
+    """The reference's default geometry (cmp=1) at full size: the resident
+    kernel's receiver columns against the FP64 oracle on four shots (two shot
+    groups forced), and the full run against the stream kernel and against
+    itself with the culls off."""
+    w = salt_cmp1
+    shots = [96, 288, 480, 672]
+    sub = w.shot_subset(shots)
+    monkeypatch.setenv("KMIG_SPLIT", "2")
+    print(f"salt cmp=1: GPU on shots {shots}, two shot groups forced", flush=True)
+    with _migrator(sub) as m:
+        m.run()
+        m.sync()
+        cig = m.cig()
+        assert m.stats()["variant"] == 2
+    monkeypatch.delenv("KMIG_SPLIT")
+    print(f"salt cmp=1: FP64 oracle on shots {shots}", flush=True)
+    ref = O.migrate(w.kv, *sub.args(), precision="f64")[0]
+    r = O.rel_l2(cig, ref)
+    print(f"salt cmp=1 sample: relL2 vs FP64 {r:.2e}")
+    assert np.abs(ref).max() > 0 and r <= 1e-4
+    with _migrator(w) as m:
+        print("salt cmp=1: full production run", flush=True)
+        m.run()
+        m.sync()
+        full = m.cig()
+        st = m.stats()
+        cnt = m.count_updates()
+        assert st["variant"] == 2 and np.all(np.isfinite(full))
+        print(f"salt cmp=1: preset {st['tile']}, {st['receiver_slots']} receiver slots, "
+              f"migrate {st['ms_migrate']:.1f} ms, tuples {st['ms_tuple_prep']:.1f} ms; evaluated "
+              f"{cnt['evaluated'] / cnt['nominal']:.3f}, accepted "
+              f"{cnt['accepted'] / cnt['nominal']:.3f}")
+        monkeypatch.setenv("KMIG_CULL", "0")
+        m.run()
+        m.sync()
+        nocull = m.cig()
+        monkeypatch.delenv("KMIG_CULL")
+    assert np.array_equal(nocull.view(np.uint32), full.view(np.uint32))
+    print("salt cmp=1: full run on the stream kernel", flush=True)
+    with _migrator(w, variant="stream") as m:
+        m.run()
+        m.sync()
+        stream = m.cig()
+        ms = m.stats()["ms_total"]
+    r_str = O.rel_l2(full, stream)
+    print(f"salt cmp=1: resident vs stream kernel ({ms:.0f} ms), full run: relL2 {r_str:.2e}")
+    assert r_str <= 1e-4
