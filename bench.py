@@ -52,6 +52,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stream-check", action="store_true",
                     help="skip the stream-kernel run that checks the timed run's CIG")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the graph-captured replays of the step")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the 16-bit table forms of the default line")
     ap.add_argument("--overlap", type=int, default=0,
@@ -132,6 +134,17 @@ def main():
         counts = m.count_updates()
         log(f"updates evaluated {counts['evaluated'] / counts['nominal']:.3f}, accepted "
             f"{counts['accepted'] / counts['nominal']:.3f} of nominal")
+    graph = None
+    if st["variant"] == 2 and not args.no_graph:
+        # the same step captured as one hipGraph and replayed
+        m.set_graph(True)
+        rg = timed_steps(m, max(2, args.warmup), args.steps, w.updates)
+        same = bool(np.array_equal(m.cig().view(np.uint32), cig_timed.view(np.uint32)))
+        graph = {"value": rg["value"], "ms_per_step": rg["ms_per_step"],
+                 "events_ms": rg["events_ms"], "replayed": bool(rg["stats"]["graph"]),
+                 "cig_bitwise_equal_to_eager": same, "vs_eager": rg["value"] / value}
+        log(f"graph replays: {rg['ms_per_step']:.2f} ms/step ({graph['vs_eager']:.4f} x eager), "
+            f"bitwise equal {same}")
     m.close()
     parity = {"timed_run_finite": bool(np.all(np.isfinite(cig_timed))),
               "timed_run_abs_max": float(np.abs(cig_timed).max())}
@@ -229,6 +242,7 @@ def main():
                          "migrate": mig_avg * launches, "events_total": r["events_ms"]},
         "parity": parity,
         "forms": forms or None,
+        "graph": graph,
         "upload_s": upload_s,
         "pcie_inclusive_updates_per_s": w.updates / (r["ms_per_step"] * 1e-3 + upload_s),
     }

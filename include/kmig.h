@@ -106,6 +106,9 @@ typedef struct kmig_stats {
     int shot_batches;
     int variant;               /* KMIG_VARIANT_* actually used              */
     int tile;                  /* migrate tile preset used (autotuned if 0) */
+    int graph;                 /* 1: the run replayed a captured step (only
+                                  ms_total is its own; the phase times are
+                                  the eager run's it was captured from)     */
 } kmig_stats;
 
 typedef struct kmig_ctx kmig_ctx;
@@ -150,6 +153,17 @@ int kmig_set_data(kmig_ctx *ctx, const float *data, int nt, float t0, float dt, 
    migration kernels on the context stream and returns without waiting. */
 int kmig_run(kmig_ctx *ctx, int shot_begin, int shot_end);
 int kmig_sync(kmig_ctx *ctx);
+
+/* Captured steps (no counterpart in the reference, which has no device
+   work): with enable = 1, a resident run (mode=angle, linear bins, one shot
+   batch, no surrogate) is captured as one hipGraph right after it is
+   enqueued eagerly, and later kmig_run calls with the same plan, shot range,
+   preset and switches replay the graph instead of launching kernel by kernel:
+   trace preparation (on a parallel branch), tuple preparation, the migrate
+   launch and its partial sums.  Any change of tables, data, range or preset
+   runs eagerly and captures again.  enable = 0 drops the graph.  Results are
+   bitwise those of eager runs. */
+int kmig_set_graph(kmig_ctx *ctx, int enable);
 
 /* Copy results to host memory after kmig_sync: cig has ng*nx*nz floats;
    qc_before/qc_after have nt*nh floats (the direct_mute_qc_shot gather). */

@@ -122,6 +122,13 @@ struct kmig_ctx {
     long long geo_version = 0, plan_version = -1;
     kmig_stats st{};
     bool ran = false, pending = false;
+    // captured step (kmig_set_graph): the graph of the last eager resident
+    // run and the plan it was captured for
+    bool graph_on = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::string gkey;
+    hipEvent_t gfork = nullptr, gjoin = nullptr;
 };
 
 namespace {
@@ -1153,6 +1160,10 @@ void kmig_destroy(kmig_ctx *c)
     dfree(c->ctab);
     dfree(c->cfirst);
     dfree(c->clast);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->gfork) (void)hipEventDestroy(c->gfork);
+    if (c->gjoin) (void)hipEventDestroy(c->gjoin);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1345,6 +1356,85 @@ static MigArgs stream_args(const kmig_ctx *c, const Batch &b)
     return M;
 }
 
+// The resident migrate launch (and its partial sums) of batch b on stream
+// st; tune = autotune first where the preset is automatic (never under graph
+// capture: the autotuner synchronises).
+static int enqueue_resident(kmig_ctx *c, const Batch &b, int ng, bool tune, hipStream_t st)
+{
+    const kmig_params &p = c->p;
+    const int nsb = b.s1 - b.s0;
+    int rc;
+    MigResArgs R = res_args(c, b.s0, nsb, ng);
+    // KMIG_SPLIT forces a shot-group count (tests): then short shot ranges
+    // split too, so their partial sums are in the checked path
+    const char *fs = getenv("KMIG_SPLIT");
+    if (nsb >= 16 || (fs && atoi(fs) > 1 && nsb >= 2)) {
+        // shot groups for the launch's tail (the launcher picks the count)
+        if ((rc = ensure(c, &c->part, &c->part_cap, (size_t)KMIG_MAX_SPLIT * ng * c->nzx)))
+            return rc;
+        R.part = c->part;
+        R.part_splits = KMIG_MAX_SPLIT;
+    }
+    int tile = p.tile;
+    if (tile <= 0) {
+        if (tune && (rc = autotune(c, R))) return rc;
+        tile = c->tuned_tile;
+    }
+    HIPCHK(c, (hipError_t)launch_migrate_resident(R, tile, p.precision, st));
+    return KMIG_OK;
+}
+
+// What a captured step depends on besides the buffers of its plan: the shot
+// range, the preset, the QC shot and the switches read while enqueueing.
+static std::string graph_key(const kmig_ctx *c, int sb, int se)
+{
+    char buf[256];
+    const char *e1 = getenv("KMIG_CULL"), *e2 = getenv("KMIG_SPLIT"),
+               *e3 = getenv("KMIG_TUPLE_FUSED");
+    snprintf(buf, sizeof(buf), "%lld %d %d %d %d %d|%s|%s|%s", c->plan_version, sb, se,
+             c->p.tile > 0 ? c->p.tile : c->tuned_tile, c->qc_valid ? 1 : 0,
+             c->variant_used, e1 ? e1 : "", e2 ? e2 : "", e3 ? e3 : "");
+    return buf;
+}
+
+// Capture one resident step over shots [sb, se) as a graph: the trace
+// preparation on stream2 as one branch (forked from and joined back into the
+// context stream by events), the tuple preparation on the context stream as
+// the other, then the migrate launch and its partial sums.  The capture
+// executes nothing; the graph is instantiated for replays.
+static int capture_step(kmig_ctx *c, int sb, int se, int ng)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->gkey.clear();
+    if (!c->gfork) HIPCHK(c, hipEventCreateWithFlags(&c->gfork, hipEventDisableTiming));
+    if (!c->gjoin) HIPCHK(c, hipEventCreateWithFlags(&c->gjoin, hipEventDisableTiming));
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = KMIG_OK;
+    hipError_t e = hipEventRecord(c->gfork, c->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(c->stream2, c->gfork, 0);
+    if (e == hipSuccess) rc = enqueue_trace_prep(c, c->stream2, sb, se);
+    if (e == hipSuccess && rc == KMIG_OK) e = hipEventRecord(c->gjoin, c->stream2);
+    if (e == hipSuccess && rc == KMIG_OK) rc = enqueue_tuple_prep(c, c->stream, c->batches[0], 0);
+    if (e == hipSuccess && rc == KMIG_OK) e = hipStreamWaitEvent(c->stream, c->gjoin, 0);
+    if (e == hipSuccess && rc == KMIG_OK && c->batches[0].s1 > c->batches[0].s0)
+        rc = enqueue_resident(c, c->batches[0], ng, false, c->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess || rc != KMIG_OK || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc != KMIG_OK) return rc;
+        return fail(c, KMIG_E_DEVICE, "graph capture of the step failed: %s",
+                    hipGetErrorString(e != hipSuccess ? e : ec));
+    }
+    c->graph = g;
+    HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    c->gkey = graph_key(c, sb, se);
+    return KMIG_OK;
+}
+
 int kmig_run(kmig_ctx *c, int sb, int se)
 {
     if (!c) return KMIG_E_PARAM;
@@ -1373,6 +1463,22 @@ int kmig_run(kmig_ctx *c, int sb, int se)
     if ((rc = plan_if_needed(c, sb, se))) return rc;
     const bool resident = c->variant_used == KMIG_VARIANT_RESIDENT;
     if ((rc = ensure_outputs(c, sb, se))) return rc;
+    // Captured step: a resident run with one batch and no surrogate, whose
+    // plan was captured after an eager run of it, replays the graph.
+    const bool graphable = c->graph_on && resident && c->batches.size() == 1 &&
+                           !c->side[0].nn && !c->side[1].nn;
+    if (graphable && c->gexec && c->gkey == graph_key(c, sb, se)) {
+        hipEvent_t g0 = next_event(c), g1 = next_event(c);
+        HIPCHK(c, hipEventRecord(g0, c->stream));
+        HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+        HIPCHK(c, hipEventRecord(g1, c->stream));
+        // the phase times stay those of the eager run the graph was captured
+        // from; the total is the replay's
+        c->st.graph = 1;
+        c->pending = true;
+        c->ran = true;
+        return KMIG_OK;
+    }
     // The trace preparation (data) runs on stream2 while the tuple preparation
     // (tables) runs on the context stream; the first migrate launch waits for
     // both.  stream2 starts after everything earlier on the context stream,
@@ -1410,23 +1516,7 @@ int kmig_run(kmig_ctx *c, int sb, int se)
             joined = true;
         }
         if (nsb > 0 && resident) {
-            MigResArgs R = res_args(c, b.s0, nsb, ng);
-            // KMIG_SPLIT forces a shot-group count (tests): then short shot
-            // ranges split too, so their partial sums are in the checked path
-            const char *fs = getenv("KMIG_SPLIT");
-            if (nsb >= 16 || (fs && atoi(fs) > 1 && nsb >= 2)) {
-                // shot groups for the launch's tail (the launcher picks the count)
-                if ((rc = ensure(c, &c->part, &c->part_cap,
-                                 (size_t)KMIG_MAX_SPLIT * ng * c->nzx)))
-                    return rc;
-                R.part = c->part;
-                R.part_splits = KMIG_MAX_SPLIT;
-            }
-            if (p.tile <= 0) {
-                if ((rc = autotune(c, R))) return rc;
-                tile = c->tuned_tile;
-            }
-            HIPCHK(c, (hipError_t)launch_migrate_resident(R, tile, p.precision, c->stream));
+            if ((rc = enqueue_resident(c, b, ng, true, c->stream))) return rc;
             launches++;
         } else if (nsb > 0) {
             HIPCHK(c, (hipError_t)launch_migrate(M, tile, c->stream));
@@ -1449,6 +1539,23 @@ int kmig_run(kmig_ctx *c, int sb, int se)
     c->st.receiver_slots = maxr;
     c->pending = true;
     c->ran = true;
+    // the next run with this plan replays a graph of this step
+    if (graphable && (rc = capture_step(c, sb, se, ng))) return rc;
+    return KMIG_OK;
+}
+
+int kmig_set_graph(kmig_ctx *c, int enable)
+{
+    if (!c) return KMIG_E_PARAM;
+    c->graph_on = enable != 0;
+    if (!c->graph_on && c->gexec) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        (void)hipGraphExecDestroy(c->gexec);
+        (void)hipGraphDestroy(c->graph);
+        c->gexec = nullptr;
+        c->graph = nullptr;
+        c->gkey.clear();
+    }
     return KMIG_OK;
 }
 
@@ -1457,7 +1564,13 @@ int kmig_sync(kmig_ctx *c)
     if (!c) return KMIG_E_PARAM;
     if (!c->stream) return fail(c, KMIG_E_STATE, "context has no device");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (c->pending && c->ev_used >= 2) {
+    if (c->pending && c->st.graph && c->ev_used >= 2) {
+        // a replayed graph: events around the launch give the step's time
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, c->events[0], c->events[1]);
+        c->st.ms_total = ms;
+        c->pending = false;
+    } else if (c->pending && c->ev_used >= 2) {
         // events: e0, e1 (trace prep), then per batch a0, a1 (tuples), m1 (migrate)
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, c->events[0], c->events[1]);

@@ -32,7 +32,7 @@ EXPORTS = ("kmig_params_default", "kmig_params_parse", "kmig_params_check", "kmi
            "kmig_stream", "kmig_version", "kmig_batch_create", "kmig_batch_run",
            "kmig_batch_sync", "kmig_batch_stats", "kmig_batch_last_error", "kmig_batch_destroy",
            "kmig_set_surrogate", "kmig_get_surrogate_stats", "kmig_batch_stitch",
-           "kmig_model", "kmig_count_updates", "kmig_batch_stitch_noalloc")
+           "kmig_model", "kmig_count_updates", "kmig_batch_stitch_noalloc", "kmig_set_graph")
 
 
 class KmigError(RuntimeError):
@@ -67,7 +67,7 @@ class KmigStats(C.Structure):
                 ("ms_tuple_prep", C.c_double), ("ms_migrate", C.c_double),
                 ("ms_total", C.c_double), ("migrate_launches", C.c_int),
                 ("receiver_slots", C.c_int), ("shot_batches", C.c_int),
-                ("variant", C.c_int), ("tile", C.c_int)]
+                ("variant", C.c_int), ("tile", C.c_int), ("graph", C.c_int)]
 
 
 _LIB = None
@@ -106,6 +106,7 @@ def lib():
         L.kmig_get_stats.argtypes = [vp, C.POINTER(KmigStats)]
         dp = C.POINTER(C.c_double)
         L.kmig_count_updates.argtypes = [vp, dp, dp, dp]
+        L.kmig_set_graph.argtypes = [vp, C.c_int]
         L.kmig_stream.argtypes = [vp]
         L.kmig_stream.restype = vp
         L.kmig_version.restype = C.c_char_p
@@ -238,6 +239,11 @@ class Migrator:
 
     def stream(self):
         return self.L.kmig_stream(self.ctx)
+
+    def set_graph(self, enable=True):
+        """Capture the step as a hipGraph after its first eager run and replay
+        it on later runs with the same plan (kmig_set_graph)."""
+        self._chk(self.L.kmig_set_graph(self.ctx, int(bool(enable))))
 
     def count_updates(self) -> dict:
         """Nominal, evaluated (groups surviving the culls) and accepted
