@@ -397,8 +397,8 @@ int plan_run(kmig_ctx *c, int sb, int se, std::vector<TraceMeta> &meta,
     // lies far below the float rounding of the traveltimes themselves.  The
     // trace records (aperture index, mute) keep each trace's own position.
     std::unordered_map<uint32_t, float> pos_rep;
-    const bool group_pos = p.cmp && angle && p.angle_interp == 1 && p.antialias >= 0.0f &&
-                           p.variant != KMIG_VARIANT_STREAM;
+    const bool group_pos = p.adj == 1 && p.cmp && angle && p.angle_interp == 1 &&
+                           p.antialias >= 0.0f && p.variant != KMIG_VARIANT_STREAM;
     if (group_pos) {
         std::vector<float> all;
         all.reserve((size_t)(se - sb) * nh);
@@ -521,7 +521,9 @@ int plan_run(kmig_ctx *c, int sb, int se, std::vector<TraceMeta> &meta,
     // kernel's bins (angle gathers, linear bins): receiver columns
     c->rcv_columns = false;
     cols = Columns{};
-    if (!invariant && c->batches.size() == 1 && p.cig && p.cig_is_angle && p.angle_interp == 1) {
+    // (migration only: the adj=0 transpose keeps receiver indices)
+    if (p.adj == 1 && !invariant && c->batches.size() == 1 && p.cig && p.cig_is_angle &&
+        p.angle_interp == 1) {
         build_columns(c, sb, se, meta, rspec, cols);
         c->rcv_columns = !cols.colslot.empty();
     }
@@ -1012,6 +1014,7 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
     A.aper_taper = p.aperture_taper;
     A.rmin = p.aperture_rmin;
     A.rtaper = p.aperture_rtaper;
+    A.aal = p.antialias;
     // packed directions in output-angle degrees (the opening angle is then
     // a difference of two directions folded at 360, or 180 for half angles)
     const float th_scale = angle ? (float)((p.half_angle ? 90.0 : 180.0) / M_PI) : 0.0f;
@@ -1054,10 +1057,10 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
     if (pack32) {
         if (!fused_s)
             HIPCHK(c, (hipError_t)launch_pack(c->ts, c->pk_s, nsb, c->nzx, c->dt, th_scale,
-                                              st));
+                                              p.antialias, st));
         if (!fused_r)
             HIPCHK(c, (hipError_t)launch_pack(c->tr, c->pk_r, b.nrcv, c->nzx, c->dt, th_scale,
-                                              st));
+                                              p.antialias, st));
     } else if (resident) {
         if (!fused_s)
             HIPCHK(c, (hipError_t)launch_pack16(c->ts, c->pk16_s, c->sc_s, nsb, c->nzx, c->dt,

@@ -69,6 +69,7 @@ struct TuplePrepArgs {
     int radius, smooth;
     int angle_mode;
     float aper, aper_taper, rmin, rtaper;
+    float aal;                           // antialias (folded into packed FP32 tuples)
 };
 
 struct TracePrepArgs {
@@ -106,7 +107,10 @@ struct MigArgs {
 
 // Packed 16-byte tuple of the resident migrate kernel, one per (slot, point):
 //   frac  fractional part of T/dt (samples, in [0,1])
-//   a     |dT/dpos * spacing| / dt (anti-alias term in samples)
+//   a     1 + antialias * |dT/dpos * spacing| / dt: one plus the anti-alias
+//         half-width in samples (the outer taps of an update lie 1 + antialias
+//         * max(a_s, a_r) samples from its time, and that is fmaf(a, aal, 1) of
+//         the larger side, so the larger folded field is it bit for bit)
 //   th    direction of the smoothed traveltime gradient, atan2(gx, gz), in
 //         output-angle degrees (x 180/pi, or x 90/pi for half opening angles)
 //   iw    bits 31..16: integer part of T/dt (int16), bits 15..0: the
@@ -289,7 +293,7 @@ int migrate_lds_bytes(int ng, int tile);
 int launch_fill(float *p, size_t n, float v, void *stream);
 // th_scale: degrees per radian of the packed direction (0: no direction)
 int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, float dt,
-                float th_scale, void *stream);
+                float th_scale, float aal, void *stream);
 int launch_pack16(const Tuples &in, PackedTuple16 *out, SlotScale *scale, int nslots,
                   size_t nzx, float dt, float th_scale, int precision, void *stream);
 int launch_pairs(const float *prep, float *pairs, long ntraces, int nt, void *stream);

@@ -310,8 +310,10 @@ __device__ __forceinline__ typename RawOf<Fmt>::type load_raw(const MigResArgs &
     }
 }
 
+// Decoded tuples carry the FP32 layout, whose a field is 1 + antialias * a
+// (PackedTuple): the 16-bit fields fold it in here, once per tuple.
 template <int Fmt>
-__device__ __forceinline__ PackedTuple decode(const typename RawOf<Fmt>::type &r)
+__device__ __forceinline__ PackedTuple decode(const typename RawOf<Fmt>::type &r, float aal)
 {
     if constexpr (Fmt == 0) {
         return r;
@@ -320,7 +322,7 @@ __device__ __forceinline__ PackedTuple decode(const typename RawOf<Fmt>::type &r
         t.iw = r.q.iw;
         t.frac = Fmt == 3 ? (float)(r.q.fa & 0xFFFFu) * (1.0f / 65536.0f)
                           : from16(r.q.fa & 0xFFFFu, Fmt);
-        t.a = from16(r.q.fa >> 16, Fmt) * r.scale;
+        t.a = fmaf(from16(r.q.fa >> 16, Fmt) * r.scale, aal, 1.0f);
         t.th = __uint_as_float(r.q.th);
         return t;
     }
@@ -330,7 +332,7 @@ template <int Fmt>
 __device__ __forceinline__ PackedTuple load_tuple(const MigResArgs &a, bool src, size_t slot,
                                                   size_t i)
 {
-    return decode<Fmt>(load_raw<Fmt>(a, src, slot, i));
+    return decode<Fmt>(load_raw<Fmt>(a, src, slot, i), a.aal);
 }
 
 // Opening angle from the two packed gradient directions (output-angle
@@ -369,17 +371,17 @@ __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int8, flo
     const float w1 = __builtin_amdgcn_fractf(uc);
     // split-sample time: integer parts add as integers (in bytes of 8-byte
     // tap pairs), fractions in float.  Taps at ff and ff -/+ (delta + 1)
-    // samples; both a fields are finite and >= 0 and antialias >= 0 (the
-    // resident variant requires it), so delta + 1 >= 1 and itm < it < itp:
-    // the outer taps bound the middle one.  Non-negative floats order as
-    // their bit patterns: one integer max.
+    // samples; both a fields are finite and >= 1 (one plus antialias >= 0
+    // times a >= 0; the resident variant requires antialias >= 0), so
+    // delta + 1 >= 1 and itm < it < itp: the outer taps bound the middle one.
     int b8 = s_int8 + (((int)r.iw >> 16) << 3);
     // kept as one value: re-associated, the three tap offsets cost two
     // integer instructions each instead of one
     asm("" : "+v"(b8));
     const float ff = s_frac + r.frac;
-    const float amax = __int_as_float(max(__float_as_int(s.a), __float_as_int(r.a)));
-    const float dA1 = fmaf(amax, a.aal, 1.0f);
+    // 1 + antialias * max(a_s, a_r): the larger folded field (positive floats
+    // order as their bit patterns: one integer max)
+    const float dA1 = __int_as_float(max(__float_as_int(s.a), __float_as_int(r.a)));
     const float fm = ff - dA1, fp = ff + dA1;
     const int it8 = b8 + (cvt_floor(ff) << 3);
     const int itm8 = b8 + (cvt_floor(fm) << 3), itp8 = b8 + (cvt_floor(fp) << 3);
@@ -488,6 +490,10 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
     const float plim = fminf(cutoff, a.ang_hi);
     const float bhi = fminf(cutoff_hi, a.ang_hi_up);
     const bool taper_on = !(a.taper_start < 0.0f || a.taper_end <= a.taper_start);
+    // whether any point of the wave has a nonempty depth-mute ramp band below
+    // the last bin (cutoff < bhi); where none has, an update's weight is its
+    // base weight up to plim and zero beyond, and the band test is skipped
+    const bool wave_band = a.oaztaper > 0.0f && __ballot(valid && cutoff < bhi) != 0;
     const float inv32k = 1.0f / 32768.0f;
     const float2 *__restrict__ pairs = (const float2 *)a.pairs;
 
@@ -636,7 +642,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
         typename RawOf<Fmt>::type nxt{};
         if (js < cje) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
         while (js < cje) {
-            const PackedTuple st = decode<Fmt>(nxt);
+            const PackedTuple st = decode<Fmt>(nxt, a.aal);
             const int jc = js;
             js = next_shot(js + 1);
             if (js < cje) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
@@ -677,10 +683,13 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                     ang[k] = pair_angle(st, rt[g0 + k], a.wrap);
                     u[k] = fmaf(ang[k], a.inv_dg, a.g0_inv_dg);  // (angle - g0) / dg
                     aw[k] = ang[k] <= plim ? base[k] : 0.0f;
-                    // bitwise, not short-circuit: no per-update branches
-                    band |= (ang[k] > cutoff) & (ang[k] < bhi);
                 }
-                if (a.oaztaper > 0.0f && __ballot(band) != 0) {
+                if (wave_band) {
+#pragma unroll
+                    for (int k = 0; k < G; k++)  // bitwise, not short-circuit: no branches
+                        band |= (ang[k] > cutoff) & (ang[k] < bhi);
+                }
+                if (wave_band && __ballot(band) != 0) {
 #pragma unroll
                     for (int k = 0; k < G; k++) {
                         const float rd = ramp_down(

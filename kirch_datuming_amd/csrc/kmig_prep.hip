@@ -451,7 +451,7 @@ __device__ inline PackFields pack_fields(float T, float A, float gx, float gz, f
 
 __global__ __launch_bounds__(256) void k_pack(const float *T, const float *A, const float *GX,
                                               const float *GZ, const float *W, PackedTuple *out,
-                                              size_t n, double inv_dt, float th_scale)
+                                              size_t n, double inv_dt, float th_scale, float aal)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(256) void k_pack(const float *T, const float *A, co
                                      th_scale != 0.0f ? GZ[k] : 0.0f, W[k], inv_dt, th_scale);
     PackedTuple q;
     q.frac = f.frac;
-    q.a = f.a;
+    q.a = fmaf(f.a, aal, 1.0f);  // 1 + the anti-alias half-width (PackedTuple)
     q.th = f.th;
     q.iw = f.iw;
     out[k] = q;
@@ -678,7 +678,7 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
         } else {
             PackedTuple q;
             q.frac = f.frac;
-            q.a = f.a;
+            q.a = fmaf(f.a, a.aal, 1.0f);  // 1 + the anti-alias half-width
             q.th = f.th;
             q.iw = f.iw;
             out[oi] = q;
@@ -889,12 +889,12 @@ int launch_tuple_prep(const TuplePrepArgs &a, Tuples &o, float *tmp1, float *tmp
 }
 
 int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, float dt,
-                float th_scale, void *stream)
+                float th_scale, float aal, void *stream)
 {
     const size_t n = (size_t)nslots * nzx;
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_pack, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, in.T, in.A,
-                       in.GX, in.GZ, in.W, out, n, 1.0 / (double)dt, th_scale);
+                       in.GX, in.GZ, in.W, out, n, 1.0 / (double)dt, th_scale, aal);
     return (int)hipGetLastError();
 }
 
