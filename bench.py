@@ -44,6 +44,11 @@ def main():
                     help="surrogate: Adam steps on the workload's own tables")
     ap.add_argument("--pretrain-steps", type=int, default=3000,
                     help="surrogate: Adam steps on the random-model records")
+    ap.add_argument("--nn-weights", default=None,
+                    help="surrogate: load the network's parameters from this .npy file and "
+                         "train nothing")
+    ap.add_argument("--save-nn-weights", default=None,
+                    help="surrogate: save the trained network's parameters to this .npy file")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "u16"])
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--variant", default="auto", choices=["auto", "stream", "resident"])
@@ -681,7 +686,7 @@ def bench_surrogate(args, cores):
     log(f"small workload and {rec_t.shape[0]} table records in {time.time() - t0:.1f} s")
     t0 = time.time()
     rec_g = configs.grf_records(nmodels=args.grf_models, threads=cores) \
-        if args.grf_models > 0 else None
+        if args.grf_models > 0 and not args.nn_weights else None
     t_grf = time.time() - t0
     if rec_g is not None:
         log(f"{rec_g.shape[0]} records from {args.grf_models} random models in {t_grf:.1f} s")
@@ -699,7 +704,12 @@ def bench_surrogate(args, cores):
                 "max_abs_dt_s": float(e.max()), "frac_within_1ms": float((e <= 1e-3).mean())}
 
     nets, report = {}, {}
-    if rec_g is not None and args.pretrain_steps > 0:
+    if args.nn_weights:
+        net = nn.Surrogate(**configs.SURROGATE)
+        net.params = np.load(args.nn_weights)
+        nets["loaded"] = net
+        report["loaded"] = {"file": os.path.basename(args.nn_weights), "check": check(net)}
+    if not args.nn_weights and rec_g is not None and args.pretrain_steps > 0:
         pre = nn.Surrogate(**configs.SURROGATE)
         pre.set_data(rec_g)
         lp = pre.train(args.pretrain_steps)
@@ -712,16 +722,18 @@ def bench_surrogate(args, cores):
         report["pretrained_then_fitted"] = {"steps": args.fit_steps,
                                             "loss_last": float(lf[-1]) if len(lf) else None,
                                             **pre.stats(), "check": check(pre)}
-    scratch = nn.Surrogate(**configs.SURROGATE)
-    scratch.set_data(rec_t)
-    ls = scratch.train(args.fit_steps)
-    nets["fitted_from_scratch"] = scratch
-    report["fitted_from_scratch"] = {"steps": args.fit_steps,
-                                     "loss_last": float(ls[-1]) if len(ls) else None,
-                                     **scratch.stats(), "check": check(scratch)}
+    if not args.nn_weights:
+        scratch = nn.Surrogate(**configs.SURROGATE)
+        scratch.set_data(rec_t)
+        ls = scratch.train(args.fit_steps)
+        nets["fitted_from_scratch"] = scratch
+        report["fitted_from_scratch"] = {"steps": args.fit_steps,
+                                         "loss_last": float(ls[-1]) if len(ls) else None,
+                                         **scratch.stats(), "check": check(scratch)}
     best = min(nets, key=lambda k: report[k]["check"]["mean_abs_dt_s"])
     net = nets[best]
-    tr = report[best]
+    if args.save_nn_weights:
+        np.save(args.save_nn_weights, net.params)
     log(f"surrogate used: {best}; check {report[best]['check']}")
 
     kv = dict(w.kv, variant=args.variant)
@@ -730,14 +742,15 @@ def bench_surrogate(args, cores):
     ax = w.data_axes
     p = kmig.parse_params(kv, int(ax[1][0]), ax[1][1], ax[1][2])
 
-    def migrator(surrogate, data_w):
+    def migrator(surrogate, data_w, per_slot=True):
         m = kmig.Migrator(p)
         m.set_tables(0, w.stab, w.sder, w.s_axes)
         m.set_tables(1, w.rtab, w.rder, w.r_axes)
         m.set_data(data_w.data, data_w.data_axes)
         if surrogate:
-            m.set_surrogate(0, net, 1e-3)
-            m.set_surrogate(1, net, 1e-3)
+            for side in (0, 1):
+                m.set_surrogate(side, net, 1e-3)
+                m.set_surrogate_fallback(side, per_slot)
         return m
 
     def timed(m):
@@ -758,6 +771,13 @@ def bench_surrogate(args, cores):
     mt = migrator(False, w)
     wall_t, tup_t, tot_t = timed(mt)
     mt.close()
+    # per-point fallback: the network wherever a point is within 1 ms
+    mq = migrator(True, w, per_slot=False)
+    wall_q, tup_q, tot_q = timed(mq)
+    sst_q = [mq.surrogate_stats(k) for k in (0, 1)]
+    mq.close()
+    # per-slot fallback (the default): a slot takes the network only where
+    # its worst point is within 1 ms
     m = migrator(True, w)
     wall, tup_n, tot_n = timed(m)
     sst = [m.surrogate_stats(k) for k in (0, 1)]
@@ -778,8 +798,9 @@ def bench_surrogate(args, cores):
     shots = [shots[0] + j * step for j in range(k)]
     subw = w.shot_subset(shots)
     cig = {}
-    for name, sur in (("table", False), ("surrogate", True)):
-        ms = migrator(sur, subw)
+    for name, sur, slot in (("table", False, True), ("surrogate", True, True),
+                            ("surrogate_per_point", True, False)):
+        ms = migrator(sur, subw, per_slot=slot)
         ms.run()
         ms.sync()
         cig[name] = ms.cig()
@@ -811,11 +832,18 @@ def bench_surrogate(args, cores):
         "table_step": {"ms_per_step": wall_t / args.steps * 1e3,
                        "updates_per_s": w.updates * args.steps / wall_t,
                        "tuple_prep_ms": tup_t, "events_total_ms": tot_t},
-        "surrogate_step": {"tuple_prep_ms": tup_n, "events_total_ms": tot_n,
+        "surrogate_step": {"fallback": "per slot (shot or receiver position)",
+                           "tuple_prep_ms": tup_n, "events_total_ms": tot_n,
                            "points_from_network": used, "points": points,
                            "max_residual_s": max(s["max_residual"] for s in sst)},
+        "surrogate_per_point_step": {
+            "fallback": "per point", "ms_per_step": wall_q / args.steps * 1e3,
+            "tuple_prep_ms": tup_q, "events_total_ms": tot_q,
+            "points_from_network": sum(s["used"] for s in sst_q), "points": points},
         "parity": {"sample": f"shots {shots}",
                    "rel_l2_surrogate_vs_f64": O.rel_l2(cig["surrogate"], cig64),
+                   "rel_l2_surrogate_per_point_vs_f64": O.rel_l2(cig["surrogate_per_point"],
+                                                                  cig64),
                    "rel_l2_table_vs_f64": O.rel_l2(cig["table"], cig64)},
         "training": report, "network_used": best, "grf_records_s": t_grf,
         "memory_bytes": {"tables": table_bytes, "network_fp32": 4 * nn.PARAMS},

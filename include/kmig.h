@@ -262,12 +262,19 @@ int kmig_batch_stitch_noalloc(kmig_batch *b, const int *start, const int *end,
    tables (load_table_slice and tinterp_hermite, src/kirchmig2d_cig.cpp:274-284,
    :987-1029, :1148-1180).  With a network set on a side, every run evaluates
    it at each (slot position, grid point) right after the table interpolation
-   and takes its traveltime and dT/ds wherever |T_nn - T_table| <= max_dt
-   (seconds); elsewhere the table values stay.  Gradients, smoothing, weights
-   and the migration then proceed as without it.  nn = NULL removes it.  The
-   network must outlive its use by the context. */
+   and compares its traveltime with the table's: with the per-slot fallback
+   (the default) a slot -- one shot position on the source side, one receiver
+   position on the receiver side -- takes the network's traveltime and dT/ds
+   at every point when its largest |T_nn - T_table| is within max_dt
+   (seconds) and keeps its tables at every point otherwise; with the
+   per-point fallback each point decides alone.  Gradients, smoothing,
+   weights and the migration then proceed as without it.  nn = NULL removes
+   it.  The network must outlive its use by the context. */
 struct kmig_nn;
 int kmig_set_surrogate(kmig_ctx *ctx, int side, struct kmig_nn *nn, float max_dt);
+#define KMIG_NN_FALLBACK_SLOT 0
+#define KMIG_NN_FALLBACK_POINT 1
+int kmig_set_surrogate_fallback(kmig_ctx *ctx, int side, int mode);
 /* After a run: grid points evaluated on that side, points that took the
    network's values, and the largest |T_nn - T_table| seen. */
 int kmig_get_surrogate_stats(kmig_ctx *ctx, int side, double *points, double *used,

@@ -36,6 +36,7 @@ struct Side {
     // learned surrogate of the table interpolation (kmig_set_surrogate)
     kmig_nn *nn = nullptr;
     float nn_max_dt = 0.0f;
+    int nn_fallback = KMIG_NN_FALLBACK_SLOT;
     int *nn_used = nullptr;          // [slot] points taken from the network
     unsigned *nn_maxres = nullptr;   // [slot] max |T_nn - T_table| (float bits)
     size_t nn_cap = 0, nn_cap_m = 0; // slots allocated
@@ -795,7 +796,8 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
         if ((rc = ensure_tuples(c, c->ts, &c->ts_cap, std::max<size_t>(1, maxs)))) return rc;
         if ((rc = ensure_tuples(c, c->tr, &c->tr_cap, std::max<size_t>(1, maxr)))) return rc;
         const size_t tslots = std::max<size_t>(1, std::max(maxs, maxr));
-        if (angle && p.angle_smooth > 0) {
+        // (the per-slot surrogate fallback keeps the network's values there)
+        if ((angle && p.angle_smooth > 0) || c->side[0].nn || c->side[1].nn) {
             if ((rc = ensure(c, &c->tmp1, &c->tmp1_cap, tslots * c->nzx))) return rc;
             if ((rc = ensure(c, &c->tmp2, &c->tmp2_cap, tslots * c->nzx))) return rc;
         }
@@ -965,10 +967,12 @@ int side_tuples(kmig_ctx *c, int side, const TuplePrepArgs &a, Tuples &o, size_t
                                           st));
         HIPCHK(c, (hipError_t)launch_fill((float *)(sd.nn_maxres + slot0), (size_t)a.nslots,
                                           0.0f, st));
+        const bool per_slot = sd.nn_fallback == KMIG_NN_FALLBACK_SLOT;
         HIPCHK(c, (hipError_t)nn_replace_tuples(sd.nn, a.spec, a.nslots, c->nz, c->nzx, c->z0,
                                                 c->dz, c->x0, c->dx, a.dside, sd.nn_max_dt, o.T,
                                                 o.A, sd.nn_used + slot0, sd.nn_maxres + slot0,
-                                                st));
+                                                per_slot ? c->tmp1 : nullptr,
+                                                per_slot ? c->tmp2 : nullptr, st));
     }
     const char *env = getenv("KMIG_TUPLE_FUSED");
     const bool use_fused = (po.packed || po.packed16) && tuple_fused_ok(a) &&
@@ -1186,6 +1190,17 @@ int kmig_set_surrogate(kmig_ctx *c, int side, kmig_nn *nn, float max_dt)
     c->side[side].nn = nn;
     c->side[side].nn_max_dt = max_dt;
     c->geo_version++;  // the next run plans its counters
+    return KMIG_OK;
+}
+
+int kmig_set_surrogate_fallback(kmig_ctx *c, int side, int mode)
+{
+    if (!c) return KMIG_E_PARAM;
+    if (side != 0 && side != 1) return fail(c, KMIG_E_PARAM, "side must be 0 or 1");
+    if (mode != KMIG_NN_FALLBACK_SLOT && mode != KMIG_NN_FALLBACK_POINT)
+        return fail(c, KMIG_E_PARAM, "fallback mode must be KMIG_NN_FALLBACK_SLOT or _POINT");
+    c->side[side].nn_fallback = mode;
+    c->geo_version++;
     return KMIG_OK;
 }
 

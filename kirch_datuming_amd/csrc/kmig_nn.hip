@@ -511,6 +511,10 @@ struct NnInferArgs {
     float *T, *A;
     int *used;          // [slot] points taken from the network
     unsigned *maxres;   // [slot] max |T_nn - T| (float bits)
+    // per-slot fallback: the network's T and A go to Tn/An for every point
+    // (T and A stay the tables'); k_nn_select then takes them for the slots
+    // whose largest residual is within max_dt
+    float *Tn, *An;
 };
 
 template <bool Tuples>
@@ -624,7 +628,10 @@ __global__ __launch_bounds__(256) void k_nn_infer(const NnInferArgs a)
             const float Ts = S0 * (y[b][3] + b5[3]);
             if (Tuples) {
                 float res = fabsf(T - a.T[p]);
-                if (res <= a.max_dt) {
+                if (a.Tn) {
+                    a.Tn[p] = T;
+                    a.An[p] = fabsf(Ts * a.dside);
+                } else if (res <= a.max_dt) {
                     a.T[p] = T;
                     a.A[p] = fabsf(Ts * a.dside);
                     rep[b] = 1;
@@ -666,6 +673,30 @@ __global__ __launch_bounds__(256) void k_nn_infer(const NnInferArgs a)
             }
         }
     }
+}
+
+// Per-slot fallback: a slot whose largest residual (maxres, float bits;
+// non-negative floats order as their bits) is within max_dt takes the
+// network's T and A at every point, any other keeps its tables at every
+// point; used[slot] becomes the slot's points taken from the network.
+__global__ __launch_bounds__(256) void k_nn_select(const float *__restrict__ Tn,
+                                                   const float *__restrict__ An, float *T, float *A,
+                                                   const unsigned *__restrict__ maxres,
+                                                   unsigned lim_bits, long npts, long nzx)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    if (maxres[p / nzx] <= lim_bits) {
+        T[p] = Tn[p];
+        A[p] = An[p];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_nn_slot_used(int *used, const unsigned *maxres,
+                                                      unsigned lim_bits, int nslots, int nzx)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nslots) used[s] = maxres[s] <= lim_bits ? nzx : 0;
 }
 
 static size_t infer_lds() { return NN_BLOB_BF16 * sizeof(bf16) + NN_BLOB_BIAS * sizeof(float); }
@@ -925,7 +956,8 @@ int nn_prepare(kmig_nn *n)
 
 int nn_replace_tuples(const kmig_nn *n, const SlotSpec *spec, int nslots, int nz, size_t nzx,
                       float z0, float dz, float x0, float dx, float dside, float max_dt, float *T,
-                      float *A, int *used, unsigned *maxres, hipStream_t st)
+                      float *A, int *used, unsigned *maxres, float *Tn, float *An,
+                      hipStream_t st)
 {
     if (!n || !n->blob_valid) return (int)hipErrorNotReady;
     if (nslots <= 0 || nzx == 0) return 0;
@@ -947,7 +979,17 @@ int nn_replace_tuples(const kmig_nn *n, const SlotSpec *spec, int nslots, int nz
     a.A = A;
     a.used = used;
     a.maxres = maxres;
-    return launch_infer(a, true, st);
+    a.Tn = Tn;
+    a.An = An;
+    int e = launch_infer(a, true, st);
+    if (e || !Tn) return e;
+    unsigned lim;
+    memcpy(&lim, &max_dt, sizeof(lim));
+    hipLaunchKernelGGL(k_nn_select, dim3(nblk((size_t)a.npts, 256)), dim3(256), 0, st, Tn, An, T,
+                       A, maxres, lim, a.npts, a.nzx);
+    hipLaunchKernelGGL(k_nn_slot_used, dim3(nblk((size_t)nslots, 256)), dim3(256), 0, st, used,
+                       maxres, lim, nslots, (int)nzx);
+    return (int)hipGetLastError();
 }
 
 }  // namespace kmig

@@ -46,8 +46,12 @@ int nn_prepare(kmig_nn *nn);
 // |T_nn - T| <= max_dt, replaces T by T_nn and A by |dT/ds_nn * dside|.
 // used[slot] counts the replaced points, maxres[slot] keeps the largest
 // |T_nn - T| (float bits); both must be cleared by the caller.
+// With Tn/An given (per-slot fallback), the network's values go there and a
+// slot takes them at every point only if its largest residual is within
+// max_dt (used = its points or 0); without them each point is decided alone.
 int nn_replace_tuples(const kmig_nn *nn, const SlotSpec *spec, int nslots, int nz, size_t nzx,
                       float z0, float dz, float x0, float dx, float dside, float max_dt, float *T,
-                      float *A, int *used, unsigned *maxres, hipStream_t st);
+                      float *A, int *used, unsigned *maxres, float *Tn, float *An,
+                      hipStream_t st);
 
 }  // namespace kmig

@@ -32,7 +32,8 @@ EXPORTS = ("kmig_params_default", "kmig_params_parse", "kmig_params_check", "kmi
            "kmig_stream", "kmig_version", "kmig_batch_create", "kmig_batch_run",
            "kmig_batch_sync", "kmig_batch_stats", "kmig_batch_last_error", "kmig_batch_destroy",
            "kmig_set_surrogate", "kmig_get_surrogate_stats", "kmig_batch_stitch",
-           "kmig_model", "kmig_count_updates", "kmig_batch_stitch_noalloc", "kmig_set_graph")
+           "kmig_model", "kmig_count_updates", "kmig_batch_stitch_noalloc", "kmig_set_graph",
+           "kmig_set_surrogate_fallback")
 
 
 class KmigError(RuntimeError):
@@ -126,6 +127,7 @@ def lib():
                                                 C.POINTER(C.c_float), C.c_int, C.c_int,
                                                 C.c_double, C.c_int, C.c_int, fp]
         L.kmig_set_surrogate.argtypes = [vp, C.c_int, vp, C.c_float]
+        L.kmig_set_surrogate_fallback.argtypes = [vp, C.c_int, C.c_int]
         L.kmig_get_surrogate_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_double),
                                                C.POINTER(C.c_double), C.POINTER(C.c_float)]
         _LIB = L
@@ -261,6 +263,11 @@ class Migrator:
             self._nets = {}
         self._nets[side] = net  # keep the network alive while the context uses it
         self._chk(self.L.kmig_set_surrogate(self.ctx, int(side), h, float(max_dt)))
+
+    def set_surrogate_fallback(self, side, per_slot=True):
+        """Per-slot (default: a shot or receiver position takes the network
+        everywhere or nowhere) or per-point fallback to the tables."""
+        self._chk(self.L.kmig_set_surrogate_fallback(self.ctx, int(side), 0 if per_slot else 1))
 
     def surrogate_stats(self, side) -> dict:
         pts, used, mx = C.c_double(), C.c_double(), C.c_float()

@@ -101,14 +101,15 @@ def test_surrogate_in_the_migration():
     # would otherwise each pick a preset by timing)
     p = kmig.parse_params(dict(w.kv, tile=9), 201, 0.0, 0.015)
 
-    def run(net):
+    def run(net, per_slot=False, max_dt=1e-3):
         m = kmig.Migrator(p)
         m.set_tables(0, w.stab, w.sder, w.s_axes)
         m.set_tables(1, w.rtab, w.rder, w.r_axes)
         m.set_data(w.data, w.data_axes)
         if net is not None:
-            m.set_surrogate(0, net, 1e-3)
-            m.set_surrogate(1, net, 1e-3)
+            for side in (0, 1):
+                m.set_surrogate(side, net, max_dt)
+                m.set_surrogate_fallback(side, per_slot)
         m.run()
         m.sync()
         c = m.cig()
@@ -143,6 +144,23 @@ def test_surrogate_in_the_migration():
     # form is relL2 <= 1e-3, which this network misses by ~400x: a 1 ms
     # traveltime error is a 0.1 rad phase error at 15 Hz.
     assert rel < 0.45
+    # Per-slot fallback (the default): a shot or receiver position takes the
+    # network at all of its points or at none.  With no limit every slot takes
+    # it, which is the per-point result with no limit, bit for bit; with a zero
+    # limit none does, which is the table result, bit for bit; at 1 ms the
+    # slots whose worst point stays within 1 ms take it.
+    cig_all_s, st_all, _ = run(net, per_slot=True, max_dt=1e3)
+    cig_all_p, _, _ = run(net, per_slot=False, max_dt=1e3)
+    assert np.array_equal(cig_all_s.view(np.uint32), cig_all_p.view(np.uint32))
+    assert sum(q["used"] for q in st_all) == sum(q["points"] for q in st_all)
+    cig_none, st_none, _ = run(net, per_slot=True, max_dt=0.0)
+    assert np.array_equal(cig_none.view(np.uint32), cig_t.view(np.uint32))
+    assert sum(q["used"] for q in st_none) == 0
+    cig_s, st_s, _ = run(net, per_slot=True, max_dt=1e-3)
+    frac_s = sum(q["used"] for q in st_s) / sum(q["points"] for q in st_s)
+    print(f"per-slot fallback at 1 ms: {frac_s:.4f} of the points (whole slots) from the "
+          f"network, CIG relL2 vs tables {_rel(cig_s, cig_t):.3f}; with every slot "
+          f"{_rel(cig_all_s, cig_t):.3f}")
     assert np.array_equal(cig_off, cig_t)
     assert 4 * nn.PARAMS < 0.01 * 4 * 2 * (full.stab.size + full.rtab.size)
     untrained.close()
