@@ -218,6 +218,8 @@ def main():
                                "vs_fp32": rp["value"] / value, **accuracy(cp, cig64, "gpu")}
                 log(f"{prec}: {rp['value']:.3e} updates/s ({forms[prec]['vs_fp32']:.3f} x fp32), "
                     f"relL2 vs f64 {forms[prec]['rel_l2_gpu_vs_f64']:.2e}")
+            if args.config.startswith("salt"):
+                forms["surrogate"] = surrogate_form(args, w, kv, sub, cig64, value)
 
     (nt, _, dt), (nh, _, _), (ns, _, _) = ax
     kernel = (f"k_migrate_ph (preset {st['tile']})" if st["variant"] == 2 else "k_migrate")
@@ -252,6 +254,58 @@ def main():
         "pcie_inclusive_updates_per_s": w.updates / (r["ms_per_step"] * 1e-3 + upload_s),
     }
     print(json.dumps(out), flush=True)
+
+
+def surrogate_form(args, w, kv, sub, cig64, value_fp32):
+    """The learned-surrogate form of the workload: the network fitted on
+    records of the workload's own tables, evaluated in every step's tuple
+    preparation on both sides, with the per-slot fallback (a shot or
+    receiver position whose worst point is more than 1 ms off keeps its
+    tables).  Timed like the FP32 line, accuracy on the same shot sample."""
+    import numpy as np
+
+    from kirch_datuming_amd import configs, kmig, nn
+    ax = w.data_axes
+    (ns, s0, ds), (nr, r0, dr) = w.s_axes[2], w.r_axes[2]
+    pos = np.concatenate([s0 + np.arange(ns) * ds, r0 + np.arange(nr) * dr])
+    t0 = time.perf_counter()
+    rec = configs.table_records(np.concatenate([w.stab, w.rtab]),
+                                np.concatenate([w.sder, w.rder]), pos, 4096, 20260707)
+    net = nn.Surrogate(**configs.SURROGATE_SALT)
+    net.set_data(rec)
+    losses = net.train(args.fit_steps)
+    log(f"surrogate: {rec.shape[0]} records, {args.fit_steps} steps in "
+        f"{time.perf_counter() - t0:.1f} s (loss {losses[-1]:.3g})")
+
+    def migrator(data, axes):
+        m = kmig.Migrator(kmig.parse_params(kv, int(axes[1][0]), axes[1][1], axes[1][2]))
+        m.set_tables(0, w.stab, w.sder, w.s_axes)
+        m.set_tables(1, w.rtab, w.rder, w.r_axes)
+        m.set_data(data, axes)
+        for side in (0, 1):
+            m.set_surrogate(side, net, 1e-3)
+        return m
+
+    with migrator(w.data, ax) as m:
+        r = timed_steps(m, args.warmup, min(args.steps, 5), w.updates)
+        sst = [m.surrogate_stats(k) for k in (0, 1)]
+    with migrator(sub.data, sub.data_axes) as m:
+        m.run()
+        m.sync()
+        cig = m.cig()
+    pts = sum(q["points"] for q in sst)
+    used = sum(q["used"] for q in sst)
+    out = {"value": r["value"], "ms_per_step": r["ms_per_step"], "vs_fp32": r["value"] / value_fp32,
+           "tuple_prep_ms": r["tuple_prep_ms"], "fallback": "per slot, |dT| > 1 ms",
+           "points_from_network_fraction": used / pts if pts else 0.0,
+           "max_residual_s": max(q["max_residual"] for q in sst),
+           "training_records": int(rec.shape[0]), "training_steps": args.fit_steps,
+           **accuracy(cig, cig64, "gpu")}
+    net.close()
+    log(f"surrogate: {out['value']:.3e} updates/s ({out['vs_fp32']:.3f} x fp32), "
+        f"{out['points_from_network_fraction']:.3f} of the points from the network, relL2 vs f64 "
+        f"{out['rel_l2_gpu_vs_f64']:.2e}")
+    return out
 
 
 def rel_l2(a, b):

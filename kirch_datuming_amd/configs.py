@@ -274,6 +274,11 @@ def stack_layers(cigs):
 # its 3 x 2 km section (positions 0..3 km), reference slowness 1/2 s/km.
 SURROGATE = dict(s_min=0.0, s_max=3.0, z_min=0.0, z_max=2.0, x_min=0.0, x_max=3.0, z_src=0.0,
                  slowness=0.5, sigma_t=0.001, sigma_g=0.3, batch=16384, lr=2e-3, lr_final=2e-5)
+# The same network over the salt section (10 x 4 km, positions 0..10 km; the
+# salt model's slowness runs 1/4.5 .. 1/1.5 s/km).
+SURROGATE_SALT = dict(s_min=0.0, s_max=10.0, z_min=0.0, z_max=4.0, x_min=0.0, x_max=10.0,
+                      z_src=0.0, slowness=0.4, sigma_t=0.001, sigma_g=0.3, batch=16384,
+                      lr=2e-3, lr_final=2e-5)
 
 
 def table_records(T, D, pos, nper, seed, dz=0.01, dx=0.01):
@@ -282,15 +287,24 @@ def table_records(T, D, pos, nper, seed, dz=0.01, dx=0.01):
     dT/dz are centred differences of the table (one-sided at the edges)."""
     rng = np.random.default_rng(seed)
     _, nx, nz = T.shape
+
+    def diff(Tk, i, j, axis, n, h):
+        # numpy.gradient's stencil at the sampled nodes only: centred inside,
+        # one-sided first order at the edges
+        lo, hi = np.maximum(i - 1, 0), np.minimum(i + 1, n - 1)
+        a = Tk[hi, j] if axis == 0 else Tk[j, hi]
+        b = Tk[lo, j] if axis == 0 else Tk[j, lo]
+        return (a - b) / ((hi - lo) * h)
+
     out = []
     for k, s in enumerate(pos):
-        Tk = T[k].astype(np.float64)
-        gx = np.gradient(Tk, dx, axis=0)
-        gz = np.gradient(Tk, dz, axis=1)
+        Tk = T[k]
         ix = rng.integers(0, nx, nper)
         iz = rng.integers(0, nz, nper)
-        out.append(np.stack([np.full(nper, s), iz * dz, ix * dx, Tk[ix, iz], gx[ix, iz],
-                             gz[ix, iz], D[k][ix, iz]], axis=1))
+        gx = diff(Tk.astype(np.float64, copy=False), ix, iz, 0, nx, dx)
+        gz = diff(Tk.astype(np.float64, copy=False), iz, ix, 1, nz, dz)
+        out.append(np.stack([np.full(nper, s), iz * dz, ix * dx, Tk[ix, iz].astype(np.float64),
+                             gx, gz, D[k][ix, iz]], axis=1))
     return np.concatenate(out).astype(np.float32)
 
 
