@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdlib>
 
+#include "kmig_index.h"
 #include "kmig_internal.h"
 
 namespace kmig {
@@ -519,8 +520,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
         for (int jx = 0; jx < NCX; jx++)
 #pragma unroll
             for (int jz = 0; jz < NCZ; jz++)
-                cell[jx * NCZ + jz] =
-                    min(cx0 + jx, a.ncx - 1) * a.ncz + min(cz0 + jz, a.ncz - 1);
+                cell[jx * NCZ + jz] = idx::cell(cx0, jx, cz0, jz, a.ncx, a.ncz);
         float tl = a.oaztaper > 0.0f ? bhi : plim;
         for (int o = 32; o > 0; o >>= 1) tl = fmaxf(tl, __shfl_xor(tl, o, 64));
         tl = fmaxf(fminf(tl, 1000.0f), -1000.0f);
@@ -536,7 +536,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
 #pragma unroll
         for (int c = 0; c < NC; c++)
             for (int b = b0; b <= b1; b++)
-                sall = min(sall, a.sblk[(size_t)cell[c] * a.scell_nblk + b].tmin);
+                sall = min(sall, a.sblk[idx::bnd(cell[c], (size_t)a.scell_nblk, b)].tmin);
     }
 
     unsigned long long n_eval = 0, n_acc = 0;  // Count only (wave-uniform)
@@ -566,7 +566,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 if (slot < 0) continue;
 #pragma unroll
                 for (int c = 0; c < NC; c++) {
-                    const ThBound b = a.rbnd[(size_t)cell[c] * a.nrslots + slot];
+                    const ThBound b = a.rbnd[idx::bnd(cell[c], (size_t)a.nrslots, slot)];
                     rlo = min(rlo, b.lo);
                     rhi = max(rhi, b.hi);
                     rtm = min(rtm, b.tmin);
@@ -597,7 +597,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             int slo = BND_EMPTY, shi = -BND_EMPTY, stm = BND_EMPTY;
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                const ThBound b = bnd[(size_t)cell[c] * stride + j];
+                const ThBound b = bnd[idx::bnd(cell[c], stride, j)];
                 slo = min(slo, b.lo);
                 shi = max(shi, b.hi);
                 stm = min(stm, b.tmin);
@@ -653,13 +653,13 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             const float s_frac = st.frac + a.c_frac;
             const float s_w = (float)(st.iw & 0xFFFFu) * (inv32k * inv32k);
             const int is = a.shot0 + jc;
-            const float *__restrict__ wrow = a.wap + (size_t)is * a.nh + r0;
-            const float2 *__restrict__ trow = pairs + ((size_t)is * a.nh + r0) * a.nt;
+            const float *__restrict__ wrow = a.wap + idx::wap(is, a.nh, r0, 0);
+            const float2 *__restrict__ trow = pairs + idx::chunk_row(is, a.nh, r0) * a.nt;
             // the chunk's traces of this shot (Cols): receiver index and
             // trace-aperture weight per column; the rows are padded, and a
             // padding column has zero tuple weight
             const ColTrace *__restrict__ crow =
-                Cols ? a.ctab + (size_t)(a.scell_off + jc) * a.ncols + r0 : nullptr;
+                Cols ? a.ctab + idx::ctab(a.scell_off, jc, a.ncols, r0, 0) : nullptr;
 #pragma unroll
             for (int g0 = 0; g0 < R; g0 += G) {
                 // stage 1: opening angle, bin coordinate and weight.  aw =
@@ -801,10 +801,9 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                     if constexpr (Cols) {
                         // a column without a trace in this shot (weight < 0)
                         // reads receiver 0's sentinel pair
-                        tr = (const char *)(pairs + ((size_t)is * a.nh + crow[g0 + k].ih) * a.nt);
+                        tr = (const char *)(pairs + idx::col_row(is, a.nh, crow[g0 + k].ih) * a.nt);
                     } else {
-                        const int kr = min(g0 + k, a.nh - 1 - r0);
-                        tr = (const char *)(trow + (size_t)kr * a.nt);
+                        tr = (const char *)(trow + (size_t)idx::tap_row_off(a.nh, r0, g0 + k) * a.nt);
                     }
                     v0[k] = *(const float2 *)(tr + q[k].o0);
                     vm[k] = *(const float2 *)(tr + q[k].om);
