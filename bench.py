@@ -37,7 +37,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="salt",
-                    choices=["salt", "salt_cmp1", "small", "layers", "surrogate", "datum"])
+                    choices=["salt", "salt_cmp1", "salt_stack", "small", "layers", "surrogate",
+                             "datum"])
     ap.add_argument("--grf-models", type=int, default=256,
                     help="surrogate: velocity models of the pretraining set")
     ap.add_argument("--fit-steps", type=int, default=8000,
@@ -105,6 +106,8 @@ def main():
         w = configs.salt(threads=cores)
     elif args.config == "salt_cmp1":
         w = configs.salt(threads=cores, cmp=1)
+    elif args.config == "salt_stack":
+        w = configs.salt(threads=cores, cmp=1, stack=True)
     else:
         w = configs.small(threads=cores)
     log(f"workload ready in {time.time() - t0:.1f} s: {w.name}, {w.updates:.4e} updates")

@@ -140,13 +140,14 @@ def small(threads=0, seed=20260101) -> Workload:
                     RT, RD, tax + [[nh, h0, dh]], vel)
 
 
-def salt(threads=0, seed=20260202, nshots=768, cmp=0) -> Workload:
+def salt(threads=0, seed=20260202, nshots=768, cmp=0, stack=False) -> Workload:
     """The salt workload.  cmp=0 (the workflow's geometry): 401 fixed
     receivers every 0.025 km.  cmp=1 (the program's default geometry): 401
     split-spread offsets -2.5 .. 2.5 km every 0.0125 km moving with the shot,
     receiver tables at the 800 positions 0 .. 9.9875 km every 0.0125 km
     (receivers beyond the model edge take the edge table, as the reference's
-    clamped table lookup does)."""
+    clamped table lookup does).  stack=True: the stacked image (cig=0, the
+    program's default output) instead of 61 angle bins."""
     L = synth()
     nz, nx, d = 400, 1000, 0.01
     vel = np.empty((nx, nz), np.float32)
@@ -169,6 +170,10 @@ def salt(threads=0, seed=20260202, nshots=768, cmp=0) -> Workload:
               direct_mute_taper=0.05)
     tax = [[nz, 0.0, d], [nx, 0.0, d]]
     geo = "401 offsets -2.5..2.5 km (cmp=1)" if cmp else "401 receivers"
+    if stack:
+        kv = {k: v for k, v in kv.items() if not k.startswith(("mode", "angle_"))}
+        kv["cig"] = 0
+        geo += ", stacked image"
     return Workload(f"salt 400x1000, {ns} shots x {geo}, nt=2001", kv, data,
                     [[nt, 0.0, dt], [nh, h0, dh], [ns, s0, ds]], ST, SD, tax + [[ns, s0, ds]],
                     RT, RD, tax + [[nr, r0, dr]], vel)

@@ -112,6 +112,10 @@ struct kmig_ctx {
     size_t ctab_cap = 0;
     int *cfirst = nullptr, *clast = nullptr;
     size_t cfirst_cap = 0, clast_cap = 0;
+    // offset gathers and the stack on the resident kernel: the output bin of
+    // each receiver index (MigResArgs::rbin)
+    int *rbin = nullptr;
+    size_t rbin_cap = 0;
     int variant_used = 0;
     int tuned_tile = 0;             // autotuned resident preset (0: not tuned yet)
     long long tuned_version = -1;
@@ -398,7 +402,8 @@ int plan_run(kmig_ctx *c, int sb, int se, std::vector<TraceMeta> &meta,
     // lies far below the float rounding of the traveltimes themselves.  The
     // trace records (aperture index, mute) keep each trace's own position.
     std::unordered_map<uint32_t, float> pos_rep;
-    const bool group_pos = p.adj == 1 && p.cmp && angle && p.angle_interp == 1 &&
+    const bool group_pos = p.adj == 1 && p.cmp &&
+                           (angle ? p.angle_interp == 1 : p.precision == KMIG_PREC_FP32) &&
                            p.antialias >= 0.0f && p.variant != KMIG_VARIANT_STREAM;
     if (group_pos) {
         std::vector<float> all;
@@ -523,8 +528,7 @@ int plan_run(kmig_ctx *c, int sb, int se, std::vector<TraceMeta> &meta,
     c->rcv_columns = false;
     cols = Columns{};
     // (migration only: the adj=0 transpose keeps receiver indices)
-    if (p.adj == 1 && !invariant && c->batches.size() == 1 && p.cig && p.cig_is_angle &&
-        p.angle_interp == 1) {
+    if (p.adj == 1 && !invariant && c->batches.size() == 1 && (!angle || p.angle_interp == 1)) {
         build_columns(c, sb, se, meta, rspec, cols);
         c->rcv_columns = !cols.colslot.empty();
     }
@@ -643,6 +647,7 @@ MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
         R.clast = c->clast;
         R.ncols = c->ncols;
     }
+    if (!(p.cig && p.cig_is_angle)) R.rbin = c->rbin;
     R.cig = c->cig;
     const double cst = ((double)p.tau - (double)c->t0) / (double)c->dt;
     const double ci = std::floor(cst);
@@ -677,10 +682,13 @@ int autotune(kmig_ctx *c, const MigResArgs &full)
     if (c->tuned_tile > 0 && c->tuned_version == c->geo_version) return KMIG_OK;
     static const int presets32[] = {9, 15, 16, 20, 11, 10, 18};
     static const int presets16[] = {9, 15, 10, 13, 11};
-    // the 16-bit formats and receiver columns are built for five presets
+    // the 16-bit formats and receiver columns are built for five presets,
+    // offset gathers and stacks for three
+    static const int presets_bin[] = {9, 15, 11};
     const bool p16 = c->p.precision != KMIG_PREC_FP32 || c->rcv_columns;
-    const int *presets = p16 ? presets16 : presets32;
-    const int npresets = p16 ? 5 : 7;
+    const bool pbin = full.rbin != nullptr;
+    const int *presets = pbin ? presets_bin : (p16 ? presets16 : presets32);
+    const int npresets = pbin ? 3 : (p16 ? 5 : 7);
     const double per_shot = (double)c->nh * (double)c->nzx;
     int S = (int)std::max(1.0, std::min((double)full.nshots, 5e9 / std::max(per_shot, 1.0)));
     // two half slices, centred at 1/4 and 3/4 of the batch (source tuples are
@@ -804,15 +812,22 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
         // resident variant: angle CIG, two-bin interpolation, receivers that do
         // not move with the shot, one batch, times within the int16 sample range
         const double cst = ((double)p.tau - (double)c->t0) / (double)c->dt;
-        const bool res_ok = angle && p.angle_interp == 1 &&
-                            (c->rcv_invariant || c->rcv_columns) &&
+        // bins the resident kernel forms: two linear angle bins, or the
+        // trace's own bin (offset gathers, the stack) for migration with FP32
+        // tuples and an axis whose LDS columns fit 8 x 8 tiles in 64 KiB
+        const int ng_out = p.cig ? p.ng : 1;
+        const bool bins_ok = angle ? p.angle_interp == 1
+                                   : (p.adj == 1 && p.precision == KMIG_PREC_FP32 &&
+                                      migrate_lds_bytes(ng_out, 15) <= 64 * 1024);
+        const bool res_ok = bins_ok && (c->rcv_invariant || c->rcv_columns) &&
                             p.antialias >= 0.0f && std::fabs(cst) + (double)c->nt < 30000.0;
         int variant = KMIG_VARIANT_STREAM;
         if (p.variant == KMIG_VARIANT_RESIDENT) {
             if (!res_ok)
                 return fail(c, KMIG_E_PARAM,
-                            "variant=resident needs mode=angle, angle_interp=1, antialias >= 0 "
-                            "and tuples that fit one batch");
+                            "variant=resident needs linear angle bins (mode=angle, angle_interp=1) "
+                            "or migration with FP32 tables into offset gathers or a stack, "
+                            "antialias >= 0 and tuples that fit one batch");
             variant = KMIG_VARIANT_RESIDENT;
         } else if (p.variant == KMIG_VARIANT_AUTO && res_ok) {
             variant = KMIG_VARIANT_RESIDENT;
@@ -869,6 +884,19 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
             HIPCHK(c, hipMemcpy(c->rslot, rs.data(), rs.size() * sizeof(int),
                                 hipMemcpyHostToDevice));
             c->ncols = c->rcv_columns ? (int)cols.colslot.size() : 0;
+            if (!angle) {
+                // per receiver index: its offset bin (offset gathers; a
+                // receiver outside the axis has no migrating trace), or 0
+                std::vector<int> rb((size_t)c->nh, 0);
+                const float dh = (c->nh == 1) ? 0.0f : c->dh;
+                for (int ih = 0; ih < c->nh && p.cig; ih++) {
+                    const float ig = nearest_bin_h(c->h0 + (float)ih * dh, p.g0, p.dg, p.ng);
+                    rb[(size_t)ih] = ig < 0 ? 0 : (int)ig;
+                }
+                if ((rc = ensure(c, &c->rbin, &c->rbin_cap, rb.size()))) return rc;
+                HIPCHK(c, hipMemcpy(c->rbin, rb.data(), rb.size() * sizeof(int),
+                                    hipMemcpyHostToDevice));
+            }
             if (c->rcv_columns) {
                 if ((rc = ensure(c, &c->ctab, &c->ctab_cap, cols.ctab.size()))) return rc;
                 if ((rc = ensure(c, &c->cfirst, &c->cfirst_cap, cols.cfirst.size()))) return rc;
@@ -1167,6 +1195,7 @@ void kmig_destroy(kmig_ctx *c)
     dfree(c->ctab);
     dfree(c->cfirst);
     dfree(c->clast);
+    dfree(c->rbin);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->gfork) (void)hipEventDestroy(c->gfork);

@@ -217,6 +217,10 @@ struct MigResArgs {
     const ColTrace *ctab;
     const int *cfirst, *clast;
     int ncols;
+    // offset gathers and the stack (cig=0) on the resident kernel: the output
+    // bin of each receiver index (offset bin, or 0 for the stack); null for
+    // angle gathers
+    const int *rbin;
     int c_int;                   // integer part of (tau - t0)/dt
     int t_cull;                  // nt - 2 - c_int: a group whose smallest source plus
                                  // receiver integer samples reach it has every

@@ -360,16 +360,26 @@ __device__ __forceinline__ int cvt_floor(float x)
 // u is the bin coordinate.  s_int8, s_frac: the source tuple's integer
 // samples (as a byte offset of 8-byte tap pairs) and fractional samples, with
 // the constant (tau - t0)/dt already added (once per shot).
+// Bin = 0: two linear angle bins from u; Bin = 1: the trace's own bin ibin
+// with all of the weight (offset gathers and the stack).
+template <int Bin>
 __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int8, float s_frac,
-                                           const PackedTuple &r, float aw, float u,
+                                           const PackedTuple &r, float aw, float u, int ibin,
                                            const MigResArgs &a)
 {
     Pend q;
     // linear bins: u clamped to [0, ng-1] gives the edge bin with w1 = 0 at
     // either end and (floor(u), u - floor(u)) inside
-    const float uc = __builtin_amdgcn_fmed3f(u, 0.0f, (float)(a.ng - 1));
-    const int ig0 = cvt_floor(uc);
-    const float w1 = __builtin_amdgcn_fractf(uc);
+    int ig0;
+    float w1;
+    if constexpr (Bin == 0) {
+        const float uc = __builtin_amdgcn_fmed3f(u, 0.0f, (float)(a.ng - 1));
+        ig0 = cvt_floor(uc);
+        w1 = __builtin_amdgcn_fractf(uc);
+    } else {
+        ig0 = ibin;
+        w1 = 0.0f;
+    }
     // split-sample time: integer parts add as integers (in bytes of 8-byte
     // tap pairs), fractions in float.  Taps at ff and ff -/+ (delta + 1)
     // samples; both a fields are finite and >= 1 (one plus antialias >= 0
@@ -451,12 +461,16 @@ __device__ __forceinline__ void wave_lds_sync()
 // over receiver columns (a.ncols; column c keeps tuple slot a.rslot[c]) and
 // each chunk sweeps only the shots with a trace in one of its columns, whose
 // receiver index and trace-aperture weight come from a.ctab.
+// Bin = 1: offset gathers and the stacked image (cig=0): no directions, no
+// angle weights; every update goes with all of its weight to its trace's bin
+// a.rbin[receiver index] (src/kirchmig2d_cig.cpp:1100-1102, :1327-1331).
 template <int TZ, int TX, int R, int G, int Fmt, bool Model = false, bool Count = false,
-          bool Cols = false>
+          bool Cols = false, int Bin = 0>
 __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, float *acc,
                                              int sg = 0, int nsplit = 1)
 {
     static_assert(!(Model && Cols), "adj=0 with receiver columns runs on the stream kernel");
+    static_assert(!(Model && Bin), "adj=0 offset gathers and stacks run on the stream kernel");
     constexpr int P = TZ * TX;
     const int lid = threadIdx.x;
     // tile order z fastest (measured faster than x fastest on the salt
@@ -525,6 +539,8 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
         for (int o = 32; o > 0; o >>= 1) tl = fmaxf(tl, __shfl_xor(tl, o, 64));
         tl = fmaxf(fminf(tl, 1000.0f), -1000.0f);
         tq = __builtin_amdgcn_readfirstlane((int)ceilf(tl * BND_Q)) + 64;
+        // per-trace bins have no directions: only the time-window tests cull
+        if constexpr (Bin == 1) tq = 1 << 30;
     }
     const int wrapq = (int)(a.wrap * BND_Q);
     // smallest source sample of the wave's cells over the shot group's
@@ -675,21 +691,31 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 // wave-level skip of rejected groups measured slower: few
                 // groups are empty.)
                 float ang[G], u[G], aw[G], base[G];
+                int ibin[G];  // per-trace bins (offset gathers, stack)
                 bool band = false;
 #pragma unroll
                 for (int k = 0; k < G; k++) {
                     const float wk = Cols ? crow[g0 + k].wap : wrow[g0 + k];
                     base[k] = wk * s_w * (float)(rt[g0 + k].iw & 0xFFFFu);
-                    ang[k] = pair_angle(st, rt[g0 + k], a.wrap);
-                    u[k] = fmaf(ang[k], a.inv_dg, a.g0_inv_dg);  // (angle - g0) / dg
-                    aw[k] = ang[k] <= plim ? base[k] : 0.0f;
+                    if constexpr (Bin == 0) {
+                        ang[k] = pair_angle(st, rt[g0 + k], a.wrap);
+                        u[k] = fmaf(ang[k], a.inv_dg, a.g0_inv_dg);  // (angle - g0) / dg
+                        aw[k] = ang[k] <= plim ? base[k] : 0.0f;
+                    } else {
+                        // the trace's own bin (a receiver index past nh pads
+                        // the chunk with zero weight: any bin of the axis)
+                        const int ih = Cols ? crow[g0 + k].ih : idx::imin(r0 + g0 + k, a.nh - 1);
+                        ibin[k] = a.rbin[ih];
+                        u[k] = 0.0f;
+                        aw[k] = base[k];
+                    }
                 }
-                if (wave_band) {
+                if (Bin == 0 && wave_band) {
 #pragma unroll
                     for (int k = 0; k < G; k++)  // bitwise, not short-circuit: no branches
                         band |= (ang[k] > cutoff) & (ang[k] < bhi);
                 }
-                if (wave_band && __ballot(band) != 0) {
+                if (Bin == 0 && wave_band && __ballot(band) != 0) {
 #pragma unroll
                     for (int k = 0; k < G; k++) {
                         const float rd = ramp_down(
@@ -699,12 +725,12 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                         aw[k] = ang[k] <= a.ang_hi ? wd * base[k] : 0.0f;
                     }
                 }
-                if (a.lo_check) {
+                if (Bin == 0 && a.lo_check) {
 #pragma unroll
                     for (int k = 0; k < G; k++) aw[k] = ang[k] >= a.ang_lo ? aw[k] : 0.0f;
                 }
                 // the angle taper (angle_taper_start/end) multiplies in here
-                if (taper_on) {
+                if (Bin == 0 && taper_on) {
 #pragma unroll
                     for (int k = 0; k < G; k++) {
                         const float rt_ = ramp_down(fminf(
@@ -718,7 +744,8 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 bool any = false;
 #pragma unroll
                 for (int k = 0; k < G; k++) {
-                    q[k] = res_phase1(st, s_int8, s_frac, rt[g0 + k], aw[k], u[k], a);
+                    q[k] = res_phase1<Bin>(st, s_int8, s_frac, rt[g0 + k], aw[k], u[k], ibin[k],
+                                           a);
                     any |= q[k].ok;
                 }
                 if constexpr (Count) {
@@ -860,20 +887,20 @@ __global__ __launch_bounds__(256) void k_sum_parts(const float *__restrict__ par
     }
 }
 
-template <int TZ, int TX, int R, int G, bool Probe, int Fmt, bool Cols = false>
+template <int TZ, int TX, int R, int G, bool Probe, int Fmt, bool Cols = false, int Bin = 0>
 __global__ __launch_bounds__(TZ *TX, 3) void k_migrate_ph(const MigResArgs a)
 {
     extern __shared__ float acc[];
-    migrate_tile<TZ, TX, R, G, Fmt, false, false, Cols>(a, xcd_tile(blockIdx.x, gridDim.x), acc,
-                                                        (int)blockIdx.y, (int)gridDim.y);
+    migrate_tile<TZ, TX, R, G, Fmt, false, false, Cols, Bin>(
+        a, xcd_tile(blockIdx.x, gridDim.x), acc, (int)blockIdx.y, (int)gridDim.y);
 }
 
-template <int TZ, int TX, int R, int G, bool Cols = false>
+template <int TZ, int TX, int R, int G, bool Cols = false, int Bin = 0>
 __global__ __launch_bounds__(TZ *TX, 3) void k_count_ph(const MigResArgs a)
 {
     extern __shared__ float acc[];
-    migrate_tile<TZ, TX, R, G, 0, false, true, Cols>(a, xcd_tile(blockIdx.x, gridDim.x), acc,
-                                                     (int)blockIdx.y, (int)gridDim.y);
+    migrate_tile<TZ, TX, R, G, 0, false, true, Cols, Bin>(a, xcd_tile(blockIdx.x, gridDim.x),
+                                                          acc, (int)blockIdx.y, (int)gridDim.y);
 }
 
 template <int TZ, int TX, int R, int G>
@@ -949,13 +976,14 @@ static int launch_mig_t(const MigArgs &a, hipStream_t st)
     return (int)hipGetLastError();
 }
 
-template <int TZ, int TX, int R, int G, bool Probe = false, int Fmt = 0, bool Cols = false>
+template <int TZ, int TX, int R, int G, bool Probe = false, int Fmt = 0, bool Cols = false,
+          int Bin = 0>
 static int launch_ph_t(const MigResArgs &a, hipStream_t st)
 {
     const int ntz = (a.nz + TZ - 1) / TZ, ntx = (a.nx + TX - 1) / TX;
     const size_t lds = (size_t)(a.ng + 1) * TZ * TX * sizeof(float);
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_ph<TZ, TX, R, G, Probe, Fmt, Cols>,
+        const hipError_t e = hipFuncSetAttribute((const void *)k_migrate_ph<TZ, TX, R, G, Probe, Fmt, Cols, Bin>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
@@ -967,7 +995,7 @@ static int launch_ph_t(const MigResArgs &a, hipStream_t st)
     // count (tests).
     const int ntiles = ntz * ntx;
     const int nsplit = res_splits(a, ntiles);
-    hipLaunchKernelGGL((k_migrate_ph<TZ, TX, R, G, Probe, Fmt, Cols>), dim3((unsigned)ntiles,
+    hipLaunchKernelGGL((k_migrate_ph<TZ, TX, R, G, Probe, Fmt, Cols, Bin>), dim3((unsigned)ntiles,
                        (unsigned)nsplit), dim3(TZ * TX), lds, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || nsplit == 1) return (int)e;
@@ -983,6 +1011,24 @@ static int launch_ph_t(const MigResArgs &a, hipStream_t st)
 template <bool Probe, int Fmt>
 static int launch_res_preset(const MigResArgs &a, int tile, hipStream_t st)
 {
+    if (a.rbin) {
+        // offset gathers and stacks: FP32 tuples, presets 9, 11 and 15
+        if constexpr (Fmt != 0) {
+            return (int)hipErrorInvalidValue;
+        } else if (a.ctab) {
+            switch (tile) {
+            case 11: return launch_ph_t<16, 8, 8, 8, Probe, 0, true, 1>(a, st);
+            case 15: return launch_ph_t<8, 8, 8, 4, Probe, 0, true, 1>(a, st);
+            default: return launch_ph_t<16, 8, 8, 4, Probe, 0, true, 1>(a, st);
+            }
+        } else {
+            switch (tile) {
+            case 11: return launch_ph_t<16, 8, 8, 8, Probe, 0, false, 1>(a, st);
+            case 15: return launch_ph_t<8, 8, 8, 4, Probe, 0, false, 1>(a, st);
+            default: return launch_ph_t<16, 8, 8, 4, Probe, 0, false, 1>(a, st);
+            }
+        }
+    }
     if (a.ctab) {
         // receiver columns (cmp=1): the five presets of the 16-bit formats
         switch (tile) {
@@ -1072,19 +1118,19 @@ int launch_migrate_resident_probe(const MigResArgs &a, int tile, int precision, 
     return launch_res_prec<true>(a, tile, precision, (hipStream_t)stream);
 }
 
-template <int TZ, int TX, int R, int G, bool Cols = false>
+template <int TZ, int TX, int R, int G, bool Cols = false, int Bin = 0>
 static int launch_count_t(const MigResArgs &a, hipStream_t st)
 {
     const int ntz = (a.nz + TZ - 1) / TZ, ntx = (a.nx + TX - 1) / TX;
     const size_t lds = (size_t)(a.ng + 1) * TZ * TX * sizeof(float);
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_count_ph<TZ, TX, R, G, Cols>,
+        const hipError_t e = hipFuncSetAttribute((const void *)k_count_ph<TZ, TX, R, G, Cols, Bin>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
     const int ntiles = ntz * ntx;
-    hipLaunchKernelGGL((k_count_ph<TZ, TX, R, G, Cols>), dim3((unsigned)ntiles,
+    hipLaunchKernelGGL((k_count_ph<TZ, TX, R, G, Cols, Bin>), dim3((unsigned)ntiles,
                        (unsigned)res_splits(a, ntiles)), dim3(TZ * TX), lds, st, a);
     return (int)hipGetLastError();
 }
@@ -1092,6 +1138,20 @@ static int launch_count_t(const MigResArgs &a, hipStream_t st)
 int launch_count_resident(const MigResArgs &a, int tile, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
+    if (a.rbin && a.ctab) {
+        switch (tile) {
+        case 11: return launch_count_t<16, 8, 8, 8, true, 1>(a, st);
+        case 15: return launch_count_t<8, 8, 8, 4, true, 1>(a, st);
+        default: return launch_count_t<16, 8, 8, 4, true, 1>(a, st);
+        }
+    }
+    if (a.rbin) {
+        switch (tile) {
+        case 11: return launch_count_t<16, 8, 8, 8, false, 1>(a, st);
+        case 15: return launch_count_t<8, 8, 8, 4, false, 1>(a, st);
+        default: return launch_count_t<16, 8, 8, 4, false, 1>(a, st);
+        }
+    }
     if (a.ctab) {
         switch (tile) {
         case 11: return launch_count_t<16, 8, 8, 8, true>(a, st);
