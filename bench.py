@@ -585,9 +585,9 @@ def bench_datum(args, cores):
             ds.sync()
             gpu_sub = ds.output()
             sst = ds.stats()
-        log("FP64 oracle on the same corner")
-        ref64 = D.datum(sub.kv, sub.data, sub.data_axes, sub.green, sub.pos_axis, sub.green,
-                        sub.pos_axis, sub.datum, precision="f64")
+        log("FP64 oracle (direct path) on the same corner")
+        ref64 = D.datum(dict(sub.kv, use_bf=0), sub.data, sub.data_axes, sub.green,
+                        sub.pos_axis, sub.green, sub.pos_axis, sub.datum, precision="f64")
         cpu = {"value": sst["cmacs"] / tcpu, "unit": "complex MAC/s", "cores": cores,
                "kind": "port",
                "sample": f"96 shots x 96 receivers of the same cube, tables and keys "

@@ -23,7 +23,9 @@ ERRORS = {-1: "bad arguments", -2: "Receiver table too small.", -3: "Source tabl
 class KmdoCfg(C.Structure):
     _fields_ = [("cmp", C.c_int), ("aper", C.c_int), ("tap", C.c_int), ("data_tap", C.c_int),
                 ("antialias", C.c_float), ("length", C.c_float), ("sdatum", C.c_float),
-                ("rdatum", C.c_float)]
+                ("rdatum", C.c_float), ("use_bf", C.c_int), ("bf_p", C.c_int),
+                ("bf_leaf", C.c_int), ("bf_levels", C.c_int), ("bf_amp_eps", C.c_float),
+                ("bf_phase_tol", C.c_float)]
 
 
 _BOUND = False
@@ -64,7 +66,8 @@ def _fp(a):
 
 
 def config(kv, sdatum, rdatum=None):
-    """The reference's keys and defaults (src/kirchdat2d_frequency_cs.cpp:1216-1270)."""
+    """The reference's keys and defaults (src/kirchdat2d_frequency_cs.cpp:1216-1270),
+    use_bf=1 and the bf_* keys included (:1244-1249)."""
     c = KmdoCfg()
     c.cmp = int(kv.get("cmp", 1))
     c.aper = int(kv.get("aperture", 50))
@@ -74,13 +77,22 @@ def config(kv, sdatum, rdatum=None):
     c.length = float(kv.get("length", 0.025))
     c.sdatum = float(sdatum)
     c.rdatum = float(sdatum if rdatum is None else rdatum)
+    c.use_bf = int(kv.get("use_bf", 1))
+    c.bf_p = int(kv.get("bf_p", 12))
+    c.bf_leaf = int(kv.get("bf_n_leaf", 16))
+    c.bf_levels = int(kv.get("bf_panel_levels", 1))
+    c.bf_amp_eps = float(kv.get("bf_amp_eps", 1e-20))
+    c.bf_phase_tol = float(kv.get("bf_phase_tol", 2.0))
     return c
 
 
 def datum(kv, data, data_axes, sgreen, sg_axis, rgreen, rg_axis, sdatum, rdatum=None,
           precision="f32", interm=False, iw_range=None):
-    """Direct-path datuming: returns out (ns, nh, nt) (and the receiver-side
-    cube when interm).  data_axes [[nt, t0, dt], [nh, h0, dh], [ns, s0, ds]];
+    """Datuming: returns out (ns, nh, nt) (and the receiver-side cube when
+    interm).  kv's use_bf (the reference's default 1) selects the butterfly
+    operators, which the oracle restates in float only (precision="f32");
+    use_bf=0 is the direct path in either precision.  data_axes [[nt, t0, dt],
+    [nh, h0, dh], [ns, s0, ds]];
     sg_axis / rg_axis = (o, d) of the table positions."""
     L = lib()
     d = np.ascontiguousarray(data, np.float32)

@@ -18,6 +18,14 @@
  *                                            (cmp 0 and 1)
  *   src/kirchdat2d_frequency_cs.cpp:1177-1413 main's order of the stages
  *   src/greensol_auto.cpp:23-101             the datum traveltime table
+ *   src/kirchdat2d_frequency_cs.cpp:674-1121 the butterfly operators
+ *                                            (use_bf=1, float build only):
+ *                                            per frequency the phase /
+ *                                            amplitude matrix of each side
+ *                                            (once for cmp=0, per shot or
+ *                                            per receiver gather for cmp=1)
+ *                                            through kbf_oracle.c's
+ *                                            segmented butterfly
  * The FFTs (FFTW in the reference, which is not built here) are a textbook
  * radix-2 Cooley-Tukey transform in double, with FFTW's sign and scaling
  * conventions (forward e^{-i}, unnormalised backward).
@@ -38,9 +46,14 @@
  *                      closed-form scalar functions (raised cosines: the
  *                      end values, midpoints and symmetry);
  *   f64 path           agrees with the f32 path to float rounding, linear in
- *                      the data.
+ *                      the data;
+ *   use_bf=1           the butterfly is pinned in kbf_oracle.c; the matrices
+ *                      carry the direct path's weights, so with no legal
+ *                      butterfly panel (bf_panel_levels=0) the two paths
+ *                      agree to float rounding.
  */
 #include "kmdat_oracle.h"
+#include "kbf_oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
@@ -131,7 +144,9 @@ int kmdo_greensol(const float *ttable, int nz, int nx, int ngx, float oz, float 
 #define R_FABS fabsf
 #define R_FMIN fminf
 #define R_FMAX fmaxf
+#define KMDO_WITH_BF 1
 #include "kmdat_oracle_impl.h"
+#undef KMDO_WITH_BF
 #undef R
 #undef SFX
 #undef R_SQRT
@@ -157,14 +172,14 @@ void kmdo_filter_f32(float tau, float dt, int nsam, float *Fk) { filter_f32(tau,
 
 void kmdo_H_f32(float tau, float dt, int nsam, int nfft, int iw, float *hr, float *hi)
 {
-    float *Fk = (float *)malloc(sizeof(float) * (nsam > 1 ? nsam : 2));
+    float *Fk = (float *)calloc(nsam > 1 ? nsam : 2, sizeof(float));
     Hw_f32(tau, dt, nsam, nfft, iw, Fk, hr, hi);
     free(Fk);
 }
 
 void kmdo_H_f64(double tau, double dt, int nsam, int nfft, int iw, double *hr, double *hi)
 {
-    double *Fk = (double *)malloc(sizeof(double) * (nsam > 1 ? nsam : 2));
+    double *Fk = (double *)calloc(nsam > 1 ? nsam : 2, sizeof(double));
     Hw_f64(tau, dt, nsam, nfft, iw, Fk, hr, hi);
     free(Fk);
 }

@@ -15,12 +15,18 @@ typedef struct {
     float antialias;  /* antialias= */
     float length;     /* length= (filter length, s) */
     float sdatum, rdatum;
+    int use_bf;       /* use_bf=: 1 applies each frequency's operator through the
+                         segmented butterfly (f32 only) */
+    int bf_p, bf_leaf, bf_levels;   /* bf_p=, bf_n_leaf=, bf_panel_levels= */
+    float bf_amp_eps, bf_phase_tol; /* bf_amp_eps=, bf_phase_tol= */
 } kmdo_cfg;
 
-/* Full direct path: data (ns, nh, nt) -> out (ns, nh, nt); interm (the
+/* Full path: data (ns, nh, nt) -> out (ns, nh, nt); interm (the
    receiver-side result, not reversed) when not NULL.  Frequencies outside
-   [iw_begin, iw_end) are left zero (iw_end <= 0: all).  Returns 0, -1 bad
-   arguments, -2 "Receiver table too small.", -3 "Source table too small.". */
+   [iw_begin, iw_end) are left zero (iw_end <= 0: all).  use_bf selects the
+   butterfly operators (kmdo_datum_f32 only; the f64 build returns -1 for it).
+   Returns 0, -1 bad arguments, -2 "Receiver table too small.", -3 "Source
+   table too small.". */
 int kmdo_datum_f32(const kmdo_cfg *cfg, const float *data, int nt, int nh, int ns, float dt,
                    float h0, float dh, float s0, float ds, const float *stable, int nsg,
                    float sg0, float dsg, const float *rtable, int nrg, float rg0, float drg,

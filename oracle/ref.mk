@@ -1,6 +1,6 @@
 # Compiles the reference's kirchmig2d_cig path (and the table/shot/compare
-# helpers and the layer stitch/extract programs used to write golden
-# fixtures) directly from its sources under
+# helpers, the layer stitch/extract programs and the segmented butterfly used
+# to write golden fixtures) directly from its sources under
 # REF, with the reference's release flags (-O3 -DNDEBUG + OpenMP).  Nothing
 # is copied, generated or stubbed: the vendored fftw3.h header and sqlite3.c
 # source are the reference's own.  Outputs go to _ref/ only.
@@ -18,7 +18,14 @@ OR := $(patsubst %.cpp,$(OUT)/obj/SERECKIRCH/src/%.o,$(SR))
 PROGS := kirchmig2d_cig eikods2d shotgen rsf_l2cmp layercom_cig extract_adcig kirchdat2d_auto_cs \
   greensol_auto
 
-all: $(addprefix $(OUT)/,$(PROGS))
+all: $(addprefix $(OUT)/,$(PROGS)) $(OUT)/libbf1d_ref.so
+
+# The butterfly of kirchdat2d_freq (SERECKIRCH/src/bf1d.cpp) as a shared
+# library: bf1d_ref_driver.cpp includes that source file where it lies and
+# adds ctypes entry points.  It uses only the vendored fftw3.h types.
+$(OUT)/libbf1d_ref.so: bf1d_ref_driver.cpp $(REF)/SERECKIRCH/src/bf1d.cpp $(REF)/SERECKIRCH/include/bf1d.h
+	@mkdir -p $(OUT)
+	$(CXX) -O3 -DNDEBUG -w -fPIC -shared -I$(REF)/SERECKIRCH/src -I$(REF)/fftw/api -o $@ $<
 
 $(OUT)/obj/%.o: $(REF)/%.cpp
 	@mkdir -p $(dir $@)

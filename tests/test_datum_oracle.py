@@ -30,8 +30,10 @@ TOL = 1e-5
 
 
 def _case(name):
+    """A golden geometry on the direct path (use_bf=0); the butterfly
+    operators are pinned in test_bf_oracle.py."""
     c = INFO["cases"][name]
-    return c["params"], GOLD[name + "/data"], c["data_axes"]
+    return dict(c["params"], use_bf=0), GOLD[name + "/data"], c["data_axes"]
 
 
 def test_greensol_is_bitwise_the_reference():
@@ -198,7 +200,7 @@ def test_tie_geometry_follows_the_frequency_program():
     rng = np.random.default_rng(4)
     data = rng.normal(size=(8, 14, 64)).astype(np.float32)
     ax = [[64, 0.0, 0.004], [14, 0.0, 0.05], [8, 0.05, 0.1]]
-    out = D.datum({"cmp": 1, "aperture": 6}, data, ax, g, POS, g, POS, SDATUM)
+    out = D.datum({"cmp": 1, "aperture": 6, "use_bf": 0}, data, ax, g, POS, g, POS, SDATUM)
     assert np.isfinite(out).all() and np.abs(out).max() > 0
 
 
@@ -206,8 +208,8 @@ def test_table_range_errors_carry_the_reference_messages():
     g = GOLD["green"]
     data = np.zeros((4, 6, 64), np.float32)
     with pytest.raises(ValueError, match="Receiver table too small."):
-        D.datum({"cmp": 0}, data, [[64, 0.0, 0.004], [6, 1.5, 0.05], [4, 0.1, 0.05]],
+        D.datum({"cmp": 0, "use_bf": 0}, data, [[64, 0.0, 0.004], [6, 1.5, 0.05], [4, 0.1, 0.05]],
                 g, POS, g, POS, SDATUM)
     with pytest.raises(ValueError, match="Source table too small."):
-        D.datum({"cmp": 0}, data, [[64, 0.0, 0.004], [6, 0.1, 0.05], [4, 1.5, 0.05]],
+        D.datum({"cmp": 0, "use_bf": 0}, data, [[64, 0.0, 0.004], [6, 0.1, 0.05], [4, 1.5, 0.05]],
                 g, POS, g, POS, SDATUM)

@@ -41,8 +41,9 @@ def _dat():
 
 
 def _case(name):
+    """A golden geometry on the direct path (use_bf=0)."""
     c = INFO["cases"][name]
-    return dict(c["params"]), GOLD[name + "/data"], c["data_axes"]
+    return dict(c["params"], use_bf=0), GOLD[name + "/data"], c["data_axes"]
 
 
 @pytest.mark.parametrize("name", NAMES)
@@ -107,6 +108,7 @@ def test_edge_geometries_against_the_oracle(name):
     arithmetic, as the float oracle does: same traces, so the two agree to
     rounding)."""
     ns, s0, ds, nh, h0, dh, nt, kv = EDGE[name]
+    kv = dict(kv, use_bf=0)
     rng = np.random.default_rng(zlib.crc32(name.encode()))
     data = rng.normal(size=(ns, nh, nt)).astype(np.float32)
     ax = [[nt, 0.0, 0.004], [nh, h0, dh], [ns, s0, ds]]
