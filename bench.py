@@ -545,7 +545,7 @@ def bench_datum(args, cores):
         dm.run()
         dm.sync()
         log(f"warmup {i + 1}/{args.warmup}: {dm.stats()['ms_total']:.1f} ms")
-    rec, fwd, inv, tab, rcv, src = [], [], [], [], [], []
+    rec, fwd, inv, tab, rcv, src, bfr, bfs = [], [], [], [], [], [], [], []
     tstart = time.perf_counter()
     for _ in range(args.steps):
         dm.run()
@@ -557,22 +557,28 @@ def bench_datum(args, cores):
         tab.append(st["ms_tables"])
         rcv.append(st["ms_receiver"])
         src.append(st["ms_source"])
+        bfr.append(st["ms_bf_receiver"])
+        bfs.append(st["ms_bf_source"])
     wall = time.perf_counter() - tstart
     out = dm.output()
     dm.close()
     ms_step = wall / args.steps * 1e3
     value = st["cmacs"] * args.steps / wall
     log(f"timed {args.steps} steps: {ms_step:.1f} ms/step, nfft {st['nfft']}, "
-        f"{st['cmacs']:.3e} complex MACs per step")
+        f"{st['cmacs']:.3e} complex MACs per step; butterfly "
+        f"{st['bf_panels_receiver']}+{st['bf_panels_source']} panels, "
+        f"{np.mean(bfr):.1f}+{np.mean(bfs):.1f} ms")
     rcv_ms = float(np.mean(rcv))
-    achieved = 8.0 * st["pairs_receiver"] * st["nw"] / (rcv_ms * 1e-3) / 1e12
+    band_ms = rcv_ms - float(np.mean(bfr))  # the banded kernel alone
+    achieved = 8.0 * st["pairs_receiver"] * st["nw"] / (band_ms * 1e-3) / 1e12
 
     cpu = parity = None
     if not args.no_cpu:
         from oracle import datum as D
         from oracle import oracle as O
         sub = w.corner(96)
-        log("CPU oracle (float, direct path) on a 96 x 96 corner of the cube")
+        bfk = "butterfly" if sub.kv.get("use_bf", 1) else "direct path"
+        log(f"CPU oracle (float, {bfk}) on a 96 x 96 corner of the cube")
         tc = time.perf_counter()
         ref32 = D.datum(sub.kv, sub.data, sub.data_axes, sub.green, sub.pos_axis, sub.green,
                         sub.pos_axis, sub.datum)
@@ -585,6 +591,8 @@ def bench_datum(args, cores):
             ds.sync()
             gpu_sub = ds.output()
             sst = ds.stats()
+        gpu_exact = dat.datum(dict(sub.kv, use_bf=0), sub.data, sub.data_axes, sub.green,
+                              sub.pos_axis, sub.green, sub.pos_axis, sub.datum)
         log("FP64 oracle (direct path) on the same corner")
         ref64 = D.datum(dict(sub.kv, use_bf=0), sub.data, sub.data_axes, sub.green,
                         sub.pos_axis, sub.green, sub.pos_axis, sub.datum, precision="f64")
@@ -592,29 +600,40 @@ def bench_datum(args, cores):
                "kind": "port",
                "sample": f"96 shots x 96 receivers of the same cube, tables and keys "
                          f"({sst['cmacs']:.3e} complex MACs, nfft {sst['nfft']}), {tcpu:.1f} s, "
-                         f"the reference's direct path with OpenMP over frequencies"}
-        parity = {"sample": "96 x 96 corner", "rel_l2_gpu_vs_f64": O.rel_l2(gpu_sub, ref64),
-                  "rel_l2_cpu_f32_vs_f64": O.rel_l2(ref32, ref64)}
-        log(f"CPU {cpu['value']:.3e} complex MAC/s; relL2 GPU vs f64 "
-            f"{parity['rel_l2_gpu_vs_f64']:.2e}, CPU f32 vs f64 {parity['rel_l2_cpu_f32_vs_f64']:.2e}")
+                         f"the reference's {bfk} (use_bf={int(sub.kv.get('use_bf', 1))}) "
+                         f"with OpenMP over frequencies"}
+        parity = {"sample": "96 x 96 corner",
+                  "rel_l2_gpu_vs_oracle": O.rel_l2(gpu_sub, ref32),
+                  "rel_l2_gpu_exact_vs_f64_direct": O.rel_l2(gpu_exact, ref64),
+                  "butterfly_effect": O.rel_l2(gpu_sub, gpu_exact),
+                  "butterfly_panels": [sst["bf_panels_receiver"], sst["bf_panels_source"]]}
+        log(f"CPU {cpu['value']:.3e} complex MAC/s; relL2 GPU vs oracle "
+            f"{parity['rel_l2_gpu_vs_oracle']:.2e}, exact GPU vs f64 direct "
+            f"{parity['rel_l2_gpu_exact_vs_f64_direct']:.2e}, butterfly effect "
+            f"{parity['butterfly_effect']:.2e}")
     (nt, _, _), (nh, _, _), (ns, _, _) = w.data_axes
     line = {
-        "metric": "datuming operator complex MACs/s (kirchdat2d_freq, direct sums)",
+        "metric": "datuming operator complex MACs/s (kirchdat2d_freq)",
         "value": value, "unit": "complex MAC/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 (complex)", "data": "synthetic",
         "config": {"workload": w.name, "config": "datum", "ns": int(ns), "nh": int(nh),
                    "nt": int(nt), "nfft": st["nfft"], "keys": w.kv,
                    "pairs_receiver": st["pairs_receiver"], "pairs_source": st["pairs_source"],
-                   "kernel": "k_band<8,4>" if st["banded"] else "k_pairs"},
+                   "kernel": "k_band<8,4>" if st["banded"] else "k_pairs",
+                   "butterfly": {"on": st["butterfly"],
+                                 "panels": [st["bf_panels_receiver"], st["bf_panels_source"]],
+                                 "items": [st["bf_items_receiver"], st["bf_items_source"]]}},
         "roofline": {"bound": "valu-fp32", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                      "kernel": "k_band<8,4> (receiver side)", "flops_per_cmac": 8,
-                     "avg_launch_ms": rcv_ms},
+                     "avg_launch_ms": band_ms},
         "cpu_baseline": cpu,
         "parity": parity,
         "breakdown_ms": {"forward": float(np.mean(fwd)), "tables": float(np.mean(tab)),
                          "receiver": rcv_ms, "source": float(np.mean(src)),
+                         "butterfly_receiver": float(np.mean(bfr)),
+                         "butterfly_source": float(np.mean(bfs)),
                          "inverse": float(np.mean(inv)), "events_total": float(np.mean(rec))},
         "upload_s": upload_s,
         "out_abs_max": float(np.abs(out).max()),

@@ -9,10 +9,13 @@
  * greensol_auto (src/greensol_auto.cpp:23-101), which cuts the datum table
  * out of a traveltime cube.
  *
- * The operators are applied exactly (the reference's direct sums) for
- * use_bf=0 and use_bf=1 alike: the butterfly is the reference's low-rank
- * approximation of the same sums on the CPU; its bf_* keys are checked as
- * the reference checks them and otherwise unused.
+ * use_bf=0 applies the operators exactly (the reference's direct sums).
+ * use_bf=1 (the default) reproduces the reference's segmented butterfly
+ * (SERECKIRCH/src/bf1d.cpp, :814-1121) for cmp=0: per frequency the panels
+ * the reference would factor get the factor's own values (the same
+ * Chebyshev nodes, Lagrange values, bilinear samples and switch kernel),
+ * every other entry its exact value.  For cmp=1 the butterfly is not
+ * reproduced and the sums stay exact (stats.butterfly = 0).
  *
  * Layouts are the reference's RSF ones: data and outputs (ns, nh, nt) with
  * time fastest, datum tables (n, n) with the output position fastest
@@ -44,7 +47,7 @@ typedef struct {
     float antialias;       /* operator antialias factor, 0 disables it (1) */
     float length;          /* datuming filter length, s (0.025) */
     float sdatum, rdatum;  /* datum depths of the source and receiver tables */
-    int use_bf;            /* (1) accepted; the sums are exact either way */
+    int use_bf;            /* (1) the butterfly panels (cmp=0), 0 exact sums */
     int bf_p, bf_n_leaf, bf_panel_levels;  /* (12, 16, 1) */
     float bf_amp_eps, bf_phase_tol;        /* (1e-20, 2) */
     int interm;            /* keep the receiver-side result (the interm= file) */
@@ -61,6 +64,10 @@ typedef struct {
     float ms_inverse;       /* c2r + reversal (and the interm cube) */
     float ms_total;
     int banded;             /* 1: cmp=0 banded kernel, 0: pair lists */
+    int butterfly;          /* 1: use_bf=1 ran the butterfly panels */
+    long long bf_panels_receiver, bf_panels_source;  /* panels that can be butterflies */
+    long long bf_items_receiver, bf_items_source;    /* (panel, 64-frequency chunk) items */
+    float ms_bf_receiver, ms_bf_source;  /* the butterfly steps, inside ms_receiver / ms_source */
 } kmdat_stats;
 
 typedef struct kmdat_ctx kmdat_ctx;

@@ -49,6 +49,18 @@ struct OpPlan {
     std::vector<int> rowptr;
     std::vector<PairRec> pairs;
     long long npairs = 0;
+    // butterfly panels (use_bf=1, banded form)
+    bool bf = false;
+    BfPlan bfp;
+};
+
+// Device copies of one operator's butterfly plan.
+struct BfDev {
+    void *nodes = nullptr, *items = nullptr, *tau = nullptr, *lag = nullptr, *sws = nullptr,
+         *omega = nullptr, *grp_ptr = nullptr, *grp_row0 = nullptr, *grp_rows = nullptr,
+         *grp_chunk = nullptr, *grp = nullptr, *quads = nullptr;
+    void **all() { return &nodes; }
+    static constexpr int N = 12;
 };
 
 struct kmdat_ctx {
@@ -79,7 +91,12 @@ struct kmdat_ctx {
     rocfft_execution_info info = nullptr;
     void *work = nullptr;
     size_t work_cap = 0;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[12] = {};
+    // butterfly plans on the device and the work buffers both sides share
+    BfDev bfd[2];
+    unsigned long long *d_bf_mask = nullptr;
+    float2 *d_bf_sw = nullptr, *d_bf_d = nullptr;
+    size_t cap_bf_mask = 0, cap_bf_sw = 0, cap_bf_d = 0;
     kmdat_stats st{};
     bool mfma = false;  // KMDAT_MFMA=1: the matrix-core banded kernel
     int super_o = 16, super_b = 8;  // KMDAT_SUPER=o,b: tile grouping of the VALU kernel
@@ -482,6 +499,50 @@ static int upload_op(kmdat_ctx *c, int k)
     return KMDAT_OK;
 }
 
+template <typename T>
+static int upload_vec(kmdat_ctx *c, void **dst, const std::vector<T> &v)
+{
+    if (*dst) (void)hipFree(*dst);
+    *dst = nullptr;
+    const size_t n = std::max<size_t>(v.size(), 1);
+    HIPCHK(c, hipMalloc(dst, n * sizeof(T)));
+    if (!v.empty())
+        HIPCHK(c, hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return KMDAT_OK;
+}
+
+static void free_bf(BfDev &d)
+{
+    void **p = d.all();
+    for (int i = 0; i < BfDev::N; i++) {
+        if (p[i]) (void)hipFree(p[i]);
+        p[i] = nullptr;
+    }
+}
+
+static int upload_bf(kmdat_ctx *c, int k)
+{
+    const BfPlan &b = c->op[k].bfp;
+    BfDev &d = c->bfd[k];
+    int rc;
+    if ((rc = upload_vec(c, &d.nodes, b.nodes)) || (rc = upload_vec(c, &d.items, b.items)) ||
+        (rc = upload_vec(c, &d.tau, b.tau)) || (rc = upload_vec(c, &d.lag, b.lag)) ||
+        (rc = upload_vec(c, &d.sws, b.sws)) || (rc = upload_vec(c, &d.omega, b.omega)) ||
+        (rc = upload_vec(c, &d.grp_ptr, b.grp_ptr)) ||
+        (rc = upload_vec(c, &d.grp_row0, b.grp_row0)) ||
+        (rc = upload_vec(c, &d.grp_rows, b.grp_rows)) ||
+        (rc = upload_vec(c, &d.grp_chunk, b.grp_chunk)) || (rc = upload_vec(c, &d.grp, b.grp)) ||
+        (rc = upload_vec(c, &d.quads, b.quads)))
+        return rc;
+    if ((rc = ensure(c, &c->d_bf_mask, &c->cap_bf_mask, std::max<size_t>(b.items.size(), 1))))
+        return rc;
+    if ((rc = ensure(c, &c->d_bf_sw, &c->cap_bf_sw, std::max<size_t>((size_t)b.sw_elems, 1))))
+        return rc;
+    if ((rc = ensure(c, &c->d_bf_d, &c->cap_bf_d, std::max<size_t>((size_t)b.d_elems, 1))))
+        return rc;
+    return KMDAT_OK;
+}
+
 // choose_nfft (:87-98) over both tables
 static int choose_nfft(const kmdat_ctx *c)
 {
@@ -512,6 +573,36 @@ static int plan(kmdat_ctx *c)
     if ((rc = plan_source(c))) return rc;
     for (int k = 0; k < 2; k++)
         if ((rc = upload_op(c, k))) return rc;
+    // use_bf=1: the butterfly panels of the banded (cmp=0) operators; the
+    // pair-list (cmp=1) operators stay exact
+    for (int k = 0; k < 2; k++) {
+        OpPlan &op = c->op[k];
+        op.bf = p.use_bf != 0 && op.banded;
+        if (!op.bf) continue;
+        BfPlanIn in{};
+        in.n = op.n_o;
+        in.nb = op.nb;
+        in.aper = op.aper;
+        in.pkey = op.pkey.data();
+        in.pw = op.pw.data();
+        in.tau_key = op.tau.data();
+        in.aarg = op.aarg.data();
+        in.nkey = (int)op.tau.size();
+        in.nw = c->nw;
+        in.nfft = c->nfft;
+        in.dt = c->dt;
+        in.antialias = p.antialias;
+        in.p = p.bf_p;
+        in.leaf = p.bf_n_leaf;
+        in.levels = p.bf_panel_levels;
+        // bf1d.cpp:949: a non-positive or non-finite tolerance reads as 2
+        in.phase_tol = (p.bf_phase_tol <= 0.0f || !std::isfinite(p.bf_phase_tol))
+                           ? 2.0f
+                           : p.bf_phase_tol;
+        std::string e;
+        if ((rc = bf_plan(in, op.bfp, e))) return fail(c, rc, "%s", e.c_str());
+        if ((rc = upload_bf(c, k))) return rc;
+    }
     // per-trace data-edge taper weight ws * wh (:290-295)
     const size_t ntr = (size_t)c->ns * c->nh;
     std::vector<float> wtr(ntr);
@@ -545,6 +636,11 @@ static int plan(kmdat_ctx *c)
     c->st.pairs_source = c->op[1].npairs;
     c->st.cmacs = (double)(c->op[0].npairs + c->op[1].npairs) * (double)c->nw;
     c->st.banded = c->op[0].banded ? 1 : 0;
+    c->st.butterfly = (c->op[0].bf || c->op[1].bf) ? 1 : 0;
+    c->st.bf_panels_receiver = (long long)c->op[0].bfp.nodes.size();
+    c->st.bf_panels_source = (long long)c->op[1].bfp.nodes.size();
+    c->st.bf_items_receiver = (long long)c->op[0].bfp.items.size();
+    c->st.bf_items_source = (long long)c->op[1].bfp.items.size();
     c->planned = true;
     return KMDAT_OK;
 }
@@ -625,10 +721,14 @@ void kmdat_destroy(kmdat_ctx *c)
                     (void *)c->d_pad, (void *)c->d_u1, (void *)c->d_u2, (void *)c->d_ktab,
                     c->work})
         if (p) (void)hipFree(p);
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < 2; k++) {
         for (void *p : {(void *)c->d_tau[k], (void *)c->d_aarg[k], (void *)c->d_pkey[k],
                         (void *)c->d_pw[k], (void *)c->d_rowptr[k], (void *)c->d_pairs[k]})
             if (p) (void)hipFree(p);
+        free_bf(c->bfd[k]);
+    }
+    for (void *p : {(void *)c->d_bf_mask, (void *)c->d_bf_sw, (void *)c->d_bf_d})
+        if (p) (void)hipFree(p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -737,6 +837,58 @@ static int apply_op(kmdat_ctx *c, int k, const float2 *in, float2 *out)
         a.super_b = c->super_b;
         HIPCHK(c, (hipError_t)(c->mfma ? launch_band_mfma(a, c->stream)
                                         : launch_band(a, c->stream)));
+        if (op.bf) {
+            const BfPlan &bp = op.bfp;
+            const BfDev &d = c->bfd[k];
+            bf::Args b{};
+            b.pkey = c->d_pkey[k];
+            b.pw = c->d_pw[k];
+            b.tau_key = c->d_tau[k];
+            b.aarg = c->d_aarg[k];
+            b.ktab = c->d_ktab;
+            b.omega = (const float *)d.omega;
+            b.nb = op.nb;
+            b.aper = op.aper;
+            b.nw = nw;
+            b.nfft = c->nfft;
+            b.dt = c->dt;
+            b.antialias = c->cfg.antialias;
+            const float eps = c->cfg.bf_amp_eps < 0.0f ? 0.0f : c->cfg.bf_amp_eps;
+            b.eps2 = eps * eps;
+            b.nodes = (const bf::Node *)d.nodes;
+            b.items = (const bf::Item *)d.items;
+            b.nitems = (int)bp.items.size();
+            b.tau = (const double *)d.tau;
+            b.lag = (const float *)d.lag;
+            b.sws = (const bf::SwSample *)d.sws;
+            for (int L = 0; L <= bf::MAX_L; L++) {
+                b.lag_off[L] = bp.lag_off[L];
+                b.sws_off[L] = bp.sws_off[L];
+            }
+            b.p = c->cfg.bf_p;
+            b.leaf = c->cfg.bf_n_leaf;
+            b.maxL = bp.maxL;
+            b.mask = c->d_bf_mask;
+            b.sw = c->d_bf_sw;
+            b.d = c->d_bf_d;
+            b.grp_ptr = (const int *)d.grp_ptr;
+            b.grp_row0 = (const int *)d.grp_row0;
+            b.grp_rows = (const int *)d.grp_rows;
+            b.grp_chunk = (const int *)d.grp_chunk;
+            b.grp = (const int *)d.grp;
+            b.quads = (const bf::Quad *)d.quads;
+            b.nquads = (int)bp.quads.size();
+            b.in = in;
+            b.out = out;
+            b.n_b = op.n_b;
+            b.in_b = a.in_b;
+            b.in_i = a.in_i;
+            b.out_b = a.out_b;
+            b.out_o = a.out_o;
+            HIPCHK(c, hipEventRecord(c->ev[7 + 2 * k], c->stream));
+            HIPCHK(c, (hipError_t)launch_bf(b, c->stream));
+            HIPCHK(c, hipEventRecord(c->ev[8 + 2 * k], c->stream));
+        }
     } else {
         PairArgs a{};
         a.in = in;
@@ -800,6 +952,11 @@ int kmdat_sync(kmdat_ctx *c)
         c->st.ms_source = t[4];
         c->st.ms_inverse = t[5];
         HIPCHK(c, hipEventElapsedTime(&c->st.ms_total, c->ev[0], c->ev[6]));
+        c->st.ms_bf_receiver = c->st.ms_bf_source = 0.0f;
+        if (c->op[0].bf)
+            HIPCHK(c, hipEventElapsedTime(&c->st.ms_bf_receiver, c->ev[7], c->ev[8]));
+        if (c->op[1].bf)
+            HIPCHK(c, hipEventElapsedTime(&c->st.ms_bf_source, c->ev[9], c->ev[10]));
     }
     return KMDAT_OK;
 }

@@ -126,29 +126,7 @@ __global__ __launch_bounds__(256) void k_ktab(float2 *__restrict__ ktab,
         const float lin_i = (1.0f - delta) * sin_m + delta * sin_m1;
         K.x = lin_r * sum_r - lin_i * sum_i;
         K.y = lin_r * sum_i + lin_i * sum_r;
-        // antialias_weight_iw
-        float aa = 1.0f;
-        if (antialias > 0.0f && iw > 0) {
-            const float dtau = fabsf(aarg[key]) * antialias;
-            if (dtau > 1e-12f && isfinite(dtau)) {
-                const float freq = (float)iw / ((float)nfft * dt);
-                const float fnyq = 0.5f / dt;
-                const float fmax = 0.5f / dtau;
-                if (fmax < fnyq) {
-                    if (fmax <= 0.0f) {
-                        aa = 0.0f;
-                    } else {
-                        const float fpass = 0.8f * fmax;
-                        if (freq >= fmax) {
-                            aa = 0.0f;
-                        } else if (freq > fpass) {
-                            const float x = (freq - fpass) / (fmax - fpass);
-                            aa = 0.5f * (1.0f + cosf(KPI_F * x));
-                        }
-                    }
-                }
-            }
-        }
+        const float aa = antialias_weight(antialias, aarg[key], iw, nfft, dt);
         K.x *= aa;
         K.y *= aa;
     }

@@ -46,7 +46,11 @@ class DatStats(C.Structure):
                 ("pairs_receiver", C.c_longlong), ("pairs_source", C.c_longlong),
                 ("cmacs", C.c_double), ("ms_forward", C.c_float), ("ms_tables", C.c_float),
                 ("ms_receiver", C.c_float), ("ms_source", C.c_float),
-                ("ms_inverse", C.c_float), ("ms_total", C.c_float), ("banded", C.c_int)]
+                ("ms_inverse", C.c_float), ("ms_total", C.c_float), ("banded", C.c_int),
+                ("butterfly", C.c_int), ("bf_panels_receiver", C.c_longlong),
+                ("bf_panels_source", C.c_longlong), ("bf_items_receiver", C.c_longlong),
+                ("bf_items_source", C.c_longlong), ("ms_bf_receiver", C.c_float),
+                ("ms_bf_source", C.c_float)]
 
 
 _LIB = None
@@ -183,10 +187,10 @@ class Datum:
 
 
 def datum(kv, data, data_axes, sgreen, sg_axis, rgreen, rg_axis, sdatum, rdatum=None,
-          interm=False):
+          interm=False, stats=False):
     """One datuming run on the GPU: returns the (ns, nh, nt) output (and the
-    receiver-side cube when interm); sg_axis / rg_axis = (o, d) of the table
-    positions."""
+    receiver-side cube when interm, the run's statistics when stats);
+    sg_axis / rg_axis = (o, d) of the table positions."""
     cfg = config(kv, sdatum, rdatum, interm)
     check(cfg)
     with Datum(cfg) as dm:
@@ -195,8 +199,12 @@ def datum(kv, data, data_axes, sgreen, sg_axis, rgreen, rg_axis, sdatum, rdatum=
         dm.set_data(data, data_axes)
         dm.run()
         dm.sync()
-        out = dm.output()
-        return (out, dm.interm()) if interm else out
+        res = [dm.output()]
+        if interm:
+            res.append(dm.interm())
+        if stats:
+            res.append(dm.stats())
+        return res[0] if len(res) == 1 else tuple(res)
 
 
 def greensol(cube, cube_axes, pos_axis, datum_depth):
