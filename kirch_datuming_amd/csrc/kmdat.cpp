@@ -49,18 +49,23 @@ struct OpPlan {
     std::vector<int> rowptr;
     std::vector<PairRec> pairs;
     long long npairs = 0;
-    // butterfly panels (use_bf=1, banded form)
+    // butterfly panels (use_bf=1)
     bool bf = false;
     BfPlan bfp;
+    // cmp=1 source side: the rows of every common-receiver gather ir, row is:
+    // output trace and pair range ([ir * ns + is], -1 where the gather has no row)
+    std::vector<int> g_o, g_q0, g_q1;
+    int ngather = 0;
 };
 
 // Device copies of one operator's butterfly plan.
 struct BfDev {
     void *nodes = nullptr, *items = nullptr, *tau = nullptr, *lag = nullptr, *sws = nullptr,
          *omega = nullptr, *grp_ptr = nullptr, *grp_row0 = nullptr, *grp_rows = nullptr,
-         *grp_chunk = nullptr, *grp = nullptr, *quads = nullptr;
+         *grp_chunk = nullptr, *grp = nullptr, *quads = nullptr, *node_pair0 = nullptr,
+         *node_out = nullptr;
     void **all() { return &nodes; }
-    static constexpr int N = 12;
+    static constexpr int N = 14;
 };
 
 struct kmdat_ctx {
@@ -94,6 +99,7 @@ struct kmdat_ctx {
     hipEvent_t ev[12] = {};
     // butterfly plans on the device and the work buffers both sides share
     BfDev bfd[2];
+    bool bf_ran[2] = {false, false};  // the butterfly kernels of a side ran (events recorded)
     unsigned long long *d_bf_mask = nullptr;
     float2 *d_bf_sw = nullptr, *d_bf_d = nullptr;
     size_t cap_bf_mask = 0, cap_bf_sw = 0, cap_bf_d = 0;
@@ -348,6 +354,10 @@ static int plan_source(kmdat_ctx *c)
     const int nr = (int)((s + h) / dr + 1.5f);
     const float astep = fabsf((float)jump * ds);
     std::vector<std::vector<PairRec>> rows((size_t)ns * nh);
+    op.ngather = nr;
+    op.g_o.assign((size_t)nr * ns, -1);
+    op.g_q0.assign((size_t)nr * ns, 0);
+    op.g_q1.assign((size_t)nr * ns, 0);
     for (int ir = 0; ir < nr; ir++) {
         const float r = ir * dr + ((ds <= 0.f) ? -1.f : 0.f) * s + ((dh <= 0.f) ? -1.f : 0.f) * h;
         int sleft = (int)((ir * dr + ((ds <= 0.f) ? -1.f : 0.f) * s +
@@ -366,6 +376,12 @@ static int plan_source(kmdat_ctx *c)
             const int ih = (int)((r - is * ds) / dh + 0.5f);
             left = (is - jump * aper < sleft) ? sleft : is - jump * aper;
             right = (is + jump * aper > sright) ? sright : is + jump * aper;
+            const size_t gi = (size_t)ir * ns + is;
+            const bool row_ok = ih >= 0 && ih < nh;
+            if (row_ok) {
+                op.g_o[gi] = is * nh + ih;
+                op.g_q0[gi] = (int)rows[(size_t)is * nh + ih].size();  // local, made absolute below
+            }
             for (int ic = left; ic <= right; ic += jump) {
                 const int cc = spos(ic);
                 if (cc < 0 || cc >= S.n) return fail(c, KMDAT_E_DIM, "Source table too small.");
@@ -387,6 +403,7 @@ static int plan_source(kmdat_ctx *c)
                 q.w = w;
                 rows[(size_t)is * nh + ih].push_back(q);
             }
+            if (row_ok) op.g_q1[gi] = (int)rows[(size_t)is * nh + ih].size();
         }
     }
     op.rowptr.assign((size_t)ns * nh + 1, 0);
@@ -400,6 +417,11 @@ static int plan_source(kmdat_ctx *c)
     op.pairs.reserve(tot);
     for (auto &rw : rows) op.pairs.insert(op.pairs.end(), rw.begin(), rw.end());
     op.npairs = (long long)tot;
+    for (size_t gi = 0; gi < op.g_o.size(); gi++)
+        if (op.g_o[gi] >= 0) {
+            op.g_q0[gi] += op.rowptr[(size_t)op.g_o[gi]];
+            op.g_q1[gi] += op.rowptr[(size_t)op.g_o[gi]];
+        }
     return KMDAT_OK;
 }
 
@@ -532,7 +554,9 @@ static int upload_bf(kmdat_ctx *c, int k)
         (rc = upload_vec(c, &d.grp_row0, b.grp_row0)) ||
         (rc = upload_vec(c, &d.grp_rows, b.grp_rows)) ||
         (rc = upload_vec(c, &d.grp_chunk, b.grp_chunk)) || (rc = upload_vec(c, &d.grp, b.grp)) ||
-        (rc = upload_vec(c, &d.quads, b.quads)))
+        (rc = upload_vec(c, &d.quads, b.quads)) ||
+        (rc = upload_vec(c, &d.node_pair0, b.node_pair0)) ||
+        (rc = upload_vec(c, &d.node_out, b.node_out)))
         return rc;
     if ((rc = ensure(c, &c->d_bf_mask, &c->cap_bf_mask, std::max<size_t>(b.items.size(), 1))))
         return rc;
@@ -573,18 +597,16 @@ static int plan(kmdat_ctx *c)
     if ((rc = plan_source(c))) return rc;
     for (int k = 0; k < 2; k++)
         if ((rc = upload_op(c, k))) return rc;
-    // use_bf=1: the butterfly panels of the banded (cmp=0) operators; the
-    // pair-list (cmp=1) operators stay exact
+    // use_bf=1: the butterfly panels of both operators: one matrix per side
+    // (cmp=0, banded), one per shot (cmp=1 receiver side) or per receiver
+    // gather (cmp=1 source side)
+    std::vector<int> ro, rq0, rq1;
     for (int k = 0; k < 2; k++) {
         OpPlan &op = c->op[k];
-        op.bf = p.use_bf != 0 && op.banded;
+        op.bf = p.use_bf != 0;
+        op.bfp = BfPlan{};
         if (!op.bf) continue;
         BfPlanIn in{};
-        in.n = op.n_o;
-        in.nb = op.nb;
-        in.aper = op.aper;
-        in.pkey = op.pkey.data();
-        in.pw = op.pw.data();
         in.tau_key = op.tau.data();
         in.aarg = op.aarg.data();
         in.nkey = (int)op.tau.size();
@@ -599,6 +621,42 @@ static int plan(kmdat_ctx *c)
         in.phase_tol = (p.bf_phase_tol <= 0.0f || !std::isfinite(p.bf_phase_tol))
                            ? 2.0f
                            : p.bf_phase_tol;
+        if (op.banded) {
+            in.n = op.n_o;
+            in.nb = op.nb;
+            in.aper = op.aper;
+            in.pkey = op.pkey.data();
+            in.pw = op.pw.data();
+        } else if (k == 0) {
+            // shot is: row ih = output trace is nh + ih, column ic = tr % nh
+            const int nh = c->nh, ns = c->ns;
+            ro.assign((size_t)ns * nh, 0);
+            rq0.assign(ro.size(), 0);
+            rq1.assign(ro.size(), 0);
+            for (size_t o = 0; o < ro.size(); o++) {
+                ro[o] = (int)o;
+                rq0[o] = op.rowptr[o];
+                rq1[o] = op.rowptr[o + 1];
+            }
+            in.n = nh;
+            in.ninst = ns;
+            in.pairs = op.pairs.data();
+            in.row_o = ro.data();
+            in.row_q0 = rq0.data();
+            in.row_q1 = rq1.data();
+            in.col_base = nh;
+            in.col_by_div = 0;
+        } else {
+            // receiver gather ir: row is, column ic = tr / nh
+            in.n = c->ns;
+            in.ninst = op.ngather;
+            in.pairs = op.pairs.data();
+            in.row_o = op.g_o.data();
+            in.row_q0 = op.g_q0.data();
+            in.row_q1 = op.g_q1.data();
+            in.col_base = c->nh;
+            in.col_by_div = 1;
+        }
         std::string e;
         if ((rc = bf_plan(in, op.bfp, e))) return fail(c, rc, "%s", e.c_str());
         if ((rc = upload_bf(c, k))) return rc;
@@ -812,6 +870,59 @@ static int transforms(kmdat_ctx *c, bool forward, float2 *U, float *dst, int rev
     return KMDAT_OK;
 }
 
+// The butterfly kernels' arguments for operator k (the batch strides of the
+// banded form are the caller's).
+static bf::Args bf_args(kmdat_ctx *c, int k, const float2 *in, float2 *out)
+{
+    const OpPlan &op = c->op[k];
+    const BfPlan &bp = op.bfp;
+    const BfDev &d = c->bfd[k];
+    bf::Args b{};
+    b.pkey = c->d_pkey[k];
+    b.pw = c->d_pw[k];
+    b.tau_key = c->d_tau[k];
+    b.aarg = c->d_aarg[k];
+    b.ktab = c->d_ktab;
+    b.omega = (const float *)d.omega;
+    b.nb = op.nb;
+    b.aper = op.aper;
+    b.nw = c->nw;
+    b.nfft = c->nfft;
+    b.dt = c->dt;
+    b.antialias = c->cfg.antialias;
+    const float eps = c->cfg.bf_amp_eps < 0.0f ? 0.0f : c->cfg.bf_amp_eps;
+    b.eps2 = eps * eps;
+    b.nodes = (const bf::Node *)d.nodes;
+    b.items = (const bf::Item *)d.items;
+    b.nitems = (int)bp.items.size();
+    b.tau = (const double *)d.tau;
+    b.lag = (const float *)d.lag;
+    b.sws = (const bf::SwSample *)d.sws;
+    for (int L = 0; L <= bf::MAX_L; L++) {
+        b.lag_off[L] = bp.lag_off[L];
+        b.sws_off[L] = bp.sws_off[L];
+    }
+    b.p = c->cfg.bf_p;
+    b.leaf = c->cfg.bf_n_leaf;
+    b.maxL = bp.maxL;
+    b.mask = c->d_bf_mask;
+    b.sw = c->d_bf_sw;
+    b.d = c->d_bf_d;
+    b.grp_ptr = (const int *)d.grp_ptr;
+    b.grp_row0 = (const int *)d.grp_row0;
+    b.grp_rows = (const int *)d.grp_rows;
+    b.grp_chunk = (const int *)d.grp_chunk;
+    b.grp = (const int *)d.grp;
+    b.quads = (const bf::Quad *)d.quads;
+    b.nquads = (int)bp.quads.size();
+    b.ngroup = (int)bp.grp_row0.size();
+    b.node_pair0 = (const int *)d.node_pair0;
+    b.node_out = (const int *)d.node_out;
+    b.in = in;
+    b.out = out;
+    return b;
+}
+
 static int apply_op(kmdat_ctx *c, int k, const float2 *in, float2 *out)
 {
     OpPlan &op = c->op[k];
@@ -837,49 +948,8 @@ static int apply_op(kmdat_ctx *c, int k, const float2 *in, float2 *out)
         a.super_b = c->super_b;
         HIPCHK(c, (hipError_t)(c->mfma ? launch_band_mfma(a, c->stream)
                                         : launch_band(a, c->stream)));
-        if (op.bf) {
-            const BfPlan &bp = op.bfp;
-            const BfDev &d = c->bfd[k];
-            bf::Args b{};
-            b.pkey = c->d_pkey[k];
-            b.pw = c->d_pw[k];
-            b.tau_key = c->d_tau[k];
-            b.aarg = c->d_aarg[k];
-            b.ktab = c->d_ktab;
-            b.omega = (const float *)d.omega;
-            b.nb = op.nb;
-            b.aper = op.aper;
-            b.nw = nw;
-            b.nfft = c->nfft;
-            b.dt = c->dt;
-            b.antialias = c->cfg.antialias;
-            const float eps = c->cfg.bf_amp_eps < 0.0f ? 0.0f : c->cfg.bf_amp_eps;
-            b.eps2 = eps * eps;
-            b.nodes = (const bf::Node *)d.nodes;
-            b.items = (const bf::Item *)d.items;
-            b.nitems = (int)bp.items.size();
-            b.tau = (const double *)d.tau;
-            b.lag = (const float *)d.lag;
-            b.sws = (const bf::SwSample *)d.sws;
-            for (int L = 0; L <= bf::MAX_L; L++) {
-                b.lag_off[L] = bp.lag_off[L];
-                b.sws_off[L] = bp.sws_off[L];
-            }
-            b.p = c->cfg.bf_p;
-            b.leaf = c->cfg.bf_n_leaf;
-            b.maxL = bp.maxL;
-            b.mask = c->d_bf_mask;
-            b.sw = c->d_bf_sw;
-            b.d = c->d_bf_d;
-            b.grp_ptr = (const int *)d.grp_ptr;
-            b.grp_row0 = (const int *)d.grp_row0;
-            b.grp_rows = (const int *)d.grp_rows;
-            b.grp_chunk = (const int *)d.grp_chunk;
-            b.grp = (const int *)d.grp;
-            b.quads = (const bf::Quad *)d.quads;
-            b.nquads = (int)bp.quads.size();
-            b.in = in;
-            b.out = out;
+        if (op.bf && !op.bfp.items.empty()) {
+            bf::Args b = bf_args(c, k, in, out);
             b.n_b = op.n_b;
             b.in_b = a.in_b;
             b.in_i = a.in_i;
@@ -888,6 +958,7 @@ static int apply_op(kmdat_ctx *c, int k, const float2 *in, float2 *out)
             HIPCHK(c, hipEventRecord(c->ev[7 + 2 * k], c->stream));
             HIPCHK(c, (hipError_t)launch_bf(b, c->stream));
             HIPCHK(c, hipEventRecord(c->ev[8 + 2 * k], c->stream));
+            c->bf_ran[k] = true;
         }
     } else {
         PairArgs a{};
@@ -898,6 +969,14 @@ static int apply_op(kmdat_ctx *c, int k, const float2 *in, float2 *out)
         a.pairs = c->d_pairs[k];
         a.nw = nw;
         HIPCHK(c, (hipError_t)launch_pairs(a, c->ns * c->nh, c->stream));
+        if (op.bf && !op.bfp.items.empty()) {
+            bf::Args b = bf_args(c, k, in, out);
+            b.pairs = c->d_pairs[k];
+            HIPCHK(c, hipEventRecord(c->ev[7 + 2 * k], c->stream));
+            HIPCHK(c, (hipError_t)launch_bf_pairs(b, c->stream));
+            HIPCHK(c, hipEventRecord(c->ev[8 + 2 * k], c->stream));
+            c->bf_ran[k] = true;
+        }
     }
     return KMDAT_OK;
 }
@@ -912,6 +991,7 @@ int kmdat_run(kmdat_ctx *c)
     if ((rc = plan(c))) return rc;
     const kmdat_config &p = c->cfg;
     hipEvent_t *e = c->ev;
+    c->bf_ran[0] = c->bf_ran[1] = false;
     HIPCHK(c, hipEventRecord(e[0], c->stream));
     // forward transforms into u1
     if ((rc = transforms(c, true, c->d_u1, nullptr, 0))) return rc;
@@ -953,9 +1033,9 @@ int kmdat_sync(kmdat_ctx *c)
         c->st.ms_inverse = t[5];
         HIPCHK(c, hipEventElapsedTime(&c->st.ms_total, c->ev[0], c->ev[6]));
         c->st.ms_bf_receiver = c->st.ms_bf_source = 0.0f;
-        if (c->op[0].bf)
+        if (c->bf_ran[0])
             HIPCHK(c, hipEventElapsedTime(&c->st.ms_bf_receiver, c->ev[7], c->ev[8]));
-        if (c->op[1].bf)
+        if (c->bf_ran[1])
             HIPCHK(c, hipEventElapsedTime(&c->st.ms_bf_source, c->ev[9], c->ev[10]));
     }
     return KMDAT_OK;

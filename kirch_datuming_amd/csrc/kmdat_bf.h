@@ -23,6 +23,9 @@
 #endif
 
 namespace kmdat {
+
+struct PairRec;
+
 namespace bf {
 
 constexpr int MAX_L = 3;        // nloc up to leaf 2^3
@@ -116,6 +119,8 @@ struct SwSample {
 // One butterfly panel candidate.
 struct Node {
     int row0, col0, L;
+    int inst;        // matrix instance (pair lists: shot or receiver gather; banded: 0)
+    int rows_off;    // pair lists: first of the node's rows in node_pair0 / node_out, else -1
     int tau_off;     // first traveltime sample (TauLayout of L)
     int iw_res;      // largest frequency index whose phase residual passes (-1: none)
     int nanc;        // candidate ancestors (panels containing this one)
@@ -169,6 +174,12 @@ struct Args {
     float2 *out;
     int n_b;
     size_t in_b, in_i, out_b, out_o;
+    // pair-list operators (cmp=1): entry (r, c) of a node is pair
+    // node_pair0[rows_off + r] + c; its column's input trace is that pair's tr
+    // (spectra [trace][nw]) and row r's output trace node_out[rows_off + r]
+    const PairRec *pairs;
+    const int *node_pair0, *node_out;
+    int ngroup;
 };
 
 }  // namespace bf

@@ -1,5 +1,7 @@
-// Host planning of the segmented butterfly (use_bf=1) for the banded
-// (cmp=0) operators of kirchdat2d_freq.
+// Host planning of the segmented butterfly (use_bf=1) for the operators of
+// kirchdat2d_freq: the banded (cmp=0) ones, one matrix per side and
+// frequency, and the pair-list (cmp=1) ones, one matrix per shot (receiver
+// side, :882-935) or per common-receiver gather (source side, :975-1087).
 //
 // The reference factors each frequency's matrix K = amp exp(-i omega tau)
 // (SERECKIRCH/src/bf1d.cpp:935-961): top panels of leaf 2^levels, each split
@@ -30,7 +32,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <functional>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "kmdat_internal.h"
@@ -142,6 +146,66 @@ double sample(const std::vector<float> &t, int n, double x, double z)
 
 bool pow2(int x) { return x > 0 && (x & (x - 1)) == 0; }
 
+// One matrix as the planner reads it: the table keys of a row's column span,
+// or false when a column of the span has no summation pair; pair0 receives
+// the pair index of the span's first column (pair lists) or -1.
+struct RowSource {
+    virtual ~RowSource() = default;
+    virtual bool span(int r, int c0, int n, int *keys, int *pair0) const = 0;
+    virtual int out_trace(int r) const = 0;
+};
+
+struct BandedRows : RowSource {
+    const BfPlanIn &in;
+    explicit BandedRows(const BfPlanIn &i) : in(i) {}
+    bool span(int r, int c0, int n, int *keys, int *pair0) const override
+    {
+        *pair0 = -1;
+        for (int c = 0; c < n; c++) {
+            const int j = c0 + c - r + in.aper;
+            if (j < 0 || j >= in.nb) return false;
+            keys[c] = in.pkey[(size_t)r * in.nb + j];
+            if (keys[c] < 0) return false;
+        }
+        return true;
+    }
+    int out_trace(int) const override { return -1; }
+};
+
+// Rows of one instance of a pair-list operator: row r holds the pairs
+// [q0, q1) of output trace o, in increasing column.
+struct PairRows : RowSource {
+    const BfPlanIn &in;
+    int inst;
+    PairRows(const BfPlanIn &i, int k) : in(i), inst(k) {}
+    int col(int q) const
+    {
+        const int tr = in.pairs[q].tr;
+        return in.col_by_div ? tr / in.col_base : tr % in.col_base;
+    }
+    bool span(int r, int c0, int n, int *keys, int *pair0) const override
+    {
+        const size_t e = (size_t)inst * in.n + r;
+        if (in.row_o[e] < 0) return false;
+        int lo = in.row_q0[e], hi = in.row_q1[e];
+        while (lo < hi) {  // first pair of column >= c0
+            const int m = (lo + hi) / 2;
+            if (col(m) < c0)
+                lo = m + 1;
+            else
+                hi = m;
+        }
+        if (lo + n > in.row_q1[e]) return false;
+        for (int c = 0; c < n; c++) {
+            if (col(lo + c) != c0 + c) return false;
+            keys[c] = in.pairs[lo + c].key;
+        }
+        *pair0 = lo;
+        return true;
+    }
+    int out_trace(int r) const override { return in.row_o[(size_t)inst * in.n + r]; }
+};
+
 struct Planner {
     const BfPlanIn &in;
     BfPlan &out;
@@ -149,15 +213,10 @@ struct Planner {
     std::vector<int> cut;       // first frequency with a zero antialias weight, per key
     std::vector<Tree> trees;    // per L
     std::string err;
+    const RowSource *src = nullptr;
+    int inst = 0;
 
     Planner(const BfPlanIn &i, BfPlan &o) : in(i), out(o) {}
-
-    int key(int r, int c) const
-    {
-        const int j = c - r + in.aper;
-        if (j < 0 || j >= in.nb) return -1;
-        return in.pkey[(size_t)r * in.nb + j];
-    }
 
     // antialias_weight_iw (kirchdat2d_frequency_cs.cpp:203-233) is zero for
     // every iw >= the returned index (nw when it never is)
@@ -189,17 +248,19 @@ struct Planner {
             const int n = rows;
             int L = 0;
             while ((leaf << L) < n) L++;
-            // fully inside the band, every weight nonzero, every traveltime
-            // positive (else H = 0 and the entry is inactive at all frequencies)
+            // every entry has a summation pair (inside the band, weight
+            // nonzero) and a positive traveltime (else H = 0 and the entry is
+            // inactive at all frequencies)
+            std::vector<int> keys((size_t)n * n), pair0(n);
             bool full = true;
             int hi = in.nw - 1;
-            for (int r = 0; r < n && full; r++)
+            for (int r = 0; r < n && full; r++) {
+                if (!src->span(r0 + r, c0, n, &keys[(size_t)r * n], &pair0[r])) {
+                    full = false;
+                    break;
+                }
                 for (int c = 0; c < n; c++) {
-                    const int k = key(r0 + r, c0 + c);
-                    if (k < 0) {
-                        full = false;
-                        break;
-                    }
+                    const int k = keys[(size_t)r * n + c];
                     const float t = in.tau_key[k];
                     if (!(t > 0.0f) || !std::isfinite(t)) {
                         full = false;
@@ -207,9 +268,12 @@ struct Planner {
                     }
                     hi = std::min(hi, cut[(size_t)k]);  // the cut itself kept as a margin
                 }
+            }
             if (full) {
                 // residual (bf1d.cpp:816-877) as the largest |difference|
-                auto T = [&](int r, int c) { return (double)in.tau_key[key(r, c)]; };
+                auto T = [&](int r, int c) {
+                    return (double)in.tau_key[keys[(size_t)(r - r0) * n + (c - c0)]];
+                };
                 const int rs[5] = {r0, r0 + n - 1, r0 + n / 2, r0 + n / 4, r0 + (3 * n) / 4};
                 const int cs[5] = {c0, c0 + n - 1, c0 + n / 2, c0 + n / 4, c0 + (3 * n) / 4};
                 const double t00 = T(r0, c0);
@@ -243,10 +307,19 @@ struct Planner {
                     nd.col0 = c0;
                     nd.L = L;
                     nd.iw_res = hi;
+                    nd.inst = inst;
+                    nd.rows_off = -1;
+                    if (pair0[0] >= 0) {
+                        nd.rows_off = (int)out.node_pair0.size();
+                        for (int r = 0; r < n; r++) {
+                            out.node_pair0.push_back(pair0[r]);
+                            out.node_out.push_back(src->out_trace(r0 + r));
+                        }
+                    }
                     nd.nanc = (int)std::min<size_t>(anc.size(), bf::MAX_L + 1);
                     for (int q = 0; q < nd.nanc; q++) nd.anc[q] = anc[anc.size() - 1 - q];
                     nd.tau_off = (int)out.tau.size();
-                    samples(nd);
+                    samples(nd, keys);
                     anc.push_back((int)out.nodes.size());
                     out.nodes.push_back(nd);
                     pushed = true;
@@ -269,15 +342,13 @@ struct Planner {
     }
 
     // every traveltime sample of the node's factor, in TauLayout order
-    void samples(const Node &nd)
+    void samples(const Node &nd, const std::vector<int> &keys)
     {
         const Tree &T = trees[nd.L];
         const int L = nd.L, p = in.p, leaf = in.leaf, nloc = leaf << L, nb = 1 << L,
                   mid = L / 2;
         std::vector<float> t((size_t)nloc * nloc);
-        for (int r = 0; r < nloc; r++)
-            for (int c = 0; c < nloc; c++)
-                t[(size_t)r * nloc + c] = in.tau_key[key(nd.row0 + r, nd.col0 + c)];
+        for (size_t e = 0; e < t.size(); e++) t[e] = in.tau_key[keys[e]];
         std::vector<double> &s = out.tau;
         const size_t base = s.size();
         const bf::TauLayout lay = bf::tau_layout(L, p, leaf);
@@ -404,9 +475,16 @@ int bf_plan(const BfPlanIn &in, BfPlan &out, std::string &err)
     P.trees.resize(bf::MAX_L + 1);
     for (int L = 1; L <= bf::MAX_L; L++) P.trees[L].build(in.leaf << L, in.leaf, in.p);
     std::vector<int> anc;
-    for (int r0 = 0; r0 < in.n && P.err.empty(); r0 += pn)
-        for (int c0 = 0; c0 < in.n && P.err.empty(); c0 += pn)
-            P.walk(r0, c0, std::min(pn, in.n - r0), std::min(pn, in.n - c0), anc);
+    BandedRows banded(in);
+    const int ninst = in.pairs ? in.ninst : 1;
+    for (int k = 0; k < ninst && P.err.empty(); k++) {
+        PairRows pr(in, k);
+        P.src = in.pairs ? (const RowSource *)&pr : (const RowSource *)&banded;
+        P.inst = k;
+        for (int r0 = 0; r0 < in.n && P.err.empty(); r0 += pn)
+            for (int c0 = 0; c0 < in.n && P.err.empty(); c0 += pn)
+                P.walk(r0, c0, std::min(pn, in.n - r0), std::min(pn, in.n - c0), anc);
+    }
     if (!P.err.empty()) {
         err = P.err;
         return KMDAT_E_PARAM;
@@ -426,46 +504,85 @@ int bf_plan(const BfPlanIn &in, BfPlan &out, std::string &err)
             it.chunk = ch;
             it.d_off = d;
             it.sw_off = sw;
-            d += (long long)nloc * nloc * 64;
-            sw += (long long)((1 << nd.L) * in.p * in.p) * 64;
+            if (!in.pairs) {  // difference blocks and switch kernels: banded form only
+                d += (long long)nloc * nloc * 64;
+                sw += (long long)((1 << nd.L) * in.p * in.p) * 64;
+            }
             out.items.push_back(it);
         }
     }
     out.d_elems = d;
     out.sw_elems = sw;
-    // groups: per chunk, items whose rows overlap share a group, so that no
-    // two waves update the same output rows
+    // groups: per chunk, items that write a common output trace (overlapping
+    // rows of one matrix, or -- pair lists -- rows of different gathers that
+    // land on one trace) share a group, so that no two waves update a trace
+    auto outputs = [&](const bf::Item &it, std::vector<long long> &o) {
+        const Node &nd = out.nodes[it.node];
+        const int nloc = in.leaf << nd.L;
+        o.clear();
+        for (int r = 0; r < nloc; r++)
+            o.push_back(nd.rows_off >= 0 ? (long long)out.node_out[(size_t)nd.rows_off + r]
+                                         : (long long)nd.row0 + r);
+    };
+    std::vector<int> byc(out.items.size());
+    for (size_t i = 0; i < byc.size(); i++) byc[i] = (int)i;
+    std::stable_sort(byc.begin(), byc.end(), [&](int a, int b) {
+        return out.items[a].chunk < out.items[b].chunk;
+    });
+    std::vector<int> parent(out.items.size());
+    for (size_t i = 0; i < parent.size(); i++) parent[i] = (int)i;
+    std::function<int(int)> root = [&](int x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    };
+    std::vector<long long> o;
+    for (size_t b = 0; b < byc.size();) {
+        size_t e = b;
+        while (e < byc.size() && out.items[byc[e]].chunk == out.items[byc[b]].chunk) e++;
+        std::unordered_map<long long, int> owner;
+        for (size_t i = b; i < e; i++) {
+            outputs(out.items[byc[i]], o);
+            for (long long t : o) {
+                auto f = owner.find(t);
+                if (f == owner.end())
+                    owner.emplace(t, byc[i]);
+                else
+                    parent[root(byc[i])] = root(f->second);
+            }
+        }
+        b = e;
+    }
+    // a group's items in (chunk, first row) order; banded groups cover the
+    // contiguous rows [grp_row0, grp_row0 + grp_rows)
     std::vector<int> order(out.items.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
     std::sort(order.begin(), order.end(), [&](int a, int b) {
-        const bf::Item &x = out.items[a], &y = out.items[b];
-        if (x.chunk != y.chunk) return x.chunk < y.chunk;
-        return out.nodes[x.node].row0 < out.nodes[y.node].row0;
+        const int ra = root(a), rb = root(b);
+        if (ra != rb) {
+            const bf::Item &x = out.items[ra], &y = out.items[rb];
+            if (x.chunk != y.chunk) return x.chunk < y.chunk;
+            return ra < rb;
+        }
+        return out.nodes[out.items[a].node].row0 < out.nodes[out.items[b].node].row0 ||
+               (out.nodes[out.items[a].node].row0 == out.nodes[out.items[b].node].row0 && a < b);
     });
     out.grp_ptr.push_back(0);
-    int lo = 0, hi = -1, ch = -1;
-    auto close = [&]() {
-        if (ch < 0) return;
+    for (size_t b = 0; b < order.size();) {
+        size_t e = b;
+        int lo = 1 << 30, hi = -1;
+        while (e < order.size() && root(order[e]) == root(order[b])) {
+            const Node &nd = out.nodes[out.items[order[e]].node];
+            lo = std::min(lo, nd.row0);
+            hi = std::max(hi, nd.row0 + (in.leaf << nd.L));
+            out.grp.push_back(order[e]);
+            e++;
+        }
         out.grp_row0.push_back(lo);
         out.grp_rows.push_back(hi - lo);
-        out.grp_chunk.push_back(ch);
+        out.grp_chunk.push_back(out.items[order[b]].chunk);
         out.grp_ptr.push_back((int)out.grp.size());
-    };
-    for (int i : order) {
-        const bf::Item &it = out.items[i];
-        const Node &nd = out.nodes[it.node];
-        const int r0 = nd.row0, r1 = nd.row0 + (in.leaf << nd.L);
-        if (it.chunk != ch || r0 >= hi) {
-            close();
-            ch = it.chunk;
-            lo = r0;
-            hi = r1;
-        } else {
-            hi = std::max(hi, r1);
-        }
-        out.grp.push_back(i);
+        b = e;
     }
-    close();
     for (size_t g = 0; g + 1 < out.grp_ptr.size(); g++)
         for (int q = 0; q < (out.grp_rows[g] + 3) / 4; q++) out.quads.push_back({(int)g, q});
     return KMDAT_OK;

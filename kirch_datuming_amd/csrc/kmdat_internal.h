@@ -83,12 +83,18 @@ int launch_pairs(const PairArgs &a, int n_out, void *stream);
 
 // ------------------------------------------------ butterfly (use_bf=1) --
 
-// One banded operator as the butterfly planner reads it.
+// One operator as the butterfly planner reads it: banded (pkey / pw, one
+// matrix) or pair lists (pairs, ninst matrices whose row r of instance k is
+// output trace row_o[k n + r] (-1: none) with pairs [row_q0, row_q1) in
+// increasing column; a pair's column is tr / col_base or tr % col_base).
 struct BfPlanIn {
     int n;                   // matrix size (outputs = summation indices)
     int nb, aper;            // band of pkey / pw: [n][nb], column c = r + j - aper
     const int *pkey;
     const float *pw;
+    const PairRec *pairs = nullptr;
+    const int *row_o = nullptr, *row_q0 = nullptr, *row_q1 = nullptr;
+    int ninst = 1, col_base = 1, col_by_div = 0;
     const float *tau_key, *aarg;  // per table key
     int nkey;
     int nw, nfft;
@@ -106,6 +112,7 @@ struct BfPlan {
     std::vector<float> lag;
     std::vector<bf::SwSample> sws;
     std::vector<float> omega;
+    std::vector<int> node_pair0, node_out;   // pair-list operators
     int lag_off[bf::MAX_L + 1] = {}, sws_off[bf::MAX_L + 1] = {};
     long long d_elems = 0, sw_elems = 0;  // float2 elements of the work buffers
     int maxL = 0;
@@ -116,5 +123,7 @@ int bf_plan(const BfPlanIn &in, BfPlan &out, std::string &err);
 // the four butterfly steps of one operator (kmdat_bf.hip), after its banded
 // sums: masks, switch kernels, difference blocks, correction of `out`
 int launch_bf(const bf::Args &a, void *stream);
+// the same for a pair-list operator after k_pairs: one fused pass per group
+int launch_bf_pairs(const bf::Args &a, void *stream);
 
 }  // namespace kmdat

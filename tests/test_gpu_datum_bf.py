@@ -1,13 +1,15 @@
-"""GPU butterfly datuming (use_bf=1, cmp=0) against the oracle's restatement
+"""GPU butterfly datuming (use_bf=1) against the oracle's restatement
 of the reference's segmented butterfly (oracle/kbf_oracle.c, pinned bitwise
 to SERECKIRCH/src/bf1d.cpp compiled from source, tests/test_bf_oracle.py).
 
 The GPU adds (B_P - K_P) u over the panels the reference factors at each
-frequency to its exact banded sums.  Held, per geometry:
+frequency to its exact sums (banded for cmp=0, pair lists for cmp=1, where
+the matrices are per shot on the receiver side and per common-receiver
+gather on the source side).  Held, per geometry:
 * the output and the receiver-side cube against the float oracle with
   use_bf=1: relative L2 <= 2e-5 (the exact path's bound against the FP64
-  oracle; every case's butterfly moves the output by more than that, so a
-  run without the panels fails it);
+  oracle), and the butterfly moves the output by at least ten times the
+  GPU's distance from the oracle;
 * the butterfly's own contribution, GPU(use_bf=1) - GPU(use_bf=0) against
   oracle(use_bf=1) - oracle(use_bf=0): relative L2 <= 2e-2 (the difference
   is 2e-5 .. 1e-3 of the output, so float rounding of the outputs is a few
@@ -18,7 +20,9 @@ x 1001 samples, aperture 200, data_taper 20, antialias 1: bf_p 12, leaf 16,
 one panel level), and on the 64-position golden table leaf 8 with p 5 or 6
 and panel levels 1-3 (panels of 16, 32 and 64 points), a narrow aperture
 (panels at the band's edge stay exact) and a 50 x 60 cube (panels cut by the
-matrix edge).  cmp=1 keeps exact sums (stats butterfly = 0).
+matrix edge); cmp=1 with leaf 8 (panels on both sides), with panel levels 2
+(source-side gathers only) and with shots at half the offset spacing (jump
+2: receiver side only).
 """
 import json
 import os
@@ -51,20 +55,31 @@ CASES = {
                             bf_panel_levels=2)),
 }
 
+# name: (ns, ds, nh, dh, keys) on the 64-position table, shots and offsets from 0
+CMP1 = {
+    "cmp1_leaf8": (20, 1.0, 40, 1.0, dict(cmp=1, aperture=39, taper=0, antialias=1.0,
+                                          bf_n_leaf=8, bf_p=5)),
+    "cmp1_levels2": (36, 1.0, 28, 1.0, dict(cmp=1, aperture=35, taper=2, antialias=1.0,
+                                            bf_n_leaf=8, bf_p=6, bf_panel_levels=2)),
+    "cmp1_jump2": (24, 0.5, 40, 1.0, dict(cmp=1, aperture=20, taper=0, antialias=1.0,
+                                          bf_n_leaf=8, bf_p=5)),
+}
+
 
 def _dat():
     from kirch_datuming_amd import dat
     return dat
 
 
-def _check(name, kv, data, ax, sg, sax, datum):
+def _check(name, kv, data, ax, sg, sax, datum, sides=(True, True)):
     dat = _dat()
     print(f"datum butterfly {name}: GPU use_bf=1 and 0", flush=True)
     out, mid, st = dat.datum(dict(kv, use_bf=1), data, ax, sg, sax, sg, sax, datum, interm=True,
                              stats=True)
     again = dat.datum(dict(kv, use_bf=1), data, ax, sg, sax, sg, sax, datum)
     ex, exm = dat.datum(dict(kv, use_bf=0), data, ax, sg, sax, sg, sax, datum, interm=True)
-    assert st["butterfly"] == 1 and st["bf_panels_receiver"] > 0 and st["bf_panels_source"] > 0
+    assert st["butterfly"] == 1
+    assert (st["bf_panels_receiver"] > 0, st["bf_panels_source"] > 0) == sides
     assert np.array_equal(out.view(np.uint32), again.view(np.uint32))
     print(f"datum butterfly {name}: oracle use_bf=1 and 0", flush=True)
     ref, rmid = D.datum(dict(kv, use_bf=1), data, ax, sg, sax, sg, sax, datum, interm=True)
@@ -78,9 +93,11 @@ def _check(name, kv, data, ax, sg, sax, datum):
           f"{st['ms_bf_receiver']:.2f}+{st['ms_bf_source']:.2f} ms; relL2 out {r_out:.2e} "
           f"interm {r_mid:.2e}; butterfly effect {effect:.2e}, its GPU-vs-oracle error "
           f"out {d_out:.2e} interm {d_mid:.2e}")
-    assert effect > TOL  # the panels matter at this bound
     assert r_out <= TOL and r_mid <= TOL
-    assert d_out <= DTOL and d_mid <= DTOL
+    assert effect > 10.0 * max(r_out, r_mid)  # the panels matter at this distance
+    assert d_out <= DTOL
+    if sides[0]:  # without receiver-side panels the interm cube's difference is rounding only
+        assert d_mid <= DTOL
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -101,12 +118,13 @@ def test_butterfly_salt_workflow_keys():
     _check("salt", w.kv, w.data, w.data_axes, w.green, w.pos_axis, w.datum)
 
 
-def test_cmp1_keeps_exact_sums():
-    kv, data, ax = dict(INFO["cases"]["cmp1_equal"]["params"]), GOLD["cmp1_equal/data"], \
-        INFO["cases"]["cmp1_equal"]["data_axes"]
-    dat = _dat()
-    g = GOLD["green"]
-    a, st = dat.datum(dict(kv, use_bf=1), data, ax, g, POS, g, POS, SDATUM, stats=True)
-    b = dat.datum(dict(kv, use_bf=0), data, ax, g, POS, g, POS, SDATUM)
-    assert st["butterfly"] == 0
-    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+@pytest.mark.parametrize("name", sorted(CMP1))
+def test_cmp1_butterfly_against_the_oracle(name):
+    ns, ds, nh, dh, kv = CMP1[name]
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    nt = 256
+    data = rng.normal(size=(ns, nh, nt)).astype(np.float32)
+    ax = [[nt, 0.0, 0.004], [nh, 0.0, dh * M["d"]], [ns, 0.0, ds * M["d"]]]
+    sides = {"cmp1_leaf8": (True, True), "cmp1_levels2": (False, True),
+             "cmp1_jump2": (True, False)}[name]
+    _check(name, kv, data, ax, GOLD["green"], POS, SDATUM, sides)
