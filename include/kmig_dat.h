@@ -11,11 +11,11 @@
  *
  * use_bf=0 applies the operators exactly (the reference's direct sums).
  * use_bf=1 (the default) reproduces the reference's segmented butterfly
- * (SERECKIRCH/src/bf1d.cpp, :814-1121) for cmp=0: per frequency the panels
- * the reference would factor get the factor's own values (the same
- * Chebyshev nodes, Lagrange values, bilinear samples and switch kernel),
- * every other entry its exact value.  For cmp=1 the butterfly is not
- * reproduced and the sums stay exact (stats.butterfly = 0).
+ * (SERECKIRCH/src/bf1d.cpp, :814-1121): per frequency the panels the
+ * reference would factor -- of each side's matrix for cmp=0, of each shot's
+ * and each receiver gather's for cmp=1 -- get the factor's own values (the
+ * same Chebyshev nodes, Lagrange values, bilinear samples and switch
+ * kernel), every other entry its exact value.
  *
  * Layouts are the reference's RSF ones: data and outputs (ns, nh, nt) with
  * time fastest, datum tables (n, n) with the output position fastest
@@ -47,7 +47,7 @@ typedef struct {
     float antialias;       /* operator antialias factor, 0 disables it (1) */
     float length;          /* datuming filter length, s (0.025) */
     float sdatum, rdatum;  /* datum depths of the source and receiver tables */
-    int use_bf;            /* (1) the butterfly panels (cmp=0), 0 exact sums */
+    int use_bf;            /* (1) the butterfly panels, 0 exact sums */
     int bf_p, bf_n_leaf, bf_panel_levels;  /* (12, 16, 1) */
     float bf_amp_eps, bf_phase_tol;        /* (1e-20, 2) */
     int interm;            /* keep the receiver-side result (the interm= file) */
