@@ -387,6 +387,17 @@ def sample_run(kmig, kv, w, sub, split):
 VALU_PEAK_LANE_INSTR = 256 * 4 * 16 * 2.4e9
 
 
+def migrate_source_sha16():
+    """Hash of the migrate kernel's sources (the kernel file and the headers
+    compiled into it): a counter profile belongs to one build of them."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in ("kmig_migrate.hip", "kmig_internal.h", "kmig_index.h"):
+        with open(os.path.join(ROOT, "kirch_datuming_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def roofline(config, precision, st, mig_ms, updates, counts, kernel):
     """The resident migrate kernel is bound by instruction issue, not by HBM:
     its receiver tuples stay in registers and culled groups load nothing, so
@@ -399,7 +410,6 @@ def roofline(config, precision, st, mig_ms, updates, counts, kernel):
     beside it, and the bytes model's 16 B/update is reported as a reuse
     factor over them.  A profile recorded for another build of the kernel
     source is marked stale and not used."""
-    import hashlib
     bpu = BYTES_PER_UPDATE.get(precision, 16.0)
     model = updates * bpu
     base = {"kernel": kernel, "avg_launch_ms": mig_ms, "updates_per_launch": updates,
@@ -407,16 +417,17 @@ def roofline(config, precision, st, mig_ms, updates, counts, kernel):
     if counts:
         base["evaluated_fraction"] = counts["evaluated"] / counts["nominal"]
         base["accepted_fraction"] = counts["accepted"] / counts["nominal"]
-    prof = os.path.join(ROOT, "profiles", f"migrate_{config}_{precision}_counters.json")
+    # the profile of the preset that ran, else the workload's default one
+    prof = os.path.join(ROOT, "profiles", f"migrate_{config}_{precision}_p{st['tile']}_counters.json")
+    if not os.path.exists(prof):
+        prof = os.path.join(ROOT, "profiles", f"migrate_{config}_{precision}_counters.json")
     if st["variant"] != 2 or not os.path.exists(prof):
         return {"bound": "hbm", "achieved": model / (mig_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": None, "traffic": None,
                 "note": "no counter profile for this kernel and workload", **base}
     with open(prof) as f:
         pc = json.load(f)
-    src = os.path.join(ROOT, "kirch_datuming_amd", "csrc", "kmig_migrate.hip")
-    with open(src, "rb") as f:
-        sha = hashlib.sha256(f.read()).hexdigest()[:16]
+    sha = migrate_source_sha16()
     stale = pc.get("source_sha16") != sha or int(pc.get("preset", -1)) != int(st["tile"])
     if stale:
         return {"bound": "valu", "achieved": None, "peak": VALU_PEAK_LANE_INSTR / 1e12,

@@ -147,7 +147,7 @@ struct ThBound {
     int lo, hi;
     int tmin, pad;
 };
-constexpr int BND_CZ = 8, BND_CX = 4;
+constexpr int BND_CZ = 16, BND_CX = 4;
 constexpr float BND_Q = 65536.0f;
 constexpr int BND_EMPTY = 1 << 28;
 
