@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "image-pt×trace updates/s, batched FP16/NN vs plain FP32 GPU at CIG relL2≤1e-3"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_UPDATE = {"fp32": 16.0, "fp16": 12.0, "bf16": 12.0, "u16": 12.0}
+BYTES_PER_UPDATE = {"fp32": 16.0, "fp16": 16.0, "bf16": 12.0, "u16": 12.0}
 
 
 def log(msg):
@@ -217,7 +217,10 @@ def main():
                     rp = timed_steps(mp, args.warmup, min(args.steps, 10), w.updates)
                 cp = sample_run(kmig, kvp, w, sub, split=2)
                 forms[prec] = {"value": rp["value"], "ms_per_step": rp["ms_per_step"],
-                               "migrate_ms": rp["migrate_ms"], "tile": rp["stats"]["tile"],
+                               "migrate_ms": rp["migrate_ms"],
+                               "tuple_prep_ms": rp["tuple_prep_ms"],
+                               "trace_prep_ms": rp["trace_prep_ms"],
+                               "events_ms": rp["events_ms"], "tile": rp["stats"]["tile"],
                                "vs_fp32": rp["value"] / value, **accuracy(cp, cig64, "gpu")}
                 log(f"{prec}: {rp['value']:.3e} updates/s ({forms[prec]['vs_fp32']:.3f} x fp32), "
                     f"relL2 vs f64 {forms[prec]['rel_l2_gpu_vs_f64']:.2e}")

@@ -143,6 +143,14 @@ void dfree(void *p)
     if (p) (void)hipFree(p);
 }
 
+// Tuple layouts of the resident kernel: FP32 and FP16 fields in 16 bytes
+// (PackedTuple: pk_s, pk_r), BF16 and UNORM16 fields in 12 bytes with
+// per-slot scales (PackedTuple16: pk16_s, pk16_r, sc_s, sc_r).
+bool full_layout(int precision)
+{
+    return precision == KMIG_PREC_FP32 || precision == KMIG_PREC_FP16;
+}
+
 int fail(kmig_ctx *c, int code, const char *fmt, ...)
 {
     char buf[1024];
@@ -819,8 +827,10 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
         const bool bins_ok = angle ? p.angle_interp == 1
                                    : (p.adj == 1 && p.precision == KMIG_PREC_FP32 &&
                                       migrate_lds_bytes(ng_out, 15) <= 64 * 1024);
+        // (FP16 tuples read their late tap 16 bytes past a pair offset: nt >= 3)
         const bool res_ok = bins_ok && (c->rcv_invariant || c->rcv_columns) &&
-                            p.antialias >= 0.0f && std::fabs(cst) + (double)c->nt < 30000.0;
+                            p.antialias >= 0.0f && std::fabs(cst) + (double)c->nt < 30000.0 &&
+                            (p.precision != KMIG_PREC_FP16 || c->nt >= 3);
         int variant = KMIG_VARIANT_STREAM;
         if (p.variant == KMIG_VARIANT_RESIDENT) {
             if (!res_ok)
@@ -835,10 +845,10 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
         if (p.precision != KMIG_PREC_FP32 && variant != KMIG_VARIANT_RESIDENT)
             return fail(c, KMIG_E_PARAM,
                         "precision=fp16/bf16 applies to the resident kernel: it needs "
-                        "mode=angle, angle_interp=1, antialias >= 0 and tuples that fit one "
-                        "batch");
+                        "mode=angle, angle_interp=1, antialias >= 0, tuples that fit one "
+                        "batch and (fp16) nt >= 3");
         c->variant_used = variant;
-        if (variant == KMIG_VARIANT_RESIDENT && p.precision != KMIG_PREC_FP32) {
+        if (variant == KMIG_VARIANT_RESIDENT && !full_layout(p.precision)) {
             if ((rc = ensure(c, &c->pk16_s, &c->pk16_s_cap, std::max<size_t>(1, maxs) * c->nzx)))
                 return rc;
             if ((rc = ensure(c, &c->pk16_r, &c->pk16_r_cap, std::max<size_t>(1, maxr) * c->nzx)))
@@ -846,7 +856,7 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
             if ((rc = ensure(c, &c->sc_s, &c->sc_s_cap, std::max<size_t>(1, maxs)))) return rc;
             if ((rc = ensure(c, &c->sc_r, &c->sc_r_cap, std::max<size_t>(1, maxr)))) return rc;
         }
-        if (variant == KMIG_VARIANT_RESIDENT && p.precision == KMIG_PREC_FP32) {
+        if (variant == KMIG_VARIANT_RESIDENT && full_layout(p.precision)) {
             if ((rc = ensure(c, &c->pk_s, &c->pk_s_cap, std::max<size_t>(1, maxs) * c->nzx)))
                 return rc;
             if ((rc = ensure(c, &c->pk_r, &c->pk_r_cap, std::max<size_t>(1, maxr) * c->nzx)))
@@ -1007,7 +1017,8 @@ int side_tuples(kmig_ctx *c, int side, const TuplePrepArgs &a, Tuples &o, size_t
                            !(env && env[0] == '0');
     if (fused) *fused = use_fused;
     if (use_fused && po.packed) {
-        HIPCHK(c, (hipError_t)launch_tuple_fused(a, o, po.packed, c->dt, po.th_scale, st));
+        HIPCHK(c, (hipError_t)launch_tuple_fused(a, o, po.packed, c->dt, po.th_scale,
+                                                 po.precision, st));
         return KMIG_OK;
     }
     if (use_fused) {
@@ -1050,7 +1061,8 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
     // packed directions in output-angle degrees (the opening angle is then
     // a difference of two directions folded at 360, or 180 for half angles)
     const float th_scale = angle ? (float)((p.half_angle ? 90.0 : 180.0) / M_PI) : 0.0f;
-    const bool pack32 = resident && p.precision == KMIG_PREC_FP32;
+    // FP32 and FP16 fields use the 16-byte layout, BF16 and UNORM16 the 12-byte one
+    const bool pack32 = resident && full_layout(p.precision);
     bool fused_s = false, fused_r = false;
     // source side
     A.tab = c->side[0].T;
@@ -1089,10 +1101,10 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
     if (pack32) {
         if (!fused_s)
             HIPCHK(c, (hipError_t)launch_pack(c->ts, c->pk_s, nsb, c->nzx, c->dt, th_scale,
-                                              p.antialias, st));
+                                              p.antialias, p.precision, st));
         if (!fused_r)
             HIPCHK(c, (hipError_t)launch_pack(c->tr, c->pk_r, b.nrcv, c->nzx, c->dt, th_scale,
-                                              p.antialias, st));
+                                              p.antialias, p.precision, st));
     } else if (resident) {
         if (!fused_s)
             HIPCHK(c, (hipError_t)launch_pack16(c->ts, c->pk16_s, c->sc_s, nsb, c->nzx, c->dt,

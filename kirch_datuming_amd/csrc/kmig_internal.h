@@ -105,7 +105,9 @@ struct MigArgs {
     float taper_start, taper_end;
 };
 
-// Packed 16-byte tuple of the resident migrate kernel, one per (slot, point):
+// Packed 16-byte tuple of the resident migrate kernel, one per (slot, point)
+// (FP32 fields; precision=fp16 stores IEEE half fields in the same 16 bytes,
+// described at tuple_h in kmig_prep.hip):
 //   frac  fractional part of T/dt (samples, in [0,1])
 //   a     1 + antialias * |dT/dpos * spacing| / dt: one plus the anti-alias
 //         half-width in samples (the outer taps of an update lie 1 + antialias
@@ -120,7 +122,7 @@ struct PackedTuple {
     uint32_t iw;
 };
 
-// 12-byte tuple of the 16-bit precision modes (FP16, BF16 or UNORM16 fields):
+// 12-byte tuple of the BF16 and UNORM16 precision modes:
 //   iw  as PackedTuple::iw (int16 integer sample | W * 2^15)
 //   fa  bits 15..0: fractional sample part f in [0,1)
 //       bits 31..16: a / a_scale of the slot, in [0,1]
@@ -278,13 +280,14 @@ int launch_tuple_prep(const TuplePrepArgs &a, Tuples &out, float *tmp1, float *t
 int launch_tuple_interp(const TuplePrepArgs &a, Tuples &out, void *stream);
 int launch_tuple_finish(const TuplePrepArgs &a, Tuples &out, float *tmp1, float *tmp2,
                         void *stream);
-// gradients, smoothing, weights and FP32 packing in one LDS-tiled pass (angle
-// mode, radius and smooth <= 4); the packed tuples equal those of
-// launch_tuple_finish + launch_pack bit for bit
+// gradients, smoothing, weights and packing into the 16-byte layout (FP32 or
+// FP16 fields: precision) in one LDS-tiled pass (angle mode, radius and
+// smooth <= 4); the packed tuples equal those of launch_tuple_finish +
+// launch_pack bit for bit
 bool tuple_fused_ok(const TuplePrepArgs &a);
 int launch_tuple_fused(const TuplePrepArgs &a, Tuples &o, PackedTuple *out, float dt,
-                       float th_scale, void *stream);
-// the same for the 16-bit formats; the per-slot anti-alias scale is the
+                       float th_scale, int precision, void *stream);
+// the same for the 12-byte formats (BF16, UNORM16); the per-slot anti-alias scale is the
 // maximum over all points of the slot (the unfused path takes it over the
 // points with W > 0)
 int launch_tuple_fused16(const TuplePrepArgs &a, Tuples &o, PackedTuple16 *out,
@@ -295,9 +298,10 @@ int launch_migrate(const MigArgs &a, int tile, void *stream);
 int launch_model(const MigArgs &a, int tile, void *stream);
 int migrate_lds_bytes(int ng, int tile);
 int launch_fill(float *p, size_t n, float v, void *stream);
-// th_scale: degrees per radian of the packed direction (0: no direction)
+// th_scale: degrees per radian of the packed direction (0: no direction);
+// precision: KMIG_PREC_FP32 or KMIG_PREC_FP16 fields
 int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, float dt,
-                float th_scale, float aal, void *stream);
+                float th_scale, float aal, int precision, void *stream);
 int launch_pack16(const Tuples &in, PackedTuple16 *out, SlotScale *scale, int nslots,
                   size_t nzx, float dt, float th_scale, int precision, void *stream);
 int launch_pairs(const float *prep, float *pairs, long ntraces, int nt, void *stream);

@@ -270,15 +270,59 @@ struct Pend {
     bool ok;
 };
 
-// Tuple loaders: FP32 (Fmt 0) or 16-bit fields decoded to the FP32 layout
-// with the slot's offset and scale (Fmt 1: IEEE half, 2: bfloat16,
-// 3: UNORM16).  The update arithmetic is the same in every mode; only the
-// stored precision differs.
+// Tuple loaders: FP32 (Fmt 0); IEEE half fields in the 16-byte layout
+// (Fmt 1: the fractional sample decoded to FP32 when the tuple is used, the
+// packed (m, -256 q) pair kept as it is, PackedTuple); or 12-byte tuples whose
+// 16-bit fields are decoded to the FP32 layout with the slot's scale (Fmt 2:
+// bfloat16, 3: UNORM16).  Interpolation and accumulation are FP32 in every
+// mode.
 __device__ __forceinline__ float from16(uint32_t h, int fmt)
 {
     if (fmt == 2) return __uint_as_float(h << 16);
     if (fmt == 3) return (float)h * (1.0f / 65535.0f);
     return (float)__builtin_bit_cast(_Float16, (unsigned short)h);
+}
+
+// Halves of a packed f16 pair as FP32 (exact conversions).
+__device__ __forceinline__ float half_lo(uint32_t x)
+{
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)(x & 0xFFFFu));
+}
+__device__ __forceinline__ float half_hi(uint32_t x)
+{
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)(x >> 16));
+}
+
+// w * -(the high f16 half of x), in FP32 (one v_fma_mix_f32)
+__device__ __forceinline__ float mul_neg_half_hi(float w, uint32_t x)
+{
+    float d;
+    asm("v_fma_mix_f32 %0, %1, -%2, 0 op_sel:[0,1,0] op_sel_hi:[0,1,0]" : "=v"(d) : "v"(w), "v"(x));
+    return d;
+}
+
+// Both halves' maxima of two packed f16 pairs in one instruction: for the
+// (m, -256 q) pairs of the FP16 tuples this is (max m, -256 min q).
+__device__ __forceinline__ uint32_t pk_max_f16(uint32_t x, uint32_t y)
+{
+    uint32_t d;
+    asm("v_pk_max_f16 %0, %1, %2" : "=v"(d) : "v"(x), "v"(y));
+    return d;
+}
+
+// f -/+ the low f16 half of x, in FP32 with one rounding (one v_fma_mix_f32:
+// the half is an f16 operand of the FP32 fused multiply-add).
+__device__ __forceinline__ float sub_half_lo(float f, uint32_t x)
+{
+    float d;
+    asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(d) : "v"(x), "v"(f));
+    return d;
+}
+__device__ __forceinline__ float add_half_lo(float f, uint32_t x)
+{
+    float d;
+    asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(d) : "v"(x), "v"(f));
+    return d;
 }
 
 // A tuple as loaded (Raw) and decoded: the loads of the next shot's tuple
@@ -295,13 +339,16 @@ template <int Fmt> struct RawOf {
 template <> struct RawOf<0> {
     using type = PackedTuple;
 };
+template <> struct RawOf<1> {
+    using type = PackedTuple;
+};
 
 template <int Fmt>
 __device__ __forceinline__ typename RawOf<Fmt>::type load_raw(const MigResArgs &a, bool src,
                                                               size_t slot, size_t i)
 {
     const size_t idx = slot * a.nzx + i;
-    if constexpr (Fmt == 0) {
+    if constexpr (Fmt <= 1) {
         return src ? a.stup[idx] : a.rtup[idx];
     } else {
         Raw16 r;
@@ -312,18 +359,25 @@ __device__ __forceinline__ typename RawOf<Fmt>::type load_raw(const MigResArgs &
 }
 
 // Decoded tuples carry the FP32 layout, whose a field is 1 + antialias * a
-// (PackedTuple): the 16-bit fields fold it in here, once per tuple.
+// (PackedTuple): the 12-byte 16-bit fields fold it in here, once per tuple.
+// FP16 tuples decode only the fractional sample (their a field stays the
+// packed (m, -256 q) pair that the update reads as it is).
 template <int Fmt>
 __device__ __forceinline__ PackedTuple decode(const typename RawOf<Fmt>::type &r, float aal)
 {
     if constexpr (Fmt == 0) {
         return r;
+    } else if constexpr (Fmt == 1) {
+        PackedTuple t = r;
+        t.frac = half_lo(__float_as_uint(r.frac));
+        return t;
     } else {
         PackedTuple t;
         t.iw = r.q.iw;
         t.frac = Fmt == 3 ? (float)(r.q.fa & 0xFFFFu) * (1.0f / 65536.0f)
                           : from16(r.q.fa & 0xFFFFu, Fmt);
-        t.a = fmaf(from16(r.q.fa >> 16, Fmt) * r.scale, aal, 1.0f);
+        // capped as the FP32 field (an update this wide is rejected anyway)
+        t.a = fminf(fmaf(from16(r.q.fa >> 16, Fmt) * r.scale, aal, 1.0f), 1048576.0f);
         t.th = __uint_as_float(r.q.th);
         return t;
     }
@@ -362,11 +416,63 @@ __device__ __forceinline__ int cvt_floor(float x)
 // the constant (tau - t0)/dt already added (once per shot).
 // Bin = 0: two linear angle bins from u; Bin = 1: the trace's own bin ibin
 // with all of the weight (offset gathers and the stack).
-template <int Bin>
+//
+// FP16 tuples (Fmt 1) carry m = antialias * a instead of 1 + m, and -256 q
+// with q = (1 / (3 + 2 m))^2 beside it: one packed f16 max gives the update's
+// larger m and smaller q (q falls as m grows) with no reciprocal per update.
+// Its tap offsets are kept one pair low (b8 - 8): the middle tap is then
+// loaded 8 bytes and the outer late tap 16 bytes past its offset (immediate
+// load offsets, TapOff below), so the outer taps floor(ff -/+ m) -/+ 1 cost
+// nothing over floor(ff -/+ m).  The caller's weights carry the 2^-8 of q.
+template <int Fmt> struct TapOff {
+    static constexpr int o0 = 0, om = 0, op = 0;
+};
+template <> struct TapOff<1> {
+    static constexpr int o0 = 8, om = 0, op = 16;
+};
+
+template <int Bin, int Fmt>
 __device__ __forceinline__ Pend res_phase1(const PackedTuple &s, int s_int8, float s_frac,
                                            const PackedTuple &r, float aw, float u, int ibin,
                                            const MigResArgs &a)
 {
+    if constexpr (Fmt == 1) {
+        Pend q;
+        int ig0;
+        float w1;
+        if constexpr (Bin == 0) {
+            const float uc = __builtin_amdgcn_fmed3f(u, 0.0f, (float)(a.ng - 1));
+            ig0 = cvt_floor(uc);
+            w1 = __builtin_amdgcn_fractf(uc);
+        } else {
+            ig0 = ibin;
+            w1 = 0.0f;
+        }
+        int b8 = s_int8 + (((int)r.iw >> 16) << 3) - 8;
+        asm("" : "+v"(b8));
+        const float ff = s_frac + r.frac;
+        const uint32_t mq = pk_max_f16(__float_as_uint(s.a), __float_as_uint(r.a));
+        const float fm = sub_half_lo(ff, mq), fp = add_half_lo(ff, mq);
+        const int it8 = b8 + (cvt_floor(ff) << 3);   // middle tap at it8 + 8
+        const int itm8 = b8 + (cvt_floor(fm) << 3);  // early tap at itm8
+        const int itp8 = b8 + (cvt_floor(fp) << 3);  // late tap at itp8 + 16
+        // taps inside pairs [0, nt - 2]: itm8 >= 0 and itp8 + 16 < lim8 (the
+        // host requires nt >= 3, so lim8 - 16 >= 0); itm8 <= itp8
+        const unsigned lim8 = (unsigned)(a.nt - 1) << 3;
+        const bool ok = (aw > 0.0f) & ((unsigned)itm8 < lim8 - 16u) & ((unsigned)itp8 < lim8 - 16u);
+        q.ok = ok;
+        q.coef = mul_neg_half_hi(aw, mq);
+        q.w1 = w1;
+        q.ft = __builtin_amdgcn_fractf(ff);
+        q.ftm = __builtin_amdgcn_fractf(fm);
+        q.ftp = __builtin_amdgcn_fractf(fp);
+        q.ig0 = ig0;
+        // a rejected update's three loads land on the trace's last pair, (0, 0)
+        q.o0 = ok ? (unsigned)it8 : lim8 - 8u;
+        q.om = ok ? (unsigned)itm8 : lim8;
+        q.op = ok ? (unsigned)itp8 : lim8 - 16u;
+        return q;
+    }
     Pend q;
     // linear bins: u clamped to [0, ng-1] gives the edge bin with w1 = 0 at
     // either end and (floor(u), u - floor(u)) inside
@@ -471,6 +577,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
 {
     static_assert(!(Model && Cols), "adj=0 with receiver columns runs on the stream kernel");
     static_assert(!(Model && Bin), "adj=0 offset gathers and stacks run on the stream kernel");
+    static_assert(!(Model && Fmt), "adj=0 reads FP32 tuples");
     constexpr int P = TZ * TX;
     const int lid = threadIdx.x;
     // tile order z fastest (measured faster than x fastest on the salt
@@ -591,17 +698,26 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             if (rlo > rhi) continue;  // no receiver weight at any point of the wave
             if (sall + rtm >= a.t_cull) continue;  // every shot's group is past the trace
         }
-        PackedTuple rt[R];
+        // the chunk's R receiver tuples: all loads issued first and decoded
+        // after (a decode next to each load would wait for it, one memory
+        // round trip per receiver); an unused receiver has zero fields
+        typename RawOf<Fmt>::type rraw[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int ih = r0 + r;
             const int slot = ih < nrec ? a.rslot[ih] : -1;
-            if (slot >= 0) {
-                rt[r] = load_tuple<Fmt>(a, false, (size_t)slot, i);
-            } else {
-                rt[r].frac = rt[r].a = rt[r].th = 0.0f;
-                rt[r].iw = 0u;
-            }
+            if (slot >= 0)
+                rraw[r] = load_raw<Fmt>(a, false, (size_t)slot, i);
+            else
+                rraw[r] = typename RawOf<Fmt>::type{};
+        }
+        PackedTuple rt[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            // (the empty asm keeps the decode of a loaded tuple from moving
+            // back next to its load, where the zero tuple's decode folds away)
+            if constexpr (Fmt == 1) asm("" : "+v"(rraw[r].frac));
+            rt[r] = decode<Fmt>(rraw[r], a.aal);
         }
         // Shots whose group survives the cull, in order: the tuple of the next
         // one is loaded while the current one is processed, and a culled shot
@@ -667,7 +783,9 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             if constexpr (Count) n_eval += n_valid * (unsigned long long)min(R, nrec - r0);
             const int s_int8 = (((int)st.iw >> 16) + a.c_int) << 3;
             const float s_frac = st.frac + a.c_frac;
-            const float s_w = (float)(st.iw & 0xFFFFu) * (inv32k * inv32k);
+            // both tuple weights' 2^-15 (and the 2^-8 of the FP16 tuples' q)
+            const float s_w = (float)(st.iw & 0xFFFFu) * (Fmt == 1 ? inv32k * inv32k * (1.0f / 256.0f)
+                                                                   : inv32k * inv32k);
             const int is = a.shot0 + jc;
             const float *__restrict__ wrow = a.wap + idx::wap(is, a.nh, r0, 0);
             const float2 *__restrict__ trow = pairs + idx::chunk_row(is, a.nh, r0) * a.nt;
@@ -744,8 +862,8 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 bool any = false;
 #pragma unroll
                 for (int k = 0; k < G; k++) {
-                    q[k] = res_phase1<Bin>(st, s_int8, s_frac, rt[g0 + k], aw[k], u[k], ibin[k],
-                                           a);
+                    q[k] = res_phase1<Bin, Fmt>(st, s_int8, s_frac, rt[g0 + k], aw[k], u[k],
+                                                ibin[k], a);
                     any |= q[k].ok;
                 }
                 if constexpr (Count) {
@@ -832,9 +950,9 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                     } else {
                         tr = (const char *)(trow + (size_t)idx::tap_row_off(a.nh, r0, g0 + k) * a.nt);
                     }
-                    v0[k] = *(const float2 *)(tr + q[k].o0);
-                    vm[k] = *(const float2 *)(tr + q[k].om);
-                    vp[k] = *(const float2 *)(tr + q[k].op);
+                    v0[k] = *(const float2 *)(tr + (size_t)q[k].o0 + TapOff<Fmt>::o0);
+                    vm[k] = *(const float2 *)(tr + (size_t)q[k].om + TapOff<Fmt>::om);
+                    vp[k] = *(const float2 *)(tr + (size_t)q[k].op + TapOff<Fmt>::op);
                 }
                 // Branch-free accumulation: a rejected update adds an exact
                 // zero, and a one-bin update adds its zero second share to the

@@ -444,25 +444,69 @@ __device__ inline PackFields pack_fields(float T, float A, float gx, float gz, f
         ip = 0.0;
         f.frac = f.a = f.th = 0.0f;
     }
+    // Times beyond the int16 sample range leave |frac| > 1; it is capped at
+    // 2^15 samples, still far past any trace (runs need |tau - t0| / dt + nt
+    // < 30000), so those updates stay rejected while the kernels' sample
+    // sums stay far inside 32-bit integers.
+    f.frac = fminf(fmaxf(f.frac, -32768.0f), 32768.0f);
     f.iw = ((uint32_t)(uint16_t)(int16_t)(int)ip << 16) |
            (uint32_t)__float2uint_rn(w * 32768.0f);
     return f;
 }
 
+// The FP32 tuple's a field: 1 + the anti-alias half-width in samples,
+// capped at 2^20 (an update with a larger half-width reads past every trace
+// and is rejected either way; the cap keeps its tap indices in 32 bits).
+__device__ inline float folded_a(const PackFields &f, float aal)
+{
+    return fminf(fmaf(f.a, aal, 1.0f), 1048576.0f);
+}
+
+// FP16 tuple in the 16-byte layout (precision=fp16, PackedTuple):
+//   frac  bits 15..0: the fractional sample as IEEE half, bits 31..16: 0
+//   a     bits 15..0: m = antialias * a, the anti-alias half-width in samples
+//         (half, capped at 2^14), bits 31..16: -256 q (half), q = (1 / (3 +
+//         2 m))^2 the kirmig_pick impulse weight of a side with this m
+//   th, iw as in the FP32 tuple
+// The update takes the larger m of its two sides and, with it, the smaller q
+// (one packed f16 max of the a fields); 256 keeps q a normal half for m up to
+// about 1000 samples.
+__device__ inline PackedTuple tuple_h(const PackFields &f, float aal)
+{
+    PackedTuple q;
+    const float m = fminf(f.a * aal, 16384.0f);
+    const float rr = 1.0f / fmaf(2.0f, m, 3.0f);
+    const _Float16 hf = (_Float16)f.frac, hm = (_Float16)m, hq = (_Float16)(-256.0f * (rr * rr));
+    q.frac = __uint_as_float((uint32_t)__builtin_bit_cast(unsigned short, hf));
+    q.a = __uint_as_float((uint32_t)__builtin_bit_cast(unsigned short, hm) |
+                          ((uint32_t)__builtin_bit_cast(unsigned short, hq) << 16));
+    q.th = f.th;
+    q.iw = f.iw;
+    return q;
+}
+
+// FP32 or FP16 fields in the 16-byte layout
+__device__ inline PackedTuple tuple_full(const PackFields &f, float aal, int precision)
+{
+    if (precision == KMIG_PREC_FP16) return tuple_h(f, aal);
+    PackedTuple q;
+    q.frac = f.frac;
+    q.a = folded_a(f, aal);
+    q.th = f.th;
+    q.iw = f.iw;
+    return q;
+}
+
 __global__ __launch_bounds__(256) void k_pack(const float *T, const float *A, const float *GX,
                                               const float *GZ, const float *W, PackedTuple *out,
-                                              size_t n, double inv_dt, float th_scale, float aal)
+                                              size_t n, double inv_dt, float th_scale, float aal,
+                                              int precision)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const PackFields f = pack_fields(T[k], A[k], th_scale != 0.0f ? GX[k] : 0.0f,
                                      th_scale != 0.0f ? GZ[k] : 0.0f, W[k], inv_dt, th_scale);
-    PackedTuple q;
-    q.frac = f.frac;
-    q.a = fmaf(f.a, aal, 1.0f);  // 1 + the anti-alias half-width (PackedTuple)
-    q.th = f.th;
-    q.iw = f.iw;
-    out[k] = q;
+    out[k] = tuple_full(f, aal, precision);
 }
 
 // 16-bit fields of PackedTuple16: IEEE half (round to nearest), bfloat16
@@ -571,8 +615,9 @@ __device__ inline float box_sum(const float *buf, int stride, int l, int g, int 
     return s / (float)(hi - lo + 1);
 }
 
-// Out = PackedTuple (FP32) or PackedTuple16 (16-bit formats: `precision`,
-// per-slot anti-alias scales in `scale`).
+// Out = PackedTuple (FP32 or FP16 fields: `precision`) or PackedTuple16
+// (BF16 / UNORM16 fields: `precision`, per-slot anti-alias scales in
+// `scale`).
 template <class Out>
 __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, const Tuples o,
                                                      Out *out, double inv_dt, float th_scale,
@@ -676,12 +721,7 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
         if constexpr (sizeof(Out) == sizeof(PackedTuple16)) {
             out[oi] = tuple16(f, scale[slot].a_scale, precision);
         } else {
-            PackedTuple q;
-            q.frac = f.frac;
-            q.a = fmaf(f.a, a.aal, 1.0f);  // 1 + the anti-alias half-width
-            q.th = f.th;
-            q.iw = f.iw;
-            out[oi] = q;
+            out[oi] = tuple_full(f, a.aal, precision);
         }
     }
 }
@@ -862,9 +902,9 @@ static int launch_fused_t(const TuplePrepArgs &a, Tuples &o, Out *out, float dt,
 }
 
 int launch_tuple_fused(const TuplePrepArgs &a, Tuples &o, PackedTuple *out, float dt,
-                       float th_scale, void *stream)
+                       float th_scale, int precision, void *stream)
 {
-    return launch_fused_t(a, o, out, dt, th_scale, (const SlotScale *)nullptr, 0,
+    return launch_fused_t(a, o, out, dt, th_scale, (const SlotScale *)nullptr, precision,
                           (hipStream_t)stream);
 }
 
@@ -889,12 +929,12 @@ int launch_tuple_prep(const TuplePrepArgs &a, Tuples &o, float *tmp1, float *tmp
 }
 
 int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, float dt,
-                float th_scale, float aal, void *stream)
+                float th_scale, float aal, int precision, void *stream)
 {
     const size_t n = (size_t)nslots * nzx;
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_pack, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, in.T, in.A,
-                       in.GX, in.GZ, in.W, out, n, 1.0 / (double)dt, th_scale, aal);
+                       in.GX, in.GZ, in.W, out, n, 1.0 / (double)dt, th_scale, aal, precision);
     return (int)hipGetLastError();
 }
 
