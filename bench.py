@@ -69,7 +69,15 @@ def main():
                     help="0: time adj=0 modelling of the workload's migrated image instead")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain device copy of this many MiB (profiler calibration)")
+    ap.add_argument("--record-counters", default=None, metavar="OUT_JSON",
+                    help="write the migrate kernel's counter profile from rocprofv3 output "
+                         "directories (--trace-dir, --pmc-dirs, --copy-dirs) and exit; no GPU")
+    ap.add_argument("--trace-dir", default=None)
+    ap.add_argument("--pmc-dirs", nargs="*", default=[])
+    ap.add_argument("--copy-dirs", nargs="*", default=[])
     args = ap.parse_args()
+    if args.record_counters:
+        return record_counters(args)
     if args.copy_probe > 0:
         # a plain copy kernel of known size: the HBM counters of a profiler run
         # on it give the scale factor for the migrate kernel's counters
@@ -388,6 +396,11 @@ def sample_run(kmig, kv, w, sub, split):
 # instructions per second (a wave64 instruction occupies a SIMD for 4 cycles);
 # 2 flops per lane for an FMA give the 78.6 TFLOP/s non-packed FP32 rate.
 VALU_PEAK_LANE_INSTR = 256 * 4 * 16 * 2.4e9
+# VALU issue cost in units of one 16-lane pass per wave64 instruction (4 cycles
+# on a SIMD): FP32 add/sub/mul/fma issue at twice that rate on this GPU
+# (157.3 TFLOP/s FP32 vector), the transcendental unit (v_rcp_f32 etc.) at
+# half of it; measured 56-60, 35-38 and 19 T lane-instr/s respectively
+VALU_COST = {"fast": 0.5, "other": 1.0, "trans": 2.0}
 
 
 def migrate_source_sha16():
@@ -440,10 +453,25 @@ def roofline(config, precision, st, mig_ms, updates, counts, kernel):
                 **base}
     valu = pc["sq_insts_valu"] * 64.0
     achieved = valu / (mig_ms * 1e-3)
+    # issue passes of the launch's instruction mix: the peak rate of this mix
+    # is VALU_PEAK_LANE_INSTR x instructions / passes
+    n = pc["sq_insts_valu"]
+    fast = sum(pc.get(k, 0.0) for k in ("sq_insts_valu_fma_f32", "sq_insts_valu_add_f32",
+                                        "sq_insts_valu_mul_f32"))
+    trans = pc.get("sq_insts_valu_trans_f32", 0.0)
+    passes = (VALU_COST["fast"] * fast + VALU_COST["trans"] * trans +
+              VALU_COST["other"] * (n - fast - trans))
+    peak = VALU_PEAK_LANE_INSTR * n / passes
     traffic = pc["hbm_bytes_per_launch"]
     hbm = traffic / (mig_ms * 1e-3) / 1e9
-    return {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK_LANE_INSTR / 1e12,
-            "unit": "T lane-instr/s", "frac": achieved / VALU_PEAK_LANE_INSTR,
+    return {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
+            "unit": "T lane-instr/s", "frac": achieved / peak,
+            "peak_note": "VALU issue peak of this launch's instruction mix: FP32 "
+                         "add/sub/mul/fma at 2x, transcendentals at 0.5x the 39.3 T "
+                         "lane-instr/s of the other VALU instructions",
+            "mix": {"fp32_fast": fast / n, "trans": trans / n,
+                    "other": (n - fast - trans) / n},
+            "frac_unweighted": achieved / VALU_PEAK_LANE_INSTR,
             "traffic": traffic, "stale": False, "profile": os.path.relpath(prof, ROOT),
             "valu_lane_instr_per_launch": valu,
             "valu_instr_per_nominal_update": valu / updates,
@@ -451,6 +479,83 @@ def roofline(config, precision, st, mig_ms, updates, counts, kernel):
                     "frac": hbm / HBM_PEAK_GBS},
             "reuse_factor_bytes_model_over_traffic": model / traffic,
             "profile_launch_ms": pc.get("avg_launch_ms"), **base}
+
+
+def record_counters(args):
+    """Counter profile of the production migrate kernel (k_migrate_ph, not the
+    autotuner's Probe instances) from rocprofv3 CSV output: the kernel-trace
+    statistics (--trace-dir), the counter passes (--pmc-dirs; per-dispatch
+    values averaged over the kernel's dispatches) and a plain device copy of
+    known size read and written (--copy-dirs, one FETCH_SIZE and one
+    WRITE_SIZE pass of `bench.py --copy-probe MiB`), which scales the HBM
+    counters to bytes."""
+    import csv
+    import glob
+
+    def rows(d, suffix):
+        out = []
+        for f in sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True)):
+            with open(f) as fh:
+                out += list(csv.DictReader(fh))
+        return out
+
+    def production(name):
+        # k_migrate_ph<TZ, TX, R, G, Probe, Fmt, Cols, Bin>: Probe is the fifth
+        if "k_migrate_ph<" not in name:
+            return False
+        targs = [t.strip() for t in name.split("<", 1)[1].split(">", 1)[0].split(",")]
+        return len(targs) >= 5 and targs[4] == "false"
+
+    def counters(d, want):
+        per = {}
+        kname = None
+        for r in rows(d, "counter_collection.csv"):
+            if not want(r["Kernel_Name"]):
+                continue
+            kname = r["Kernel_Name"]
+            per.setdefault(r["Dispatch_Id"], {}).setdefault(r["Counter_Name"], 0.0)
+            per[r["Dispatch_Id"]][r["Counter_Name"]] += float(r["Counter_Value"])
+        if not per:
+            raise SystemExit(f"no dispatch of the kernel in {d}")
+        names = set(c for v in per.values() for c in v)
+        return kname, {c: sum(v.get(c, 0.0) for v in per.values()) / len(per) for c in names}
+
+    prof = {}
+    kname = None
+    for d in args.pmc_dirs:
+        k, c = counters(d, production)
+        kname = kname or k
+        for name, v in c.items():
+            prof[name.lower()] = v
+    calls, avg_ns = 0, 0.0
+    if args.trace_dir:
+        for r in rows(args.trace_dir, "kernel_stats.csv"):
+            if production(r["Name"]):
+                calls = int(r["Calls"])
+                avg_ns = float(r["AverageNs"])
+    factors = {}
+    for d in args.copy_dirs:
+        _, c = counters(d, lambda n: "copyBuffer" in n)
+        nbytes = args.copy_probe * 1024 * 1024 if args.copy_probe else 1024.0 ** 3
+        for key in ("FETCH_SIZE", "WRITE_SIZE"):
+            if key in c:
+                factors[key] = nbytes / (c[key] * 1024.0)
+    targs = [t.strip() for t in kname.split("<", 1)[1].split(">", 1)[0].split(",")]
+    out = {"kernel": "k_migrate_ph<" + ", ".join(targs) + ">", "workload": args.config,
+           "precision": args.precision, "preset": args.tile,
+           "source_sha16": migrate_source_sha16(), "avg_launch_ms": avg_ns * 1e-6,
+           "trace_calls": calls, **{k: v for k, v in sorted(prof.items())
+                                    if k.startswith("sq_")}}
+    if "fetch_size" in prof and "write_size" in prof:
+        ff = round(factors.get("FETCH_SIZE", 1.0), 2)
+        wf = round(factors.get("WRITE_SIZE", 1.0), 2)
+        out.update({"fetch_kib_raw": prof["fetch_size"], "fetch_factor": ff,
+                    "write_kib_raw": prof["write_size"], "write_factor": wf,
+                    "hbm_bytes_per_launch": 1024.0 * (prof["fetch_size"] * ff +
+                                                      prof["write_size"] * wf)})
+    with open(args.record_counters, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
 
 
 def bench_model(args, cores, w):

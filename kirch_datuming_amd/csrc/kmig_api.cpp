@@ -991,6 +991,7 @@ struct PackOut {
     PackedTuple *packed = nullptr;
     PackedTuple16 *packed16 = nullptr;
     SlotScale *scale = nullptr;
+    ThBound *bnd = nullptr;  // the fused pass also writes the slots' bounds here
     int precision = KMIG_PREC_FP32;
     float th_scale = 0.0f;
 };
@@ -1018,12 +1019,12 @@ int side_tuples(kmig_ctx *c, int side, const TuplePrepArgs &a, Tuples &o, size_t
     if (fused) *fused = use_fused;
     if (use_fused && po.packed) {
         HIPCHK(c, (hipError_t)launch_tuple_fused(a, o, po.packed, c->dt, po.th_scale,
-                                                 po.precision, st));
+                                                 po.precision, po.bnd, st));
         return KMIG_OK;
     }
     if (use_fused) {
         HIPCHK(c, (hipError_t)launch_tuple_fused16(a, o, po.packed16, po.scale, c->dt,
-                                                   po.th_scale, po.precision, st));
+                                                   po.th_scale, po.precision, po.bnd, st));
         return KMIG_OK;
     }
     HIPCHK(c, (hipError_t)launch_tuple_finish(a, o, c->tmp1, c->tmp2, st));
@@ -1081,6 +1082,7 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
         po.packed16 = c->pk16_s;
         po.scale = c->sc_s;
     }
+    if (resident) po.bnd = c->bnd_s;
     int rc = side_tuples(c, 0, A, c->ts, spec_s_off, st, po, &fused_s);
     if (rc) return rc;
     // receiver side
@@ -1097,6 +1099,7 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
         po.packed16 = c->pk16_r;
         po.scale = c->sc_r;
     }
+    if (resident) po.bnd = c->bnd_r;
     if ((rc = side_tuples(c, 1, A, c->tr, b.spec_r0, st, po, &fused_r))) return rc;
     if (pack32) {
         if (!fused_s)
@@ -1114,14 +1117,17 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
                                                 c->dt, th_scale, p.precision, st));
     }
     if (resident) {
-        HIPCHK(c, (hipError_t)launch_th_bounds(pack32 ? c->pk_s : nullptr,
-                                               pack32 ? nullptr : c->pk16_s, nsb, c->nz, c->nx,
-                                               c->bnd_s, st));
+        // the fused pass writes the bounds of its tuples itself
+        if (!fused_s)
+            HIPCHK(c, (hipError_t)launch_th_bounds(pack32 ? c->pk_s : nullptr,
+                                                   pack32 ? nullptr : c->pk16_s, nsb, c->nz,
+                                                   c->nx, c->bnd_s, st));
         const int ncell = ((c->nz + BND_CZ - 1) / BND_CZ) * ((c->nx + BND_CX - 1) / BND_CX);
         HIPCHK(c, (hipError_t)launch_cell_blocks(c->bnd_s, nsb, ncell, c->bnd_sblk, st));
-        HIPCHK(c, (hipError_t)launch_th_bounds(pack32 ? c->pk_r : nullptr,
-                                               pack32 ? nullptr : c->pk16_r, b.nrcv, c->nz,
-                                               c->nx, c->bnd_r, st));
+        if (!fused_r)
+            HIPCHK(c, (hipError_t)launch_th_bounds(pack32 ? c->pk_r : nullptr,
+                                                   pack32 ? nullptr : c->pk16_r, b.nrcv, c->nz,
+                                                   c->nx, c->bnd_r, st));
     }
     return KMIG_OK;
 }

@@ -284,15 +284,17 @@ int launch_tuple_finish(const TuplePrepArgs &a, Tuples &out, float *tmp1, float 
 // FP16 fields: precision) in one LDS-tiled pass (angle mode, radius and
 // smooth <= 4); the packed tuples equal those of launch_tuple_finish +
 // launch_pack bit for bit
+// With bnd set the pass also writes the slots' direction and sample bounds
+// (launch_th_bounds' output for the same tuples).
 bool tuple_fused_ok(const TuplePrepArgs &a);
 int launch_tuple_fused(const TuplePrepArgs &a, Tuples &o, PackedTuple *out, float dt,
-                       float th_scale, int precision, void *stream);
+                       float th_scale, int precision, ThBound *bnd, void *stream);
 // the same for the 12-byte formats (BF16, UNORM16); the per-slot anti-alias scale is the
 // maximum over all points of the slot (the unfused path takes it over the
 // points with W > 0)
 int launch_tuple_fused16(const TuplePrepArgs &a, Tuples &o, PackedTuple16 *out,
                          SlotScale *scale, float dt, float th_scale, int precision,
-                         void *stream);
+                         ThBound *bnd, void *stream);
 int launch_migrate(const MigArgs &a, int tile, void *stream);
 // adj=0 with the same kernel: a.cig is the image, a.spray the zeroed traces
 int launch_model(const MigArgs &a, int tile, void *stream);

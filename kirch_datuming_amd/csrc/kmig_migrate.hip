@@ -250,8 +250,19 @@ __device__ __forceinline__ float ramp_down(float u)
 #define KMIG_XCD_MAP 0
 #endif
 
+#ifndef KMIG_XCD_CLUSTER
+#define KMIG_XCD_CLUSTER 0
+#endif
+
 __device__ __forceinline__ int xcd_tile(int b, int nb)
 {
+    if (KMIG_XCD_CLUSTER > 0) {
+        constexpr int C = KMIG_XCD_CLUSTER;
+        const int nfull = nb / (8 * C) * (8 * C);
+        if (b >= nfull) return b;
+        const int x = b & 7, k = b >> 3;
+        return ((k / C) * 8 + x) * C + (k % C);
+    }
     if (!KMIG_XCD_MAP) return b;
     const int xcd = b & 7, k = b >> 3, q = nb >> 3, rem = nb & 7;
     return xcd < rem ? xcd * (q + 1) + k : rem * (q + 1) + (xcd - rem) * q + k;

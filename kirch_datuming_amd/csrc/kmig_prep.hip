@@ -601,37 +601,76 @@ struct TileWalk {
     }
 };
 
-__device__ inline float box_sum(const float *buf, int stride, int l, int g, int r, int n_img,
-                                int l_lo, int l_hi)
+// One box sum of the smoothing passes for the element at `buf` (taps `stride`
+// apart, global coordinate g on an axis of n image samples): the 2 RS + 1
+// taps summed from the low end with the taps outside the image added as
+// +0.0, over the count of taps inside.  This is the clipped loop of k_box_z /
+// k_box_x bit for bit: its sum starts at +0.0 and so never becomes -0.0, and
+// adding +0.0 leaves such a sum as it is.  The caller's regions keep RS
+// elements of margin on the box's axis, so every tap lies in the LDS region.
+template <int RS>
+__device__ __forceinline__ float box_masked(const float *buf, int stride, int g, int n)
 {
-    // window [g - r, g + r] of image positions, clipped to the image and to the
-    // tile's local range [l_lo, l_hi]; summed from its low end as k_box_*
-    int lo = g - r < 0 ? 0 : g - r, hi = g + r > n_img - 1 ? n_img - 1 : g + r;
-    const int off = l - g;  // local = global + off
-    if (lo + off < l_lo) lo = l_lo - off;
-    if (hi + off > l_hi) hi = l_hi - off;
     float s = 0.0f;
-    for (int k = lo; k <= hi; k++) s += buf[(k + off) * stride];
+#pragma unroll
+    for (int t = -RS; t <= RS; t++) {
+        const float v = buf[t * stride];
+        s += (g + t >= 0 && g + t < n) ? v : 0.0f;
+    }
+    const int lo = g - RS < 0 ? 0 : g - RS, hi = g + RS > n - 1 ? n - 1 : g + RS;
     return s / (float)(hi - lo + 1);
+}
+
+// One smoothing pass (box along z, or along x) of both gradient components
+// over the local region [zlo, zhi) x [xlo, xhi) of the w1z-high LDS arrays
+// whose element (0, 0) is image point (gz0, gx0); image points only.
+template <int RS, bool ZDIR>
+__device__ __forceinline__ void box_pass(const float *sx, const float *sz, float *dx, float *dz,
+                                         int w1z, int zlo, int zhi, int xlo, int xhi, int gz0,
+                                         int gx0, int nz, int nx)
+{
+    const int wz = zhi - zlo, n = wz * (xhi - xlo);
+    TileWalk t(threadIdx.x, wz);
+    for (int k = threadIdx.x; k < n; k += 256, t.next()) {
+        const int lz = zlo + t.l, lx = xlo + t.m;
+        const int iz = gz0 + lz, ix = gx0 + lx;
+        if (iz < 0 || iz >= nz || ix < 0 || ix >= nx) continue;
+        const int e = lx * w1z + lz;
+        if (ZDIR) {
+            dx[e] = box_masked<RS>(sx + e, 1, iz, nz);
+            dz[e] = box_masked<RS>(sz + e, 1, iz, nz);
+        } else {
+            dx[e] = box_masked<RS>(sx + e, w1z, ix, nx);
+            dz[e] = box_masked<RS>(sz + e, w1z, ix, nx);
+        }
+    }
+    __syncthreads();
 }
 
 // Out = PackedTuple (FP32 or FP16 fields: `precision`) or PackedTuple16
 // (BF16 / UNORM16 fields: `precision`, per-slot anti-alias scales in
-// `scale`).
-template <class Out>
+// `scale`); RS = the smoothing radius.  With bnd set, the block also writes
+// the direction and sample bounds of its bound cells (k_th_bounds' values):
+// each wave's output points are one cell of BND_CZ x BND_CX points.
+template <class Out, int RS>
 __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, const Tuples o,
                                                      Out *out, double inv_dt, float th_scale,
-                                                     const SlotScale *scale, int precision)
+                                                     const SlotScale *scale, int precision,
+                                                     ThBound *bnd)
 {
+    static_assert(FZ % BND_CZ == 0 && FX % BND_CX == 0 && BND_CZ * BND_CX == 64 &&
+                      FZ * FX == 4 * 64 * ((FZ / BND_CZ) * (FX / BND_CX) / 4),
+                  "a tile is whole bound cells, one per wave and round");
     extern __shared__ float sm[];
     const int slot = blockIdx.y;
     const int nz = a.nz, nx = a.nx;
     const int ntz = (nz + FZ - 1) / FZ;
     const int z0 = (blockIdx.x % ntz) * FZ, x0 = (blockIdx.x / ntz) * FX;
-    const int rf = a.radius < 1 ? 1 : a.radius, rs = a.smooth > 0 ? a.smooth : 0;
-    const int H1 = 2 * rs, H0 = H1 + rf;
+    const int rf = a.radius < 1 ? 1 : a.radius;
+    constexpr int H1 = 2 * RS;
+    const int H0 = H1 + rf;
     const int w0z = FZ + 2 * H0, w0x = FX + 2 * H0;
-    const int w1z = FZ + 2 * H1, w1x = FX + 2 * H1, n1 = w1z * w1x;
+    constexpr int w1z = FZ + 2 * H1, w1x = FX + 2 * H1, n1 = w1z * w1x;
     float *sT = sm;
     float *ax = sT + (a.use_grad_files ? 0 : w0z * w0x), *az = ax + n1;
     float *bx = az + n1, *bz = bx + n1;
@@ -678,50 +717,76 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
         az[k] = vz;
     }
     __syncthreads();
-    // two passes of (box z, box x) per component: a -> b -> a -> b -> a
-    float *src_x = ax, *src_z = az, *dst_x = bx, *dst_z = bz;
-    for (int pass = 0; rs > 0 && pass < 4; pass++) {
-        const bool zdir = (pass % 2) == 0;
-        TileWalk tp(tid, w1z);
-        for (int k = tid; k < n1; k += 256, tp.next()) {
-            const int lz = tp.l, lx = tp.m;
-            const int iz = z0 - H1 + lz, ix = x0 - H1 + lx;
-            if (!(iz >= 0 && iz < nz && ix >= 0 && ix < nx)) continue;
-            if (zdir) {
-                dst_x[k] = box_sum(src_x + lx * w1z, 1, lz, iz, rs, nz, 0, w1z - 1);
-                dst_z[k] = box_sum(src_z + lx * w1z, 1, lz, iz, rs, nz, 0, w1z - 1);
-            } else {
-                dst_x[k] = box_sum(src_x + lz, w1z, lx, ix, rs, nx, 0, w1x - 1);
-                dst_z[k] = box_sum(src_z + lz, w1z, lx, ix, rs, nx, 0, w1x - 1);
-            }
-        }
-        __syncthreads();
-        float *t = src_x; src_x = dst_x; dst_x = t;
-        t = src_z; src_z = dst_z; dst_z = t;
+    // two passes of (box z, box x) per component, a -> b -> a -> b -> a, each
+    // over only the region the next pass reads (the last: the output tile)
+    if constexpr (RS > 0) {
+        const int gz0 = z0 - H1, gx0 = x0 - H1;
+        box_pass<RS, true>(ax, az, bx, bz, w1z, RS, w1z - RS, 0, w1x, gz0, gx0, nz, nx);
+        box_pass<RS, false>(bx, bz, ax, az, w1z, RS, w1z - RS, RS, w1x - RS, gz0, gx0, nz, nx);
+        box_pass<RS, true>(ax, az, bx, bz, w1z, 2 * RS, w1z - 2 * RS, RS, w1x - RS, gz0, gx0,
+                           nz, nx);
+        box_pass<RS, false>(bx, bz, ax, az, w1z, 2 * RS, w1z - 2 * RS, 2 * RS, w1x - 2 * RS,
+                            gz0, gx0, nz, nx);
     }
-    // weights and packing of the output tile
+    // weights and packing of the output tile: wave w, round j takes bound cell
+    // 4 j + w of the tile's (FZ / BND_CZ) x (FX / BND_CX), lane = (z, x) in it
     const float dzabs = fabsf(a.dz);
-    for (int k = tid; k < FZ * FX; k += 256) {
-        const int iz = z0 + k % FZ, ix = x0 + k / FZ;
-        if (iz >= nz || ix >= nx) continue;
-        const int l = (ix - x0 + H1) * w1z + (iz - z0 + H1);
-        const float gx = src_x[l], gz = src_z[l];
-        const float zd0 = fabsf((float)iz * a.dz), zd1 = 1.0e-6f * dzabs;
-        const float zd2 = 1.0e-12f > zd1 ? 1.0e-12f : zd1;
-        const float zdist = zd0 > zd2 ? zd0 : zd2;
-        const float ximg = a.x0 + (float)ix * a.dx;
-        const float d = fabsf(a.spec[slot].surf_x - ximg);
-        const float deg = atanf(d / zdist) * 180.0f / (float)KPI;
-        const float wc = leg_weight(deg, a.aper, a.aper_taper);
-        const float wf = floor_weight(d, a.rmin, a.rtaper);
-        float w = wc > wf ? wc : wf;
-        if (sqrtf(gx * gx + gz * gz) <= 1.0e-12f) w = 0.0f;
-        const size_t oi = base + (size_t)ix * nz + iz;
-        const PackFields f = pack_fields(o.T[oi], o.A[oi], gx, gz, w, inv_dt, th_scale);
-        if constexpr (sizeof(Out) == sizeof(PackedTuple16)) {
-            out[oi] = tuple16(f, scale[slot].a_scale, precision);
-        } else {
-            out[oi] = tuple_full(f, a.aal, precision);
+    const int lane = tid & 63, wv = tid >> 6;
+    constexpr int TCZ = FZ / BND_CZ;
+    for (int c = wv; c < TCZ * (FX / BND_CX); c += 4) {
+        const int cz = c % TCZ, cx = c / TCZ;
+        const int iz = z0 + cz * BND_CZ + lane % BND_CZ, ix = x0 + cx * BND_CX + lane / BND_CZ;
+        const bool valid = iz < nz && ix < nx;
+        float th = 0.0f;
+        uint32_t iw = 0u;
+        if (valid) {
+            const int l = (ix - x0 + H1) * w1z + (iz - z0 + H1);
+            const float gx = ax[l], gz = az[l];
+            const float zd0 = fabsf((float)iz * a.dz), zd1 = 1.0e-6f * dzabs;
+            const float zd2 = 1.0e-12f > zd1 ? 1.0e-12f : zd1;
+            const float zdist = zd0 > zd2 ? zd0 : zd2;
+            const float ximg = a.x0 + (float)ix * a.dx;
+            const float d = fabsf(a.spec[slot].surf_x - ximg);
+            const float deg = atanf(d / zdist) * 180.0f / (float)KPI;
+            const float wc = leg_weight(deg, a.aper, a.aper_taper);
+            const float wf = floor_weight(d, a.rmin, a.rtaper);
+            float w = wc > wf ? wc : wf;
+            if (sqrtf(gx * gx + gz * gz) <= 1.0e-12f) w = 0.0f;
+            const size_t oi = base + (size_t)ix * nz + iz;
+            const PackFields f = pack_fields(o.T[oi], o.A[oi], gx, gz, w, inv_dt, th_scale);
+            if constexpr (sizeof(Out) == sizeof(PackedTuple16)) {
+                out[oi] = tuple16(f, scale[slot].a_scale, precision);
+            } else {
+                out[oi] = tuple_full(f, a.aal, precision);
+            }
+            th = f.th;
+            iw = f.iw;
+        }
+        if (!bnd) continue;
+        // the cell's bounds over its points of nonzero weight (k_th_bounds)
+        const bool use = valid && (iw & 0xFFFFu) != 0u;
+        float lo = use ? th : INFINITY, hi = use ? th : -INFINITY;
+        int tmin = use ? (int)iw >> 16 : BND_EMPTY;
+        for (int m = 1; m < 64; m <<= 1) {
+            lo = fminf(lo, __shfl_xor(lo, m, 64));
+            hi = fmaxf(hi, __shfl_xor(hi, m, 64));
+            tmin = min(tmin, __shfl_xor(tmin, m, 64));
+        }
+        const int gcz = z0 / BND_CZ + cz, gcx = x0 / BND_CX + cx;
+        const int ncz = (nz + BND_CZ - 1) / BND_CZ, ncx = (nx + BND_CX - 1) / BND_CX;
+        if (lane == 0 && gcz < ncz && gcx < ncx) {
+            ThBound b;
+            b.pad = 0;
+            if (lo > hi) {
+                b.lo = BND_EMPTY;
+                b.hi = -BND_EMPTY;
+                b.tmin = BND_EMPTY;
+            } else {
+                b.lo = (int)floorf(lo * BND_Q);
+                b.hi = (int)ceilf(hi * BND_Q);
+                b.tmin = tmin;
+            }
+            bnd[(size_t)(gcx * ncz + gcz) * a.nslots + slot] = b;
         }
     }
 }
@@ -879,38 +944,52 @@ bool tuple_fused_ok(const TuplePrepArgs &a)
            a.nslots <= 65535;
 }
 
-template <class Out>
-static int launch_fused_t(const TuplePrepArgs &a, Tuples &o, Out *out, float dt, float th_scale,
-                          const SlotScale *scale, int precision, hipStream_t st)
+template <class Out, int RS>
+static int launch_fused_rs(const TuplePrepArgs &a, Tuples &o, Out *out, float dt, float th_scale,
+                           const SlotScale *scale, int precision, ThBound *bnd, hipStream_t st)
 {
-    if (a.nslots <= 0) return 0;
-    const int rf = a.radius < 1 ? 1 : a.radius, rs = a.smooth > 0 ? a.smooth : 0;
-    const int H1 = 2 * rs, H0 = H1 + rf;
+    const int rf = a.radius < 1 ? 1 : a.radius;
+    const int H1 = 2 * RS, H0 = H1 + rf;
     const size_t n0 = a.use_grad_files ? 0 : (size_t)(FZ + 2 * H0) * (FX + 2 * H0);
     const size_t n1 = (size_t)(FZ + 2 * H1) * (FX + 2 * H1);
     const size_t lds = (n0 + 4 * n1) * sizeof(float);
     const unsigned tiles = (unsigned)(((a.nz + FZ - 1) / FZ) * ((a.nx + FX - 1) / FX));
     if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_tuple_fused<Out>,
+        const hipError_t e = hipFuncSetAttribute((const void *)k_tuple_fused<Out, RS>,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  (int)lds);
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL(k_tuple_fused<Out>, dim3(tiles, (unsigned)a.nslots), dim3(256), lds, st,
-                       a, o, out, 1.0 / (double)dt, th_scale, scale, precision);
+    hipLaunchKernelGGL((k_tuple_fused<Out, RS>), dim3(tiles, (unsigned)a.nslots), dim3(256), lds,
+                       st, a, o, out, 1.0 / (double)dt, th_scale, scale, precision, bnd);
     return (int)hipGetLastError();
 }
 
-int launch_tuple_fused(const TuplePrepArgs &a, Tuples &o, PackedTuple *out, float dt,
-                       float th_scale, int precision, void *stream)
+template <class Out>
+static int launch_fused_t(const TuplePrepArgs &a, Tuples &o, Out *out, float dt, float th_scale,
+                          const SlotScale *scale, int precision, ThBound *bnd, hipStream_t st)
 {
-    return launch_fused_t(a, o, out, dt, th_scale, (const SlotScale *)nullptr, precision,
+    if (a.nslots <= 0) return 0;
+    switch (a.smooth > 0 ? a.smooth : 0) {
+    case 0: return launch_fused_rs<Out, 0>(a, o, out, dt, th_scale, scale, precision, bnd, st);
+    case 1: return launch_fused_rs<Out, 1>(a, o, out, dt, th_scale, scale, precision, bnd, st);
+    case 2: return launch_fused_rs<Out, 2>(a, o, out, dt, th_scale, scale, precision, bnd, st);
+    case 3: return launch_fused_rs<Out, 3>(a, o, out, dt, th_scale, scale, precision, bnd, st);
+    case 4: return launch_fused_rs<Out, 4>(a, o, out, dt, th_scale, scale, precision, bnd, st);
+    default: return (int)hipErrorInvalidValue;  // tuple_fused_ok keeps smooth <= 4
+    }
+}
+
+int launch_tuple_fused(const TuplePrepArgs &a, Tuples &o, PackedTuple *out, float dt,
+                       float th_scale, int precision, ThBound *bnd, void *stream)
+{
+    return launch_fused_t(a, o, out, dt, th_scale, (const SlotScale *)nullptr, precision, bnd,
                           (hipStream_t)stream);
 }
 
 int launch_tuple_fused16(const TuplePrepArgs &a, Tuples &o, PackedTuple16 *out,
                          SlotScale *scale, float dt, float th_scale, int precision,
-                         void *stream)
+                         ThBound *bnd, void *stream)
 {
     if (a.nslots <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
@@ -919,7 +998,7 @@ int launch_tuple_fused16(const TuplePrepArgs &a, Tuples &o, PackedTuple16 *out,
     hipLaunchKernelGGL(k_slot_range, dim3((unsigned)a.nslots), dim3(256), 0, st,
                        (const float *)o.A, (const float *)nullptr, scale, a.nzx,
                        1.0 / (double)dt);
-    return launch_fused_t(a, o, out, dt, th_scale, (const SlotScale *)scale, precision, st);
+    return launch_fused_t(a, o, out, dt, th_scale, (const SlotScale *)scale, precision, bnd, st);
 }
 
 int launch_tuple_prep(const TuplePrepArgs &a, Tuples &o, float *tmp1, float *tmp2, void *stream)
