@@ -88,8 +88,8 @@ struct kmig_ctx {
     SlotScale *sc_s = nullptr, *sc_r = nullptr;
     size_t sc_s_cap = 0, sc_r_cap = 0;
     ThBound *bnd_s = nullptr, *bnd_r = nullptr;  // direction bounds per (slot, cell)
-    ThBound *bnd_sblk = nullptr;  // source bounds per (cell, block of TMIN_BLK shots)
-    size_t bnd_sblk_cap = 0;
+    unsigned long long *mask = nullptr;  // survivor masks of the group culls (MigResArgs)
+    size_t mask_cap = 0;
     size_t bnd_s_cap = 0, bnd_r_cap = 0;
     float *pairs = nullptr;
     size_t pairs_cap = 0;
@@ -641,9 +641,8 @@ MigResArgs res_args(kmig_ctx *c, int shot0, int nshots, int ng)
     const char *cull_env = std::getenv("KMIG_CULL");
     const bool cull = !(cull_env && std::atoi(cull_env) == 0);
     R.sbnd = cull ? c->bnd_s : nullptr;
+    R.mask = cull ? c->mask : nullptr;
     R.rbnd = cull ? c->bnd_r : nullptr;
-    R.sblk = cull ? c->bnd_sblk : nullptr;
-    R.scell_nblk = (nshots + TMIN_BLK - 1) / TMIN_BLK;
     R.scell_off = 0;
     R.ncz = (c->nz + BND_CZ - 1) / BND_CZ;
     R.ncx = (c->nx + BND_CX - 1) / BND_CX;
@@ -867,10 +866,12 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
                                  ((c->nx + BND_CX - 1) / BND_CX);
             if ((rc = ensure(c, &c->bnd_s, &c->bnd_s_cap, std::max<size_t>(1, maxs) * ncell)))
                 return rc;
-            if ((rc = ensure(c, &c->bnd_sblk, &c->bnd_sblk_cap,
-                             ncell * ((std::max<size_t>(1, maxs) + TMIN_BLK - 1) / TMIN_BLK))))
-                return rc;
             if ((rc = ensure(c, &c->bnd_r, &c->bnd_r_cap, std::max<size_t>(1, maxr) * ncell)))
+                return rc;
+            // survivor masks for the smallest receiver chunk of any preset (4)
+            const size_t nrec = c->rcv_columns ? cols.colslot.size() : (size_t)c->nh;
+            if ((rc = ensure(c, &c->mask, &c->mask_cap,
+                             ncell * ((nrec + 3) / 4) * ((std::max<size_t>(1, maxs) + 63) / 64))))
                 return rc;
             if ((rc = ensure(c, &c->pairs, &c->pairs_cap, (size_t)c->ns * c->nh * c->nt * 2)))
                 return rc;
@@ -1122,8 +1123,6 @@ int enqueue_tuple_prep(kmig_ctx *c, hipStream_t st, const Batch &b, size_t spec_
             HIPCHK(c, (hipError_t)launch_th_bounds(pack32 ? c->pk_s : nullptr,
                                                    pack32 ? nullptr : c->pk16_s, nsb, c->nz,
                                                    c->nx, c->bnd_s, st));
-        const int ncell = ((c->nz + BND_CZ - 1) / BND_CZ) * ((c->nx + BND_CX - 1) / BND_CX);
-        HIPCHK(c, (hipError_t)launch_cell_blocks(c->bnd_s, nsb, ncell, c->bnd_sblk, st));
         if (!fused_r)
             HIPCHK(c, (hipError_t)launch_th_bounds(pack32 ? c->pk_r : nullptr,
                                                    pack32 ? nullptr : c->pk16_r, b.nrcv, c->nz,
@@ -1199,7 +1198,7 @@ void kmig_destroy(kmig_ctx *c)
     dfree(c->pk_r);
     dfree(c->pk16_s);
     dfree(c->bnd_s);
-    dfree(c->bnd_sblk);
+    dfree(c->mask);
     dfree(c->bnd_r);
     dfree(c->pk16_r);
     dfree(c->sc_s);

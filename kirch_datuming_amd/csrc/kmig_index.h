@@ -56,5 +56,12 @@ KMIG_HD inline size_t ctab(int scell_off, int jc, int ncols, int r0, int g)
 // Trace of shot is at receiver index ih (a column's trace in that shot).
 KMIG_HD inline size_t col_row(int is, int nh, int ih) { return (size_t)is * nh + ih; }
 
+// Survivor word w (launch-local shots 64 w .. 64 w + 63) of receiver chunk k
+// in bound cell `cell`, in a mask of nchunk chunks and nwords words per cell.
+KMIG_HD inline size_t mask_word(int cell, int nchunk, int k, int nwords, int w)
+{
+    return ((size_t)cell * nchunk + k) * nwords + w;
+}
+
 }  // namespace idx
 }  // namespace kmig

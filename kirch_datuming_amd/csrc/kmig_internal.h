@@ -153,11 +153,6 @@ constexpr int BND_CZ = 16, BND_CX = 4;
 constexpr float BND_Q = 65536.0f;
 constexpr int BND_EMPTY = 1 << 28;
 
-// out[cell * nblk + b] = union over slots [b TMIN_BLK, (b + 1) TMIN_BLK) of
-// bnd[cell * nslots + slot] (min lo, max hi, min tmin), nblk = ceil(nslots /
-// TMIN_BLK)
-constexpr int TMIN_BLK = 32;
-int launch_cell_blocks(const ThBound *bnd, int nslots, int ncell, ThBound *out, void *stream);
 
 // Bounds of every (slot, cell) of FP32 (pk) or 16-bit (pk16) packed tuples,
 // cell-major: out[cell * nslots + slot], cell = cx * ncz + cz.
@@ -189,11 +184,14 @@ struct MigResArgs {
                                  // shots (entry js of a row: batch-local shot js),
                                  // or null (no culling)
     const ThBound *rbnd;         // [ncz * ncx][nrslots]
-    const ThBound *sblk;         // [ncz * ncx][scell_nblk] union of the source bounds of
-                                 // each cell over each block of TMIN_BLK batch shots
-                                 // (smallest lo and tmin, largest hi): a block culled
-                                 // with them has every shot culled
-    int scell_nblk, scell_off;   // blocks per cell; batch index of local shot 0
+    int scell_off;               // batch index of local shot 0
+    // Survivor masks of the group culls, written at the start of every launch
+    // (k_cull_mask) from sbnd and rbnd: bit j of word w of (cell, receiver
+    // chunk k) is set where launch-local shot 64 w + j may add a nonzero
+    // update at a point of the cell with a receiver of chunk k
+    // (idx::mask_word; chunks of the launch preset's R receivers, ceil(nshots
+    // / 64) words).  Null: no culling.
+    unsigned long long *mask;
     int ncz, ncx;                // cells of BND_CZ x BND_CX points
     int sbnd_stride, nrslots;    // shots per sbnd row, receiver slots
     float *spray;                // adj=0: [ns][nh][nt] traces the image is sprayed into

@@ -566,6 +566,111 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Survivor masks of the group culls for one launch (MigResArgs::mask).  For
+// bound cell c (blockIdx.x) and launch-local shots 64 w .. 64 w + 63
+// (blockIdx.y = w; lane = shot), over the receiver chunks of R receivers: a
+// group is culled where the cell's source bounds (sbnd) and the union of the
+// chunk's receiver bounds in the cell (rbnd) show that no update can add a
+// nonzero value:
+//  * no point of the cell has source or receiver weight (empty bounds);
+//  * time window: an update's last tap lies at s_int + r_int + c_int +
+//    floor(ff + delta + 1) >= s_int + r_int + c_int + 1 (ff, delta >= 0), so
+//    where the cell's smallest source plus receiver integer samples reach
+//    nt - 2 - c_int every tap pair lies past the trace's last pair;
+//  * direction: th_s - th_r lies in [dlo, dhi], and the folded opening angle
+//    min(|d|, wrap - |d|) is smallest at an end of the |d| interval; where it
+//    exceeds tq, the largest angle that receives weight at any row of the
+//    cell (plus 1e-3 degrees, far above the rounding of the kernel's angle),
+//    every update has zero weight (no_dir: per-trace bins, no direction
+//    test);
+//  * receiver columns (Cols): the chunk's columns have no trace in the shot.
+// All bounds are integers (directions x BND_Q, floor / ceil), so the tests
+// are exact; a wave whose points span several cells evaluates a group where
+// any of its cells keeps it.
+__device__ __forceinline__ bool group_culled(const ThBound &s, int rlo, int rhi, int rtm,
+                                             int t_cull, int tq, int wrapq)
+{
+    if (s.lo > s.hi || rlo > rhi) return true;
+    if (s.tmin + rtm >= t_cull) return true;
+    const int dlo = s.lo - rhi, dhi = s.hi - rlo;
+    const int amin = (dlo <= 0 && dhi >= 0) ? 0 : min(abs(dlo), abs(dhi));
+    const int bmax = max(abs(dlo), abs(dhi));
+    return min(amin, wrapq - bmax) > tq;
+}
+
+template <bool Cols>
+__global__ __launch_bounds__(64) void k_cull_mask(const MigResArgs a, int R, int no_dir)
+{
+    const int c = blockIdx.x, w = blockIdx.y, lane = threadIdx.x;
+    const int nwords = gridDim.y;
+    const int nrec = Cols ? a.ncols : a.nh;
+    const int nchunk = (nrec + R - 1) / R;
+    const int j = w * 64 + lane;
+    const bool in = j < a.nshots;
+    ThBound sb;
+    sb.lo = BND_EMPTY;
+    sb.hi = -BND_EMPTY;
+    sb.tmin = BND_EMPTY;
+    sb.pad = 0;
+    if (in) sb = a.sbnd[idx::bnd(c, (size_t)a.sbnd_stride, j)];
+    // the largest angle that receives weight at any row of the cell: the depth
+    // mute's cutoff with its taper band, capped by the bin range (rows past
+    // nz are included, which only widens it)
+    int tq = 1 << 30;
+    if (!no_dir) {
+        const int iz = (c % a.ncz) * BND_CZ + (lane % BND_CZ);
+        const float zf = (a.nz > 1) ? (float)iz / (float)(a.nz - 1) : 0.0f;
+        const float cutoff = a.oazmin + zf * (a.oazmax - a.oazmin);
+        const float plim = fminf(cutoff, a.ang_hi);
+        const float bhi = fminf(cutoff + a.oaztaper, a.ang_hi_up);
+        float tl = a.oaztaper > 0.0f ? bhi : plim;
+        for (int o = 32; o > 0; o >>= 1) tl = fmaxf(tl, __shfl_xor(tl, o, 64));
+        tl = fmaxf(fminf(tl, 1000.0f), -1000.0f);
+        tq = __builtin_amdgcn_readfirstlane((int)ceilf(tl * BND_Q)) + 64;
+    }
+    const int wrapq = (int)(a.wrap * BND_Q);
+    for (int k = 0; k < nchunk; k++) {
+        // union of the chunk's receiver bounds in the cell (lanes 0 .. R - 1)
+        const int ih = k * R + lane;
+        int rlo = BND_EMPTY, rhi = -BND_EMPTY, rtm = BND_EMPTY, cf = 1 << 30, cl = -1;
+        if (lane < R && ih < nrec) {
+            const int slot = a.rslot[ih];
+            if (slot >= 0) {
+                const ThBound rb = a.rbnd[idx::bnd(c, (size_t)a.nrslots, slot)];
+                rlo = rb.lo;
+                rhi = rb.hi;
+                rtm = rb.tmin;
+            }
+            if (Cols) {
+                cf = a.cfirst[ih];
+                cl = a.clast[ih];
+            }
+        }
+        rlo = wave_min_i32(rlo);
+        rhi = wave_max_i32(rhi);
+        rtm = wave_min_i32(rtm);
+        bool keep = in && !group_culled(sb, rlo, rhi, rtm, a.t_cull, tq, wrapq);
+        if (Cols) {
+            cf = wave_min_i32(cf);
+            cl = wave_max_i32(cl);
+            keep = keep && a.scell_off + j >= cf && a.scell_off + j <= cl;
+        }
+        const unsigned long long m = __ballot(keep);
+        if (lane == 0) a.mask[idx::mask_word(c, nchunk, k, nwords, w)] = m;
+    }
+}
+
+// The survivor masks of a launch with R receivers per chunk (before the
+// launch, on its stream).
+template <bool Cols>
+static int launch_cull_mask(const MigResArgs &a, int R, int no_dir, hipStream_t st)
+{
+    if (!a.mask || a.nshots <= 0) return 0;
+    hipLaunchKernelGGL(k_cull_mask<Cols>, dim3((unsigned)(a.ncz * a.ncx),
+                       (unsigned)((a.nshots + 63) / 64)), dim3(64), 0, st, a, R, no_dir);
+    return (int)hipGetLastError();
+}
+
 // Probe = true only renames the kernel for the autotuner's timing launches,
 // so profiler statistics of the production kernel cover production launches.
 // One tile of TZ x TX image points of one problem.
@@ -630,21 +735,18 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
     const float inv32k = 1.0f / 32768.0f;
     const float2 *__restrict__ pairs = (const float2 *)a.pairs;
 
-    // Group culling.  The direction bounds of the wave's cells give the
-    // smallest opening angle any update of a (shot, receiver chunk) group can
-    // have; where it exceeds tq, the largest angle that receives weight at
-    // any point of the wave (plus 1e-3 degrees, far above the rounding of the
-    // kernel's angle), every update of the group would add an exact zero, so
-    // the group is skipped and the sums are unchanged.  All of it is
-    // wave-uniform integer work.  A wave covers all TZ rows of WX columns;
-    // tiles start on cell boundaries, so NC cells hold its points.
+    // Group culling (survivor masks, k_cull_mask): a (shot, receiver chunk)
+    // group is evaluated only where one of the bound cells holding the
+    // wave's points has its bit set, i.e. where the group can add a nonzero
+    // update; a skipped group's updates would all add exact zeros, so the
+    // sums are unchanged.  A wave covers all TZ rows of WX columns; tiles
+    // start on cell boundaries, so NC cells hold its points.
     constexpr int WX = 64 / TZ;
     constexpr int NCZ = (TZ + BND_CZ - 1) / BND_CZ;
     constexpr int NCX = (WX + BND_CX - 1) / BND_CX;
     constexpr int NC = NCZ * NCX;
-    const bool cull = a.sbnd != nullptr;
+    const bool cull = a.mask != nullptr;
     int cell[NC];
-    int tq;
     {
         const int wv = __builtin_amdgcn_readfirstlane(lid >> 6);
         const int cz0 = (bz * TZ) / BND_CZ, cx0 = (bx * TX + wv * WX) / BND_CX;
@@ -653,35 +755,19 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
 #pragma unroll
             for (int jz = 0; jz < NCZ; jz++)
                 cell[jx * NCZ + jz] = idx::cell(cx0, jx, cz0, jz, a.ncx, a.ncz);
-        float tl = a.oaztaper > 0.0f ? bhi : plim;
-        for (int o = 32; o > 0; o >>= 1) tl = fmaxf(tl, __shfl_xor(tl, o, 64));
-        tl = fmaxf(fminf(tl, 1000.0f), -1000.0f);
-        tq = __builtin_amdgcn_readfirstlane((int)ceilf(tl * BND_Q)) + 64;
-        // per-trace bins have no directions: only the time-window tests cull
-        if constexpr (Bin == 1) tq = 1 << 30;
-    }
-    const int wrapq = (int)(a.wrap * BND_Q);
-    // smallest source sample of the wave's cells over the shot group's
-    // blocks of shots: with a chunk's smallest receiver sample it bounds
-    // every (shot, chunk) group of the chunk
-    int sall = BND_EMPTY;
-    if (cull && a.sblk && je > jb) {
-        const int b0 = (a.scell_off + jb) / TMIN_BLK, b1 = (a.scell_off + je - 1) / TMIN_BLK;
-#pragma unroll
-        for (int c = 0; c < NC; c++)
-            for (int b = b0; b <= b1; b++)
-                sall = min(sall, a.sblk[idx::bnd(cell[c], (size_t)a.scell_nblk, b)].tmin);
     }
 
     unsigned long long n_eval = 0, n_acc = 0;  // Count only (wave-uniform)
     const unsigned long long n_valid = Count ? __popcll(__ballot(valid)) : 0;
     // receiver indices (cmp=0) or receiver columns (Cols) of the chunks
     const int nrec = Cols ? a.ncols : a.nh;
+    const int nchunk = (nrec + R - 1) / R, nwords = (a.nshots + 63) >> 6;
     for (int r0 = 0; r0 < nrec; r0 += R) {
         // the chunk's shots: the shot group's, and with columns only those
-        // with a trace in one of the chunk's columns
+        // with a trace in one of the chunk's columns (the masks hold those
+        // ranges already)
         int cjb = jb, cje = je;
-        if constexpr (Cols) {
+        if (Cols && !cull) {
             int f = 1 << 30, l = -1;
             for (int r = 0; r < R && r0 + r < nrec; r++) {
                 f = min(f, a.cfirst[r0 + r]);
@@ -691,24 +777,30 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             cje = min(je, l + 1 - a.scell_off);
             if (cjb >= cje) continue;
         }
-        // bounds of the chunk's receivers over the wave's cells
-        int rlo = BND_EMPTY, rhi = -BND_EMPTY, rtm = BND_EMPTY;
-        if (cull) {
-            for (int r = 0; r < R; r++) {
-                const int ih = r0 + r;
-                const int slot = ih < nrec ? a.rslot[ih] : -1;
-                if (slot < 0) continue;
+        // Surviving shots in order, from the chunk's mask words (scalar loads,
+        // one per 64 shots and cell; the next shot is a scalar bit scan)
+        const int kc = r0 / R;
+        int wi = -1;
+        unsigned long long mw = 0;
+        auto next_shot = [&](int j) {
+            if (!cull) return j;
+            while (j < cje) {
+                const int w = j >> 6;
+                if (w != wi) {
+                    wi = w;
+                    mw = 0;
 #pragma unroll
-                for (int c = 0; c < NC; c++) {
-                    const ThBound b = a.rbnd[idx::bnd(cell[c], (size_t)a.nrslots, slot)];
-                    rlo = min(rlo, b.lo);
-                    rhi = max(rhi, b.hi);
-                    rtm = min(rtm, b.tmin);
+                    for (int c = 0; c < NC; c++)
+                        mw |= a.mask[idx::mask_word(cell[c], nchunk, kc, nwords, w)];
                 }
+                const unsigned long long m = mw & (~0ull << (j & 63));
+                if (m) return min(cje, (w << 6) + (int)__builtin_ctzll(m));
+                j = (w + 1) << 6;
             }
-            if (rlo > rhi) continue;  // no receiver weight at any point of the wave
-            if (sall + rtm >= a.t_cull) continue;  // every shot's group is past the trace
-        }
+            return cje;
+        };
+        int js = next_shot(cjb);
+        if (js >= cje) continue;  // no group of the chunk survives
         // the chunk's R receiver tuples: all loads issued first and decoded
         // after (a decode next to each load would wait for it, one memory
         // round trip per receiver); an unused receiver has zero fields
@@ -730,58 +822,6 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             if constexpr (Fmt == 1) asm("" : "+v"(rraw[r].frac));
             rt[r] = decode<Fmt>(rraw[r], a.aal);
         }
-        // Shots whose group survives the cull, in order: the tuple of the next
-        // one is loaded while the current one is processed, and a culled shot
-        // costs a few scalar instructions (its tuple is never loaded).  The
-        // bounds rows are cell-major, so consecutive shots share cache lines.
-        // the same test for one shot (bnd = sbnd, stride sbnd_stride, index
-        // j) or for a block of TMIN_BLK shots (their union bounds)
-        auto culled_by = [&](const ThBound *bnd, size_t stride, int j) -> bool {
-            int slo = BND_EMPTY, shi = -BND_EMPTY, stm = BND_EMPTY;
-#pragma unroll
-            for (int c = 0; c < NC; c++) {
-                const ThBound b = bnd[idx::bnd(cell[c], stride, j)];
-                slo = min(slo, b.lo);
-                shi = max(shi, b.hi);
-                stm = min(stm, b.tmin);
-            }
-            if (slo > shi) return true;  // no source weight at any point of the wave
-            // Time window: an update's last tap is at s_int + r_int + c_int +
-            // floor(ff + delta + 1) >= s_int + r_int + c_int + 1 (ff >= 0,
-            // delta >= 0), so where the group's smallest integer samples reach
-            // nt - 2 - c_int every tap pair lies past the trace's last pair
-            // and every update is rejected.
-            if (stm + rtm >= a.t_cull) return true;
-            // th_s - th_r lies in [dlo, dhi]; the folded angle min(|d|,
-            // wrap - |d|) is smallest at an end of the |d| interval
-            const int dlo = slo - rhi, dhi = shi - rlo;
-            const int amin = (dlo <= 0 && dhi >= 0) ? 0 : min(abs(dlo), abs(dhi));
-            const int bmax = max(abs(dlo), abs(dhi));
-            return min(amin, wrapq - bmax) > tq;
-        };
-        auto culled = [&](int j) { return culled_by(a.sbnd, a.sbnd_stride, j); };
-        // Shots in blocks of TMIN_BLK: a block whose union bounds cull it is
-        // passed in one test; a block that may hold live shots is scanned shot
-        // by shot (and not tested again).
-        int blk_open = -1;
-        auto next_shot = [&](int j) {
-            if (cull) {
-                while (j < cje) {
-                    const int bj = (a.scell_off + j) / TMIN_BLK;
-                    if (a.sblk && bj != blk_open) {
-                        if (culled_by(a.sblk, (size_t)a.scell_nblk, bj)) {
-                            j = min(cje, (bj + 1) * TMIN_BLK - a.scell_off);
-                            continue;
-                        }
-                        blk_open = bj;
-                    }
-                    if (!culled(j)) break;
-                    j++;
-                }
-            }
-            return j;
-        };
-        int js = next_shot(cjb);
         typename RawOf<Fmt>::type nxt{};
         if (js < cje) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
         while (js < cje) {
@@ -1124,6 +1164,7 @@ static int launch_ph_t(const MigResArgs &a, hipStream_t st)
     // count (tests).
     const int ntiles = ntz * ntx;
     const int nsplit = res_splits(a, ntiles);
+    if (const int e = launch_cull_mask<Cols>(a, R, Bin == 1, st)) return e;
     hipLaunchKernelGGL((k_migrate_ph<TZ, TX, R, G, Probe, Fmt, Cols, Bin>), dim3((unsigned)ntiles,
                        (unsigned)nsplit), dim3(TZ * TX), lds, st, a);
     hipError_t e = hipGetLastError();
@@ -1227,6 +1268,7 @@ static int launch_model_t(const MigResArgs &a, hipStream_t st)
     nsplit = std::min(nsplit, std::max(1, a.nshots / 8));
     if (const char *v = std::getenv("KMIG_SPLIT"))
         nsplit = std::min(std::max(1, std::atoi(v)), std::min(KMIG_MAX_SPLIT, a.nshots));
+    if (const int e = launch_cull_mask<false>(a, R, 0, st)) return e;
     hipLaunchKernelGGL((k_model_ph<TZ, TX, R, G>), dim3((unsigned)ntiles, (unsigned)nsplit),
                        dim3(TZ * TX), lds, st, a);
     return (int)hipGetLastError();
@@ -1259,6 +1301,7 @@ static int launch_count_t(const MigResArgs &a, hipStream_t st)
         if (e != hipSuccess) return (int)e;
     }
     const int ntiles = ntz * ntx;
+    if (const int e = launch_cull_mask<Cols>(a, R, Bin == 1, st)) return e;
     hipLaunchKernelGGL((k_count_ph<TZ, TX, R, G, Cols, Bin>), dim3((unsigned)ntiles,
                        (unsigned)res_splits(a, ntiles)), dim3(TZ * TX), lds, st, a);
     return (int)hipGetLastError();
@@ -1337,6 +1380,8 @@ static int launch_multi_t(const MigMultiArgs &m, hipStream_t st)
         if (const char *v = std::getenv("KMIG_SPLIT"))
             nsplit = std::min(std::max(1, std::atoi(v)), std::min(splits, min_shots));
     }
+    for (int q = 0; q < m.nprob; q++)
+        if (const int e = launch_cull_mask<false>(m.prob[q], R, 0, st)) return e;
     hipLaunchKernelGGL((k_migrate_multi<TZ, TX, R, G, Fmt>), dim3((unsigned)ntiles,
                        (unsigned)nsplit), dim3(TZ * TX), lds, st, m);
     hipError_t e = hipGetLastError();

@@ -1179,34 +1179,6 @@ __global__ __launch_bounds__(64) void k_th_bounds(const PackedTuple *__restrict_
     out[(size_t)(cx * ncz + cz) * nslots + slot] = b;
 }
 
-__global__ __launch_bounds__(64) void k_cell_blocks(const ThBound *__restrict__ bnd, int nslots,
-                                                    int nblk, ThBound *__restrict__ out)
-{
-    const ThBound *row = bnd + (size_t)blockIdx.x * nslots;
-    for (int b = threadIdx.x; b < nblk; b += 64) {
-        ThBound u;
-        u.lo = BND_EMPTY;
-        u.hi = -BND_EMPTY;
-        u.tmin = BND_EMPTY;
-        u.pad = 0;
-        for (int s = b * TMIN_BLK; s < min(nslots, (b + 1) * TMIN_BLK); s++) {
-            const ThBound e = row[s];
-            u.lo = min(u.lo, e.lo);
-            u.hi = max(u.hi, e.hi);
-            u.tmin = min(u.tmin, e.tmin);
-        }
-        out[(size_t)blockIdx.x * nblk + b] = u;
-    }
-}
-
-int launch_cell_blocks(const ThBound *bnd, int nslots, int ncell, ThBound *out, void *stream)
-{
-    if (ncell <= 0) return 0;
-    hipLaunchKernelGGL(k_cell_blocks, dim3((unsigned)ncell), dim3(64), 0, (hipStream_t)stream,
-                       bnd, nslots, (nslots + TMIN_BLK - 1) / TMIN_BLK, out);
-    return (int)hipGetLastError();
-}
-
 int launch_th_bounds(const PackedTuple *pk, const PackedTuple16 *pk16, int nslots, int nz,
                      int nx, ThBound *out, void *stream)
 {

@@ -5,7 +5,9 @@
 // It walks the launch's grid as the kernel does (tiles, waves, shot groups,
 // receiver chunks or columns, shots, receivers of a chunk) and visits every
 // shot of a chunk's range, as if no group were culled, so it covers every
-// index any cull decision can leave.  Built with g++ by tests/test_bounds_replay.py.
+// index any cull decision can leave; and the survivor-mask words the launch's
+// k_cull_mask writes and the launch reads.  Built with g++ by
+// tests/test_bounds_replay.py.
 //
 // usage: bounds_replay nz nx TZ TX R ns nh nt sb se nsplit cols unclamped tile_step
 //   cols       0: fixed receivers; 1: receivers moving with the shot on a
@@ -76,13 +78,32 @@ int main(int argc, char **argv)
     }
     const int ncz = (nz + BND_CZ - 1) / BND_CZ, ncx = (nx + BND_CX - 1) / BND_CX;
     const size_t ncell = (size_t)ncz * ncx;
-    const int nblk = (nsb + TMIN_BLK - 1) / TMIN_BLK;
-    // sizes as kmig_api.cpp allocates them
+    const int nchunk = (nrec + R - 1) / R, nwords = (nsb + 63) / 64;
+    // sizes as kmig_api.cpp allocates them (masks: for chunks of 4 receivers)
     Arr pairs{"pairs", (size_t)ns * nh * nt}, wap{"wap", (size_t)ns * nh + RES_WAP_PAD},
-        ctabA{"ctab", ctab.size()}, sbnd{"sbnd", ncell * nsb}, sblk{"sblk", ncell * nblk},
+        ctabA{"ctab", ctab.size()}, sbnd{"sbnd", ncell * nsb},
+        mask{"mask", ncell * (size_t)((nrec + 3) / 4) * (size_t)((std::max(1, nsb) + 63) / 64)},
         rbnd{"rbnd", ncell * nrslots}, stup{"stup", (size_t)nsb * nzx},
         rtup{"rtup", (size_t)nrslots * nzx}, rslot{"rslot", (size_t)nrec},
         cfA{"cfirst", (size_t)ncols}, clA{"clast", (size_t)ncols};
+    // k_cull_mask: block (cell, word), lane = shot, a loop over the chunks
+    for (size_t c = 0; c < ncell; c++)
+        for (int w = 0; w < nwords; w++) {
+            for (int lane = 0; lane < 64; lane++)
+                if (w * 64 + lane < nsb) sbnd.at(idx::bnd((int)c, (size_t)nsb, w * 64 + lane));
+            for (int k = 0; k < nchunk; k++) {
+                for (int lane = 0; lane < R && k * R + lane < nrec; lane++) {
+                    const int ih = k * R + lane;
+                    rslot.at(ih);
+                    rbnd.at(idx::bnd((int)c, (size_t)nrslots, ih));  // slot = rslot[ih]
+                    if (cols) {
+                        cfA.at(ih);
+                        clA.at(ih);
+                    }
+                }
+                mask.at(idx::mask_word((int)c, nchunk, k, nwords, w));
+            }
+        }
     const int ntz = (nz + TZ - 1) / TZ, ntx = (nx + TX - 1) / TX, ntiles = ntz * ntx;
     const int WX = 64 / TZ, NCZ = (TZ + BND_CZ - 1) / BND_CZ, NCX = (WX + BND_CX - 1) / BND_CX;
     const int nwave = TZ * TX / 64;
@@ -105,11 +126,12 @@ int main(int argc, char **argv)
                     cell[jx * NCZ + jz] = idx::cell(cx0, jx, cz0, jz, ncx, ncz);
             for (int sg = 0; sg < nsplit; sg++) {
                 const int jb = (int)((long)sg * nsb / nsplit), je = (int)((long)(sg + 1) * nsb / nsplit);
-                if (je > jb)
-                    for (int c : cell)
-                        for (int b = jb / TMIN_BLK; b <= (je - 1) / TMIN_BLK; b++)
-                            sblk.at(idx::bnd(c, (size_t)nblk, b));
                 for (int r0 = 0; r0 < nrec; r0 += R) {
+                    // the chunk's survivor words over the shot group
+                    if (je > jb)
+                        for (int c : cell)
+                            for (int w = jb >> 6; w <= (je - 1) >> 6; w++)
+                                mask.at(idx::mask_word(c, nchunk, r0 / R, nwords, w));
                     int cjb = jb, cje = je;
                     if (cols) {
                         int f = 1 << 30, l = -1;
@@ -128,14 +150,9 @@ int main(int argc, char **argv)
                         if (ih >= nrec) continue;
                         rslot.at(ih);
                         const int slot = ih;  // rslot[ih]
-                        for (int c : cell) rbnd.at(idx::bnd(c, (size_t)nrslots, slot));
                         rtup.at((size_t)slot * nzx + imax);
                     }
                     for (int j = cjb; j < cje; j++) {
-                        for (int c : cell) {
-                            sbnd.at(idx::bnd(c, (size_t)nsb, j));
-                            sblk.at(idx::bnd(c, (size_t)nblk, j / TMIN_BLK));
-                        }
                         stup.at((size_t)j * nzx + imax);
                         const int is = sb + j;
                         for (int g = 0; g < R; g++) {
@@ -161,7 +178,7 @@ int main(int argc, char **argv)
         if (tile == ntiles - 1) break;
     }
     long bad = 0;
-    for (Arr *a : {&pairs, &wap, &ctabA, &sbnd, &sblk, &rbnd, &stup, &rtup, &rslot, &cfA, &clA}) {
+    for (Arr *a : {&pairs, &wap, &ctabA, &sbnd, &mask, &rbnd, &stup, &rtup, &rslot, &cfA, &clA}) {
         printf("%s %zu %zu\n", a->name, a->hi, a->size);
         bad += a->bad;
     }
