@@ -621,27 +621,65 @@ __device__ __forceinline__ float box_masked(const float *buf, int stride, int g,
     return s / (float)(hi - lo + 1);
 }
 
+// s / (2 RS + 1) as a product with the reciprocal and one fused correction,
+// (q0 = s r, e = fma(-q0, d, s), q = fma(e, r, q0)): equal to the IEEE
+// quotient for every finite float s but -0.0 (checked over all 2^32 inputs
+// for d = 3, 5, 7, 9; a box sum is never -0.0).
+template <int RS>
+__device__ __forceinline__ float div_box(float s)
+{
+    constexpr float d = (float)(2 * RS + 1), r = 1.0f / d;
+    const float q0 = s * r;
+    const float e = __builtin_fmaf(-q0, d, s);
+    return __builtin_fmaf(e, r, q0);
+}
+
+// The interior box sum: all 2 RS + 1 taps inside the image, summed from the
+// low end from +0.0 (bitwise the clipped loop with nothing clipped).
+template <int RS>
+__device__ __forceinline__ float box_interior(const float *buf, int stride)
+{
+    float s = 0.0f;
+#pragma unroll
+    for (int t = -RS; t <= RS; t++) s += buf[t * stride];
+    return div_box<RS>(s);
+}
+
 // One smoothing pass (box along z, or along x) of both gradient components
 // over the local region [zlo, zhi) x [xlo, xhi) of the w1z-high LDS arrays
-// whose element (0, 0) is image point (gz0, gx0); image points only.
+// whose element (0, 0) is image point (gz0, gx0).  Where every window of the
+// region lies inside the image on the box's axis (decided once per block),
+// the taps are summed without masks over every element of the region: an
+// element off the image then holds a value no image element's window reads.
+// Elsewhere image points only, with masked taps.
 template <int RS, bool ZDIR>
 __device__ __forceinline__ void box_pass(const float *sx, const float *sz, float *dx, float *dz,
                                          int w1z, int zlo, int zhi, int xlo, int xhi, int gz0,
                                          int gx0, int nz, int nx)
 {
     const int wz = zhi - zlo, n = wz * (xhi - xlo);
+    const bool interior = ZDIR ? (gz0 + zlo - RS >= 0 && gz0 + zhi - 1 + RS <= nz - 1)
+                               : (gx0 + xlo - RS >= 0 && gx0 + xhi - 1 + RS <= nx - 1);
     TileWalk t(threadIdx.x, wz);
-    for (int k = threadIdx.x; k < n; k += 256, t.next()) {
-        const int lz = zlo + t.l, lx = xlo + t.m;
-        const int iz = gz0 + lz, ix = gx0 + lx;
-        if (iz < 0 || iz >= nz || ix < 0 || ix >= nx) continue;
-        const int e = lx * w1z + lz;
-        if (ZDIR) {
-            dx[e] = box_masked<RS>(sx + e, 1, iz, nz);
-            dz[e] = box_masked<RS>(sz + e, 1, iz, nz);
-        } else {
-            dx[e] = box_masked<RS>(sx + e, w1z, ix, nx);
-            dz[e] = box_masked<RS>(sz + e, w1z, ix, nx);
+    if (interior) {
+        for (int k = threadIdx.x; k < n; k += 256, t.next()) {
+            const int e = (xlo + t.m) * w1z + zlo + t.l;
+            dx[e] = box_interior<RS>(sx + e, ZDIR ? 1 : w1z);
+            dz[e] = box_interior<RS>(sz + e, ZDIR ? 1 : w1z);
+        }
+    } else {
+        for (int k = threadIdx.x; k < n; k += 256, t.next()) {
+            const int lz = zlo + t.l, lx = xlo + t.m;
+            const int iz = gz0 + lz, ix = gx0 + lx;
+            if (iz < 0 || iz >= nz || ix < 0 || ix >= nx) continue;
+            const int e = lx * w1z + lz;
+            if (ZDIR) {
+                dx[e] = box_masked<RS>(sx + e, 1, iz, nz);
+                dz[e] = box_masked<RS>(sz + e, 1, iz, nz);
+            } else {
+                dx[e] = box_masked<RS>(sx + e, w1z, ix, nx);
+                dz[e] = box_masked<RS>(sz + e, w1z, ix, nx);
+            }
         }
     }
     __syncthreads();
