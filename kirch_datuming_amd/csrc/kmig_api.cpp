@@ -889,9 +889,10 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
                 HIPCHK(c, hipMemcpy(c->wap + (size_t)sb * c->nh, wapv.data(),
                                     wapv.size() * sizeof(float), hipMemcpyHostToDevice));
             // receiver slot per receiver index (cmp=0) or per receiver column
-            const std::vector<int> &rs = c->rcv_columns ? cols.colslot : rslot;
-            if ((rc = ensure(c, &c->rslot, &c->rslot_cap, std::max<size_t>(1, rs.size()))))
-                return rc;
+            // (-1 past the last, so a chunk's R slots load without bounds tests)
+            std::vector<int> rs = c->rcv_columns ? cols.colslot : rslot;
+            rs.resize(rs.size() + RSLOT_PAD, -1);
+            if ((rc = ensure(c, &c->rslot, &c->rslot_cap, rs.size()))) return rc;
             HIPCHK(c, hipMemcpy(c->rslot, rs.data(), rs.size() * sizeof(int),
                                 hipMemcpyHostToDevice));
             c->ncols = c->rcv_columns ? (int)cols.colslot.size() : 0;

@@ -177,7 +177,8 @@ struct MigResArgs {
     const PackedTuple16 *rtup16;
     const SlotScale *sscale;     // [nshots] (batch-local), 16-bit modes
     const SlotScale *rscale;     // [slots]
-    const int *rslot;            // [nh] receiver slot of each receiver index (-1: unused)
+    const int *rslot;            // [nh + RSLOT_PAD] receiver slot of each receiver index
+                                 // (or column), -1: unused (and past the last)
     const float *wap;            // [ns][nh] trace-aperture weight, < 0: trace skipped
     const float *pairs;          // [ns][nh][nt][2] (tr[k], tr[k+1] - tr[k])
     const ThBound *sbnd;         // [ncz * ncx][sbnd_stride] direction bounds of the
@@ -243,6 +244,9 @@ struct MigResArgs {
 
 // wap entries past the last trace: at least the largest receiver chunk (R)
 constexpr int RES_WAP_PAD = 64;
+
+// rslot entries (-1) past the last receiver: at least the largest chunk (R)
+constexpr int RSLOT_PAD = 16;
 
 // Most shot groups of one resident launch.
 constexpr int KMIG_MAX_SPLIT = 16;

@@ -245,24 +245,16 @@ __device__ __forceinline__ float ramp_down(float u)
 // round-robin to the 8 XCDs, so this gives each XCD a contiguous run of tiles.
 // Off by default: measured slower on the salt workload, because contiguous
 // regions carry unequal work and the slowest XCD sets the kernel time, while
-// the plain order spreads every region over all XCDs.
+// the plain order spreads every region over all XCDs.  Clusters of 2 to 16
+// z- or x-adjacent tiles per XCD, dealt round-robin (no imbalance), were
+// measured too and gained nothing: the tap windows of concurrent neighbours
+// do not meet in the L2.
 #ifndef KMIG_XCD_MAP
 #define KMIG_XCD_MAP 0
 #endif
 
-#ifndef KMIG_XCD_CLUSTER
-#define KMIG_XCD_CLUSTER 0
-#endif
-
 __device__ __forceinline__ int xcd_tile(int b, int nb)
 {
-    if (KMIG_XCD_CLUSTER > 0) {
-        constexpr int C = KMIG_XCD_CLUSTER;
-        const int nfull = nb / (8 * C) * (8 * C);
-        if (b >= nfull) return b;
-        const int x = b & 7, k = b >> 3;
-        return ((k / C) * 8 + x) * C + (k % C);
-    }
     if (!KMIG_XCD_MAP) return b;
     const int xcd = b & 7, k = b >> 3, q = nb >> 3, rem = nb & 7;
     return xcd < rem ? xcd * (q + 1) + k : rem * (q + 1) + (xcd - rem) * q + k;
@@ -804,31 +796,58 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
         // the chunk's R receiver tuples: all loads issued first and decoded
         // after (a decode next to each load would wait for it, one memory
         // round trip per receiver); an unused receiver has zero fields
+        // The chunk's slots are one scalar load (rslot holds -1 past the last
+        // receiver); an unused receiver loads slot 0's tuple and gets zero
+        // weight, so its updates are rejected as those of a zero tuple.
+        int rsl[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) rsl[r] = a.rslot[r0 + r];
         typename RawOf<Fmt>::type rraw[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            const int ih = r0 + r;
-            const int slot = ih < nrec ? a.rslot[ih] : -1;
-            if (slot >= 0)
-                rraw[r] = load_raw<Fmt>(a, false, (size_t)slot, i);
-            else
-                rraw[r] = typename RawOf<Fmt>::type{};
-        }
+        for (int r = 0; r < R; r++) rraw[r] = load_raw<Fmt>(a, false, (size_t)max(rsl[r], 0), i);
         PackedTuple rt[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            // (the empty asm keeps the decode of a loaded tuple from moving
-            // back next to its load, where the zero tuple's decode folds away)
-            if constexpr (Fmt == 1) asm("" : "+v"(rraw[r].frac));
             rt[r] = decode<Fmt>(rraw[r], a.aal);
+            if (rsl[r] < 0) rt[r].iw = 0u;
         }
+        // The shot's traces of the chunk: trace-aperture weights, and with
+        // columns the receiver index per column (the column table's rows are
+        // padded; a padding column has zero tuple weight).  Scalar loads; for
+        // fixed receivers issued with the shot's tuple one surviving shot
+        // ahead (with columns, twice the scalar registers of a prefetched row
+        // spill, and the row is read when the shot is processed).
+        ColTrace tw[R], twn[R];  // fixed receivers only
+        auto load_traces = [&](int j) {
+            if constexpr (!Cols) {
+                const float *__restrict__ wr = a.wap + idx::wap(a.shot0 + j, a.nh, r0, 0);
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    twn[r].ih = 0;
+                    twn[r].wap = wr[r];
+                }
+            }
+        };
         typename RawOf<Fmt>::type nxt{};
-        if (js < cje) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
+        if (js < cje) {
+            nxt = load_raw<Fmt>(a, true, (size_t)js, i);
+            load_traces(js);
+        }
         while (js < cje) {
             const PackedTuple st = decode<Fmt>(nxt, a.aal);
             const int jc = js;
+            if constexpr (!Cols) {
+#pragma unroll
+                for (int r = 0; r < R; r++) tw[r] = twn[r];
+            }
+            // the column table's row of this shot (Cols)
+            const ColTrace *__restrict__ crow =
+                Cols ? a.ctab + idx::ctab(a.scell_off, jc, a.ncols, r0, 0) : nullptr;
             js = next_shot(js + 1);
-            if (js < cje) nxt = load_raw<Fmt>(a, true, (size_t)js, i);
+            if (js < cje) {
+                nxt = load_raw<Fmt>(a, true, (size_t)js, i);
+                load_traces(js);
+            }
             const bool live = valid && (st.iw & 0xFFFFu) != 0u;
             if (__ballot(live) == 0) continue;
             if constexpr (Count) n_eval += n_valid * (unsigned long long)min(R, nrec - r0);
@@ -838,13 +857,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
             const float s_w = (float)(st.iw & 0xFFFFu) * (Fmt == 1 ? inv32k * inv32k * (1.0f / 256.0f)
                                                                    : inv32k * inv32k);
             const int is = a.shot0 + jc;
-            const float *__restrict__ wrow = a.wap + idx::wap(is, a.nh, r0, 0);
             const float2 *__restrict__ trow = pairs + idx::chunk_row(is, a.nh, r0) * a.nt;
-            // the chunk's traces of this shot (Cols): receiver index and
-            // trace-aperture weight per column; the rows are padded, and a
-            // padding column has zero tuple weight
-            const ColTrace *__restrict__ crow =
-                Cols ? a.ctab + idx::ctab(a.scell_off, jc, a.ncols, r0, 0) : nullptr;
 #pragma unroll
             for (int g0 = 0; g0 < R; g0 += G) {
                 // stage 1: opening angle, bin coordinate and weight.  aw =
@@ -864,7 +877,7 @@ __device__ __forceinline__ void migrate_tile(const MigResArgs &a, int tile, floa
                 bool band = false;
 #pragma unroll
                 for (int k = 0; k < G; k++) {
-                    const float wk = Cols ? crow[g0 + k].wap : wrow[g0 + k];
+                    const float wk = Cols ? crow[g0 + k].wap : tw[g0 + k].wap;
                     base[k] = wk * s_w * (float)(rt[g0 + k].iw & 0xFFFFu);
                     if constexpr (Bin == 0) {
                         ang[k] = pair_angle(st, rt[g0 + k], a.wrap);

@@ -84,7 +84,7 @@ int main(int argc, char **argv)
         ctabA{"ctab", ctab.size()}, sbnd{"sbnd", ncell * nsb},
         mask{"mask", ncell * (size_t)((nrec + 3) / 4) * (size_t)((std::max(1, nsb) + 63) / 64)},
         rbnd{"rbnd", ncell * nrslots}, stup{"stup", (size_t)nsb * nzx},
-        rtup{"rtup", (size_t)nrslots * nzx}, rslot{"rslot", (size_t)nrec},
+        rtup{"rtup", (size_t)nrslots * nzx}, rslot{"rslot", (size_t)nrec + RSLOT_PAD},
         cfA{"cfirst", (size_t)ncols}, clA{"clast", (size_t)ncols};
     // k_cull_mask: block (cell, word), lane = shot, a loop over the chunks
     for (size_t c = 0; c < ncell; c++)
@@ -146,10 +146,11 @@ int main(int argc, char **argv)
                         if (cjb >= cje) continue;
                     }
                     for (int r = 0; r < R; r++) {
+                        // every slot of the chunk is read (-1 past the last
+                        // receiver, whose tuple load then reads slot 0)
                         const int ih = r0 + r;
-                        if (ih >= nrec) continue;
                         rslot.at(ih);
-                        const int slot = ih;  // rslot[ih]
+                        const int slot = ih < nrec ? ih : 0;  // max(rslot[ih], 0)
                         rtup.at((size_t)slot * nzx + imax);
                     }
                     for (int j = cjb; j < cje; j++) {
