@@ -275,8 +275,11 @@ int kmig_set_surrogate(kmig_ctx *ctx, int side, struct kmig_nn *nn, float max_dt
 #define KMIG_NN_FALLBACK_SLOT 0
 #define KMIG_NN_FALLBACK_POINT 1
 int kmig_set_surrogate_fallback(kmig_ctx *ctx, int side, int mode);
-/* After a run: grid points evaluated on that side, points that took the
-   network's values, and the largest |T_nn - T_table| seen. */
+/* After a run: grid points of that side's slots, points that took the
+   network's values, and the largest |T_nn - T_table| seen.  With the
+   per-slot fallback a slot whose residual already exceeds max_dt at a sample
+   of its points (every 1021st) is not evaluated further, so for such a slot the
+   residual seen is that of the sample: a lower bound, above max_dt. */
 int kmig_get_surrogate_stats(kmig_ctx *ctx, int side, double *points, double *used,
                              float *max_residual);
 

@@ -515,6 +515,14 @@ struct NnInferArgs {
     // (T and A stay the tables'); k_nn_select then takes them for the slots
     // whose largest residual is within max_dt
     float *Tn, *An;
+    // Point p of the launch is grid point p * pstride (1: every point).
+    // sample_only: the residuals only (maxres), no T / A / Tn / An / used.
+    // skip (per-slot fallback): a tile whose slots all have skip[slot] above
+    // skip_lim (a residual already past the limit) is not evaluated.
+    long pstride;
+    int sample_only;
+    const unsigned *skip;
+    unsigned skip_lim;
 };
 
 template <bool Tuples>
@@ -530,15 +538,17 @@ __global__ __launch_bounds__(256) void k_nn_infer(const NnInferArgs a)
     const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
     const long ntiles = (a.npts + 63) / 64;
     const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6), nwaves = (long)gridDim.x * 4;
-    for (long tile = wave; tile < ntiles; tile += nwaves) {
+    const long stride = Tuples ? a.pstride : 1, nphys = Tuples ? a.npts * stride : a.npts;
+    auto run_tile = [&](const long tile) {
         // the lane's four points n = 16 b + r
         float u[4][3], dist[4];
         long pidx[4];
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             long p = tile * 64 + 16 * b + r;
-            pidx[b] = p;
             if (p >= a.npts) p = a.npts - 1;
+            p *= stride;  // the grid point
+            pidx[b] = (tile * 64 + 16 * b + r) < a.npts ? p : nphys;
             float s, z, x;
             if (Tuples) {
                 const int slot = (int)(p / a.nzx);
@@ -622,13 +632,14 @@ __global__ __launch_bounds__(256) void k_nn_infer(const NnInferArgs a)
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             const long p = pidx[b];
-            if (q != 0 || p >= a.npts) continue;
+            if (q != 0 || p >= nphys) continue;
             const float S0 = a.sc.S0;
             const float T = dist[b] * S0 * (1.0f + y[b][0] + b5[0]);
             const float Ts = S0 * (y[b][3] + b5[3]);
             if (Tuples) {
                 float res = fabsf(T - a.T[p]);
-                if (a.Tn) {
+                if (a.sample_only) {
+                } else if (a.Tn) {
                     a.Tn[p] = T;
                     a.An[p] = fabsf(Ts * a.dside);
                 } else if (res <= a.max_dt) {
@@ -648,7 +659,7 @@ __global__ __launch_bounds__(256) void k_nn_infer(const NnInferArgs a)
         if (Tuples) {
             // per-slot counters: one atomic per wave when its 64 points share
             // a slot, per point otherwise (tiles across a slot boundary)
-            const long pf = tile * 64, pl = min(pf + 63, a.npts - 1);
+            const long pf = tile * 64 * stride, pl = min(tile * 64 + 63, a.npts - 1) * stride;
             const int s0 = (int)(pf / a.nzx), s1 = (int)(pl / a.nzx);
             if (s0 == s1) {
                 int cnt = rep[0] + rep[1] + rep[2] + rep[3];
@@ -665,12 +676,38 @@ __global__ __launch_bounds__(256) void k_nn_infer(const NnInferArgs a)
             } else if (q == 0) {
 #pragma unroll
                 for (int b = 0; b < 4; b++) {
-                    if (pidx[b] >= a.npts) continue;
+                    if (pidx[b] >= nphys) continue;
                     const int slot = (int)(pidx[b] / a.nzx);
                     if (rep[b]) atomicAdd(a.used + slot, 1);
                     atomicMax(a.maxres + slot, __float_as_uint(resb[b]));
                 }
             }
+        }
+    };
+    if (!(Tuples && a.skip)) {
+        for (long tile = wave; tile < ntiles; tile += nwaves) run_tile(tile);
+        return;
+    }
+    // Only the slots whose residual is still within the limit: the wave
+    // reads 64 slots' verdicts at a time and runs its own tiles (tile index =
+    // wave mod nwaves) of each slot taken.  A tile across the boundary of two
+    // taken slots runs once, with the first; a slot this wave reads as taken
+    // may have failed since, which costs only time (its values are not used).
+    const int nslots = (int)((a.npts + a.nzx - 1) / a.nzx);
+    int last = -2;  // the last slot taken
+    for (int sb = 0; sb < nslots; sb += 64) {
+        const int sl = sb + lane;
+        unsigned long long m = __ballot(sl < nslots && a.skip[sl] <= a.skip_lim);
+        while (m) {
+            const int s = sb + __builtin_ctzll(m);
+            m &= m - 1;
+            const long pf = (long)s * a.nzx, pl = min(pf + a.nzx, a.npts) - 1;
+            long t0 = pf / 64;
+            const long t1 = pl / 64;
+            if (last == s - 1 && pf % 64 != 0) t0++;
+            last = s;
+            long tile = t0 + ((wave - t0) % nwaves + nwaves) % nwaves;
+            for (; tile <= t1; tile += nwaves) run_tile(tile);
         }
     }
 }
@@ -679,16 +716,20 @@ __global__ __launch_bounds__(256) void k_nn_infer(const NnInferArgs a)
 // non-negative floats order as their bits) is within max_dt takes the
 // network's T and A at every point, any other keeps its tables at every
 // point; used[slot] becomes the slot's points taken from the network.
+// Grid: x blocks of 256 threads over a slot's nzx points (a thread takes
+// points i, i + 256 x, ...), y = slot.
 __global__ __launch_bounds__(256) void k_nn_select(const float *__restrict__ Tn,
                                                    const float *__restrict__ An, float *T, float *A,
                                                    const unsigned *__restrict__ maxres,
-                                                   unsigned lim_bits, long npts, long nzx)
+                                                   unsigned lim_bits, long nzx)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npts) return;
-    if (maxres[p / nzx] <= lim_bits) {
-        T[p] = Tn[p];
-        A[p] = An[p];
+    const int slot = blockIdx.y;
+    if (maxres[slot] > lim_bits) return;
+    const long base = (long)slot * nzx;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nzx;
+         i += (long)gridDim.x * blockDim.x) {
+        T[base + i] = Tn[base + i];
+        A[base + i] = An[base + i];
     }
 }
 
@@ -698,6 +739,10 @@ __global__ __launch_bounds__(256) void k_nn_slot_used(int *used, const unsigned 
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < nslots) used[s] = maxres[s] <= lim_bits ? nzx : 0;
 }
+
+// Grid-point stride of the residual pre-check of the per-slot fallback: a
+// prime, so that the sample walks through every depth row of the grid.
+constexpr long kNnSampleStride = 1021;
 
 static size_t infer_lds() { return NN_BLOB_BF16 * sizeof(bf16) + NN_BLOB_BIAS * sizeof(float); }
 
@@ -981,12 +1026,34 @@ int nn_replace_tuples(const kmig_nn *n, const SlotSpec *spec, int nslots, int nz
     a.maxres = maxres;
     a.Tn = Tn;
     a.An = An;
-    int e = launch_infer(a, true, st);
-    if (e || !Tn) return e;
+    a.pstride = 1;
     unsigned lim;
     memcpy(&lim, &max_dt, sizeof(lim));
-    hipLaunchKernelGGL(k_nn_select, dim3(nblk((size_t)a.npts, 256)), dim3(256), 0, st, Tn, An, T,
-                       A, maxres, lim, a.npts, a.nzx);
+    const char *pre = getenv("KMIG_NN_PRECHECK");
+    if (Tn && !(pre && pre[0] == '0')) {
+        // Per-slot fallback: a slot takes the network only if every one of
+        // its points is within max_dt, so a slot that misses it at a sample
+        // of its points (every kNnSampleStride-th grid point) misses it
+        // overall, and the full evaluation runs only for the other slots.
+        // (Their maxres is then exact; a skipped slot's is the sample's.)
+        NnInferArgs sa = a;
+        sa.pstride = kNnSampleStride;
+        sa.npts = (a.npts + kNnSampleStride - 1) / kNnSampleStride;
+        sa.sample_only = 1;
+        int es = launch_infer(sa, true, st);
+        if (es) return es;
+        a.skip = maxres;
+        a.skip_lim = lim;
+    }
+    int e = launch_infer(a, true, st);
+    if (e || !Tn) return e;
+    const unsigned sx = std::max(1u, std::min(nblk((size_t)a.nzx, 256), 64u));
+    for (int s0 = 0; s0 < nslots; s0 += 65535) {
+        const long off = (long)s0 * a.nzx;
+        hipLaunchKernelGGL(k_nn_select, dim3(sx, (unsigned)std::min(nslots - s0, 65535)),
+                           dim3(256), 0, st, Tn + off, An + off, T + off, A + off, maxres + s0,
+                           lim, a.nzx);
+    }
     hipLaunchKernelGGL(k_nn_slot_used, dim3(nblk((size_t)nslots, 256)), dim3(256), 0, st, used,
                        maxres, lim, nslots, (int)nzx);
     return (int)hipGetLastError();

@@ -1,6 +1,8 @@
 """GPU checks of the learned traveltime surrogate (kmig_nn.hip) against the
 CPU oracle (oracle/mlp.py): BF16-MFMA forward pass, loss and parameter
 gradient on the same BF16 operands, and training on v(z) traveltimes."""
+import os
+
 import numpy as np
 import pytest
 
@@ -161,6 +163,25 @@ def test_surrogate_in_the_migration():
     print(f"per-slot fallback at 1 ms: {frac_s:.4f} of the points (whole slots) from the "
           f"network, CIG relL2 vs tables {_rel(cig_s, cig_t):.3f}; with every slot "
           f"{_rel(cig_all_s, cig_t):.3f}")
+    # The per-slot fallback first evaluates every 1021st point and skips the
+    # slots already past the limit there (KMIG_NN_PRECHECK=0 turns that off):
+    # at a limit where some slots take the network and some do not, the CIG
+    # and the points taken are the same with and without it, bit for bit.
+    try:
+        for lim in (1.5e-3, 2e-3, 2.5e-3, 3e-3, 4e-3):
+            os.environ["KMIG_NN_PRECHECK"] = "0"
+            cig_f, st_f, _ = run(net, per_slot=True, max_dt=lim)
+            frac_f = sum(q["used"] for q in st_f) / sum(q["points"] for q in st_f)
+            if 0.0 < frac_f < 1.0:
+                break
+        del os.environ["KMIG_NN_PRECHECK"]
+        cig_p, st_p, _ = run(net, per_slot=True, max_dt=lim)
+    finally:
+        os.environ.pop("KMIG_NN_PRECHECK", None)
+    print(f"per-slot fallback at {lim:.1e} s: {frac_f:.4f} of the points from the network")
+    assert 0.0 < frac_f < 1.0
+    assert np.array_equal(cig_p.view(np.uint32), cig_f.view(np.uint32))
+    assert [q["used"] for q in st_p] == [q["used"] for q in st_f]
     assert np.array_equal(cig_off, cig_t)
     assert 4 * nn.PARAMS < 0.01 * 4 * 2 * (full.stab.size + full.rtab.size)
     untrained.close()
