@@ -873,7 +873,8 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
             if ((rc = ensure(c, &c->mask, &c->mask_cap,
                              ncell * ((nrec + 3) / 4) * ((std::max<size_t>(1, maxs) + 63) / 64))))
                 return rc;
-            if ((rc = ensure(c, &c->pairs, &c->pairs_cap, (size_t)c->ns * c->nh * c->nt * 2)))
+            if ((rc = ensure(c, &c->pairs, &c->pairs_cap,
+                             trace_alloc((size_t)c->ns * c->nh, c->nt) * 2)))
                 return rc;
             // The resident kernels read a whole receiver chunk of wap
             // entries: past nh in the last chunk of a shot, i.e. the first
@@ -1355,13 +1356,13 @@ int kmig_set_data(kmig_ctx *c, const float *data, int nt, float t0, float dt, in
     char err[512] = {0};
     int rc = kmig_params_check(&c->p, ns, err, sizeof(err));
     if (rc) return fail(c, rc, "%s", err);
-    const size_t n = (size_t)nt * nh * ns;
+    const size_t n = (size_t)nt * nh * ns, npad = trace_alloc((size_t)nh * ns, nt);
     size_t cap = c->raw_cap;
     // a run still in flight may read the traces being replaced
     HIPCHK(c, hipStreamSynchronize(c->stream2));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if ((rc = ensure(c, &c->raw, &c->raw_cap, n))) return rc;
-    if ((rc = ensure(c, &c->prep, &cap, n))) return rc;
+    if ((rc = ensure(c, &c->raw, &c->raw_cap, npad))) return rc;
+    if ((rc = ensure(c, &c->prep, &cap, npad))) return rc;
     HIPCHK(c, hipMemcpy(c->raw, data, n * sizeof(float), hipMemcpyHostToDevice));
     c->nt = nt;
     c->t0 = t0;
@@ -1741,10 +1742,10 @@ int kmig_model(kmig_ctx *c, const float *image, int nt, float t0, float dt, int 
                     "rgradz=, or provide none of them.");
     // the data axes of the adj=0 keys (src/kirchmig2d_cig.cpp:663-701) and the
     // trace buffers: raw takes the spray, prep the doubly integrated traces
-    const size_t n = (size_t)nt * nh * ns;
+    const size_t n = (size_t)nt * nh * ns, npad = trace_alloc((size_t)nh * ns, nt);
     size_t cap = c->raw_cap;
-    if ((rc = ensure(c, &c->raw, &c->raw_cap, n))) return rc;
-    if ((rc = ensure(c, &c->prep, &cap, n))) return rc;
+    if ((rc = ensure(c, &c->raw, &c->raw_cap, npad))) return rc;
+    if ((rc = ensure(c, &c->prep, &cap, npad))) return rc;
     c->nt = nt;
     c->t0 = t0;
     c->dt = dt;

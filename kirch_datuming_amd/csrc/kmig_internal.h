@@ -270,10 +270,23 @@ struct MigMultiArgs {
     MigResArgs prob[KMIG_MAX_PROBLEMS];
 };
 
+// Elements of the raw, prepared and pair buffers for ntraces traces of nt
+// samples: TRACE_PAD traces and 64 samples more, never written and never
+// used (the trace preparation loads a block's 64 rows of a chunk of 32
+// samples unclamped: from a block's first trace at most 64 rows plus 31
+// samples past the last valid trace's end).
+constexpr int TRACE_PAD = 64;
+inline size_t trace_alloc(size_t ntraces, int nt)
+{
+    return (ntraces + TRACE_PAD) * (size_t)nt + 64;
+}
+
 // Launchers (kmig_prep.hip, kmig_migrate.hip).  Return 0 or a hipError_t value.
 int launch_trace_prep(const TracePrepArgs &a, void *stream);
-// coalesced trace preparation; pairs (nullable) gets (tr[k], tr[k+1] - tr[k]); tmp is a
-// trace-sized scratch volume used only when diff=1
+// coalesced trace preparation; pairs (nullable) gets (tr[k], tr[k+1] - tr[k]); prep
+// keeps the prepared samples only where pairs is null or diff=1 (otherwise it is
+// left holding the causal sums); tmp is a trace-sized scratch volume used only
+// when diff=1
 int launch_trace_prep2(const TracePrepArgs &a, float *pairs, float *tmp, void *stream);
 int launch_tuple_prep(const TuplePrepArgs &a, Tuples &out, float *tmp1, float *tmp2,
                       void *stream);

@@ -85,123 +85,236 @@ __global__ __launch_bounds__(256) void k_trace_prep(const TracePrepArgs a)
     }
 }
 
-// Coalesced form of the trace preparation: a block of 64 lanes owns 64
-// traces and walks them in chunks of 64 samples staged through LDS, so every
-// global access is a 256-byte row while each lane still forms its own trace's
-// running sums sequentially (the reference's rounding order).
+// Coalesced form of the trace preparation: a block of one wave owns TR = 64
+// traces and walks them in chunks of TS = 32 samples staged through LDS, so
+// every global access is a 128-byte row (two rows per load instruction)
+// while each lane still forms its own trace's running sums sequentially (the
+// reference's rounding order).  The next chunk's 32 loads are issued before
+// the current chunk is scanned, all at once, so they overlap the scan and
+// the stores; the swizzled tile (below) is 8 KB, so every block of a salt
+// batch (4,812) is resident at once.
 //   k_trace_fwd: mute (+QC copies) and the causal sum of doubint
-//   k_trace_bwd: the anticausal sum, written as samples and, when asked for,
-//                as (tr[k], tr[k+1] - tr[k]) pairs
-constexpr int TB = 64;  // traces per block = samples per chunk
+//   k_trace_bwd: the anticausal sum, written as samples (when asked for) and
+//                as (tr[k], tr[k+1] - tr[k]) pairs (when asked for)
+constexpr int TR = 64;  // traces per block (lane t scans trace t)
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+constexpr int TS = 32;  // samples per chunk
 
-__global__ __launch_bounds__(TB) void k_trace_fwd(const TracePrepArgs a)
+// Element (trace j, sample s) of a TR x TS chunk: rows of TS floats with the
+// sample index XORed by j / 2, so both access patterns are free of bank
+// conflicts: a row-wise load (lanes 0-31 row j, lanes 32-63 row j + 1: banks
+// (s ^ j/2) and 32 + (s ^ j/2)) and the scan (lane t at row t, one sample:
+// banks 32 (t & 1) + (k ^ t/2), all distinct).
+__device__ __forceinline__ int chunk_at(int j, int s) { return j * TS + (s ^ (j >> 1)); }
+
+// The direct-wave mute weight of sample it (src/kirchmig2d_cig.cpp:110-129).
+__device__ __forceinline__ float mute_weight(const TracePrepArgs &a, float boundary, int it)
 {
-    __shared__ float tile[TB][TB + 1];
-    const int t = threadIdx.x;
+    const float time = a.t0 + (float)it * a.dt;
+    float w = 0.0f;
+    if (time >= boundary + a.taper) {
+        w = 1.0f;
+    } else if (time > boundary) {
+        const float u = (time - boundary) / a.taper;
+        w = 0.5f * (1.0f - cosf((float)KPI * u));
+    }
+    return w;
+}
+
+// The chunk of samples c0 .. c0 + TS - 1 of the block's traces into
+// registers: lane (h, s) = (t / 32, t % 32) takes sample c0 + s of trace
+// 2 i + h (base = the block's first trace).  Every lane loads, with no clamp
+// and no branch, so all TR / 2 loads are in flight together: trace buffers
+// are trace_alloc() long, which covers a block's TR rows from any valid first
+// trace plus the TS - 1 samples a chunk reaches past a row's end (values past
+// nloc traces or nt samples are never used).  Buffer loads: the lane's part of the offset, (h nt + s)
+// samples, is one register for all TR / 2 loads, and the row and chunk part,
+// (2 i nt + c0) samples, is a scalar (flat loads would hold a 64-bit address
+// per row).  Offsets within a block's rows fit in 32 bits (TR x nt samples).
+// (The same split of offsets serves the stores: a block's rows are a buffer
+// from its first trace, with no range limit; every access is inside the
+// allocation by construction.)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const void *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, -1, 0x00020000);
+}
+
+__device__ __forceinline__ void load_chunk(float (&v)[TR / 2], const float *base, int nt, int c0,
+                                           int h, int s)
+{
+    const __amdgpu_buffer_rsrc_t rs = rows_rsrc(base);
+    const int voff = (h * nt + s) * (int)sizeof(float);
+#pragma unroll
+    for (int i = 0; i < TR / 2; i++)
+        v[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                             rs, voff, (2 * i * nt + c0) * (int)sizeof(float), 0));
+}
+
+__global__ __launch_bounds__(TR, 5) void k_trace_fwd(const TracePrepArgs a)
+{
+    __shared__ float tile[TR * TS];
+    const int t = threadIdx.x, h = t >> 5, s = t & (TS - 1);
+    const int nt = a.nt;
     const long ntr = (long)(a.shot_end - a.shot_begin) * a.nh;
-    const long g0 = (long)a.shot_begin * a.nh + (long)blockIdx.x * TB;
-    const int nloc = (int)min((long)TB, ntr - (long)blockIdx.x * TB);
-    const long gt = g0 + t;
+    const long g0 = (long)a.shot_begin * a.nh + (long)blockIdx.x * TR;
+    const int nloc = (int)min((long)TR, ntr - (long)blockIdx.x * TR);
     const bool mine = t < nloc;
     TraceMeta m{};
-    if (mine) m = a.meta[gt];
+    if (mine) m = a.meta[g0 + t];
+    const bool keep = m.wap >= 0.0f;
     const float offset_time = a.slope * fabsf(m.rcv_x - m.src_x);
     const float boundary = a.invert ? a.intercept - offset_time : a.intercept + offset_time;
+    // the block's rows [qlo, qhi) of the QC shot, whose trace j is the shot's
+    // receiver j - qoff
     bool any_qc = false;
+    int qlo = 0, qhi = 0;
+    long qoff = 0;
     if (a.mute && a.qc_shot >= 0) {
         const long q0 = (long)a.qc_shot * a.nh, q1 = q0 + a.nh;
         any_qc = g0 < q1 && g0 + nloc > q0;
+        qoff = q0 - g0;
+        qlo = (int)max(0L, qoff);
+        qhi = (int)min((long)nloc, q1 - g0);
     }
+    // QC copies of the chunk's rows of the QC shot (before or after the mute)
+    auto qc_copy = [&](float *dst, int c0, int nc) {
+        for (int j = qlo + h; j < qhi; j += 2)
+            if (s < nc) dst[(size_t)(j - qoff) * nt + c0 + s] = tile[chunk_at(j, s)];
+    };
+    const float *__restrict__ raw = a.raw + g0 * nt;
+    float *__restrict__ prep = a.prep + g0 * nt;
+    float pre[TR / 2];
+    load_chunk(pre, raw, nt, 0, h, s);
     float tt = 0.0f;
-    for (int c0 = 0; c0 < a.nt; c0 += TB) {
-        const int nc = min(TB, a.nt - c0);
-        for (int j = 0; j < nloc; j++)
-            if (t < nc) tile[j][t] = a.raw[(g0 + j) * a.nt + c0 + t];
+    for (int c0 = 0; c0 < nt; c0 += TS) {
+        const int nc = min(TS, nt - c0);
+        // the tile addresses of the row-wise accesses are formed anew in each
+        // chunk (one XOR each), not held in registers across chunks
+        int sv = s;
+        asm volatile("" : "+v"(sv));
+#pragma unroll
+        for (int i = 0; i < TR / 2; i++) tile[chunk_at(2 * i + h, sv)] = pre[i];
         __syncthreads();
-        if (any_qc)
-            for (int j = 0; j < nloc; j++) {
-                const long g = g0 + j;
-                if (g / a.nh == a.qc_shot && t < nc)
-                    a.qcb[(size_t)(g % a.nh) * a.nt + c0 + t] = tile[j][t];
-            }
-        if (mine && a.mute) {
-            for (int k = 0; k < nc; k++) {
-                const float time = a.t0 + (float)(c0 + k) * a.dt;
-                float w = 0.0f;
-                if (time >= boundary + a.taper) {
-                    w = 1.0f;
-                } else if (time > boundary) {
-                    const float u = (time - boundary) / a.taper;
-                    w = 0.5f * (1.0f - cosf((float)KPI * u));
-                }
-                tile[t][k] *= w;
-            }
-        }
-        __syncthreads();
-        if (any_qc)
-            for (int j = 0; j < nloc; j++) {
-                const long g = g0 + j;
-                if (g / a.nh == a.qc_shot && t < nc)
-                    a.qca[(size_t)(g % a.nh) * a.nt + c0 + t] = tile[j][t];
-            }
-        if (mine && m.wap >= 0.0f) {
-            for (int k = 0; k < nc; k++) {
-                const float v = tile[t][k];
+        if (c0 + TS < nt) load_chunk(pre, raw, nt, c0 + TS, h, s);
+        // mute, then the causal running sum, of the lane's own trace
+        auto step = [&](int k, bool do_mute, bool do_sum) {
+            float v = tile[chunk_at(t, k)];
+            if (do_mute) v *= mute_weight(a, boundary, c0 + k);
+            if (do_sum) {
                 tt = (c0 + k == 0) ? v : tt + v;
-                tile[t][k] = tt;
+                v = tt;
+            }
+            tile[chunk_at(t, k)] = v;
+        };
+        if (any_qc) {
+            qc_copy(a.qcb, c0, nc);
+            __syncthreads();
+            if (mine && a.mute)
+                for (int k = 0; k < nc; k++) step(k, true, false);
+            __syncthreads();
+            qc_copy(a.qca, c0, nc);
+            __syncthreads();
+            if (mine && keep)
+                for (int k = 0; k < nc; k++) step(k, false, true);
+        } else if (mine && (a.mute || keep)) {
+            const bool mu = a.mute != 0;
+            if (nc == TS) {
+#pragma unroll 8
+                for (int k = 0; k < TS; k++) step(k, mu, keep);
+            } else {
+                for (int k = 0; k < nc; k++) step(k, mu, keep);
             }
         }
         __syncthreads();
-        for (int j = 0; j < nloc; j++)
-            if (t < nc) a.prep[(g0 + j) * a.nt + c0 + t] = tile[j][t];
+        {
+            const __amdgpu_buffer_rsrc_t rs = rows_rsrc(prep);
+            const int voff = (h * nt + s) * (int)sizeof(float);
+#pragma unroll 8
+            for (int i = 0; i < TR / 2; i++) {
+                const int j = 2 * i + h;
+                if (j < nloc && s < nc)
+                    __builtin_amdgcn_raw_buffer_store_b32(
+                        __builtin_bit_cast(int, tile[chunk_at(j, sv)]), rs, voff,
+                        (2 * i * nt + c0) * (int)sizeof(float), 0);
+            }
+        }
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(TB) void k_trace_bwd(const TracePrepArgs a, float2 *pairs)
+// write_prep: the anticausal sums also go back to prep (the stream kernel
+// and diff reads them; the resident kernels read only the pairs).
+__global__ __launch_bounds__(TR, 5) void k_trace_bwd(const TracePrepArgs a, float2 *pairs,
+                                                  int write_prep)
 {
-    __shared__ float tile[TB][TB + 1];
-    __shared__ float bnd[TB];
-    const int t = threadIdx.x;
+    __shared__ float tile[TR * TS];
+    __shared__ float bnd[TR];  // first sample of the chunk after this one
+    const int t = threadIdx.x, h = t >> 5, s = t & (TS - 1);
+    const int nt = a.nt;
     const long ntr = (long)(a.shot_end - a.shot_begin) * a.nh;
-    const long g0 = (long)a.shot_begin * a.nh + (long)blockIdx.x * TB;
-    const int nloc = (int)min((long)TB, ntr - (long)blockIdx.x * TB);
+    const long g0 = (long)a.shot_begin * a.nh + (long)blockIdx.x * TR;
+    const int nloc = (int)min((long)TR, ntr - (long)blockIdx.x * TR);
     const bool mine = t < nloc && a.meta[g0 + t].wap >= 0.0f;
+    const int last = ((nt - 1) / TS) * TS;
+    float *prep = a.prep + g0 * nt;
+    float2 *pr = pairs ? pairs + g0 * nt : nullptr;
+    float pre[TR / 2];
+    load_chunk(pre, prep, nt, last, h, s);
     float tt = 0.0f;
-    const int last = ((a.nt - 1) / TB) * TB;
-    for (int c0 = last; c0 >= 0; c0 -= TB) {
-        const int nc = min(TB, a.nt - c0);
-        for (int j = 0; j < nloc; j++)
-            if (t < nc) tile[j][t] = a.prep[(g0 + j) * a.nt + c0 + t];
+    for (int c0 = last; c0 >= 0; c0 -= TS) {
+        const int nc = min(TS, nt - c0);
+        int sv = s;  // as in k_trace_fwd
+        asm volatile("" : "+v"(sv));
+#pragma unroll
+        for (int i = 0; i < TR / 2; i++) tile[chunk_at(2 * i + h, sv)] = pre[i];
         __syncthreads();
+        if (c0 > 0) load_chunk(pre, prep, nt, c0 - TS, h, s);
         if (mine) {
-            for (int k = nc - 1; k >= 0; k--) {
+            auto step = [&](int k) {
                 const int it = c0 + k;
-                if (it == a.nt - 1) {
-                    tt = tile[t][k];
+                if (it == nt - 1) {
+                    tt = tile[chunk_at(t, k)];
                 } else {
-                    tt += tile[t][k];
-                    tile[t][k] = tt;
+                    tt += tile[chunk_at(t, k)];
+                    tile[chunk_at(t, k)] = tt;
+                }
+            };
+            if (nc == TS) {
+#pragma unroll 8
+                for (int k = TS - 1; k >= 0; k--) step(k);
+            } else {
+                for (int k = nc - 1; k >= 0; k--) step(k);
+            }
+        }
+        __syncthreads();
+        {
+            const __amdgpu_buffer_rsrc_t rs = rows_rsrc(prep), rp = rows_rsrc(pr);
+            const int voff = (h * nt + s) * (int)sizeof(float);
+#pragma unroll 8
+            for (int i = 0; i < TR / 2; i++) {
+                const int j = 2 * i + h;
+                if (j < nloc && s < nc) {
+                    const int so = 2 * i * nt + c0;  // samples
+                    const float v = tile[chunk_at(j, sv)];
+                    if (write_prep)
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), rs, voff,
+                                                              so * (int)sizeof(float), 0);
+                    if (pr) {
+                        const float nx = (s + 1 < nc) ? tile[chunk_at(j, sv + 1)]
+                                                      : (c0 + s + 1 < nt ? bnd[j] : 0.0f);
+                        // the last pair of a trace is (0, 0): no update reads
+                        // it, and rejected updates are pointed at it
+                        const float2 q = c0 + s + 1 < nt ? make_float2(v, nx - v)
+                                                         : make_float2(0.0f, 0.0f);
+                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, q), rp,
+                                                              2 * voff,
+                                                              so * (int)sizeof(float2), 0);
+                    }
                 }
             }
         }
         __syncthreads();
-        for (int j = 0; j < nloc; j++) {
-            if (t < nc) {
-                const float v = tile[j][t];
-                a.prep[(g0 + j) * a.nt + c0 + t] = v;
-                if (pairs) {
-                    const float nx = (t + 1 < nc) ? tile[j][t + 1]
-                                                  : (c0 + t + 1 < a.nt ? bnd[j] : 0.0f);
-                    // the last pair of a trace is (0, 0): no update reads
-                    // it, and rejected updates are pointed at it
-                    pairs[(g0 + j) * a.nt + c0 + t] =
-                        c0 + t + 1 < a.nt ? make_float2(v, nx - v) : make_float2(0.0f, 0.0f);
-                }
-            }
-        }
-        __syncthreads();
-        if (t < nloc) bnd[t] = tile[t][0];
+        if (t < nloc) bnd[t] = tile[chunk_at(t, 0)];
         __syncthreads();
     }
 }
@@ -913,11 +1026,13 @@ int launch_trace_prep2(const TracePrepArgs &a, float *pairs, float *tmp, void *s
     const long ntr = (long)(a.shot_end - a.shot_begin) * a.nh;
     if (ntr <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
-    const unsigned nb = (unsigned)((ntr + TB - 1) / TB);
-    hipLaunchKernelGGL(k_trace_fwd, dim3(nb), dim3(TB), 0, st, a);
+    const unsigned nb = (unsigned)((ntr + TR - 1) / TR);
+    hipLaunchKernelGGL(k_trace_fwd, dim3(nb), dim3(TR), 0, st, a);
     const bool diff = a.diff == 1;
-    hipLaunchKernelGGL(k_trace_bwd, dim3(nb), dim3(TB), 0, st, a,
-                       diff ? (float2 *)nullptr : (float2 *)pairs);
+    // without diff the pairs (when asked for) are the last output: the
+    // samples go back to prep only for the callers that read them
+    hipLaunchKernelGGL(k_trace_bwd, dim3(nb), dim3(TR), 0, st, a,
+                       diff ? (float2 *)nullptr : (float2 *)pairs, (diff || !pairs) ? 1 : 0);
     if (diff) {
         // diff2 into tmp, back into prep, then the pairs from the final trace
         const size_t off = (size_t)a.shot_begin * a.nh * a.nt, n = (size_t)ntr * a.nt;

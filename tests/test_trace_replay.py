@@ -1,0 +1,249 @@
+"""CPU replay of the coalesced trace preparation (k_trace_fwd, k_trace_bwd in
+kirch_datuming_amd/csrc/kmig_prep.hip; no GPU).
+
+The replay walks each block as the kernels do -- 64 traces per block, chunks
+of 32 samples loaded as rows 2 i + h at offsets (2 i nt + c0) + (h nt + s)
+with no clamp, the swizzled tile j * 32 + (s ^ j / 2), the per-lane scans,
+the chunk-boundary sample of the backward pass -- and checks (1) every load
+against the buffer length the library allocates (trace_alloc: 64 traces and
+64 samples past the last trace), (2) that the swizzle is a permutation whose
+row-wise and per-lane accesses are free of bank conflicts, and (3) the
+prepared samples, the pairs and the QC copies against a plain per-trace
+statement of the same float32 arithmetic (src/kirchmig2d_cig.cpp:110-129,
+SERECKIRCH/src/kirmig.cpp:8-24).  Geometries: trace counts around the block
+size, nt below, at and around the chunk size, a shot range that starts
+inside the volume, skipped traces (wap < 0) and the QC shot."""
+import numpy as np
+import pytest
+
+TR, TS, PAD = 64, 32, 64
+f32 = np.float32
+
+
+def trace_alloc(ntraces, nt):
+    return (ntraces + PAD) * nt + 64
+
+
+def chunk_at(j, s):
+    return j * TS + (s ^ (j >> 1))
+
+
+def mute_w(t0, dt, boundary, taper, it):
+    time = f32(f32(t0) + f32(f32(it) * f32(dt)))
+    if time >= f32(boundary + taper):
+        return f32(1.0)
+    if time > boundary:
+        u = f32((time - boundary) / f32(taper))
+        return f32(f32(0.5) * f32(f32(1.0) - f32(np.cos(f32(np.pi) * u))))
+    return f32(0.0)
+
+
+class Geo:
+    def __init__(self, ns, nh, nt, sb, se, mute, qc_shot, seed):
+        rng = np.random.default_rng(seed)
+        self.ns, self.nh, self.nt, self.sb, self.se = ns, nh, nt, sb, se
+        ntot = ns * nh
+        self.alloc = trace_alloc(ntot, nt)
+        self.raw = np.full(self.alloc, np.nan, f32)
+        self.raw[:ntot * nt] = rng.standard_normal(ntot * nt).astype(f32)
+        self.wap = np.where(rng.random(ntot) < 0.2, -1.0, 1.0).astype(f32)
+        self.off = rng.random(ntot).astype(f32)
+        self.mute, self.qc_shot = mute, qc_shot
+        self.t0, self.dt, self.slope, self.intercept, self.taper = 0.0, 0.004, 0.5, 0.02, 0.03
+
+    def boundary(self, g):
+        return f32(f32(self.intercept) + f32(f32(self.slope) * self.off[g]))
+
+
+def replay(G):
+    nt, ntot = G.nt, G.ns * G.nh
+    prep = np.full(G.alloc, np.nan, f32)
+    pairs = np.full((G.alloc, 2), np.nan, f32)
+    qcb = np.full(G.nh * nt, np.nan, f32)
+    qca = np.full(G.nh * nt, np.nan, f32)
+    ntr = (G.se - G.sb) * G.nh
+    nblk = (ntr + TR - 1) // TR
+    loads = []
+
+    def load_chunk(buf, g0, c0):
+        pre = np.empty((TR // 2, TR), f32)  # [i][lane]
+        for t in range(TR):
+            h, s = t >> 5, t & (TS - 1)
+            for i in range(TR // 2):
+                e = g0 * nt + c0 + (2 * i * nt) + (h * nt + s)
+                loads.append(e)
+                assert 0 <= e < G.alloc, (g0, c0, t, i, e, G.alloc)
+                pre[i, t] = buf[e]
+        return pre
+
+    for b in range(nblk):
+        g0 = G.sb * G.nh + b * TR
+        nloc = min(TR, ntr - b * TR)
+        # forward
+        tile = np.full(TR * TS, np.nan, f32)
+        qoff = G.qc_shot * G.nh - g0
+        any_qc = G.mute and G.qc_shot >= 0 and g0 < G.qc_shot * G.nh + G.nh and \
+            g0 + nloc > G.qc_shot * G.nh
+        qlo, qhi = max(0, qoff), min(nloc, G.qc_shot * G.nh + G.nh - g0)
+        tt = np.zeros(TR, f32)
+        pre = load_chunk(G.raw, g0, 0)
+        for c0 in range(0, nt, TS):
+            nc = min(TS, nt - c0)
+            for t in range(TR):
+                h, s = t >> 5, t & (TS - 1)
+                for i in range(TR // 2):
+                    tile[chunk_at(2 * i + h, s)] = pre[i, t]
+            if c0 + TS < nt:
+                pre = load_chunk(G.raw, g0, c0 + TS)
+
+            def qc_copy(dst):
+                for t in range(TR):
+                    h, s = t >> 5, t & (TS - 1)
+                    for j in range(qlo + h, qhi, 2):
+                        if s < nc:
+                            dst[(j - qoff) * nt + c0 + s] = tile[chunk_at(j, s)]
+
+            def steps(t, do_mute, do_sum):
+                g = g0 + t
+                for k in range(nc):
+                    v = tile[chunk_at(t, k)]
+                    if do_mute:
+                        v = f32(v * mute_w(G.t0, G.dt, G.boundary(g), G.taper, c0 + k))
+                    if do_sum:
+                        tt[t] = v if c0 + k == 0 else f32(tt[t] + v)
+                        v = tt[t]
+                    tile[chunk_at(t, k)] = v
+
+            if any_qc:
+                qc_copy(qcb)
+                for t in range(nloc):
+                    if G.mute:
+                        steps(t, True, False)
+                qc_copy(qca)
+                for t in range(nloc):
+                    if G.wap[g0 + t] >= 0:
+                        steps(t, False, True)
+            else:
+                for t in range(nloc):
+                    keep = G.wap[g0 + t] >= 0
+                    if G.mute or keep:
+                        steps(t, bool(G.mute), keep)
+            for t in range(TR):
+                h, s = t >> 5, t & (TS - 1)
+                for i in range(TR // 2):
+                    j = 2 * i + h
+                    if j < nloc and s < nc:
+                        prep[(g0 + j) * nt + c0 + s] = tile[chunk_at(j, s)]
+    for b in range(nblk):
+        g0 = G.sb * G.nh + b * TR
+        nloc = min(TR, ntr - b * TR)
+        tile = np.full(TR * TS, np.nan, f32)
+        bnd = np.full(TR, np.nan, f32)
+        tt = np.zeros(TR, f32)
+        last = ((nt - 1) // TS) * TS
+        pre = load_chunk(prep, g0, last)
+        out = prep.copy()
+        for c0 in range(last, -1, -TS):
+            nc = min(TS, nt - c0)
+            for t in range(TR):
+                h, s = t >> 5, t & (TS - 1)
+                for i in range(TR // 2):
+                    tile[chunk_at(2 * i + h, s)] = pre[i, t]
+            if c0 > 0:
+                pre = load_chunk(prep, g0, c0 - TS)
+            for t in range(nloc):
+                if G.wap[g0 + t] < 0:
+                    continue
+                for k in range(nc - 1, -1, -1):
+                    if c0 + k == nt - 1:
+                        tt[t] = tile[chunk_at(t, k)]
+                    else:
+                        tt[t] = f32(tt[t] + tile[chunk_at(t, k)])
+                        tile[chunk_at(t, k)] = tt[t]
+            for t in range(TR):
+                h, s = t >> 5, t & (TS - 1)
+                for i in range(TR // 2):
+                    j = 2 * i + h
+                    if j < nloc and s < nc:
+                        o = (g0 + j) * nt + c0 + s
+                        v = tile[chunk_at(j, s)]
+                        out[o] = v
+                        nx = tile[chunk_at(j, s + 1)] if s + 1 < nc else \
+                            (bnd[j] if c0 + s + 1 < nt else f32(0))
+                        pairs[o] = (v, f32(nx - v)) if c0 + s + 1 < nt else (0, 0)
+            for t in range(nloc):
+                bnd[t] = tile[chunk_at(t, 0)]
+        prep[g0 * nt:(g0 + nloc) * nt] = out[g0 * nt:(g0 + nloc) * nt]
+    return prep, pairs, qcb, qca, loads
+
+
+def direct(G):
+    """Per trace, in sample order: the reference's mute, QC copies, doubint."""
+    nt = G.nt
+    prep = np.full(G.alloc, np.nan, f32)
+    pairs = np.full((G.alloc, 2), np.nan, f32)
+    qcb = np.full(G.nh * nt, np.nan, f32)
+    qca = np.full(G.nh * nt, np.nan, f32)
+    for g in range(G.sb * G.nh, G.se * G.nh):
+        x = G.raw[g * nt:(g + 1) * nt].copy()
+        qc = G.mute and g // G.nh == G.qc_shot
+        if qc:
+            qcb[(g % G.nh) * nt:(g % G.nh + 1) * nt] = x
+        if G.mute:
+            for it in range(nt):
+                x[it] = f32(x[it] * mute_w(G.t0, G.dt, G.boundary(g), G.taper, it))
+        if qc:
+            qca[(g % G.nh) * nt:(g % G.nh + 1) * nt] = x
+        if G.wap[g] >= 0:
+            tt = f32(0)
+            for it in range(nt):
+                tt = x[it] if it == 0 else f32(tt + x[it])
+                x[it] = tt
+            tt = x[nt - 1]
+            for it in range(nt - 2, -1, -1):
+                tt = f32(tt + x[it])
+                x[it] = tt
+        prep[g * nt:(g + 1) * nt] = x
+        for it in range(nt):
+            pairs[g * nt + it] = (x[it], f32(x[it + 1] - x[it])) if it + 1 < nt else (0, 0)
+    return prep, pairs, qcb, qca
+
+
+@pytest.mark.parametrize("ns,nh,nt,sb,se,mute,qc", [
+    (1, 1, 2, 0, 1, 0, -1),        # one trace, two samples
+    (2, 33, 17, 0, 2, 1, 1),       # 66 traces (two blocks, the second of 2), nt < chunk
+    (3, 32, 32, 1, 3, 1, 2),       # shot range from the second shot, nt = chunk
+    (2, 65, 33, 0, 2, 0, -1),      # 130 traces, nt one past a chunk
+    (1, 64, 100, 0, 1, 1, 0),      # one full block, QC shot, partial last chunk
+])
+def test_trace_prep_replay(ns, nh, nt, sb, se, mute, qc):
+    G = Geo(ns, nh, nt, sb, se, mute, qc, seed=ns * 1000 + nh * 10 + nt)
+    prep, pairs, qcb, qca, loads = replay(G)
+    rp, rpairs, rqcb, rqca = direct(G)
+    lo, hi = G.sb * G.nh * nt, G.se * G.nh * nt
+    assert np.array_equal(prep[lo:hi].view(np.uint32), rp[lo:hi].view(np.uint32))
+    assert np.array_equal(pairs[lo:hi].view(np.uint32), rpairs[lo:hi].view(np.uint32))
+    if mute and qc >= 0 and sb <= qc < se:
+        assert np.array_equal(qcb.view(np.uint32), rqcb.view(np.uint32))
+        assert np.array_equal(qca.view(np.uint32), rqca.view(np.uint32))
+    # the loads reach past the valid traces (into the allocation's pad) only
+    assert max(loads) < G.alloc
+
+
+def test_swizzle_is_conflict_free():
+    idx = [chunk_at(j, s) for j in range(TR) for s in range(TS)]
+    assert sorted(idx) == list(range(TR * TS))
+    for i in range(TR // 2):  # row-wise: lanes (h, s) at row 2 i + h
+        banks = [chunk_at(2 * i + (t >> 5), t & 31) % 64 for t in range(64)]
+        assert len(set(banks)) == 64
+    for k in range(TS):  # scan: lane t at row t
+        banks = [chunk_at(t, k) % 64 for t in range(64)]
+        assert len(set(banks)) == 64
+
+
+def test_replay_catches_a_short_allocation():
+    """A buffer without the pad (ntraces x nt) is caught by the bound check."""
+    G = Geo(1, 65, 17, 0, 1, 0, -1, seed=3)
+    G.alloc = 65 * 17
+    with pytest.raises(AssertionError):
+        replay(G)
