@@ -109,6 +109,9 @@ typedef struct kmig_stats {
     int graph;                 /* 1: the run replayed a captured step (only
                                   ms_total is its own; the phase times are
                                   the eager run's it was captured from)     */
+    int tuple_direct;          /* tuple passes (side x shot batch) that read
+                                  the table nodes directly: every slot sits
+                                  on a node, so no interpolation pass ran   */
 } kmig_stats;
 
 typedef struct kmig_ctx kmig_ctx;

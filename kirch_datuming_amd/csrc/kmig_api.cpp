@@ -33,6 +33,9 @@ struct Side {
     float o = 0.0f, d = 0.0f;
     bool set = false;
     size_t cap = 0;  // elements allocated per volume
+    // T is finite and never -0.0, D finite (k_table_check): Hermite
+    // interpolation at coordinate 0 then gives the node's values exactly
+    bool plain = false;
     // learned surrogate of the table interpolation (kmig_set_surrogate)
     kmig_nn *nn = nullptr;
     float nn_max_dt = 0.0f;
@@ -107,6 +110,12 @@ struct kmig_ctx {
     // receivers that move with the shot (cmp=1), one batch: receiver columns
     // (rslot then holds each column's slot; see MigResArgs)
     bool rcv_columns = false;
+    // every source (receiver) slot of the plan copies a table node: the fused
+    // tuple pass then reads the tables directly and k_interp does not run
+    bool direct_s = false, direct_r = false;
+    unsigned *chk = nullptr;  // k_table_check's flag
+    size_t chk_cap = 0;
+    int n_direct = 0;         // fused tuple passes of the run that read nodes directly
     int ncols = 0;
     ColTrace *ctab = nullptr;
     size_t ctab_cap = 0;
@@ -788,6 +797,17 @@ int plan_if_needed(kmig_ctx *c, int sb, int se)
             return rc;
         if ((rc = ensure(c, &c->spec_r, &c->spec_r_cap, std::max<size_t>(1, rspec.size()))))
             return rc;
+        // slots that copy a node: no interpolation, or Hermite interpolation
+        // at coordinate 0 of plain tables (k_interp gives the node's T and
+        // dT/ds bit for bit there)
+        auto all_direct = [&](const std::vector<SlotSpec> &v, const Side &sd) {
+            for (const SlotSpec &sp : v)
+                if (sp.interp && !(p.interp_type == 'h' && sp.coord == 0.0f && sd.plain))
+                    return false;
+            return true;
+        };
+        c->direct_s = all_direct(sspec, c->side[0]);
+        c->direct_r = all_direct(rspec, c->side[1]);
         if (!sspec.empty())
             HIPCHK(c, hipMemcpy(c->spec_s, sspec.data(), sspec.size() * sizeof(SlotSpec),
                                 hipMemcpyHostToDevice));
@@ -1003,7 +1023,19 @@ int side_tuples(kmig_ctx *c, int side, const TuplePrepArgs &a, Tuples &o, size_t
                 hipStream_t st, const PackOut &po, bool *fused)
 {
     Side &sd = c->side[side];
-    HIPCHK(c, (hipError_t)launch_tuple_interp(a, o, st));
+    const char *env = getenv("KMIG_TUPLE_FUSED");
+    const bool use_fused = (po.packed || po.packed16) && tuple_fused_ok(a) &&
+                           !(env && env[0] == '0');
+    if (fused) *fused = use_fused;
+    // the fused pass reads copied table nodes itself (no surrogate, which
+    // replaces the interpolated planes; no gradient files, interpolated with
+    // them); KMIG_TUPLE_DIRECT=0 keeps k_interp
+    const char *denv = getenv("KMIG_TUPLE_DIRECT");
+    TuplePrepArgs af = a;
+    af.direct = use_fused && !sd.nn && !a.use_grad_files &&
+                (side == 0 ? c->direct_s : c->direct_r) && !(denv && denv[0] == '0');
+    if (!af.direct) HIPCHK(c, (hipError_t)launch_tuple_interp(a, o, st));
+    c->n_direct += af.direct ? 1 : 0;
     if (sd.nn && a.nslots > 0) {
         HIPCHK(c, (hipError_t)launch_fill((float *)(sd.nn_used + slot0), (size_t)a.nslots, 0.0f,
                                           st));
@@ -1016,17 +1048,13 @@ int side_tuples(kmig_ctx *c, int side, const TuplePrepArgs &a, Tuples &o, size_t
                                                 per_slot ? c->tmp1 : nullptr,
                                                 per_slot ? c->tmp2 : nullptr, st));
     }
-    const char *env = getenv("KMIG_TUPLE_FUSED");
-    const bool use_fused = (po.packed || po.packed16) && tuple_fused_ok(a) &&
-                           !(env && env[0] == '0');
-    if (fused) *fused = use_fused;
     if (use_fused && po.packed) {
-        HIPCHK(c, (hipError_t)launch_tuple_fused(a, o, po.packed, c->dt, po.th_scale,
+        HIPCHK(c, (hipError_t)launch_tuple_fused(af, o, po.packed, c->dt, po.th_scale,
                                                  po.precision, po.bnd, st));
         return KMIG_OK;
     }
     if (use_fused) {
-        HIPCHK(c, (hipError_t)launch_tuple_fused16(a, o, po.packed16, po.scale, c->dt,
+        HIPCHK(c, (hipError_t)launch_tuple_fused16(af, o, po.packed16, po.scale, c->dt,
                                                    po.th_scale, po.precision, po.bnd, st));
         return KMIG_OK;
     }
@@ -1201,6 +1229,7 @@ void kmig_destroy(kmig_ctx *c)
     dfree(c->pk16_s);
     dfree(c->bnd_s);
     dfree(c->mask);
+    dfree(c->chk);
     dfree(c->bnd_r);
     dfree(c->pk16_r);
     dfree(c->sc_s);
@@ -1323,6 +1352,14 @@ int kmig_set_tables(kmig_ctx *c, int side, const float *T, const float *dTds, co
     if ((rc = ensure(c, &s.D, &cap2, n))) return rc;
     HIPCHK(c, hipMemcpy(s.T, T, n * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(s.D, dTds, n * sizeof(float), hipMemcpyHostToDevice));
+    {
+        if ((rc = ensure(c, &c->chk, &c->chk_cap, 1))) return rc;
+        HIPCHK(c, (hipError_t)launch_table_check(s.T, s.D, n, c->chk, c->stream));
+        unsigned bad = 1;
+        HIPCHK(c, hipMemcpyAsync(&bad, c->chk, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        s.plain = bad == 0;
+    }
     if (gx) {
         size_t cg = 0;
         dfree(s.GX);
@@ -1560,6 +1597,7 @@ int kmig_run(kmig_ctx *c, int sb, int se)
 
     size_t spec_s_off = 0;
     int launches = 0;
+    c->n_direct = 0;
     for (const Batch &b : c->batches) {
         const int nsb = b.s1 - b.s0;
         hipEvent_t a0 = next_event(c);
@@ -1603,6 +1641,7 @@ int kmig_run(kmig_ctx *c, int sb, int se)
     int maxr = 0;
     for (const Batch &b : c->batches) maxr = std::max(maxr, b.nrcv);
     c->st.receiver_slots = maxr;
+    c->st.tuple_direct = c->n_direct;
     c->pending = true;
     c->ran = true;
     // the next run with this plan replays a graph of this step

@@ -70,6 +70,10 @@ struct TuplePrepArgs {
     int angle_mode;
     float aper, aper_taper, rmin, rtaper;
     float aal;                           // antialias (folded into packed FP32 tuples)
+    // fused pass only: every slot copies its table node (no interpolation),
+    // so T and |dT/ds * dside| are read from tab / der at the slot's node
+    // instead of the interpolation's planes (which are then not written)
+    int direct;
 };
 
 struct TracePrepArgs {
@@ -315,6 +319,8 @@ int launch_migrate(const MigArgs &a, int tile, void *stream);
 int launch_model(const MigArgs &a, int tile, void *stream);
 int migrate_lds_bytes(int ng, int tile);
 int launch_fill(float *p, size_t n, float v, void *stream);
+// *bad = 1 where a traveltime is non-finite or -0.0 or a derivative non-finite
+int launch_table_check(const float *T, const float *D, size_t n, unsigned *bad, void *stream);
 // th_scale: degrees per radian of the packed direction (0: no direction);
 // precision: KMIG_PREC_FP32 or KMIG_PREC_FP16 fields
 int launch_pack(const Tuples &in, PackedTuple *out, int nslots, size_t nzx, float dt,

@@ -17,6 +17,7 @@
 // bit for bit except where the device libm (atanf, cosf) rounds differently.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 
@@ -827,11 +828,15 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
     float *bx = az + n1, *bz = bx + n1;
     const size_t base = (size_t)slot * a.nzx;
     const int tid = threadIdx.x;
+    // the slot's traveltimes and position derivatives: the table node's
+    // (a.direct, k_interp's copy case) or the interpolated planes
+    const size_t node = a.direct ? (size_t)a.spec[slot].node * a.nzx : 0;
+    const float *__restrict__ Tsrc = a.direct ? a.tab + node : o.T + base;
     if (!a.use_grad_files) {
         TileWalk t(tid, w0z);
         for (int k = tid; k < w0z * w0x; k += 256, t.next()) {
             const int iz = z0 - H0 + t.l, ix = x0 - H0 + t.m;
-            sT[k] = (iz >= 0 && iz < nz && ix >= 0 && ix < nx) ? o.T[base + (size_t)ix * nz + iz]
+            sT[k] = (iz >= 0 && iz < nz && ix >= 0 && ix < nx) ? Tsrc[(size_t)ix * nz + iz]
                                                                  : 0.0f;
         }
         __syncthreads();
@@ -903,8 +908,12 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
             const float wf = floor_weight(d, a.rmin, a.rtaper);
             float w = wc > wf ? wc : wf;
             if (sqrtf(gx * gx + gz * gz) <= 1.0e-12f) w = 0.0f;
-            const size_t oi = base + (size_t)ix * nz + iz;
-            const PackFields f = pack_fields(o.T[oi], o.A[oi], gx, gz, w, inv_dt, th_scale);
+            const size_t pi = (size_t)ix * nz + iz, oi = base + pi;
+            // T from the staged region where there is one; A as k_interp
+            // forms it for a copied node (|dT/ds * dside|)
+            const float T = a.use_grad_files ? Tsrc[pi] : sT[(ix - x0 + H0) * w0z + (iz - z0 + H0)];
+            const float A = a.direct ? fabsf(a.der[node + pi] * a.dside) : o.A[oi];
+            const PackFields f = pack_fields(T, A, gx, gz, w, inv_dt, th_scale);
             if constexpr (sizeof(Out) == sizeof(PackedTuple16)) {
                 out[oi] = tuple16(f, scale[slot].a_scale, precision);
             } else {
@@ -952,6 +961,24 @@ __global__ __launch_bounds__(256) void k_pairs(const float *prep, float2 *pairs,
     const int it = (int)(k % nt);
     const float v = prep[k];
     pairs[k] = it + 1 < nt ? make_float2(v, prep[k + 1] - v) : make_float2(0.0f, 0.0f);
+}
+
+// Sets *bad where any traveltime of the n entries is non-finite or -0.0 or
+// any position derivative is non-finite.  Tables without such entries give
+// k_interp's Hermite interpolation at an in-interval coordinate of exactly
+// 0 the node's own values bit for bit (h = (1, 0, 0, 0) and dT/ds = (dx m0)
+// / dx): the fused tuple pass then reads the node directly.
+__global__ __launch_bounds__(256) void k_table_check(const float *__restrict__ T,
+                                                     const float *__restrict__ D, size_t n,
+                                                     unsigned *bad)
+{
+    bool b = false;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const float t = T[i], d = D[i];
+        b |= !isfinite(t) || (t == 0.0f && signbit(t)) || !isfinite(d);
+    }
+    if (__ballot(b) != 0 && (threadIdx.x & 63) == 0) atomicOr(bad, 1u);
 }
 
 __global__ void k_fill(float *p, size_t n, float v)
@@ -1354,6 +1381,16 @@ int launch_pairs(const float *prep, float *pairs, long ntraces, int nt, void *st
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_pairs, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, prep,
                        (float2 *)pairs, n, nt);
+    return (int)hipGetLastError();
+}
+
+int launch_table_check(const float *T, const float *D, size_t n, unsigned *bad, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    const int e = launch_fill((float *)bad, 1, 0.0f, stream);  // +0.0f: all bits 0
+    if (e || n == 0) return e;
+    const unsigned nb = std::min<unsigned>(nblk(n, 256), 4096u);
+    hipLaunchKernelGGL(k_table_check, dim3(nb), dim3(256), 0, st, T, D, n, bad);
     return (int)hipGetLastError();
 }
 

@@ -68,7 +68,8 @@ class KmigStats(C.Structure):
                 ("ms_tuple_prep", C.c_double), ("ms_migrate", C.c_double),
                 ("ms_total", C.c_double), ("migrate_launches", C.c_int),
                 ("receiver_slots", C.c_int), ("shot_batches", C.c_int),
-                ("variant", C.c_int), ("tile", C.c_int), ("graph", C.c_int)]
+                ("variant", C.c_int), ("tile", C.c_int), ("graph", C.c_int),
+                ("tuple_direct", C.c_int)]
 
 
 _LIB = None

@@ -255,6 +255,49 @@ def test_fused_tuple_pass_is_bitwise_the_separate_passes(monkeypatch, radius, sm
     assert r < max(TOL, 1.5 * r32)
 
 
+def test_tuple_pass_reads_table_nodes_directly(monkeypatch):
+    """Shots and receivers on table nodes (power-of-two spacings, so every
+    interior slot has Hermite coordinate exactly 0): the fused tuple pass
+    reads the nodes' T and dT/ds itself, no interpolation pass runs, and the
+    CIG is the interpolating run's bit for bit.  A traveltime table holding
+    one -0.0 (or a non-finite entry) is not plain: interpolation runs there."""
+    K = _kmig()
+    rng = np.random.default_rng(5)
+    nz, nx, d = 96, 80, 0.02
+    npos, dpos = 26, 0.0625
+    T, D = _vz_tables(nz, nx, d, dpos * np.arange(npos))
+    nt, dt, ns, nh = 300, 0.004, 6, 24
+    data = rng.normal(size=(ns, nh, nt)).astype(np.float32)
+    data_axes = [[nt, 0.0, dt], [nh, 0.0625, 0.0625], [ns, 0.125, 0.25]]
+    tax = [[nz, 0.0, d], [nx, 0.0, d], [npos, 0.0, dpos]]
+    kv = dict(cmp=0, mode="angle", angle_min=0, angle_max=60, angle_step=1, aperture=60,
+              variant="resident", tile=9)
+    p = K.parse_params(kv, nh, 0.0625, 0.0625)
+
+    def run(stab):
+        with K.Migrator(p) as m:
+            m.set_tables(0, stab, D, tax)
+            m.set_tables(1, T, D, tax)
+            m.set_data(data, data_axes)
+            m.run()
+            m.sync()
+            return m.cig(), m.stats()["tuple_direct"]
+
+    cig, nd = run(T)
+    monkeypatch.setenv("KMIG_TUPLE_DIRECT", "0")
+    cig_i, nd_i = run(T)
+    monkeypatch.delenv("KMIG_TUPLE_DIRECT")
+    print(f"direct tuple passes: {nd} (switched off: {nd_i})")
+    assert nd == 2 and nd_i == 0
+    assert np.array_equal(cig.view(np.uint32), cig_i.view(np.uint32))
+    ref64 = O.migrate(kv, data, data_axes, T, D, tax, T, D, tax, precision="f64")[0]
+    assert O.rel_l2(cig, ref64) < TOL
+    Tz = np.array(T, np.float32)
+    Tz.reshape(-1)[7] = -0.0
+    _, nd_z = run(Tz)
+    assert nd_z == 1  # the receiver side still reads its (plain) nodes
+
+
 def test_fused_tuple_pass_with_gradient_files(monkeypatch):
     # one preset for both runs: the autotuner picks by timing, and presets of
     # other receiver-chunk sizes sum each point in another order
