@@ -504,6 +504,16 @@ __device__ inline float floor_weight(float d, float rmin, float rt)
     return 0.5f * (1.0f + cosf((float)KPI * u));
 }
 
+// sqrtf(gx * gx + gz * gz) <= 1e-12 (src/kirchmig2d_cig.cpp:176) without the
+// square root: sqrtf rounds correctly, so it is monotone and the test equals
+// s <= S, S = 0x179abe15 (about 1.0000000195e-24) the largest float whose
+// rounded root is <= 1e-12f (checked over every float by tests/sqrt_cut.c).
+// NaN and infinite sums fail both forms.
+__device__ __forceinline__ bool grad_vanishes(float gx, float gz)
+{
+    return gx * gx + gz * gz <= __uint_as_float(0x179abe15u);
+}
+
 __global__ __launch_bounds__(256) void k_weight(const TuplePrepArgs a, Tuples o)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -523,7 +533,7 @@ __global__ __launch_bounds__(256) void k_weight(const TuplePrepArgs a, Tuples o)
     const size_t oi = (size_t)slot * a.nzx + i;
     if (a.angle_mode) {
         const float gx = o.GX[oi], gz = o.GZ[oi];
-        if (sqrtf(gx * gx + gz * gz) <= 1.0e-12f) w = 0.0f;
+        if (grad_vanishes(gx, gz)) w = 0.0f;
     }
     o.W[oi] = w;
 }
@@ -723,16 +733,19 @@ struct TileWalk {
 // adding +0.0 leaves such a sum as it is.  The caller's regions keep RS
 // elements of margin on the box's axis, so every tap lies in the LDS region.
 template <int RS>
-__device__ __forceinline__ float box_masked(const float *buf, int stride, int g, int n)
+__device__ __forceinline__ float2 box_masked(const float2 *buf, int stride, int g, int n)
 {
-    float s = 0.0f;
+    float sx = 0.0f, sz = 0.0f;
 #pragma unroll
     for (int t = -RS; t <= RS; t++) {
-        const float v = buf[t * stride];
-        s += (g + t >= 0 && g + t < n) ? v : 0.0f;
+        const float2 v = buf[t * stride];
+        const bool in = g + t >= 0 && g + t < n;
+        sx += in ? v.x : 0.0f;
+        sz += in ? v.y : 0.0f;
     }
     const int lo = g - RS < 0 ? 0 : g - RS, hi = g + RS > n - 1 ? n - 1 : g + RS;
-    return s / (float)(hi - lo + 1);
+    const float cnt = (float)(hi - lo + 1);
+    return make_float2(sx / cnt, sz / cnt);
 }
 
 // s / (2 RS + 1) as a product with the reciprocal and one fused correction,
@@ -751,25 +764,30 @@ __device__ __forceinline__ float div_box(float s)
 // The interior box sum: all 2 RS + 1 taps inside the image, summed from the
 // low end from +0.0 (bitwise the clipped loop with nothing clipped).
 template <int RS>
-__device__ __forceinline__ float box_interior(const float *buf, int stride)
+__device__ __forceinline__ float2 box_interior(const float2 *buf, int stride)
 {
-    float s = 0.0f;
+    float sx = 0.0f, sz = 0.0f;
 #pragma unroll
-    for (int t = -RS; t <= RS; t++) s += buf[t * stride];
-    return div_box<RS>(s);
+    for (int t = -RS; t <= RS; t++) {
+        const float2 v = buf[t * stride];
+        sx += v.x;
+        sz += v.y;
+    }
+    return make_float2(div_box<RS>(sx), div_box<RS>(sz));
 }
 
 // One smoothing pass (box along z, or along x) of both gradient components
-// over the local region [zlo, zhi) x [xlo, xhi) of the w1z-high LDS arrays
-// whose element (0, 0) is image point (gz0, gx0).  Where every window of the
-// region lies inside the image on the box's axis (decided once per block),
-// the taps are summed without masks over every element of the region: an
-// element off the image then holds a value no image element's window reads.
-// Elsewhere image points only, with masked taps.
+// (x, z of each float2) over the local region [zlo, zhi) x [xlo, xhi) of the
+// w1z-high LDS arrays whose element (0, 0) is image point (gz0, gx0).  Where
+// every window of the region lies inside the image on the box's axis
+// (decided once per block), the taps are summed without masks over every
+// element of the region: an element off the image then holds a value no
+// image element's window reads.  Elsewhere image points only, with masked
+// taps.
 template <int RS, bool ZDIR>
-__device__ __forceinline__ void box_pass(const float *sx, const float *sz, float *dx, float *dz,
-                                         int w1z, int zlo, int zhi, int xlo, int xhi, int gz0,
-                                         int gx0, int nz, int nx)
+__device__ __forceinline__ void box_pass(const float2 *src, float2 *dst, int w1z, int zlo,
+                                         int zhi, int xlo, int xhi, int gz0, int gx0, int nz,
+                                         int nx)
 {
     const int wz = zhi - zlo, n = wz * (xhi - xlo);
     const bool interior = ZDIR ? (gz0 + zlo - RS >= 0 && gz0 + zhi - 1 + RS <= nz - 1)
@@ -778,8 +796,7 @@ __device__ __forceinline__ void box_pass(const float *sx, const float *sz, float
     if (interior) {
         for (int k = threadIdx.x; k < n; k += 256, t.next()) {
             const int e = (xlo + t.m) * w1z + zlo + t.l;
-            dx[e] = box_interior<RS>(sx + e, ZDIR ? 1 : w1z);
-            dz[e] = box_interior<RS>(sz + e, ZDIR ? 1 : w1z);
+            dst[e] = box_interior<RS>(src + e, ZDIR ? 1 : w1z);
         }
     } else {
         for (int k = threadIdx.x; k < n; k += 256, t.next()) {
@@ -787,16 +804,25 @@ __device__ __forceinline__ void box_pass(const float *sx, const float *sz, float
             const int iz = gz0 + lz, ix = gx0 + lx;
             if (iz < 0 || iz >= nz || ix < 0 || ix >= nx) continue;
             const int e = lx * w1z + lz;
-            if (ZDIR) {
-                dx[e] = box_masked<RS>(sx + e, 1, iz, nz);
-                dz[e] = box_masked<RS>(sz + e, 1, iz, nz);
-            } else {
-                dx[e] = box_masked<RS>(sx + e, w1z, ix, nx);
-                dz[e] = box_masked<RS>(sz + e, w1z, ix, nx);
-            }
+            dst[e] = ZDIR ? box_masked<RS>(src + e, 1, iz, nz) : box_masked<RS>(src + e, w1z, ix, nx);
         }
     }
     __syncthreads();
+}
+
+// Wave-wide reduction for full waves (every lane active) with an associative,
+// commutative op: four DPP steps inside each row of 16 lanes (quad_perm
+// [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror: every lane
+// ends with its row's result), then the four rows' results read as scalars.
+template <class Op>
+__device__ __forceinline__ int wave_reduce(int v, Op op)
+{
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, true));
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, true));
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, true));
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, true));
+    return op(op(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+              op(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
 
 // Out = PackedTuple (FP32 or FP16 fields: `precision`) or PackedTuple16
@@ -823,9 +849,12 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
     const int H0 = H1 + rf;
     const int w0z = FZ + 2 * H0, w0x = FX + 2 * H0;
     constexpr int w1z = FZ + 2 * H1, w1x = FX + 2 * H1, n1 = w1z * w1x;
+    // sT (w0z x w0x traveltimes) then two w1z x w1x arrays of gradient (x, z)
+    // pairs; an odd traveltime count leaves one float of padding, so the
+    // pairs stay 8-byte aligned
     float *sT = sm;
-    float *ax = sT + (a.use_grad_files ? 0 : w0z * w0x), *az = ax + n1;
-    float *bx = az + n1, *bz = bx + n1;
+    const int nT = a.use_grad_files ? 0 : (w0z * w0x + 1) & ~1;
+    float2 *ga = (float2 *)(sT + nT), *gb = ga + n1;
     const size_t base = (size_t)slot * a.nzx;
     const int tid = threadIdx.x;
     // the slot's traveltimes and position derivatives: the table node's
@@ -869,20 +898,18 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
                 }
             }
         }
-        ax[k] = vx;
-        az[k] = vz;
+        ga[k] = make_float2(vx, vz);
     }
     __syncthreads();
     // two passes of (box z, box x) per component, a -> b -> a -> b -> a, each
     // over only the region the next pass reads (the last: the output tile)
     if constexpr (RS > 0) {
         const int gz0 = z0 - H1, gx0 = x0 - H1;
-        box_pass<RS, true>(ax, az, bx, bz, w1z, RS, w1z - RS, 0, w1x, gz0, gx0, nz, nx);
-        box_pass<RS, false>(bx, bz, ax, az, w1z, RS, w1z - RS, RS, w1x - RS, gz0, gx0, nz, nx);
-        box_pass<RS, true>(ax, az, bx, bz, w1z, 2 * RS, w1z - 2 * RS, RS, w1x - RS, gz0, gx0,
-                           nz, nx);
-        box_pass<RS, false>(bx, bz, ax, az, w1z, 2 * RS, w1z - 2 * RS, 2 * RS, w1x - 2 * RS,
-                            gz0, gx0, nz, nx);
+        box_pass<RS, true>(ga, gb, w1z, RS, w1z - RS, 0, w1x, gz0, gx0, nz, nx);
+        box_pass<RS, false>(gb, ga, w1z, RS, w1z - RS, RS, w1x - RS, gz0, gx0, nz, nx);
+        box_pass<RS, true>(ga, gb, w1z, 2 * RS, w1z - 2 * RS, RS, w1x - RS, gz0, gx0, nz, nx);
+        box_pass<RS, false>(gb, ga, w1z, 2 * RS, w1z - 2 * RS, 2 * RS, w1x - 2 * RS, gz0, gx0,
+                            nz, nx);
     }
     // weights and packing of the output tile: wave w, round j takes bound cell
     // 4 j + w of the tile's (FZ / BND_CZ) x (FX / BND_CX), lane = (z, x) in it
@@ -897,7 +924,7 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
         uint32_t iw = 0u;
         if (valid) {
             const int l = (ix - x0 + H1) * w1z + (iz - z0 + H1);
-            const float gx = ax[l], gz = az[l];
+            const float gx = ga[l].x, gz = ga[l].y;
             const float zd0 = fabsf((float)iz * a.dz), zd1 = 1.0e-6f * dzabs;
             const float zd2 = 1.0e-12f > zd1 ? 1.0e-12f : zd1;
             const float zdist = zd0 > zd2 ? zd0 : zd2;
@@ -907,7 +934,7 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
             const float wc = leg_weight(deg, a.aper, a.aper_taper);
             const float wf = floor_weight(d, a.rmin, a.rtaper);
             float w = wc > wf ? wc : wf;
-            if (sqrtf(gx * gx + gz * gz) <= 1.0e-12f) w = 0.0f;
+            if (grad_vanishes(gx, gz)) w = 0.0f;
             const size_t pi = (size_t)ix * nz + iz, oi = base + pi;
             // T from the staged region where there is one; A as k_interp
             // forms it for a copied node (|dT/ds * dside|)
@@ -927,11 +954,13 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
         const bool use = valid && (iw & 0xFFFFu) != 0u;
         float lo = use ? th : INFINITY, hi = use ? th : -INFINITY;
         int tmin = use ? (int)iw >> 16 : BND_EMPTY;
-        for (int m = 1; m < 64; m <<= 1) {
-            lo = fminf(lo, __shfl_xor(lo, m, 64));
-            hi = fmaxf(hi, __shfl_xor(hi, m, 64));
-            tmin = min(tmin, __shfl_xor(tmin, m, 64));
-        }
+        lo = __int_as_float(wave_reduce(__float_as_int(lo), [](int x, int y) {
+            return __float_as_int(fminf(__int_as_float(x), __int_as_float(y)));
+        }));
+        hi = __int_as_float(wave_reduce(__float_as_int(hi), [](int x, int y) {
+            return __float_as_int(fmaxf(__int_as_float(x), __int_as_float(y)));
+        }));
+        tmin = wave_reduce(tmin, [](int x, int y) { return min(x, y); });
         const int gcz = z0 / BND_CZ + cz, gcx = x0 / BND_CX + cx;
         const int ncz = (nz + BND_CZ - 1) / BND_CZ, ncx = (nx + BND_CX - 1) / BND_CX;
         if (lane == 0 && gcz < ncz && gcx < ncx) {
@@ -1130,7 +1159,7 @@ static int launch_fused_rs(const TuplePrepArgs &a, Tuples &o, Out *out, float dt
 {
     const int rf = a.radius < 1 ? 1 : a.radius;
     const int H1 = 2 * RS, H0 = H1 + rf;
-    const size_t n0 = a.use_grad_files ? 0 : (size_t)(FZ + 2 * H0) * (FX + 2 * H0);
+    const size_t n0 = a.use_grad_files ? 0 : (((size_t)(FZ + 2 * H0) * (FX + 2 * H0) + 1) & ~(size_t)1);
     const size_t n1 = (size_t)(FZ + 2 * H1) * (FX + 2 * H1);
     const size_t lds = (n0 + 4 * n1) * sizeof(float);
     const unsigned tiles = (unsigned)(((a.nz + FZ - 1) / FZ) * ((a.nx + FX - 1) / FX));
