@@ -87,26 +87,32 @@ __global__ __launch_bounds__(256) void k_trace_prep(const TracePrepArgs a)
 }
 
 // Coalesced form of the trace preparation: a block of one wave owns TR = 64
-// traces and walks them in chunks of TS = 32 samples staged through LDS, so
-// every global access is a 128-byte row (two rows per load instruction)
-// while each lane still forms its own trace's running sums sequentially (the
-// reference's rounding order).  The next chunk's 32 loads are issued before
-// the current chunk is scanned, all at once, so they overlap the scan and
-// the stores; the swizzled tile (below) is 8 KB, so every block of a salt
-// batch (4,812) is resident at once.
+// traces and walks them in chunks of TS samples staged through LDS, so every
+// global access is a row segment of TS samples, while each lane still forms
+// its own trace's running sums sequentially (the reference's rounding order).
+// The next chunk's loads are issued before the current chunk is scanned, all
+// at once, so they overlap the scan and the stores.
 //   k_trace_fwd: mute (+QC copies) and the causal sum of doubint
 //   k_trace_bwd: the anticausal sum, written as samples (when asked for) and
 //                as (tr[k], tr[k+1] - tr[k]) pairs (when asked for)
 constexpr int TR = 64;  // traces per block (lane t scans trace t)
 typedef int i32x2 __attribute__((ext_vector_type(2)));
-constexpr int TS = 32;  // samples per chunk
 
-// Element (trace j, sample s) of a TR x TS chunk: rows of TS floats with the
-// sample index XORed by j / 2, so both access patterns are free of bank
-// conflicts: a row-wise load (lanes 0-31 row j, lanes 32-63 row j + 1: banks
-// (s ^ j/2) and 32 + (s ^ j/2)) and the scan (lane t at row t, one sample:
-// banks 32 (t & 1) + (k ^ t/2), all distinct).
-__device__ __forceinline__ int chunk_at(int j, int s) { return j * TS + (s ^ (j >> 1)); }
+// Chunks of TS samples (32 or 64): one load instruction covers RPI = 64 / TS
+// trace rows (lanes (h, s) = (t / TS, t % TS) take sample s of row RPI i + h),
+// NL = TR / RPI instructions a chunk.  Element (trace j, sample s) sits at
+// j * TS + (s ^ swz(j)) with swz(j) = j / 2 (TS = 32) or j (TS = 64), so
+// both access patterns are free of bank conflicts: a row-wise access (banks
+// 32 h + (s ^ swz), or s ^ j) and the scan (lane t at row t, one sample k:
+// banks 32 (t & 1) + (k ^ t/2), or k ^ t, all distinct).
+template <int TS> struct Chunk {
+    static_assert(TS == 32 || TS == 64, "chunks of 32 or 64 samples");
+    static constexpr int RPI = 64 / TS, NL = TR / RPI;
+    static __device__ __forceinline__ int at(int j, int s)
+    {
+        return j * TS + (s ^ (TS == 32 ? (j >> 1) : j));
+    }
+};
 
 // The direct-wave mute weight of sample it (src/kirchmig2d_cig.cpp:110-129).
 __device__ __forceinline__ float mute_weight(const TracePrepArgs &a, float boundary, int it)
@@ -123,38 +129,42 @@ __device__ __forceinline__ float mute_weight(const TracePrepArgs &a, float bound
 }
 
 // The chunk of samples c0 .. c0 + TS - 1 of the block's traces into
-// registers: lane (h, s) = (t / 32, t % 32) takes sample c0 + s of trace
-// 2 i + h (base = the block's first trace).  Every lane loads, with no clamp
-// and no branch, so all TR / 2 loads are in flight together: trace buffers
-// are trace_alloc() long, which covers a block's TR rows from any valid first
-// trace plus the TS - 1 samples a chunk reaches past a row's end (values past
-// nloc traces or nt samples are never used).  Buffer loads: the lane's part of the offset, (h nt + s)
-// samples, is one register for all TR / 2 loads, and the row and chunk part,
-// (2 i nt + c0) samples, is a scalar (flat loads would hold a 64-bit address
-// per row).  Offsets within a block's rows fit in 32 bits (TR x nt samples).
-// (The same split of offsets serves the stores: a block's rows are a buffer
-// from its first trace, with no range limit; every access is inside the
-// allocation by construction.)
+// registers: lane (h, s) takes sample c0 + s of trace RPI i + h (base = the
+// block's first trace).  Every lane loads, with no clamp and no branch, so
+// all NL loads are in flight together: trace buffers are trace_alloc() long,
+// which covers a block's TR rows from any valid first trace plus the TS - 1
+// samples a chunk reaches past a row's end (values past nloc traces or nt
+// samples are never used).  Buffer loads: the lane's part of the offset,
+// (h nt + s) samples, is one register for all NL loads, and the row and
+// chunk part, (RPI i nt + c0) samples, is a scalar (flat loads would hold a
+// 64-bit address per row).  Offsets within a block's rows fit in 32 bits
+// (TR x nt samples).  (The same split of offsets serves the stores: a
+// block's rows are a buffer from its first trace, with no range limit;
+// every access is inside the allocation by construction.)
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const void *base)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, -1, 0x00020000);
 }
 
-__device__ __forceinline__ void load_chunk(float (&v)[TR / 2], const float *base, int nt, int c0,
-                                           int h, int s)
+template <int TS>
+__device__ __forceinline__ void load_chunk(float (&v)[Chunk<TS>::NL], const float *base, int nt,
+                                           int c0, int h, int s)
 {
     const __amdgpu_buffer_rsrc_t rs = rows_rsrc(base);
     const int voff = (h * nt + s) * (int)sizeof(float);
 #pragma unroll
-    for (int i = 0; i < TR / 2; i++)
-        v[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                             rs, voff, (2 * i * nt + c0) * (int)sizeof(float), 0));
+    for (int i = 0; i < Chunk<TS>::NL; i++)
+        v[i] = __builtin_bit_cast(
+            float, __builtin_amdgcn_raw_buffer_load_b32(
+                       rs, voff, (Chunk<TS>::RPI * i * nt + c0) * (int)sizeof(float), 0));
 }
 
+template <int TS>
 __global__ __launch_bounds__(TR, 5) void k_trace_fwd(const TracePrepArgs a)
 {
+    using C = Chunk<TS>;
     __shared__ float tile[TR * TS];
-    const int t = threadIdx.x, h = t >> 5, s = t & (TS - 1);
+    const int t = threadIdx.x, h = t / TS, s = t % TS;
     const int nt = a.nt;
     const long ntr = (long)(a.shot_end - a.shot_begin) * a.nh;
     const long g0 = (long)a.shot_begin * a.nh + (long)blockIdx.x * TR;
@@ -179,13 +189,13 @@ __global__ __launch_bounds__(TR, 5) void k_trace_fwd(const TracePrepArgs a)
     }
     // QC copies of the chunk's rows of the QC shot (before or after the mute)
     auto qc_copy = [&](float *dst, int c0, int nc) {
-        for (int j = qlo + h; j < qhi; j += 2)
-            if (s < nc) dst[(size_t)(j - qoff) * nt + c0 + s] = tile[chunk_at(j, s)];
+        for (int j = qlo + h; j < qhi; j += C::RPI)
+            if (s < nc) dst[(size_t)(j - qoff) * nt + c0 + s] = tile[C::at(j, s)];
     };
     const float *__restrict__ raw = a.raw + g0 * nt;
     float *__restrict__ prep = a.prep + g0 * nt;
-    float pre[TR / 2];
-    load_chunk(pre, raw, nt, 0, h, s);
+    float pre[C::NL];
+    load_chunk<TS>(pre, raw, nt, 0, h, s);
     float tt = 0.0f;
     for (int c0 = 0; c0 < nt; c0 += TS) {
         const int nc = min(TS, nt - c0);
@@ -194,18 +204,18 @@ __global__ __launch_bounds__(TR, 5) void k_trace_fwd(const TracePrepArgs a)
         int sv = s;
         asm volatile("" : "+v"(sv));
 #pragma unroll
-        for (int i = 0; i < TR / 2; i++) tile[chunk_at(2 * i + h, sv)] = pre[i];
+        for (int i = 0; i < C::NL; i++) tile[C::at(C::RPI * i + h, sv)] = pre[i];
         __syncthreads();
-        if (c0 + TS < nt) load_chunk(pre, raw, nt, c0 + TS, h, s);
+        if (c0 + TS < nt) load_chunk<TS>(pre, raw, nt, c0 + TS, h, s);
         // mute, then the causal running sum, of the lane's own trace
         auto step = [&](int k, bool do_mute, bool do_sum) {
-            float v = tile[chunk_at(t, k)];
+            float v = tile[C::at(t, k)];
             if (do_mute) v *= mute_weight(a, boundary, c0 + k);
             if (do_sum) {
                 tt = (c0 + k == 0) ? v : tt + v;
                 v = tt;
             }
-            tile[chunk_at(t, k)] = v;
+            tile[C::at(t, k)] = v;
         };
         if (any_qc) {
             qc_copy(a.qcb, c0, nc);
@@ -231,12 +241,12 @@ __global__ __launch_bounds__(TR, 5) void k_trace_fwd(const TracePrepArgs a)
             const __amdgpu_buffer_rsrc_t rs = rows_rsrc(prep);
             const int voff = (h * nt + s) * (int)sizeof(float);
 #pragma unroll 8
-            for (int i = 0; i < TR / 2; i++) {
-                const int j = 2 * i + h;
+            for (int i = 0; i < C::NL; i++) {
+                const int j = C::RPI * i + h;
                 if (j < nloc && s < nc)
                     __builtin_amdgcn_raw_buffer_store_b32(
-                        __builtin_bit_cast(int, tile[chunk_at(j, sv)]), rs, voff,
-                        (2 * i * nt + c0) * (int)sizeof(float), 0);
+                        __builtin_bit_cast(int, tile[C::at(j, sv)]), rs, voff,
+                        (C::RPI * i * nt + c0) * (int)sizeof(float), 0);
             }
         }
         __syncthreads();
@@ -245,12 +255,14 @@ __global__ __launch_bounds__(TR, 5) void k_trace_fwd(const TracePrepArgs a)
 
 // write_prep: the anticausal sums also go back to prep (the stream kernel
 // and diff reads them; the resident kernels read only the pairs).
+template <int TS>
 __global__ __launch_bounds__(TR, 5) void k_trace_bwd(const TracePrepArgs a, float2 *pairs,
-                                                  int write_prep)
+                                                     int write_prep)
 {
+    using C = Chunk<TS>;
     __shared__ float tile[TR * TS];
     __shared__ float bnd[TR];  // first sample of the chunk after this one
-    const int t = threadIdx.x, h = t >> 5, s = t & (TS - 1);
+    const int t = threadIdx.x, h = t / TS, s = t % TS;
     const int nt = a.nt;
     const long ntr = (long)(a.shot_end - a.shot_begin) * a.nh;
     const long g0 = (long)a.shot_begin * a.nh + (long)blockIdx.x * TR;
@@ -259,25 +271,25 @@ __global__ __launch_bounds__(TR, 5) void k_trace_bwd(const TracePrepArgs a, floa
     const int last = ((nt - 1) / TS) * TS;
     float *prep = a.prep + g0 * nt;
     float2 *pr = pairs ? pairs + g0 * nt : nullptr;
-    float pre[TR / 2];
-    load_chunk(pre, prep, nt, last, h, s);
+    float pre[C::NL];
+    load_chunk<TS>(pre, prep, nt, last, h, s);
     float tt = 0.0f;
     for (int c0 = last; c0 >= 0; c0 -= TS) {
         const int nc = min(TS, nt - c0);
         int sv = s;  // as in k_trace_fwd
         asm volatile("" : "+v"(sv));
 #pragma unroll
-        for (int i = 0; i < TR / 2; i++) tile[chunk_at(2 * i + h, sv)] = pre[i];
+        for (int i = 0; i < C::NL; i++) tile[C::at(C::RPI * i + h, sv)] = pre[i];
         __syncthreads();
-        if (c0 > 0) load_chunk(pre, prep, nt, c0 - TS, h, s);
+        if (c0 > 0) load_chunk<TS>(pre, prep, nt, c0 - TS, h, s);
         if (mine) {
             auto step = [&](int k) {
                 const int it = c0 + k;
                 if (it == nt - 1) {
-                    tt = tile[chunk_at(t, k)];
+                    tt = tile[C::at(t, k)];
                 } else {
-                    tt += tile[chunk_at(t, k)];
-                    tile[chunk_at(t, k)] = tt;
+                    tt += tile[C::at(t, k)];
+                    tile[C::at(t, k)] = tt;
                 }
             };
             if (nc == TS) {
@@ -292,16 +304,16 @@ __global__ __launch_bounds__(TR, 5) void k_trace_bwd(const TracePrepArgs a, floa
             const __amdgpu_buffer_rsrc_t rs = rows_rsrc(prep), rp = rows_rsrc(pr);
             const int voff = (h * nt + s) * (int)sizeof(float);
 #pragma unroll 8
-            for (int i = 0; i < TR / 2; i++) {
-                const int j = 2 * i + h;
+            for (int i = 0; i < C::NL; i++) {
+                const int j = C::RPI * i + h;
                 if (j < nloc && s < nc) {
-                    const int so = 2 * i * nt + c0;  // samples
-                    const float v = tile[chunk_at(j, sv)];
+                    const int so = C::RPI * i * nt + c0;  // samples
+                    const float v = tile[C::at(j, sv)];
                     if (write_prep)
                         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), rs, voff,
                                                               so * (int)sizeof(float), 0);
                     if (pr) {
-                        const float nx = (s + 1 < nc) ? tile[chunk_at(j, sv + 1)]
+                        const float nx = (s + 1 < nc) ? tile[C::at(j, sv + 1)]
                                                       : (c0 + s + 1 < nt ? bnd[j] : 0.0f);
                         // the last pair of a trace is (0, 0): no update reads
                         // it, and rejected updates are pointed at it
@@ -315,7 +327,7 @@ __global__ __launch_bounds__(TR, 5) void k_trace_bwd(const TracePrepArgs a, floa
             }
         }
         __syncthreads();
-        if (t < nloc) bnd[t] = tile[chunk_at(t, 0)];
+        if (t < nloc) bnd[t] = tile[C::at(t, 0)];
         __syncthreads();
     }
 }
@@ -1083,12 +1095,18 @@ int launch_trace_prep2(const TracePrepArgs &a, float *pairs, float *tmp, void *s
     if (ntr <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
     const unsigned nb = (unsigned)((ntr + TR - 1) / TR);
-    hipLaunchKernelGGL(k_trace_fwd, dim3(nb), dim3(TR), 0, st, a);
     const bool diff = a.diff == 1;
     // without diff the pairs (when asked for) are the last output: the
     // samples go back to prep only for the callers that read them
-    hipLaunchKernelGGL(k_trace_bwd, dim3(nb), dim3(TR), 0, st, a,
-                       diff ? (float2 *)nullptr : (float2 *)pairs, (diff || !pairs) ? 1 : 0);
+    float2 *pr = diff ? (float2 *)nullptr : (float2 *)pairs;
+    const int wp = (diff || !pairs) ? 1 : 0;
+    // chunks of 64 samples: each visit of a trace row is 256 contiguous bytes
+    // (salt, one run: forward 1.84 ms and backward 2.51 ms, against 2.13 and
+    // 2.94 ms with chunks of 32 samples, although those let every block be
+    // resident at once)
+    constexpr int CS = 64;
+    hipLaunchKernelGGL(k_trace_fwd<CS>, dim3(nb), dim3(TR), 0, st, a);
+    hipLaunchKernelGGL(k_trace_bwd<CS>, dim3(nb), dim3(TR), 0, st, a, pr, wp);
     if (diff) {
         // diff2 into tmp, back into prep, then the pairs from the final trace
         const size_t off = (size_t)a.shot_begin * a.nh * a.nt, n = (size_t)ntr * a.nt;

@@ -2,9 +2,10 @@
 kirch_datuming_amd/csrc/kmig_prep.hip; no GPU).
 
 The replay walks each block as the kernels do -- 64 traces per block, chunks
-of 32 samples loaded as rows 2 i + h at offsets (2 i nt + c0) + (h nt + s)
-with no clamp, the swizzled tile j * 32 + (s ^ j / 2), the per-lane scans,
-the chunk-boundary sample of the backward pass -- and checks (1) every load
+of TS = 32 or 64 samples loaded as rows RPI i + h (RPI = 64 / TS) at offsets
+(RPI i nt + c0) + (h nt + s) with no clamp, the swizzled tile
+j * TS + (s ^ swz(j)), the per-lane scans, the chunk-boundary sample of the
+backward pass -- and checks (1) every load
 against the buffer length the library allocates (trace_alloc: 64 traces and
 64 samples past the last trace), (2) that the swizzle is a permutation whose
 row-wise and per-lane accesses are free of bank conflicts, and (3) the
@@ -16,7 +17,7 @@ inside the volume, skipped traces (wap < 0) and the QC shot."""
 import numpy as np
 import pytest
 
-TR, TS, PAD = 64, 32, 64
+TR, PAD = 64, 64
 f32 = np.float32
 
 
@@ -24,8 +25,8 @@ def trace_alloc(ntraces, nt):
     return (ntraces + PAD) * nt + 64
 
 
-def chunk_at(j, s):
-    return j * TS + (s ^ (j >> 1))
+def chunk_at(j, s, TS):
+    return j * TS + (s ^ ((j >> 1) if TS == 32 else j))
 
 
 def mute_w(t0, dt, boundary, taper, it):
@@ -55,7 +56,9 @@ class Geo:
         return f32(f32(self.intercept) + f32(f32(self.slope) * self.off[g]))
 
 
-def replay(G):
+def replay(G, TS):
+    RPI = 64 // TS
+    NL = TR // RPI
     nt, ntot = G.nt, G.ns * G.nh
     prep = np.full(G.alloc, np.nan, f32)
     pairs = np.full((G.alloc, 2), np.nan, f32)
@@ -65,16 +68,28 @@ def replay(G):
     nblk = (ntr + TR - 1) // TR
     loads = []
 
+    def at(j, s):
+        return chunk_at(j, s, TS)
+
+    def lane(t):
+        return t // TS, t % TS
+
     def load_chunk(buf, g0, c0):
-        pre = np.empty((TR // 2, TR), f32)  # [i][lane]
+        pre = np.empty((NL, TR), f32)  # [i][lane]
         for t in range(TR):
-            h, s = t >> 5, t & (TS - 1)
-            for i in range(TR // 2):
-                e = g0 * nt + c0 + (2 * i * nt) + (h * nt + s)
+            h, s = lane(t)
+            for i in range(NL):
+                e = g0 * nt + c0 + (RPI * i * nt) + (h * nt + s)
                 loads.append(e)
                 assert 0 <= e < G.alloc, (g0, c0, t, i, e, G.alloc)
                 pre[i, t] = buf[e]
         return pre
+
+    def stage(tile, pre):
+        for t in range(TR):
+            h, s = lane(t)
+            for i in range(NL):
+                tile[at(RPI * i + h, s)] = pre[i, t]
 
     for b in range(nblk):
         g0 = G.sb * G.nh + b * TR
@@ -89,30 +104,27 @@ def replay(G):
         pre = load_chunk(G.raw, g0, 0)
         for c0 in range(0, nt, TS):
             nc = min(TS, nt - c0)
-            for t in range(TR):
-                h, s = t >> 5, t & (TS - 1)
-                for i in range(TR // 2):
-                    tile[chunk_at(2 * i + h, s)] = pre[i, t]
+            stage(tile, pre)
             if c0 + TS < nt:
                 pre = load_chunk(G.raw, g0, c0 + TS)
 
             def qc_copy(dst):
                 for t in range(TR):
-                    h, s = t >> 5, t & (TS - 1)
-                    for j in range(qlo + h, qhi, 2):
+                    h, s = lane(t)
+                    for j in range(qlo + h, qhi, RPI):
                         if s < nc:
-                            dst[(j - qoff) * nt + c0 + s] = tile[chunk_at(j, s)]
+                            dst[(j - qoff) * nt + c0 + s] = tile[at(j, s)]
 
             def steps(t, do_mute, do_sum):
                 g = g0 + t
                 for k in range(nc):
-                    v = tile[chunk_at(t, k)]
+                    v = tile[at(t, k)]
                     if do_mute:
                         v = f32(v * mute_w(G.t0, G.dt, G.boundary(g), G.taper, c0 + k))
                     if do_sum:
                         tt[t] = v if c0 + k == 0 else f32(tt[t] + v)
                         v = tt[t]
-                    tile[chunk_at(t, k)] = v
+                    tile[at(t, k)] = v
 
             if any_qc:
                 qc_copy(qcb)
@@ -129,11 +141,11 @@ def replay(G):
                     if G.mute or keep:
                         steps(t, bool(G.mute), keep)
             for t in range(TR):
-                h, s = t >> 5, t & (TS - 1)
-                for i in range(TR // 2):
-                    j = 2 * i + h
+                h, s = lane(t)
+                for i in range(NL):
+                    j = RPI * i + h
                     if j < nloc and s < nc:
-                        prep[(g0 + j) * nt + c0 + s] = tile[chunk_at(j, s)]
+                        prep[(g0 + j) * nt + c0 + s] = tile[at(j, s)]
     for b in range(nblk):
         g0 = G.sb * G.nh + b * TR
         nloc = min(TR, ntr - b * TR)
@@ -145,10 +157,7 @@ def replay(G):
         out = prep.copy()
         for c0 in range(last, -1, -TS):
             nc = min(TS, nt - c0)
-            for t in range(TR):
-                h, s = t >> 5, t & (TS - 1)
-                for i in range(TR // 2):
-                    tile[chunk_at(2 * i + h, s)] = pre[i, t]
+            stage(tile, pre)
             if c0 > 0:
                 pre = load_chunk(prep, g0, c0 - TS)
             for t in range(nloc):
@@ -156,23 +165,23 @@ def replay(G):
                     continue
                 for k in range(nc - 1, -1, -1):
                     if c0 + k == nt - 1:
-                        tt[t] = tile[chunk_at(t, k)]
+                        tt[t] = tile[at(t, k)]
                     else:
-                        tt[t] = f32(tt[t] + tile[chunk_at(t, k)])
-                        tile[chunk_at(t, k)] = tt[t]
+                        tt[t] = f32(tt[t] + tile[at(t, k)])
+                        tile[at(t, k)] = tt[t]
             for t in range(TR):
-                h, s = t >> 5, t & (TS - 1)
-                for i in range(TR // 2):
-                    j = 2 * i + h
+                h, s = lane(t)
+                for i in range(NL):
+                    j = RPI * i + h
                     if j < nloc and s < nc:
                         o = (g0 + j) * nt + c0 + s
-                        v = tile[chunk_at(j, s)]
+                        v = tile[at(j, s)]
                         out[o] = v
-                        nx = tile[chunk_at(j, s + 1)] if s + 1 < nc else \
+                        nx = tile[at(j, s + 1)] if s + 1 < nc else \
                             (bnd[j] if c0 + s + 1 < nt else f32(0))
                         pairs[o] = (v, f32(nx - v)) if c0 + s + 1 < nt else (0, 0)
             for t in range(nloc):
-                bnd[t] = tile[chunk_at(t, 0)]
+                bnd[t] = tile[at(t, 0)]
         prep[g0 * nt:(g0 + nloc) * nt] = out[g0 * nt:(g0 + nloc) * nt]
     return prep, pairs, qcb, qca, loads
 
@@ -209,16 +218,17 @@ def direct(G):
     return prep, pairs, qcb, qca
 
 
+@pytest.mark.parametrize("TS", [32, 64])
 @pytest.mark.parametrize("ns,nh,nt,sb,se,mute,qc", [
     (1, 1, 2, 0, 1, 0, -1),        # one trace, two samples
     (2, 33, 17, 0, 2, 1, 1),       # 66 traces (two blocks, the second of 2), nt < chunk
-    (3, 32, 32, 1, 3, 1, 2),       # shot range from the second shot, nt = chunk
-    (2, 65, 33, 0, 2, 0, -1),      # 130 traces, nt one past a chunk
+    (3, 32, 32, 1, 3, 1, 2),       # shot range from the second shot, nt = 32
+    (2, 65, 65, 0, 2, 0, -1),      # 130 traces, nt one past 64
     (1, 64, 100, 0, 1, 1, 0),      # one full block, QC shot, partial last chunk
 ])
-def test_trace_prep_replay(ns, nh, nt, sb, se, mute, qc):
+def test_trace_prep_replay(ns, nh, nt, sb, se, mute, qc, TS):
     G = Geo(ns, nh, nt, sb, se, mute, qc, seed=ns * 1000 + nh * 10 + nt)
-    prep, pairs, qcb, qca, loads = replay(G)
+    prep, pairs, qcb, qca, loads = replay(G, TS)
     rp, rpairs, rqcb, rqca = direct(G)
     lo, hi = G.sb * G.nh * nt, G.se * G.nh * nt
     assert np.array_equal(prep[lo:hi].view(np.uint32), rp[lo:hi].view(np.uint32))
@@ -230,14 +240,16 @@ def test_trace_prep_replay(ns, nh, nt, sb, se, mute, qc):
     assert max(loads) < G.alloc
 
 
-def test_swizzle_is_conflict_free():
-    idx = [chunk_at(j, s) for j in range(TR) for s in range(TS)]
+@pytest.mark.parametrize("TS", [32, 64])
+def test_swizzle_is_conflict_free(TS):
+    RPI = 64 // TS
+    idx = [chunk_at(j, s, TS) for j in range(TR) for s in range(TS)]
     assert sorted(idx) == list(range(TR * TS))
-    for i in range(TR // 2):  # row-wise: lanes (h, s) at row 2 i + h
-        banks = [chunk_at(2 * i + (t >> 5), t & 31) % 64 for t in range(64)]
+    for i in range(TR // RPI):  # row-wise: lanes (h, s) at row RPI i + h
+        banks = [chunk_at(RPI * i + t // TS, t % TS, TS) % 64 for t in range(64)]
         assert len(set(banks)) == 64
     for k in range(TS):  # scan: lane t at row t
-        banks = [chunk_at(t, k) % 64 for t in range(64)]
+        banks = [chunk_at(t, k, TS) % 64 for t in range(64)]
         assert len(set(banks)) == 64
 
 
@@ -246,4 +258,4 @@ def test_replay_catches_a_short_allocation():
     G = Geo(1, 65, 17, 0, 1, 0, -1, seed=3)
     G.alloc = 65 * 17
     with pytest.raises(AssertionError):
-        replay(G)
+        replay(G, 64)
