@@ -86,31 +86,38 @@ __global__ __launch_bounds__(256) void k_trace_prep(const TracePrepArgs a)
     }
 }
 
-// Coalesced form of the trace preparation: a block of one wave owns TR = 64
-// traces and walks them in chunks of TS samples staged through LDS, so every
-// global access is a row segment of TS samples, while each lane still forms
-// its own trace's running sums sequentially (the reference's rounding order).
-// The next chunk's loads are issued before the current chunk is scanned, all
-// at once, so they overlap the scan and the stores.
+// Coalesced form of the trace preparation: a block of one wave owns NTR
+// traces and walks them in chunks of CS samples staged through LDS, each lane
+// loading 16 bytes (4 samples) of a row, so every visit of a trace row is CS
+// contiguous samples; lanes 0 .. NTR - 1 then form their own traces' running
+// sums sequentially (the reference's rounding order).  The next chunk's
+// loads are issued before the current chunk is scanned, all at once, so they
+// overlap the scan and the stores.
 //   k_trace_fwd: mute (+QC copies) and the causal sum of doubint
 //   k_trace_bwd: the anticausal sum, written as samples (when asked for) and
 //                as (tr[k], tr[k+1] - tr[k]) pairs (when asked for)
-constexpr int TR = 64;  // traces per block (lane t scans trace t)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x2 __attribute__((ext_vector_type(2)));
 
-// Chunks of TS samples (32 or 64): one load instruction covers RPI = 64 / TS
-// trace rows (lanes (h, s) = (t / TS, t % TS) take sample s of row RPI i + h),
-// NL = TR / RPI instructions a chunk.  Element (trace j, sample s) sits at
-// j * TS + (s ^ swz(j)) with swz(j) = j / 2 (TS = 32) or j (TS = 64), so
-// both access patterns are free of bank conflicts: a row-wise access (banks
-// 32 h + (s ^ swz), or s ^ j) and the scan (lane t at row t, one sample k:
-// banks 32 (t & 1) + (k ^ t/2), or k ^ t, all distinct).
-template <int TS> struct Chunk {
-    static_assert(TS == 32 || TS == 64, "chunks of 32 or 64 samples");
-    static constexpr int RPI = 64 / TS, NL = TR / RPI;
+// A chunk of NTR traces x CS samples: QPR = CS / 4 quads (16 bytes) per row,
+// one load instruction covers RPI = 64 / QPR rows (lane (h, q) = (t / QPR,
+// t % QPR) takes samples 4 q .. 4 q + 3 of row RPI i + h), NL = NTR / RPI
+// instructions a chunk.  Element (trace j, sample s) sits in quad (s / 4) ^ j
+// of row j, so the scan (lane t at row t, one sample) touches NTR distinct
+// quads -- distinct banks for NTR <= 16 -- while the row-wise 16-byte
+// accesses stay whole quads.
+template <int NTR, int CS> struct Chunk {
+    static constexpr int QPR = CS / 4, RPI = 64 / QPR, NL = NTR / RPI;
+    static_assert(64 % QPR == 0 && NTR % RPI == 0 && NTR <= QPR, "chunk shape");
     static __device__ __forceinline__ int at(int j, int s)
     {
-        return j * TS + (s ^ (TS == 32 ? (j >> 1) : j));
+        return j * CS + ((((s >> 2) ^ j) & (QPR - 1)) << 2) + (s & 3);
+    }
+    // the quad of samples 4 q .. 4 q + 3 of row j
+    static __device__ __forceinline__ f32x4 *quad(float *tile, int j, int q)
+    {
+        return (f32x4 *)(tile + j * CS + (((q ^ j) & (QPR - 1)) << 2));
     }
 };
 
@@ -128,47 +135,46 @@ __device__ __forceinline__ float mute_weight(const TracePrepArgs &a, float bound
     return w;
 }
 
-// The chunk of samples c0 .. c0 + TS - 1 of the block's traces into
-// registers: lane (h, s) takes sample c0 + s of trace RPI i + h (base = the
-// block's first trace).  Every lane loads, with no clamp and no branch, so
-// all NL loads are in flight together: trace buffers are trace_alloc() long,
-// which covers a block's TR rows from any valid first trace plus the TS - 1
-// samples a chunk reaches past a row's end (values past nloc traces or nt
-// samples are never used).  Buffer loads: the lane's part of the offset,
-// (h nt + s) samples, is one register for all NL loads, and the row and
-// chunk part, (RPI i nt + c0) samples, is a scalar (flat loads would hold a
-// 64-bit address per row).  Offsets within a block's rows fit in 32 bits
-// (TR x nt samples).  (The same split of offsets serves the stores: a
-// block's rows are a buffer from its first trace, with no range limit;
-// every access is inside the allocation by construction.)
+// A block's rows as a buffer from its first trace, with no range limit:
+// every access is inside the allocation by construction (below).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const void *base)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, -1, 0x00020000);
 }
 
-template <int TS>
-__device__ __forceinline__ void load_chunk(float (&v)[Chunk<TS>::NL], const float *base, int nt,
-                                           int c0, int h, int s)
+// The chunk of samples c0 .. c0 + CS - 1 of the block's traces into
+// registers (base = the block's first trace).  Every lane loads, with no
+// clamp and no branch, so all NL loads are in flight together: trace buffers
+// are trace_alloc() long, which covers a block's rows from any valid first
+// trace plus the CS - 1 samples a chunk reaches past a row's end (values past
+// nloc traces or nt samples are never used).  Buffer loads: the lane's part
+// of the offset, (h nt + 4 q) samples, is one register for all NL loads, and
+// the row and chunk part, (RPI i nt + c0) samples, is a scalar; offsets
+// within a block's rows fit in 32 bits.  Rows of an odd nt are not 16-byte
+// aligned: the loads need 4-byte alignment only.
+template <class C>
+__device__ __forceinline__ void load_chunk(f32x4 (&v)[C::NL], const float *base, int nt, int c0,
+                                           int h, int q)
 {
     const __amdgpu_buffer_rsrc_t rs = rows_rsrc(base);
-    const int voff = (h * nt + s) * (int)sizeof(float);
+    const int voff = (h * nt + 4 * q) * (int)sizeof(float);
 #pragma unroll
-    for (int i = 0; i < Chunk<TS>::NL; i++)
+    for (int i = 0; i < C::NL; i++)
         v[i] = __builtin_bit_cast(
-            float, __builtin_amdgcn_raw_buffer_load_b32(
-                       rs, voff, (Chunk<TS>::RPI * i * nt + c0) * (int)sizeof(float), 0));
+            f32x4, __builtin_amdgcn_raw_buffer_load_b128(
+                       rs, voff, (C::RPI * i * nt + c0) * (int)sizeof(float), 0));
 }
 
-template <int TS>
-__global__ __launch_bounds__(TR, 5) void k_trace_fwd(const TracePrepArgs a)
+template <int NTR, int CS>
+__global__ __launch_bounds__(64, 4) void k_trace_fwd(const TracePrepArgs a)
 {
-    using C = Chunk<TS>;
-    __shared__ float tile[TR * TS];
-    const int t = threadIdx.x, h = t / TS, s = t % TS;
+    using C = Chunk<NTR, CS>;
+    __shared__ __align__(16) float tile[NTR * CS];
+    const int t = threadIdx.x, h = t / C::QPR, q = t % C::QPR;
     const int nt = a.nt;
     const long ntr = (long)(a.shot_end - a.shot_begin) * a.nh;
-    const long g0 = (long)a.shot_begin * a.nh + (long)blockIdx.x * TR;
-    const int nloc = (int)min((long)TR, ntr - (long)blockIdx.x * TR);
+    const long g0 = (long)a.shot_begin * a.nh + (long)blockIdx.x * NTR;
+    const int nloc = (int)min((long)NTR, ntr - (long)blockIdx.x * NTR);
     const bool mine = t < nloc;
     TraceMeta m{};
     if (mine) m = a.meta[g0 + t];
@@ -190,23 +196,25 @@ __global__ __launch_bounds__(TR, 5) void k_trace_fwd(const TracePrepArgs a)
     // QC copies of the chunk's rows of the QC shot (before or after the mute)
     auto qc_copy = [&](float *dst, int c0, int nc) {
         for (int j = qlo + h; j < qhi; j += C::RPI)
-            if (s < nc) dst[(size_t)(j - qoff) * nt + c0 + s] = tile[C::at(j, s)];
+            for (int e = 0; e < 4; e++)
+                if (4 * q + e < nc)
+                    dst[(size_t)(j - qoff) * nt + c0 + 4 * q + e] = tile[C::at(j, 4 * q + e)];
     };
     const float *__restrict__ raw = a.raw + g0 * nt;
     float *__restrict__ prep = a.prep + g0 * nt;
-    float pre[C::NL];
-    load_chunk<TS>(pre, raw, nt, 0, h, s);
+    f32x4 pre[C::NL];
+    load_chunk<C>(pre, raw, nt, 0, h, q);
     float tt = 0.0f;
-    for (int c0 = 0; c0 < nt; c0 += TS) {
-        const int nc = min(TS, nt - c0);
+    for (int c0 = 0; c0 < nt; c0 += CS) {
+        const int nc = min(CS, nt - c0);
         // the tile addresses of the row-wise accesses are formed anew in each
         // chunk (one XOR each), not held in registers across chunks
-        int sv = s;
-        asm volatile("" : "+v"(sv));
+        int qv = q;
+        asm volatile("" : "+v"(qv));
 #pragma unroll
-        for (int i = 0; i < C::NL; i++) tile[C::at(C::RPI * i + h, sv)] = pre[i];
+        for (int i = 0; i < C::NL; i++) *C::quad(tile, C::RPI * i + h, qv) = pre[i];
         __syncthreads();
-        if (c0 + TS < nt) load_chunk<TS>(pre, raw, nt, c0 + TS, h, s);
+        if (c0 + CS < nt) load_chunk<C>(pre, raw, nt, c0 + CS, h, q);
         // mute, then the causal running sum, of the lane's own trace
         auto step = [&](int k, bool do_mute, bool do_sum) {
             float v = tile[C::at(t, k)];
@@ -229,24 +237,33 @@ __global__ __launch_bounds__(TR, 5) void k_trace_fwd(const TracePrepArgs a)
                 for (int k = 0; k < nc; k++) step(k, false, true);
         } else if (mine && (a.mute || keep)) {
             const bool mu = a.mute != 0;
-            if (nc == TS) {
+            if (nc == CS) {
 #pragma unroll 8
-                for (int k = 0; k < TS; k++) step(k, mu, keep);
+                for (int k = 0; k < CS; k++) step(k, mu, keep);
             } else {
                 for (int k = 0; k < nc; k++) step(k, mu, keep);
             }
         }
         __syncthreads();
         {
+            // whole quads inside the chunk go out as one 16-byte store, the
+            // last chunk's partial quad sample by sample
             const __amdgpu_buffer_rsrc_t rs = rows_rsrc(prep);
-            const int voff = (h * nt + s) * (int)sizeof(float);
-#pragma unroll 8
+            const int voff = (h * nt + 4 * q) * (int)sizeof(float);
+#pragma unroll 4
             for (int i = 0; i < C::NL; i++) {
                 const int j = C::RPI * i + h;
-                if (j < nloc && s < nc)
-                    __builtin_amdgcn_raw_buffer_store_b32(
-                        __builtin_bit_cast(int, tile[C::at(j, sv)]), rs, voff,
-                        (C::RPI * i * nt + c0) * (int)sizeof(float), 0);
+                if (j >= nloc || 4 * q >= nc) continue;
+                const int so = (C::RPI * i * nt + c0) * (int)sizeof(float);
+                const f32x4 v = *C::quad(tile, j, qv);
+                if (4 * q + 4 <= nc) {
+                    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(i32x4, v), rs, voff,
+                                                           so, 0);
+                } else {
+                    for (int e = 0; e < 4 && 4 * q + e < nc; e++)
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v[e]), rs,
+                                                              voff + 4 * e, so, 0);
+                }
             }
         }
         __syncthreads();
@@ -255,33 +272,33 @@ __global__ __launch_bounds__(TR, 5) void k_trace_fwd(const TracePrepArgs a)
 
 // write_prep: the anticausal sums also go back to prep (the stream kernel
 // and diff reads them; the resident kernels read only the pairs).
-template <int TS>
-__global__ __launch_bounds__(TR, 5) void k_trace_bwd(const TracePrepArgs a, float2 *pairs,
+template <int NTR, int CS>
+__global__ __launch_bounds__(64, 4) void k_trace_bwd(const TracePrepArgs a, float2 *pairs,
                                                      int write_prep)
 {
-    using C = Chunk<TS>;
-    __shared__ float tile[TR * TS];
-    __shared__ float bnd[TR];  // first sample of the chunk after this one
-    const int t = threadIdx.x, h = t / TS, s = t % TS;
+    using C = Chunk<NTR, CS>;
+    __shared__ __align__(16) float tile[NTR * CS];
+    __shared__ float bnd[NTR];  // first sample of the chunk after this one
+    const int t = threadIdx.x, h = t / C::QPR, q = t % C::QPR;
     const int nt = a.nt;
     const long ntr = (long)(a.shot_end - a.shot_begin) * a.nh;
-    const long g0 = (long)a.shot_begin * a.nh + (long)blockIdx.x * TR;
-    const int nloc = (int)min((long)TR, ntr - (long)blockIdx.x * TR);
+    const long g0 = (long)a.shot_begin * a.nh + (long)blockIdx.x * NTR;
+    const int nloc = (int)min((long)NTR, ntr - (long)blockIdx.x * NTR);
     const bool mine = t < nloc && a.meta[g0 + t].wap >= 0.0f;
-    const int last = ((nt - 1) / TS) * TS;
+    const int last = ((nt - 1) / CS) * CS;
     float *prep = a.prep + g0 * nt;
     float2 *pr = pairs ? pairs + g0 * nt : nullptr;
-    float pre[C::NL];
-    load_chunk<TS>(pre, prep, nt, last, h, s);
+    f32x4 pre[C::NL];
+    load_chunk<C>(pre, prep, nt, last, h, q);
     float tt = 0.0f;
-    for (int c0 = last; c0 >= 0; c0 -= TS) {
-        const int nc = min(TS, nt - c0);
-        int sv = s;  // as in k_trace_fwd
-        asm volatile("" : "+v"(sv));
+    for (int c0 = last; c0 >= 0; c0 -= CS) {
+        const int nc = min(CS, nt - c0);
+        int qv = q;  // as in k_trace_fwd
+        asm volatile("" : "+v"(qv));
 #pragma unroll
-        for (int i = 0; i < C::NL; i++) tile[C::at(C::RPI * i + h, sv)] = pre[i];
+        for (int i = 0; i < C::NL; i++) *C::quad(tile, C::RPI * i + h, qv) = pre[i];
         __syncthreads();
-        if (c0 > 0) load_chunk<TS>(pre, prep, nt, c0 - TS, h, s);
+        if (c0 > 0) load_chunk<C>(pre, prep, nt, c0 - CS, h, q);
         if (mine) {
             auto step = [&](int k) {
                 const int it = c0 + k;
@@ -292,9 +309,9 @@ __global__ __launch_bounds__(TR, 5) void k_trace_bwd(const TracePrepArgs a, floa
                     tile[C::at(t, k)] = tt;
                 }
             };
-            if (nc == TS) {
+            if (nc == CS) {
 #pragma unroll 8
-                for (int k = TS - 1; k >= 0; k--) step(k);
+                for (int k = CS - 1; k >= 0; k--) step(k);
             } else {
                 for (int k = nc - 1; k >= 0; k--) step(k);
             }
@@ -302,26 +319,54 @@ __global__ __launch_bounds__(TR, 5) void k_trace_bwd(const TracePrepArgs a, floa
         __syncthreads();
         {
             const __amdgpu_buffer_rsrc_t rs = rows_rsrc(prep), rp = rows_rsrc(pr);
-            const int voff = (h * nt + s) * (int)sizeof(float);
-#pragma unroll 8
+            const int voff = (h * nt + 4 * q) * (int)sizeof(float);
+#pragma unroll 4
             for (int i = 0; i < C::NL; i++) {
                 const int j = C::RPI * i + h;
-                if (j < nloc && s < nc) {
-                    const int so = C::RPI * i * nt + c0;  // samples
-                    const float v = tile[C::at(j, sv)];
-                    if (write_prep)
-                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), rs, voff,
-                                                              so * (int)sizeof(float), 0);
-                    if (pr) {
-                        const float nx = (s + 1 < nc) ? tile[C::at(j, sv + 1)]
-                                                      : (c0 + s + 1 < nt ? bnd[j] : 0.0f);
-                        // the last pair of a trace is (0, 0): no update reads
-                        // it, and rejected updates are pointed at it
-                        const float2 q = c0 + s + 1 < nt ? make_float2(v, nx - v)
-                                                         : make_float2(0.0f, 0.0f);
-                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(i32x2, q), rp,
-                                                              2 * voff,
-                                                              so * (int)sizeof(float2), 0);
+                if (j >= nloc || 4 * q >= nc) continue;
+                const int so = C::RPI * i * nt + c0;  // samples
+                const f32x4 v = *C::quad(tile, j, qv);
+                // the sample after the quad: the next quad's first, the first
+                // of the chunk after this one, or none past the trace's end
+                const int sn = 4 * q + 4;
+                const float after = sn < nc ? tile[C::at(j, sn)] : (c0 + sn < nt ? bnd[j] : 0.0f);
+                const bool whole = 4 * q + 4 <= nc;
+                if (write_prep) {
+                    if (whole) {
+                        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(i32x4, v), rs,
+                                                               voff, so * (int)sizeof(float), 0);
+                    } else {
+                        for (int e = 0; e < 4 && 4 * q + e < nc; e++)
+                            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v[e]), rs,
+                                                                  voff + 4 * e,
+                                                                  so * (int)sizeof(float), 0);
+                    }
+                }
+                if (pr) {
+                    // pairs (tr[k], tr[k+1] - tr[k]); the last pair of a trace
+                    // is (0, 0): no update reads it, and rejected updates are
+                    // pointed at it
+                    float pv[8];
+                    for (int e = 0; e < 4; e++) {
+                        const float nx = e < 3 ? v[e + 1] : after;
+                        const bool in = c0 + 4 * q + e + 1 < nt;
+                        pv[2 * e] = in ? v[e] : 0.0f;
+                        pv[2 * e + 1] = in ? nx - v[e] : 0.0f;
+                    }
+                    if (whole) {
+                        const f32x4 p0 = {pv[0], pv[1], pv[2], pv[3]};
+                        const f32x4 p1 = {pv[4], pv[5], pv[6], pv[7]};
+                        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(i32x4, p0), rp,
+                                                               2 * voff,
+                                                               so * (int)sizeof(float2), 0);
+                        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(i32x4, p1), rp,
+                                                               2 * voff + 16,
+                                                               so * (int)sizeof(float2), 0);
+                    } else {
+                        for (int e = 0; e < 4 && 4 * q + e < nc; e++)
+                            __builtin_amdgcn_raw_buffer_store_b64(
+                                __builtin_bit_cast(i32x2, make_float2(pv[2 * e], pv[2 * e + 1])),
+                                rp, 2 * voff + 8 * e, so * (int)sizeof(float2), 0);
                     }
                 }
             }
@@ -1094,19 +1139,31 @@ int launch_trace_prep2(const TracePrepArgs &a, float *pairs, float *tmp, void *s
     const long ntr = (long)(a.shot_end - a.shot_begin) * a.nh;
     if (ntr <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
-    const unsigned nb = (unsigned)((ntr + TR - 1) / TR);
     const bool diff = a.diff == 1;
     // without diff the pairs (when asked for) are the last output: the
     // samples go back to prep only for the callers that read them
     float2 *pr = diff ? (float2 *)nullptr : (float2 *)pairs;
     const int wp = (diff || !pairs) ? 1 : 0;
-    // chunks of 64 samples: each visit of a trace row is 256 contiguous bytes
-    // (salt, one run: forward 1.84 ms and backward 2.51 ms, against 2.13 and
-    // 2.94 ms with chunks of 32 samples, although those let every block be
-    // resident at once)
-    constexpr int CS = 64;
-    hipLaunchKernelGGL(k_trace_fwd<CS>, dim3(nb), dim3(TR), 0, st, a);
-    hipLaunchKernelGGL(k_trace_bwd<CS>, dim3(nb), dim3(TR), 0, st, a, pr, wp);
+    if (a.nt < 16) {
+        // traces too short for the chunk kernels' unclamped loads to stay
+        // inside trace_alloc()'s padding (a block reaches NTR nt + CS - 2
+        // samples past its first trace, the padding covers 64 nt + 64): one
+        // thread per trace, then the pairs
+        hipLaunchKernelGGL(k_trace_prep, dim3(nblk((size_t)ntr, 256)), dim3(256), 0, st, a);
+        if (pairs) {
+            const size_t off = (size_t)a.shot_begin * a.nh * a.nt, n = (size_t)ntr * a.nt;
+            hipLaunchKernelGGL(k_pairs, dim3(nblk(n, 256)), dim3(256), 0, st, a.prep + off,
+                               (float2 *)pairs + off, n, a.nt);
+        }
+        return (int)hipGetLastError();
+    }
+    // Chunk shapes measured on salt (one rocprof run each): forward 32 x 128
+    // 1.52 ms, 16 x 256 1.79 ms; backward 16 x 256 1.84 ms, 32 x 128 2.07 ms
+    // (64 traces x 64 samples, 4-byte lanes: 1.84 and 2.51 ms)
+    hipLaunchKernelGGL((k_trace_fwd<32, 128>), dim3((unsigned)((ntr + 31) / 32)), dim3(64), 0, st,
+                       a);
+    hipLaunchKernelGGL((k_trace_bwd<16, 256>), dim3((unsigned)((ntr + 15) / 16)), dim3(64), 0, st,
+                       a, pr, wp);
     if (diff) {
         // diff2 into tmp, back into prep, then the pairs from the final trace
         const size_t off = (size_t)a.shot_begin * a.nh * a.nt, n = (size_t)ntr * a.nt;

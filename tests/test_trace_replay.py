@@ -1,11 +1,12 @@
 """CPU replay of the coalesced trace preparation (k_trace_fwd, k_trace_bwd in
 kirch_datuming_amd/csrc/kmig_prep.hip; no GPU).
 
-The replay walks each block as the kernels do -- 64 traces per block, chunks
-of TS = 32 or 64 samples loaded as rows RPI i + h (RPI = 64 / TS) at offsets
-(RPI i nt + c0) + (h nt + s) with no clamp, the swizzled tile
-j * TS + (s ^ swz(j)), the per-lane scans, the chunk-boundary sample of the
-backward pass -- and checks (1) every load
+The replay walks each block as the kernels do -- NTR traces per block, chunks
+of CS samples loaded as 16-byte quads, rows RPI i + h (RPI = 256 / CS) at
+offsets (RPI i nt + c0) + (h nt + 4 q) with no clamp, the swizzled tile
+(quad (s / 4) ^ j of row j), the per-lane scans, the chunk-boundary sample of
+the backward pass -- for both shapes the library has (16 x 256, 32 x 128),
+and checks (1) every load
 against the buffer length the library allocates (trace_alloc: 64 traces and
 64 samples past the last trace), (2) that the swizzle is a permutation whose
 row-wise and per-lane accesses are free of bank conflicts, and (3) the
@@ -25,8 +26,9 @@ def trace_alloc(ntraces, nt):
     return (ntraces + PAD) * nt + 64
 
 
-def chunk_at(j, s, TS):
-    return j * TS + (s ^ ((j >> 1) if TS == 32 else j))
+def chunk_at(j, s, CS):
+    QPR = CS // 4
+    return j * CS + ((((s >> 2) ^ j) & (QPR - 1)) << 2) + (s & 3)
 
 
 def mute_w(t0, dt, boundary, taper, it):
@@ -56,64 +58,68 @@ class Geo:
         return f32(f32(self.intercept) + f32(f32(self.slope) * self.off[g]))
 
 
-def replay(G, TS):
-    RPI = 64 // TS
-    NL = TR // RPI
+def replay(G, NTR, CS):
+    QPR = CS // 4
+    RPI = 64 // QPR
+    NL = NTR // RPI
     nt, ntot = G.nt, G.ns * G.nh
     prep = np.full(G.alloc, np.nan, f32)
     pairs = np.full((G.alloc, 2), np.nan, f32)
     qcb = np.full(G.nh * nt, np.nan, f32)
     qca = np.full(G.nh * nt, np.nan, f32)
     ntr = (G.se - G.sb) * G.nh
-    nblk = (ntr + TR - 1) // TR
+    nblk = (ntr + NTR - 1) // NTR
     loads = []
 
     def at(j, s):
-        return chunk_at(j, s, TS)
+        return chunk_at(j, s, CS)
 
     def lane(t):
-        return t // TS, t % TS
+        return t // QPR, t % QPR
 
     def load_chunk(buf, g0, c0):
-        pre = np.empty((NL, TR), f32)  # [i][lane]
-        for t in range(TR):
-            h, s = lane(t)
+        pre = np.empty((NL, 64, 4), f32)  # [i][lane][sample of the quad]
+        for t in range(64):
+            h, q = lane(t)
             for i in range(NL):
-                e = g0 * nt + c0 + (RPI * i * nt) + (h * nt + s)
-                loads.append(e)
-                assert 0 <= e < G.alloc, (g0, c0, t, i, e, G.alloc)
-                pre[i, t] = buf[e]
+                for e in range(4):
+                    x = g0 * nt + c0 + (RPI * i * nt) + (h * nt + 4 * q) + e
+                    loads.append(x)
+                    assert 0 <= x < G.alloc, (g0, c0, t, i, x, G.alloc)
+                    pre[i, t, e] = buf[x]
         return pre
 
     def stage(tile, pre):
-        for t in range(TR):
-            h, s = lane(t)
+        for t in range(64):
+            h, q = lane(t)
             for i in range(NL):
-                tile[at(RPI * i + h, s)] = pre[i, t]
+                for e in range(4):
+                    tile[at(RPI * i + h, 4 * q + e)] = pre[i, t, e]
 
     for b in range(nblk):
-        g0 = G.sb * G.nh + b * TR
-        nloc = min(TR, ntr - b * TR)
+        g0 = G.sb * G.nh + b * NTR
+        nloc = min(NTR, ntr - b * NTR)
         # forward
-        tile = np.full(TR * TS, np.nan, f32)
+        tile = np.full(NTR * CS, np.nan, f32)
         qoff = G.qc_shot * G.nh - g0
         any_qc = G.mute and G.qc_shot >= 0 and g0 < G.qc_shot * G.nh + G.nh and \
             g0 + nloc > G.qc_shot * G.nh
         qlo, qhi = max(0, qoff), min(nloc, G.qc_shot * G.nh + G.nh - g0)
-        tt = np.zeros(TR, f32)
+        tt = np.zeros(NTR, f32)
         pre = load_chunk(G.raw, g0, 0)
-        for c0 in range(0, nt, TS):
-            nc = min(TS, nt - c0)
+        for c0 in range(0, nt, CS):
+            nc = min(CS, nt - c0)
             stage(tile, pre)
-            if c0 + TS < nt:
-                pre = load_chunk(G.raw, g0, c0 + TS)
+            if c0 + CS < nt:
+                pre = load_chunk(G.raw, g0, c0 + CS)
 
             def qc_copy(dst):
-                for t in range(TR):
-                    h, s = lane(t)
+                for t in range(64):
+                    h, q = lane(t)
                     for j in range(qlo + h, qhi, RPI):
-                        if s < nc:
-                            dst[(j - qoff) * nt + c0 + s] = tile[at(j, s)]
+                        for e in range(4):
+                            if 4 * q + e < nc:
+                                dst[(j - qoff) * nt + c0 + 4 * q + e] = tile[at(j, 4 * q + e)]
 
             def steps(t, do_mute, do_sum):
                 g = g0 + t
@@ -140,26 +146,29 @@ def replay(G, TS):
                     keep = G.wap[g0 + t] >= 0
                     if G.mute or keep:
                         steps(t, bool(G.mute), keep)
-            for t in range(TR):
-                h, s = lane(t)
+            for t in range(64):
+                h, q = lane(t)
                 for i in range(NL):
                     j = RPI * i + h
-                    if j < nloc and s < nc:
-                        prep[(g0 + j) * nt + c0 + s] = tile[at(j, s)]
+                    if j >= nloc or 4 * q >= nc:
+                        continue
+                    for e in range(4):
+                        if 4 * q + e < nc:
+                            prep[(g0 + j) * nt + c0 + 4 * q + e] = tile[at(j, 4 * q + e)]
     for b in range(nblk):
-        g0 = G.sb * G.nh + b * TR
-        nloc = min(TR, ntr - b * TR)
-        tile = np.full(TR * TS, np.nan, f32)
-        bnd = np.full(TR, np.nan, f32)
-        tt = np.zeros(TR, f32)
-        last = ((nt - 1) // TS) * TS
+        g0 = G.sb * G.nh + b * NTR
+        nloc = min(NTR, ntr - b * NTR)
+        tile = np.full(NTR * CS, np.nan, f32)
+        bnd = np.full(NTR, np.nan, f32)
+        tt = np.zeros(NTR, f32)
+        last = ((nt - 1) // CS) * CS
         pre = load_chunk(prep, g0, last)
         out = prep.copy()
-        for c0 in range(last, -1, -TS):
-            nc = min(TS, nt - c0)
+        for c0 in range(last, -1, -CS):
+            nc = min(CS, nt - c0)
             stage(tile, pre)
             if c0 > 0:
-                pre = load_chunk(prep, g0, c0 - TS)
+                pre = load_chunk(prep, g0, c0 - CS)
             for t in range(nloc):
                 if G.wap[g0 + t] < 0:
                     continue
@@ -169,17 +178,22 @@ def replay(G, TS):
                     else:
                         tt[t] = f32(tt[t] + tile[at(t, k)])
                         tile[at(t, k)] = tt[t]
-            for t in range(TR):
-                h, s = lane(t)
+            for t in range(64):
+                h, q = lane(t)
                 for i in range(NL):
                     j = RPI * i + h
-                    if j < nloc and s < nc:
-                        o = (g0 + j) * nt + c0 + s
-                        v = tile[at(j, s)]
-                        out[o] = v
-                        nx = tile[at(j, s + 1)] if s + 1 < nc else \
-                            (bnd[j] if c0 + s + 1 < nt else f32(0))
-                        pairs[o] = (v, f32(nx - v)) if c0 + s + 1 < nt else (0, 0)
+                    if j >= nloc or 4 * q >= nc:
+                        continue
+                    v = [tile[at(j, 4 * q + e)] if 4 * q + e < CS else np.nan for e in range(4)]
+                    sn = 4 * q + 4
+                    after = tile[at(j, sn)] if sn < nc else (bnd[j] if c0 + sn < nt else f32(0))
+                    for e in range(4):
+                        if 4 * q + e >= nc:
+                            break
+                        o = (g0 + j) * nt + c0 + 4 * q + e
+                        out[o] = v[e]
+                        nx = v[e + 1] if e < 3 else after
+                        pairs[o] = (v[e], f32(nx - v[e])) if c0 + 4 * q + e + 1 < nt else (0, 0)
             for t in range(nloc):
                 bnd[t] = tile[at(t, 0)]
         prep[g0 * nt:(g0 + nloc) * nt] = out[g0 * nt:(g0 + nloc) * nt]
@@ -218,17 +232,21 @@ def direct(G):
     return prep, pairs, qcb, qca
 
 
-@pytest.mark.parametrize("TS", [32, 64])
+SHAPES = [(16, 256), (32, 128)]
+
+
+@pytest.mark.parametrize("ntr_cs", SHAPES)
 @pytest.mark.parametrize("ns,nh,nt,sb,se,mute,qc", [
-    (1, 1, 2, 0, 1, 0, -1),        # one trace, two samples
-    (2, 33, 17, 0, 2, 1, 1),       # 66 traces (two blocks, the second of 2), nt < chunk
-    (3, 32, 32, 1, 3, 1, 2),       # shot range from the second shot, nt = 32
-    (2, 65, 65, 0, 2, 0, -1),      # 130 traces, nt one past 64
-    (1, 64, 100, 0, 1, 1, 0),      # one full block, QC shot, partial last chunk
+    (1, 1, 16, 0, 1, 0, -1),       # one trace of the shortest length the chunk kernels take
+    (2, 33, 17, 0, 2, 1, 1),       # 66 traces (the last block partial), nt < chunk
+    (3, 12, 32, 1, 3, 1, 2),       # shot range from the second shot
+    (2, 20, 257, 0, 2, 0, -1),     # nt one past a 256-sample chunk
+    (1, 40, 300, 0, 1, 1, 0),      # QC shot, partial last chunk of both shapes
 ])
-def test_trace_prep_replay(ns, nh, nt, sb, se, mute, qc, TS):
+def test_trace_prep_replay(ns, nh, nt, sb, se, mute, qc, ntr_cs):
+    NTR, CS = ntr_cs
     G = Geo(ns, nh, nt, sb, se, mute, qc, seed=ns * 1000 + nh * 10 + nt)
-    prep, pairs, qcb, qca, loads = replay(G, TS)
+    prep, pairs, qcb, qca, loads = replay(G, NTR, CS)
     rp, rpairs, rqcb, rqca = direct(G)
     lo, hi = G.sb * G.nh * nt, G.se * G.nh * nt
     assert np.array_equal(prep[lo:hi].view(np.uint32), rp[lo:hi].view(np.uint32))
@@ -240,17 +258,35 @@ def test_trace_prep_replay(ns, nh, nt, sb, se, mute, qc, TS):
     assert max(loads) < G.alloc
 
 
-@pytest.mark.parametrize("TS", [32, 64])
-def test_swizzle_is_conflict_free(TS):
-    RPI = 64 // TS
-    idx = [chunk_at(j, s, TS) for j in range(TR) for s in range(TS)]
-    assert sorted(idx) == list(range(TR * TS))
-    for i in range(TR // RPI):  # row-wise: lanes (h, s) at row RPI i + h
-        banks = [chunk_at(RPI * i + t // TS, t % TS, TS) % 64 for t in range(64)]
-        assert len(set(banks)) == 64
-    for k in range(TS):  # scan: lane t at row t
-        banks = [chunk_at(t, k, TS) % 64 for t in range(64)]
-        assert len(set(banks)) == 64
+@pytest.mark.parametrize("ntr_cs", SHAPES)
+def test_chunk_bound_for_every_trace_length(ntr_cs):
+    """A block's loads reach NTR nt + CS - 2 samples past its first trace
+    (rows up to NTR - 1, the last chunk starting at most nt - 1, a quad up to
+    CS - 1); from the last valid trace that stays inside trace_alloc() for
+    every nt >= 16, the shortest trace the chunk kernels take."""
+    NTR, CS = ntr_cs
+    for nt in range(16, 5000):
+        last = ((nt - 1) // CS) * CS
+        reach = (NTR - 1) * nt + last + CS - 1   # from the block's first trace
+        assert reach < trace_alloc(1, nt)  # a block starting at the last valid trace
+
+
+@pytest.mark.parametrize("ntr_cs", SHAPES)
+def test_swizzle_is_conflict_free(ntr_cs):
+    NTR, CS = ntr_cs
+    QPR = CS // 4
+    idx = [chunk_at(j, s, CS) for j in range(NTR) for s in range(CS)]
+    assert sorted(idx) == list(range(NTR * CS))
+    # quads stay whole and 16-byte aligned
+    for j in range(NTR):
+        for qd in range(QPR):
+            base = chunk_at(j, 4 * qd, CS)
+            assert base % 4 == 0
+            assert [chunk_at(j, 4 * qd + e, CS) for e in range(4)] == [base + e for e in range(4)]
+    # the scan: lane t at row t reads sample k; distinct banks for up to 16 lanes
+    for k in range(CS):
+        banks = [chunk_at(t, k, CS) % 64 for t in range(min(NTR, 16))]
+        assert len(set(banks)) == len(banks)
 
 
 def test_replay_catches_a_short_allocation():
@@ -258,4 +294,4 @@ def test_replay_catches_a_short_allocation():
     G = Geo(1, 65, 17, 0, 1, 0, -1, seed=3)
     G.alloc = 65 * 17
     with pytest.raises(AssertionError):
-        replay(G, 64)
+        replay(G, 16, 256)
