@@ -1009,27 +1009,28 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
         if (!bnd) continue;
         // the cell's bounds over its points of nonzero weight (k_th_bounds)
         const bool use = valid && (iw & 0xFFFFu) != 0u;
-        float lo = use ? th : INFINITY, hi = use ? th : -INFINITY;
+        // each point's quantized direction first: floor and ceil are
+        // monotone and th * BND_Q is exact (a power of two; |th| <= 360), so
+        // the reduced integers are floor(min th * BND_Q) and ceil(max th *
+        // BND_Q), and integer reductions need one DPP instruction a step
+        int lo = use ? (int)floorf(th * BND_Q) : BND_EMPTY;
+        int hi = use ? (int)ceilf(th * BND_Q) : -BND_EMPTY;
         int tmin = use ? (int)iw >> 16 : BND_EMPTY;
-        lo = __int_as_float(wave_reduce(__float_as_int(lo), [](int x, int y) {
-            return __float_as_int(fminf(__int_as_float(x), __int_as_float(y)));
-        }));
-        hi = __int_as_float(wave_reduce(__float_as_int(hi), [](int x, int y) {
-            return __float_as_int(fmaxf(__int_as_float(x), __int_as_float(y)));
-        }));
+        lo = wave_reduce(lo, [](int x, int y) { return min(x, y); });
+        hi = wave_reduce(hi, [](int x, int y) { return max(x, y); });
         tmin = wave_reduce(tmin, [](int x, int y) { return min(x, y); });
         const int gcz = z0 / BND_CZ + cz, gcx = x0 / BND_CX + cx;
         const int ncz = (nz + BND_CZ - 1) / BND_CZ, ncx = (nx + BND_CX - 1) / BND_CX;
         if (lane == 0 && gcz < ncz && gcx < ncx) {
             ThBound b;
             b.pad = 0;
-            if (lo > hi) {
+            if (lo > hi) {  // no point of the cell has weight
                 b.lo = BND_EMPTY;
                 b.hi = -BND_EMPTY;
                 b.tmin = BND_EMPTY;
             } else {
-                b.lo = (int)floorf(lo * BND_Q);
-                b.hi = (int)ceilf(hi * BND_Q);
+                b.lo = lo;
+                b.hi = hi;
                 b.tmin = tmin;
             }
             bnd[(size_t)(gcx * ncz + gcz) * a.nslots + slot] = b;

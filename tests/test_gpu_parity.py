@@ -298,6 +298,31 @@ def test_tuple_pass_reads_table_nodes_directly(monkeypatch):
     assert nd_z == 1  # the receiver side still reads its (plain) nodes
 
 
+@pytest.mark.parametrize("nt", [12, 16])
+def test_short_traces(nt):
+    """Traces of fewer than 16 samples take the one-thread-per-trace
+    preparation (the chunk kernels' unclamped loads need the buffers'
+    padding to cover NTR nt + CS - 2 samples); 16 samples take the chunk
+    kernels.  Both against the FP64 oracle, with the direct mute on."""
+    K = _kmig()
+    rng = np.random.default_rng(nt)
+    nz, nx, d = 40, 50, 0.02
+    npos, dpos = 21, 0.05
+    T, D = _vz_tables(nz, nx, d, dpos * np.arange(npos))
+    dt, ns, nh = 0.05, 4, 20
+    data = rng.normal(size=(ns, nh, nt)).astype(np.float32)
+    data_axes = [[nt, 0.0, dt], [nh, 0.02, 0.037], [ns, 0.1, 0.2]]
+    tax = [[nz, 0.0, d], [nx, 0.0, d], [npos, 0.0, dpos]]
+    kv = dict(cmp=0, mode="angle", angle_min=0, angle_max=60, angle_step=1, variant="resident",
+              tile=9, direct_mute=1, direct_mute_slope=0.2, direct_mute_intercept=0.05,
+              direct_mute_taper=0.1)
+    cig = K.migrate(kv, data, data_axes, T, D, tax, T, D, tax)[0]
+    ref = O.migrate(kv, data, data_axes, T, D, tax, T, D, tax, precision="f64")[0]
+    r = O.rel_l2(cig, ref)
+    print(f"nt={nt}: relL2 vs FP64 {r:.2e}, |cig| max {np.abs(cig).max():.3e}")
+    assert np.abs(ref).max() > 0 and r < TOL
+
+
 def test_fused_tuple_pass_with_gradient_files(monkeypatch):
     # one preset for both runs: the autotuner picks by timing, and presets of
     # other receiver-chunk sizes sum each point in another order
