@@ -69,15 +69,26 @@ def main():
                     help="0: time adj=0 modelling of the workload's migrated image instead")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain device copy of this many MiB (profiler calibration)")
+    ap.add_argument("--gather-probe", type=int, default=0, metavar="LANES_PER_LINE",
+                    help="only run the L1 gather-rate probe (kmig_probe_gather) with this "
+                         "many lanes per 128-byte line (profiler calibration)")
     ap.add_argument("--record-counters", default=None, metavar="OUT_JSON",
                     help="write the migrate kernel's counter profile from rocprofv3 output "
                          "directories (--trace-dir, --pmc-dirs, --copy-dirs) and exit; no GPU")
     ap.add_argument("--trace-dir", default=None)
     ap.add_argument("--pmc-dirs", nargs="*", default=[])
     ap.add_argument("--copy-dirs", nargs="*", default=[])
+    ap.add_argument("--gather-dirs", nargs="*", default=[],
+                    help="rocprofv3 counter directories of `bench.py --gather-probe 1`")
     args = ap.parse_args()
     if args.record_counters:
         return record_counters(args)
+    if args.gather_probe > 0:
+        from kirch_datuming_amd import kmig
+        ms, loads = kmig.probe_gather(args.gather_probe, 2000)
+        print(json.dumps({"gather_probe_lanes_per_line": args.gather_probe, "ms": ms,
+                          "wave_loads": loads}))
+        return
     if args.copy_probe > 0:
         # a plain copy kernel of known size: the HBM counters of a profiler run
         # on it give the scale factor for the migrate kernel's counters
@@ -401,6 +412,8 @@ VALU_PEAK_LANE_INSTR = 256 * 4 * 16 * 2.4e9
 # (157.3 TFLOP/s FP32 vector), the transcendental unit (v_rcp_f32 etc.) at
 # half of it; measured 56-60, 35-38 and 19 T lane-instr/s respectively
 VALU_COST = {"fast": 0.5, "other": 1.0, "trans": 2.0}
+# MI355X: compute units, and XCDs (each counts its own GRBM_GUI_ACTIVE cycles)
+N_CU, N_XCD = 256, 8
 
 
 def migrate_source_sha16():
@@ -415,17 +428,20 @@ def migrate_source_sha16():
 
 
 def roofline(config, precision, st, mig_ms, updates, counts, kernel):
-    """The resident migrate kernel is bound by instruction issue, not by HBM:
-    its receiver tuples stay in registers and culled groups load nothing, so
-    the per-update streamed tuple of the bytes model (16 B/update FP32) is
-    not its traffic.  `frac` is therefore its VALU issue rate against the
-    VALU peak: the VALU instructions of one launch, from the rocprofv3
-    counters of the same kernel build under profiles/, over the launch time
-    measured here with HIP events.  The HBM bytes per launch (FETCH_SIZE and
-    WRITE_SIZE of the same profile, copy-calibrated) give the HBM fraction
-    beside it, and the bytes model's 16 B/update is reported as a reuse
-    factor over them.  A profile recorded for another build of the kernel
-    source is marked stale and not used."""
+    """The resident migrate kernel is bound by its tap gathers in the vector
+    memory path, not by HBM or VALU issue: its receiver tuples stay in
+    registers and culled groups load nothing, so the per-update streamed
+    tuple of the bytes model (16 B/update FP32) is not its traffic, and its
+    three 8-byte gathers per update keep the texture-data unit busy at the
+    rate a pure gather loop reaches (kmig_probe_gather).  `frac` is that busy
+    fraction (busy cycles from the rocprofv3 counters of the same kernel
+    build under profiles/, launch cycles from the time measured here with
+    HIP events) over the probe's; the VALU issue rate of the launch's
+    instruction mix and the HBM bytes (FETCH_SIZE and WRITE_SIZE,
+    copy-calibrated) are reported beside it, the bytes model's 16 B/update
+    as a reuse factor over the HBM bytes.  A profile without the L1 passes
+    reports the VALU rate as the bound; one recorded for another build of the
+    kernel source is marked stale and not used."""
     bpu = BYTES_PER_UPDATE.get(precision, 16.0)
     model = updates * bpu
     base = {"kernel": kernel, "avg_launch_ms": mig_ms, "updates_per_launch": updates,
@@ -464,6 +480,42 @@ def roofline(config, precision, st, mig_ms, updates, counts, kernel):
     peak = VALU_PEAK_LANE_INSTR * n / passes
     traffic = pc["hbm_bytes_per_launch"]
     hbm = traffic / (mig_ms * 1e-3) / 1e9
+    valu_obj = {"achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "T lane-instr/s",
+                "frac": achieved / peak}
+    l1 = None
+    if "tcp_total_cache_accesses_sum" in pc and "l1_probe_accesses_per_clk_per_cu" in pc:
+        # the vector memory pipeline: L1 accesses (a 64-lane 8-byte gather is
+        # at least 16, one per 4 lanes) against the probe's peak rate, at the
+        # clock the profiled launch ran (its GRBM cycles over its duration)
+        cyc = pc["grbm_gui_active"] / N_XCD
+        clk = cyc / (pc["avg_launch_ms"] * 1e-3)
+        live = mig_ms * 1e-3 * clk  # cycles of the launch timed here
+        acc = pc["tcp_total_cache_accesses_sum"]
+        busy = pc["td_td_busy_sum"] / N_CU  # texture-data busy cycles per CU
+        l1 = {"achieved": busy / live, "peak": pc["l1_probe_td_busy"],
+              "unit": "texture-data busy fraction", "frac": busy / live / pc["l1_probe_td_busy"],
+              "l1_accesses_per_clk_per_cu": acc / N_CU / live,
+              "probe_l1_accesses_per_clk_per_cu": pc["l1_probe_accesses_per_clk_per_cu"],
+              "accesses_per_load": acc / pc["sq_insts_vmem_rd"], "clock_ghz": clk / 1e9}
+    if l1 is not None:
+        return {"bound": "l1", "achieved": l1["achieved"], "peak": l1["peak"],
+                "unit": l1["unit"], "frac": l1["frac"],
+                "peak_note": "the vector memory path of the tap gathers: the texture-data "
+                             "unit's busy fraction over the launch (busy cycles from the "
+                             "counter profile, launch cycles from the time measured here) "
+                             "against the busy fraction of kmig_probe_gather, a pure loop of "
+                             "64-lane 8-byte L1-resident gathers on this GPU; the L1 access "
+                             "rate is beside it",
+                "l1": l1, "valu": {**valu_obj,
+                                   "mix": {"fp32_fast": fast / n, "trans": trans / n,
+                                           "other": (n - fast - trans) / n}},
+                "traffic": traffic, "stale": False, "profile": os.path.relpath(prof, ROOT),
+                "valu_lane_instr_per_launch": valu,
+                "valu_instr_per_nominal_update": valu / updates,
+                "hbm": {"achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": hbm / HBM_PEAK_GBS},
+                "reuse_factor_bytes_model_over_traffic": model / traffic,
+                "profile_launch_ms": pc.get("avg_launch_ms"), **base}
     return {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
             "unit": "T lane-instr/s", "frac": achieved / peak,
             "peak_note": "VALU issue peak of this launch's instruction mix: FP32 "
@@ -545,7 +597,15 @@ def record_counters(args):
            "precision": args.precision, "preset": args.tile,
            "source_sha16": migrate_source_sha16(), "avg_launch_ms": avg_ns * 1e-6,
            "trace_calls": calls, **{k: v for k, v in sorted(prof.items())
-                                    if k.startswith("sq_")}}
+                                    if k.startswith(("sq_", "tcp_", "td_", "ta_", "grbm_"))}}
+    # the L1 gather probe's rate on this GPU: accesses and texture-data busy
+    # cycles per CU per cycle (GRBM_GUI_ACTIVE counts the cycles of each of
+    # the XCDs)
+    for d in args.gather_dirs:
+        _, c = counters(d, lambda n: "k_probe_gather" in n)
+        cyc = c["GRBM_GUI_ACTIVE"] / N_XCD
+        out["l1_probe_accesses_per_clk_per_cu"] = c["TCP_TOTAL_CACHE_ACCESSES_sum"] / N_CU / cyc
+        out["l1_probe_td_busy"] = c["TD_TD_BUSY_sum"] / N_CU / cyc
     if "fetch_size" in prof and "write_size" in prof:
         ff = round(factors.get("FETCH_SIZE", 1.0), 2)
         wf = round(factors.get("WRITE_SIZE", 1.0), 2)

@@ -286,6 +286,15 @@ int kmig_set_surrogate_fallback(kmig_ctx *ctx, int side, int mode);
 int kmig_get_surrogate_stats(kmig_ctx *ctx, int side, double *points, double *used,
                              float *max_residual);
 
+/* Measurement aid, no counterpart in the reference: times `iters` rounds of
+   64-lane 8-byte gathers from an L1-resident 16 KB buffer (every lane its own
+   128-byte line for lanes_per_line = 1, or lanes_per_line consecutive lanes
+   sharing one) over 2048 blocks of 256 threads.  Under a profiler its L1 and
+   texture-data counters give the peak rate of the gathers the migrate
+   kernels issue.  *ms = the timed launch, *wave_loads = its wave-level load
+   instructions.  Allocates its own buffers. */
+int kmig_probe_gather(int lanes_per_line, int iters, double *ms, double *wave_loads);
+
 /* Library version string. */
 const char *kmig_version(void);
 

@@ -1506,4 +1506,62 @@ int launch_fill(float *p, size_t n, float v, void *stream)
     return (int)hipGetLastError();
 }
 
+// L1 gather-rate probe: waves of 64 lanes read 8-byte elements of a 16 KB
+// buffer that stays in each CU's L1, every lane from a pseudo-random 128-byte
+// line (lanes_per_line = 1) or lanes_per_line consecutive lanes sharing one.
+// Profiler counters of this kernel (L1 accesses, TD busy cycles) give the
+// vector memory pipeline's peak rate for the gathers the migrate kernels
+// issue.  Indices are taken modulo the buffer.
+__global__ __launch_bounds__(256) void k_probe_gather(const float2 *__restrict__ p, int nlines,
+                                                      int iters, int lpl, float *out)
+{
+    unsigned x = blockIdx.x * 256u + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    float acc = 0.0f;
+    for (int it = 0; it < iters; it++) {
+        x = x * 1664525u + 1013904223u;
+        const unsigned lead = __shfl(x, lane - lane % lpl, 64);
+        const int e = (int)(lead % (unsigned)nlines) * 16 + (lane % lpl) % 16;
+        const float2 v = p[e];
+        acc += v.x + v.y;
+    }
+    if (acc == 12345.0f) out[0] = acc;  // keeps the loads; the buffer is zero
+}
+
 }  // namespace kmig
+
+extern "C" int kmig_probe_gather(int lanes_per_line, int iters, double *ms, double *wave_loads)
+{
+    using namespace kmig;
+    if (lanes_per_line < 1 || lanes_per_line > 64 || iters < 1 || !ms || !wave_loads)
+        return KMIG_E_PARAM;
+    const int nlines = 128;  // 16 KB of float2
+    const unsigned blocks = 256 * 8;
+    float2 *p = nullptr;
+    float *out = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    float t = 0.0f;
+    hipError_t e = hipMalloc((void **)&p, (size_t)nlines * 16 * sizeof(float2));
+    if (e == hipSuccess) e = hipMalloc((void **)&out, sizeof(float));
+    if (e == hipSuccess) e = hipMemset(p, 0, (size_t)nlines * 16 * sizeof(float2));
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_probe_gather, dim3(blocks), dim3(256), 0, 0, p, nlines, 10,
+                           lanes_per_line, out);
+        hipEventRecord(a, 0);
+        hipLaunchKernelGGL(k_probe_gather, dim3(blocks), dim3(256), 0, 0, p, nlines, iters,
+                           lanes_per_line, out);
+        hipEventRecord(b, 0);
+        e = hipEventSynchronize(b);
+    }
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    if (p) hipFree(p);
+    if (out) hipFree(out);
+    if (e != hipSuccess) return KMIG_E_DEVICE;
+    *ms = t;
+    *wave_loads = (double)blocks * 4.0 * iters;
+    return KMIG_OK;
+}

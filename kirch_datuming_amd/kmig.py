@@ -33,7 +33,7 @@ EXPORTS = ("kmig_params_default", "kmig_params_parse", "kmig_params_check", "kmi
            "kmig_batch_sync", "kmig_batch_stats", "kmig_batch_last_error", "kmig_batch_destroy",
            "kmig_set_surrogate", "kmig_get_surrogate_stats", "kmig_batch_stitch",
            "kmig_model", "kmig_count_updates", "kmig_batch_stitch_noalloc", "kmig_set_graph",
-           "kmig_set_surrogate_fallback")
+           "kmig_set_surrogate_fallback", "kmig_probe_gather")
 
 
 class KmigError(RuntimeError):
@@ -112,6 +112,8 @@ def lib():
         L.kmig_stream.argtypes = [vp]
         L.kmig_stream.restype = vp
         L.kmig_version.restype = C.c_char_p
+        L.kmig_probe_gather.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]
         L.kmig_batch_create.argtypes = [C.POINTER(vp), C.POINTER(vp), C.c_int]
         L.kmig_batch_run.argtypes = [vp]
         L.kmig_batch_sync.argtypes = [vp]
@@ -435,3 +437,14 @@ def migrate(kv, data, data_axes, stab, sder, s_axes, rtab, rder, r_axes, grads=N
             if sb <= p.direct_mute_qc_shot < se:
                 qcb, qca = m.qc()
         return cig, qcb, qca
+
+
+def probe_gather(lanes_per_line=1, iters=2000):
+    """Time the L1 gather-rate probe (kmig_probe_gather): returns (ms of the
+    timed launch, its wave-level load instructions)."""
+    L = lib()
+    ms, n = C.c_double(), C.c_double()
+    rc = L.kmig_probe_gather(int(lanes_per_line), int(iters), C.byref(ms), C.byref(n))
+    if rc:
+        raise KmigError(rc, "kmig_probe_gather failed")
+    return ms.value, n.value

@@ -333,3 +333,12 @@ def test_fused_tuple_pass_with_gradient_files(monkeypatch):
     monkeypatch.setenv("KMIG_TUPLE_FUSED", "1")
     fus = _kmig().migrate(kv, *_args(run), grads=run["grads"])[0]
     assert np.array_equal(sep.view(np.uint32), fus.view(np.uint32))
+
+
+def test_gather_probe():
+    """The L1 gather-rate probe behind the bench line's roofline runs and
+    reports its own load count."""
+    ms, loads = _kmig().probe_gather(1, 50)
+    assert ms > 0 and loads == 2048 * 4 * 50
+    with pytest.raises(_kmig().KmigError):
+        _kmig().probe_gather(0, 50)
