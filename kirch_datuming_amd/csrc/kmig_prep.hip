@@ -867,6 +867,25 @@ __device__ __forceinline__ void box_pass(const float2 *src, float2 *dst, int w1z
     __syncthreads();
 }
 
+// x / d for a divisor d fixed per launch, given r = 1.0 / (double)d: the float
+// of the double product x r equals the IEEE float quotient for every x with
+// |x| >= 2^-60, infinities and zero included.  The product is within 2^-52 of
+// x / d relative; a float quotient that is not a rounding midpoint lies at
+// least 2^-49 (relative) from every midpoint (x - m d is a nonzero multiple of
+// 2^(e_x - 48) for a midpoint m), and none is one (m d has an odd significand
+// of at least 25 bits unless d is a power of two, when x / d is exact), so
+// both round alike; tests/recip_exact.c checks it over every such float for
+// several divisors.  Smaller |x| (and NaN) take the division.
+__device__ __forceinline__ float div_fixed(float x, float d, double r)
+{
+    float q = (float)((double)x * r);
+    const bool slow = !(fabsf(x) >= 0x1p-60f || x == 0.0f);
+    // a wave-uniform test: no lane of a wave normally needs the division
+    if (__builtin_expect(__ballot(slow) != 0ull, 0)) q = slow ? x / d : q;
+    return q;
+}
+constexpr double kRecipPi = 1.0 / (double)(float)KPI;
+
 // Wave-wide reduction for full waves (every lane active) with an associative,
 // commutative op: four DPP steps inside each row of 16 lanes (quad_perm
 // [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror: every lane
@@ -927,7 +946,14 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
         }
         __syncthreads();
     }
-    // gradients on the region with halo H1
+    // gradients on the region with halo H1; where every point of the region
+    // has its full stencil inside the image along an axis (xin, zin: decided
+    // once per block), the divisor on that axis is (2 rf) dx or (2 rf) dz
+    // throughout (div_fixed); elsewhere each point divides by its own span
+    const float dfx = (float)(2 * rf) * a.dx, dfz = (float)(2 * rf) * a.dz;
+    const double rfx = 1.0 / (double)dfx, rfz = 1.0 / (double)dfz;
+    const bool xin = x0 - H1 - rf >= 0 && x0 - H1 + w1x - 1 + rf <= nx - 1;
+    const bool zin = z0 - H1 - rf >= 0 && z0 - H1 + w1z - 1 + rf <= nz - 1;
     TileWalk tg(tid, w1z);
     for (int k = tid; k < n1; k += 256, tg.next()) {
         const int lz = tg.l, lx = tg.m;
@@ -941,17 +967,19 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
                 // sT local index of global (z, x): (x - x0 + H0) * w0z + (z - z0 + H0)
                 if (!(nx <= 1 || a.dx == 0.0f)) {
                     const int il = max(0, ix - rf), ir = min(nx - 1, ix + rf);
-                    if (ir != il)
-                        vx = (sT[(ir - x0 + H0) * w0z + (iz - z0 + H0)] -
-                              sT[(il - x0 + H0) * w0z + (iz - z0 + H0)]) /
-                             ((float)(ir - il) * a.dx);
+                    if (ir != il) {
+                        const float num = sT[(ir - x0 + H0) * w0z + (iz - z0 + H0)] -
+                                          sT[(il - x0 + H0) * w0z + (iz - z0 + H0)];
+                        vx = xin ? div_fixed(num, dfx, rfx) : num / ((float)(ir - il) * a.dx);
+                    }
                 }
                 if (!(nz <= 1 || a.dz == 0.0f)) {
                     const int it = max(0, iz - rf), ib = min(nz - 1, iz + rf);
-                    if (ib != it)
-                        vz = (sT[(ix - x0 + H0) * w0z + (ib - z0 + H0)] -
-                              sT[(ix - x0 + H0) * w0z + (it - z0 + H0)]) /
-                             ((float)(ib - it) * a.dz);
+                    if (ib != it) {
+                        const float num = sT[(ix - x0 + H0) * w0z + (ib - z0 + H0)] -
+                                          sT[(ix - x0 + H0) * w0z + (it - z0 + H0)];
+                        vz = zin ? div_fixed(num, dfz, rfz) : num / ((float)(ib - it) * a.dz);
+                    }
                 }
             }
         }
@@ -987,7 +1015,8 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
             const float zdist = zd0 > zd2 ? zd0 : zd2;
             const float ximg = a.x0 + (float)ix * a.dx;
             const float d = fabsf(a.spec[slot].surf_x - ximg);
-            const float deg = atanf(d / zdist) * 180.0f / (float)KPI;
+            // (x * 180) / pi as the float of the double product, exact as IEEE division
+            const float deg = div_fixed(atanf(d / zdist) * 180.0f, (float)KPI, kRecipPi);
             const float wc = leg_weight(deg, a.aper, a.aper_taper);
             const float wf = floor_weight(d, a.rmin, a.rtaper);
             float w = wc > wf ? wc : wf;

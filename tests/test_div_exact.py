@@ -10,7 +10,14 @@ must be caught.
 The same pass also tests whether a smoothed gradient vanishes as |g|^2 <= S
 instead of sqrtf(|g|^2) <= 1e-12 (grad_vanishes); tests/sqrt_cut.c checks the
 two forms agree on every non-negative float for S = 0x179abe15, and that the
-next float up, 0x179abe16, is caught."""
+next float up, 0x179abe16, is caught.
+
+The fused pass also divides by divisors fixed per launch -- the interior
+finite-difference spans (2 r) dx, (2 r) dz and pi in the aperture angle -- as
+the float of x times the divisor's double reciprocal (div_fixed), equal to the
+IEEE quotient for |x| >= 2^-60 by a rounding argument; tests/recip_exact.c
+checks every such float for the divisors of the workloads here and others,
+and that the float reciprocal product (a plausible slip) is caught."""
 import os
 import subprocess
 
@@ -55,3 +62,30 @@ def sqrt_cut(tmp_path_factory):
 
 def test_root_free_gradient_test_is_the_rooted_one(sqrt_cut):
     assert sqrt_cut("179abe15", "179abe16") == {"179abe15": 0, "179abe16": 1}
+
+
+@pytest.fixture(scope="module")
+def recip_exact(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("recip") / "recip_exact")
+    subprocess.run(["gcc", "-O2", "-fopenmp", "-ffp-contract=off",
+                    os.path.join(ROOT, "tests", "recip_exact.c"), "-o", exe, "-lm"], check=True)
+
+    def run(*ds):
+        r = subprocess.run([exe] + list(ds), capture_output=True, text=True, check=True,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(min(16, os.cpu_count() or 1))))
+        return {f.split()[0][2:]: int(f.split()[2]) for f in r.stdout.strip().split("\n")}
+    return run
+
+
+def test_double_reciprocal_product_is_ieee_division(recip_exact):
+    # (2 r) d for radii 1..4 and the grid spacings 0.01, 0.02, 0.015; pi as a
+    # float; and a few others
+    import numpy as np
+    ds = sorted({repr(float(np.float32(2 * r) * np.float32(h)))
+                 for r in (1, 2, 3, 4) for h in (0.01, 0.02, 0.015)})
+    ds += [repr(float(np.float32(np.pi))), "7.0", "0.001", "123.456"]
+    assert all(v == 0 for v in recip_exact(*ds).values())
+
+
+def test_check_catches_the_float_reciprocal(recip_exact):
+    assert recip_exact("f:0.04")["f:0.04"] > 0
