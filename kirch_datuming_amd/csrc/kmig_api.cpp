@@ -1027,26 +1027,32 @@ int side_tuples(kmig_ctx *c, int side, const TuplePrepArgs &a, Tuples &o, size_t
     const bool use_fused = (po.packed || po.packed16) && tuple_fused_ok(a) &&
                            !(env && env[0] == '0');
     if (fused) *fused = use_fused;
-    // the fused pass reads copied table nodes itself (no surrogate, which
-    // replaces the interpolated planes; no gradient files, interpolated with
-    // them); KMIG_TUPLE_DIRECT=0 keeps k_interp
+    // the fused pass reads copied table nodes itself (not with gradient
+    // files, interpolated with the tables; with a surrogate only under the
+    // per-slot fallback, whose taken slots' values go to the planes);
+    // KMIG_TUPLE_DIRECT=0 keeps k_interp
     const char *denv = getenv("KMIG_TUPLE_DIRECT");
+    const bool per_slot = sd.nn_fallback == KMIG_NN_FALLBACK_SLOT;
     TuplePrepArgs af = a;
-    af.direct = use_fused && !sd.nn && !a.use_grad_files &&
+    af.direct = use_fused && (!sd.nn || per_slot) && !a.use_grad_files &&
                 (side == 0 ? c->direct_s : c->direct_r) && !(denv && denv[0] == '0');
     if (!af.direct) HIPCHK(c, (hipError_t)launch_tuple_interp(a, o, st));
     c->n_direct += af.direct ? 1 : 0;
+    if (sd.nn && a.nslots > 0 && af.direct) {
+        af.nn_maxres = sd.nn_maxres + slot0;
+        memcpy(&af.nn_lim, &sd.nn_max_dt, sizeof(af.nn_lim));
+    }
     if (sd.nn && a.nslots > 0) {
         HIPCHK(c, (hipError_t)launch_fill((float *)(sd.nn_used + slot0), (size_t)a.nslots, 0.0f,
                                           st));
         HIPCHK(c, (hipError_t)launch_fill((float *)(sd.nn_maxres + slot0), (size_t)a.nslots,
                                           0.0f, st));
-        const bool per_slot = sd.nn_fallback == KMIG_NN_FALLBACK_SLOT;
         HIPCHK(c, (hipError_t)nn_replace_tuples(sd.nn, a.spec, a.nslots, c->nz, c->nzx, c->z0,
                                                 c->dz, c->x0, c->dx, a.dside, sd.nn_max_dt, o.T,
                                                 o.A, sd.nn_used + slot0, sd.nn_maxres + slot0,
                                                 per_slot ? c->tmp1 : nullptr,
-                                                per_slot ? c->tmp2 : nullptr, st));
+                                                per_slot ? c->tmp2 : nullptr,
+                                                af.direct ? a.tab : nullptr, st));
     }
     if (use_fused && po.packed) {
         HIPCHK(c, (hipError_t)launch_tuple_fused(af, o, po.packed, c->dt, po.th_scale,

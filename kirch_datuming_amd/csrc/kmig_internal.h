@@ -74,6 +74,11 @@ struct TuplePrepArgs {
     // so T and |dT/ds * dside| are read from tab / der at the slot's node
     // instead of the interpolation's planes (which are then not written)
     int direct;
+    // with direct and a surrogate (per-slot fallback): the slots whose
+    // largest residual nn_maxres[slot] is within nn_lim (float bits) took
+    // the network, whose T and A are in the planes; the others read the node
+    const unsigned *nn_maxres;
+    unsigned nn_lim;
 };
 
 struct TracePrepArgs {

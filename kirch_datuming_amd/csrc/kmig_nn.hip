@@ -523,6 +523,8 @@ struct NnInferArgs {
     int sample_only;
     const unsigned *skip;
     unsigned skip_lim;
+    // per-slot fallback with the tables read at the slot's node ([npos][nzx])
+    const float *tab;
 };
 
 template <bool Tuples>
@@ -637,7 +639,9 @@ __global__ __launch_bounds__(256) void k_nn_infer(const NnInferArgs a)
             const float T = dist[b] * S0 * (1.0f + y[b][0] + b5[0]);
             const float Ts = S0 * (y[b][3] + b5[3]);
             if (Tuples) {
-                float res = fabsf(T - a.T[p]);
+                const float Tt = a.tab ? a.tab[(size_t)a.spec[p / a.nzx].node * a.nzx + p % a.nzx]
+                                       : a.T[p];
+                float res = fabsf(T - Tt);
                 if (a.sample_only) {
                 } else if (a.Tn) {
                     a.Tn[p] = T;
@@ -1002,7 +1006,7 @@ int nn_prepare(kmig_nn *n)
 int nn_replace_tuples(const kmig_nn *n, const SlotSpec *spec, int nslots, int nz, size_t nzx,
                       float z0, float dz, float x0, float dx, float dside, float max_dt, float *T,
                       float *A, int *used, unsigned *maxres, float *Tn, float *An,
-                      hipStream_t st)
+                      const float *tab, hipStream_t st)
 {
     if (!n || !n->blob_valid) return (int)hipErrorNotReady;
     if (nslots <= 0 || nzx == 0) return 0;
@@ -1026,6 +1030,7 @@ int nn_replace_tuples(const kmig_nn *n, const SlotSpec *spec, int nslots, int nz
     a.maxres = maxres;
     a.Tn = Tn;
     a.An = An;
+    a.tab = Tn ? tab : nullptr;
     a.pstride = 1;
     unsigned lim;
     memcpy(&lim, &max_dt, sizeof(lim));

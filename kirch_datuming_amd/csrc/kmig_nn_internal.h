@@ -49,9 +49,13 @@ int nn_prepare(kmig_nn *nn);
 // With Tn/An given (per-slot fallback), the network's values go there and a
 // slot takes them at every point only if its largest residual is within
 // max_dt (used = its points or 0); without them each point is decided alone.
+// With tab given (per-slot fallback only), the tables' T is read at each
+// slot's node of tab ([npos][nzx]) instead of from T, which the table
+// interpolation then need not fill: T/A receive the network's values for the
+// slots that take it, and only those.
 int nn_replace_tuples(const kmig_nn *nn, const SlotSpec *spec, int nslots, int nz, size_t nzx,
                       float z0, float dz, float x0, float dx, float dside, float max_dt, float *T,
                       float *A, int *used, unsigned *maxres, float *Tn, float *An,
-                      hipStream_t st);
+                      const float *tab, hipStream_t st);
 
 }  // namespace kmig

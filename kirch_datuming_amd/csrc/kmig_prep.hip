@@ -934,9 +934,11 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
     const size_t base = (size_t)slot * a.nzx;
     const int tid = threadIdx.x;
     // the slot's traveltimes and position derivatives: the table node's
-    // (a.direct, k_interp's copy case) or the interpolated planes
-    const size_t node = a.direct ? (size_t)a.spec[slot].node * a.nzx : 0;
-    const float *__restrict__ Tsrc = a.direct ? a.tab + node : o.T + base;
+    // (a.direct, k_interp's copy case) or the planes (interpolated, or the
+    // network's for a slot that took it)
+    const bool node_read = a.direct && !(a.nn_maxres && a.nn_maxres[slot] <= a.nn_lim);
+    const size_t node = node_read ? (size_t)a.spec[slot].node * a.nzx : 0;
+    const float *__restrict__ Tsrc = node_read ? a.tab + node : o.T + base;
     if (!a.use_grad_files) {
         TileWalk t(tid, w0z);
         for (int k = tid; k < w0z * w0x; k += 256, t.next()) {
@@ -1025,7 +1027,7 @@ __global__ __launch_bounds__(256) void k_tuple_fused(const TuplePrepArgs a, cons
             // T from the staged region where there is one; A as k_interp
             // forms it for a copied node (|dT/ds * dside|)
             const float T = a.use_grad_files ? Tsrc[pi] : sT[(ix - x0 + H0) * w0z + (iz - z0 + H0)];
-            const float A = a.direct ? fabsf(a.der[node + pi] * a.dside) : o.A[oi];
+            const float A = node_read ? fabsf(a.der[node + pi] * a.dside) : o.A[oi];
             const PackFields f = pack_fields(T, A, gx, gz, w, inv_dt, th_scale);
             if constexpr (sizeof(Out) == sizeof(PackedTuple16)) {
                 out[oi] = tuple16(f, scale[slot].a_scale, precision);
