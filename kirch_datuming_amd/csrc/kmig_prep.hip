@@ -763,7 +763,11 @@ __global__ __launch_bounds__(256) void k_pack16(const float *T, const float *A, 
 // carries a halo of 2 rs (+ rf for the T tile); a stage's values are exact
 // wherever the remaining halo still covers its window (positions nearer the
 // tile border read a window clipped by the tile and are never used).
-constexpr int FZ = 64, FX = 16, FUSED_MAX_R = 4;
+// 32 x 24 points (twelve bound cells): 26.8 KB of LDS at smooth = radius = 2,
+// five blocks per CU; measured on the salt workload against 64 x 16 (36.2 KB,
+// four blocks), 32 x 16, 16 x 32, 32 x 32, 16 x 48, 48 x 16, 48 x 32 and
+// 64 x 32, where more halo or fewer blocks per CU cost more.
+constexpr int FZ = 32, FX = 24, FUSED_MAX_R = 4;
 
 // (k % w, k / w) for k = tid, tid + 256, tid + 512, ...: one division up
 // front, then steps of (256 % w, 256 / w) with a carry (the tile loops of

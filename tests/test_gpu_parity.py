@@ -227,7 +227,7 @@ def test_fused_tuple_pass_is_bitwise_the_separate_passes(monkeypatch, radius, sm
     """The one-pass LDS-tiled tuple finish (FD, 2x smoothing, weights, packing)
     gives the same packed tuples, hence the same CIG bit for bit, as the
     separate passes, on a grid of several tiles in z and x with partial edge
-    tiles (130 x 75 points against 64 x 16 tiles)."""
+    tiles (130 x 75 points against 32 x 24 tiles)."""
     rng = np.random.default_rng(21)
     nz, nx, d = 130, 75, 0.02
     npos, dpos = 31, 0.05
